@@ -1,0 +1,208 @@
+/*
+ * mpdpft.h -- C ABI of the MI355X-native self-consistent DPFT loop.
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every entry point returns an
+ * int status (MPDPFT_OK or a negative MPDPFT_E* code) and never exits the
+ * process.  A context owns its device buffers and one HIP stream; it is not
+ * thread-safe, distinct contexts may be driven from distinct host threads.
+ *
+ * Field layout (host and device): row-major over the (steps+1)^DIM grid with x
+ * slowest, index = i*EL*EL + j*EL + k, EL = steps+1; per-species fields are
+ * stacked species-major ([S][G]).  This is the layout of the reference's
+ * datastruct fields (mpDPFT.h:71-89, mpDPFT.cpp:7169-7186).
+ */
+#ifndef MPDPFT_H
+#define MPDPFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPDPFT_ABI_VERSION 1
+
+#define MPDPFT_MAX_SPECIES 8
+#define MPDPFT_MAX_LIST 16
+#define MPDPFT_MAX_MPP 64
+#define MPDPFT_MAX_CENTRES 256
+
+/* status codes */
+#define MPDPFT_OK 0
+#define MPDPFT_EINVAL (-1)     /* invalid parameter or unsupported feature       */
+#define MPDPFT_ENOMEM (-2)     /* host or device allocation failed               */
+#define MPDPFT_EHIP (-3)       /* a HIP runtime call failed                       */
+#define MPDPFT_ENOTCONV (-4)   /* SC loop hit maxSCcount (results still valid)    */
+#define MPDPFT_ENOBRACKET (-5) /* mu bracket not found (mpDPFT.cpp:4657-4661)     */
+#define MPDPFT_EPULAY (-6)     /* Pulay Gram singular, fell back to linear mixing */
+#define MPDPFT_EIO (-7)        /* file could not be read or written               */
+#define MPDPFT_ENODEV (-8)     /* no HIP device                                   */
+
+/* Extension environment types (the reference's GetEnv knows 0-25, 30, 61;
+ * mpDPFT.cpp:1385-1698).  40: softened Coulomb -mpp[s]/sqrt(r^2+mpp[S+s]^2)
+ * centred at the origin.  41: sum over the centres list of softened Coulomb
+ * wells -Z_c/sqrt(|r-R_c|^2+a_c^2). */
+#define MPDPFT_ENV_SOFT_COULOMB 40
+#define MPDPFT_ENV_SOFT_COULOMB_CENTRES 41
+
+/* Interaction ids follow mpDPFT.input line 27 (mpDPFT.cpp:2240-2318):
+ * 0 none, 1 contact, 8 Hartree, 10 single-species contact, 11 Dirac exchange,
+ * -1 LDA exchange, -12 PW92 LDA correlation (native, replaces libxc). */
+
+/*
+ * All parameters of one solve.  Fields 'System'..'Print' mirror the 47
+ * positional lines of mpDPFT.input as parsed by GetInputParameters
+ * (mpDPFT.cpp:621-721); the rest are the hard-coded defaults of
+ * InitializeHardCodedParameters (mpDPFT.cpp:858-1383, mpDPFT.h:79) exposed as
+ * overridable values, and the extension keys.  mpdpft_params_default() fills
+ * every field; mpdpft_parse_input() overwrites what the file states.
+ */
+typedef struct mpdpft_params {
+    int abi_version;
+    /* mpDPFT.input, lines 1-47 */
+    int System;
+    int Symmetry;
+    int TaskType;
+    int TaskParameterCount;
+    int DIM;
+    int Units;
+    double edge;
+    int steps;
+    int S;
+    double Abundances[MPDPFT_MAX_SPECIES];
+    double RelAcc;
+    double InternalAcc;
+    int K;
+    int DensityExpression;
+    int n_mpp;
+    double mpp[MPDPFT_MAX_MPP];
+    int n_Mpp;
+    double Mpp[MPDPFT_MAX_LIST];
+    double tauVec[MPDPFT_MAX_SPECIES];
+    double tVec[MPDPFT_MAX_SPECIES];
+    double TVec[MPDPFT_MAX_SPECIES];
+    int Environments[MPDPFT_MAX_SPECIES];
+    double stretchfactor;
+    double Wall;
+    double Noise;
+    int InterpolVQ;
+    int n_interactions;
+    int Interactions[MPDPFT_MAX_LIST];
+    double incrementalV;
+    double muVec[MPDPFT_MAX_SPECIES];
+    double DeltamuModifier;
+    double Mixer[3];
+    double thetaVec[MPDPFT_MAX_SPECIES];
+    double SCcriterion;
+    int maxSCcount;
+    int method;
+    int DelFieldMethod;
+    int MovieQ;
+    int n_schedule;
+    int Schedule[MPDPFT_MAX_LIST];
+    int n_ekin;
+    int EkinTypes[MPDPFT_MAX_LIST];
+    double alpha;
+    double beta;
+    double gamma;
+    double gammaH;
+    double degeneracy;
+    int n_regularize;
+    int regularize[MPDPFT_MAX_LIST];
+    double RegularizationThreshold;
+    int Print;
+    /* hard-coded defaults of the reference, overridable */
+    int TaperOff;                    /* mpDPFT.h:79, default 1 */
+    int RegularizeCoulombSingularity;/* mpDPFT.h:79, default 1 */
+    double SmearingExtentDx;         /* mpDPFT.cpp:1105, default 6 (x Deltax) */
+    int OutPrecision;                /* mpDPFT.cpp:1363, default 16 */
+    int MaxIteration;                /* mpDPFT.cpp:1101, default 4 (EkinAiT03D) */
+    /* extension keys ("value //Key" lines after line 47) */
+    int HartreeMode;       /* 0: periodic + Gaussian compensation (reference), 1: isolated, zero-padded 2s grid */
+    double SmoothingKWidth;/* >0: DelField smoothing exp(-k^2/(2 w^2)); <=0: reference exp(-(6dx)^2 k^2/2) */
+    int ConvergenceMode;   /* 0: reference overlap rule CC<SCcriterion; 1: max|V-V_old| < SCcriterion */
+    int Precision;         /* 0: FP64 throughout; 1: FP32 bulk + FP64 reductions/mu/Pulay + FP64 polish */
+    int WriteFiles;        /* 1: mpdpft_run_dir writes Den/V/Control files (default 1) */
+    int n_centres;         /* environment 41 */
+    double centres[MPDPFT_MAX_CENTRES][5]; /* x, y, z, Z, a */
+} mpdpft_params;
+
+/* Results of one solve (the quantities the reference prints in its final
+ * block, mpDPFT.cpp:12716-12800). */
+typedef struct mpdpft_result {
+    int S;
+    int SCcount;          /* loop counter at exit; the file prints SCcount-1 */
+    int converged;        /* 1 if ExitSCcount reached 3 */
+    int LinMix, PulMix;
+    int errorCount, warningCount;
+    double CC;
+    double muVec[MPDPFT_MAX_SPECIES];
+    double FinalAbundances[MPDPFT_MAX_SPECIES];
+    double DeltamuModifier;
+    double TotalEnergy;
+    double DispersalEnergies[MPDPFT_MAX_SPECIES];
+    double EnvironmentalEnergies[MPDPFT_MAX_SPECIES];
+    double InteractionEnergies[MPDPFT_MAX_LIST];
+    double AuxEkin[MPDPFT_MAX_SPECIES][4];
+    double solve_seconds;  /* wall time of the SC loop (device-synchronised) */
+    int bisection_steps;   /* total density evaluations inside mu bisection */
+} mpdpft_result;
+
+typedef struct mpdpft_ctx mpdpft_ctx;
+
+/* ---- parameters and files ------------------------------------------------ */
+int mpdpft_abi_version(void);
+/* sizeof(mpdpft_params) and sizeof(mpdpft_result), for FFI bindings to check layouts. */
+size_t mpdpft_params_sizeof(void);
+size_t mpdpft_result_sizeof(void);
+/* Defaults for every field (the reference's hard-coded values). */
+void mpdpft_params_default(mpdpft_params *p);
+/* Parse an mpDPFT.input file (47 positional lines + optional extension keys);
+ * replaces GetInputParameters, mpDPFT.cpp:621-721. */
+int mpdpft_parse_input(const char *path, mpdpft_params *p);
+
+/* ---- context ---------------------------------------------------------------- */
+/* Validates p, allocates device memory, builds grids and environments
+ * (InitializeHardCodedParameters + InitializeEnvironments, mpDPFT.cpp:858,
+ * 6362).  device < 0 picks the current HIP device. */
+int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **out);
+void mpdpft_destroy(mpdpft_ctx *ctx);
+const char *mpdpft_last_error(const mpdpft_ctx *ctx);
+size_t mpdpft_grid_size(const mpdpft_ctx *ctx);
+
+/* Whole self-consistent loop on the device (GetData, mpDPFT.cpp:135-244,
+ * loop body :210-225), then energies (ExportData's GetEnergy, :12403). */
+int mpdpft_solve(mpdpft_ctx *ctx, mpdpft_result *res);
+
+/* Copy-out of [S][G] fields (caller-owned host arrays of S*G doubles). */
+int mpdpft_get_density(mpdpft_ctx *ctx, double *host);
+int mpdpft_get_potential(mpdpft_ctx *ctx, double *host);
+int mpdpft_get_environment(mpdpft_ctx *ctx, double *host);
+/* Replace V (and the initial density is recomputed from it at solve time). */
+int mpdpft_set_potential(mpdpft_ctx *ctx, const double *host);
+
+/* ---- stage entry points (one call = one reference routine) ---------------- */
+/* Centred continuous-FT approximation on a complex [(s+1)^D][2] field,
+ * sign -1 forward / +1 backward (fftParallel, mpDPFT.cpp:8173-8335). */
+int mpdpft_fft(mpdpft_ctx *ctx, int sign, double *host_complex);
+/* Spectral first/second derivatives of field f (host, [G]) with method 4/5
+ * (DelField, mpDPFT.cpp:7243-7582); outputs [G] each, any may be NULL. */
+int mpdpft_delfield(mpdpft_ctx *ctx, const double *f, double *dx, double *dy, double *dz,
+                    double *gradsq, double *laplacian);
+/* Interaction potential of one type for species s given densities [S][G]
+ * (getVint, mpDPFT.cpp:2240-2318). */
+int mpdpft_interaction_potential(mpdpft_ctx *ctx, int type, int s, const double *den, double *vint);
+/* Density of species s at chemical potential mu for potential v [G]
+ * (GetDensity, mpDPFT.cpp:2142-2238); returns the trapezoid integral in *N. */
+int mpdpft_density(mpdpft_ctx *ctx, int s, double mu, const double *v, double *den, double *N);
+
+/* ---- drop-in: run in a directory like ./mpDPFT --------------------------- */
+/* Reads dir/mpDPFT.input, solves, writes dir/mpDPFT_Den.dat, mpDPFT_V.dat and
+ * mpDPFT_Control.dat in the reference formats (ExportData, :12399-12520). */
+int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,128 @@
+"""mpdpft_amd -- MI355X-native self-consistent DPFT loop (drop-in for mpDPFT's GetData).
+
+The compute path is libmpdpft.so (gfx950 HIP kernels + C++ host loop) behind
+the C ABI in include/mpdpft.h; this package is its Python binding.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import Params, Result, load_library
+
+__all__ = ["Params", "Result", "default_params", "parse_input", "Context", "run_dir", "load_library",
+           "MpdpftError"]
+
+
+class MpdpftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mpdpft error %d: %s" % (code, msg))
+        self.code = code
+
+
+def default_params():
+    p = Params()
+    load_library().mpdpft_params_default(ctypes.byref(p))
+    return p
+
+
+def parse_input(path, base=None):
+    p = base if base is not None else default_params()
+    rc = load_library().mpdpft_parse_input(str(path).encode(), ctypes.byref(p))
+    if rc != _abi.OK:
+        raise MpdpftError(rc, "cannot parse %s" % path)
+    return p
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """One solver context on one GPU (owns its device buffers and stream)."""
+
+    def __init__(self, params, device=-1):
+        self._lib = load_library()
+        self._h = ctypes.c_void_p()
+        self.params = params
+        rc = self._lib.mpdpft_create(ctypes.byref(params), device, ctypes.byref(self._h))
+        if rc != _abi.OK:
+            msg = self._lib.mpdpft_last_error(self._h).decode() if self._h else ""
+            self._lib.mpdpft_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            raise MpdpftError(rc, msg)
+        self.G = self._lib.mpdpft_grid_size(self._h)
+        self.S = params.S
+
+    def close(self):
+        if self._h:
+            self._lib.mpdpft_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def _check(self, rc, allow=(_abi.OK,)):
+        if rc not in allow:
+            raise MpdpftError(rc, self._lib.mpdpft_last_error(self._h).decode())
+        return rc
+
+    def solve(self):
+        r = Result()
+        rc = self._lib.mpdpft_solve(self._h, ctypes.byref(r))
+        self._check(rc, (_abi.OK, _abi.ENOTCONV))
+        return r
+
+    def _field(self, fn):
+        a = np.empty(self.S * self.G)
+        self._check(fn(self._h, _dp(a)))
+        return a.reshape(self.S, self.G)
+
+    def density(self):
+        return self._field(self._lib.mpdpft_get_density)
+
+    def potential(self):
+        return self._field(self._lib.mpdpft_get_potential)
+
+    def environment(self):
+        return self._field(self._lib.mpdpft_get_environment)
+
+    def set_potential(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        self._check(self._lib.mpdpft_set_potential(self._h, _dp(v)))
+
+    def fft(self, field, sign):
+        """Centred transform of a complex field of G points (fftParallel)."""
+        a = np.ascontiguousarray(field, dtype=np.complex128).copy()
+        self._check(self._lib.mpdpft_fft(self._h, sign, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return a
+
+    def delfield(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        outs = [np.zeros(self.G) for _ in range(5)]
+        self._check(self._lib.mpdpft_delfield(self._h, _dp(f), *[_dp(o) for o in outs]))
+        return dict(zip(("dx", "dy", "dz", "gradsq", "laplacian"), outs))
+
+    def interaction_potential(self, type_, s, den):
+        den = np.ascontiguousarray(den, dtype=np.float64).reshape(-1)
+        out = np.zeros(self.G)
+        self._check(self._lib.mpdpft_interaction_potential(self._h, type_, s, _dp(den), _dp(out)))
+        return out
+
+    def density_at(self, s, mu, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.G)
+        N = ctypes.c_double()
+        self._check(self._lib.mpdpft_density(self._h, s, mu, _dp(v), _dp(out), ctypes.byref(N)))
+        return out, N.value
+
+
+def run_dir(path, device=-1):
+    r = Result()
+    rc = load_library().mpdpft_run_dir(str(path).encode(), device, ctypes.byref(r))
+    if rc not in (_abi.OK, _abi.ENOTCONV):
+        raise MpdpftError(rc, "run_dir(%s)" % path)
+    return r

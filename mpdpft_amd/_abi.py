@@ -1,0 +1,128 @@
+"""ctypes mirror of include/mpdpft.h (the binding a Python caller uses).
+
+The library is loaded from mpdpft_amd/lib/libmpdpft.so (built in-tree by
+mpdpft_amd/build.py).  A missing library raises: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+MAX_SPECIES = 8
+MAX_LIST = 16
+MAX_MPP = 64
+MAX_CENTRES = 256
+
+OK = 0
+EINVAL = -1
+ENOMEM = -2
+EHIP = -3
+ENOTCONV = -4
+ENOBRACKET = -5
+EPULAY = -6
+EIO = -7
+ENODEV = -8
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", _i),
+        ("System", _i), ("Symmetry", _i), ("TaskType", _i), ("TaskParameterCount", _i),
+        ("DIM", _i), ("Units", _i), ("edge", _d), ("steps", _i), ("S", _i),
+        ("Abundances", _d * MAX_SPECIES), ("RelAcc", _d), ("InternalAcc", _d), ("K", _i),
+        ("DensityExpression", _i), ("n_mpp", _i), ("mpp", _d * MAX_MPP), ("n_Mpp", _i),
+        ("Mpp", _d * MAX_LIST), ("tauVec", _d * MAX_SPECIES), ("tVec", _d * MAX_SPECIES),
+        ("TVec", _d * MAX_SPECIES), ("Environments", _i * MAX_SPECIES), ("stretchfactor", _d),
+        ("Wall", _d), ("Noise", _d), ("InterpolVQ", _i), ("n_interactions", _i),
+        ("Interactions", _i * MAX_LIST), ("incrementalV", _d), ("muVec", _d * MAX_SPECIES),
+        ("DeltamuModifier", _d), ("Mixer", _d * 3), ("thetaVec", _d * MAX_SPECIES),
+        ("SCcriterion", _d), ("maxSCcount", _i), ("method", _i), ("DelFieldMethod", _i),
+        ("MovieQ", _i), ("n_schedule", _i), ("Schedule", _i * MAX_LIST), ("n_ekin", _i),
+        ("EkinTypes", _i * MAX_LIST), ("alpha", _d), ("beta", _d), ("gamma", _d), ("gammaH", _d),
+        ("degeneracy", _d), ("n_regularize", _i), ("regularize", _i * MAX_LIST),
+        ("RegularizationThreshold", _d), ("Print", _i),
+        ("TaperOff", _i), ("RegularizeCoulombSingularity", _i), ("SmearingExtentDx", _d),
+        ("OutPrecision", _i), ("MaxIteration", _i),
+        ("HartreeMode", _i), ("SmoothingKWidth", _d), ("ConvergenceMode", _i), ("Precision", _i),
+        ("WriteFiles", _i), ("n_centres", _i), ("centres", (_d * 5) * MAX_CENTRES),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("S", _i), ("SCcount", _i), ("converged", _i), ("LinMix", _i), ("PulMix", _i),
+        ("errorCount", _i), ("warningCount", _i), ("CC", _d), ("muVec", _d * MAX_SPECIES),
+        ("FinalAbundances", _d * MAX_SPECIES), ("DeltamuModifier", _d), ("TotalEnergy", _d),
+        ("DispersalEnergies", _d * MAX_SPECIES), ("EnvironmentalEnergies", _d * MAX_SPECIES),
+        ("InteractionEnergies", _d * MAX_LIST), ("AuxEkin", (_d * 4) * MAX_SPECIES),
+        ("solve_seconds", _d), ("bisection_steps", _i),
+    ]
+
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmpdpft.so")
+
+# Every entry point declared in include/mpdpft.h.
+EXPORTS = [
+    "mpdpft_abi_version", "mpdpft_params_sizeof", "mpdpft_result_sizeof", "mpdpft_params_default",
+    "mpdpft_parse_input", "mpdpft_create", "mpdpft_destroy", "mpdpft_last_error", "mpdpft_grid_size",
+    "mpdpft_solve", "mpdpft_get_density", "mpdpft_get_potential", "mpdpft_get_environment",
+    "mpdpft_set_potential", "mpdpft_fft", "mpdpft_delfield", "mpdpft_interaction_potential",
+    "mpdpft_density", "mpdpft_run_dir",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libmpdpft.so; raises OSError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError("libmpdpft.so not built (%s); run mpdpft_amd/build.py" % path)
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER(Params)
+    R = ctypes.POINTER(Result)
+    dp = ctypes.POINTER(ctypes.c_double)
+    vp = ctypes.c_void_p
+    lib.mpdpft_abi_version.restype = _i
+    lib.mpdpft_params_sizeof.restype = ctypes.c_size_t
+    lib.mpdpft_result_sizeof.restype = ctypes.c_size_t
+    lib.mpdpft_params_default.argtypes = [P]
+    lib.mpdpft_params_default.restype = None
+    lib.mpdpft_parse_input.argtypes = [ctypes.c_char_p, P]
+    lib.mpdpft_parse_input.restype = _i
+    lib.mpdpft_create.argtypes = [P, _i, ctypes.POINTER(vp)]
+    lib.mpdpft_create.restype = _i
+    lib.mpdpft_destroy.argtypes = [vp]
+    lib.mpdpft_destroy.restype = None
+    lib.mpdpft_last_error.argtypes = [vp]
+    lib.mpdpft_last_error.restype = ctypes.c_char_p
+    lib.mpdpft_grid_size.argtypes = [vp]
+    lib.mpdpft_grid_size.restype = ctypes.c_size_t
+    lib.mpdpft_solve.argtypes = [vp, R]
+    lib.mpdpft_solve.restype = _i
+    for n in ("mpdpft_get_density", "mpdpft_get_potential", "mpdpft_get_environment"):
+        getattr(lib, n).argtypes = [vp, dp]
+        getattr(lib, n).restype = _i
+    lib.mpdpft_set_potential.argtypes = [vp, dp]
+    lib.mpdpft_set_potential.restype = _i
+    lib.mpdpft_fft.argtypes = [vp, _i, dp]
+    lib.mpdpft_fft.restype = _i
+    lib.mpdpft_delfield.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    lib.mpdpft_delfield.restype = _i
+    lib.mpdpft_interaction_potential.argtypes = [vp, _i, _i, dp, dp]
+    lib.mpdpft_interaction_potential.restype = _i
+    lib.mpdpft_density.argtypes = [vp, _i, _d, dp, dp, dp]
+    lib.mpdpft_density.restype = _i
+    lib.mpdpft_run_dir.argtypes = [ctypes.c_char_p, _i, R]
+    lib.mpdpft_run_dir.restype = _i
+    if lib.mpdpft_params_sizeof() != ctypes.sizeof(Params):
+        raise OSError("mpdpft_params layout mismatch: C %d vs ctypes %d"
+                      % (lib.mpdpft_params_sizeof(), ctypes.sizeof(Params)))
+    if lib.mpdpft_result_sizeof() != ctypes.sizeof(Result):
+        raise OSError("mpdpft_result layout mismatch")
+    _lib = lib
+    return lib

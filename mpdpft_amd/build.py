@@ -1,0 +1,67 @@
+"""Build the gfx950 library, the drop-in CLI and (in-tree) nothing else.
+
+    python -m mpdpft_amd.build        # or: python mpdpft_amd/build.py
+
+Compiles every source under mpdpft_amd/csrc with hipcc for gfx950 into
+mpdpft_amd/lib/libmpdpft.so and mpdpft_amd/bin/mpdpft_hip.  Object files go
+to mpdpft_amd/_build/ (ignored by git).
+"""
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "lib", "libmpdpft.so")
+CLI = os.path.join(HERE, "bin", "mpdpft_hip")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+# Packed FP32 instructions stay disabled in all device code.
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+LIB_SOURCES = ["params.cpp", "kernels.hip", "solver.cpp", "nait.hip", "output.cpp"]
+
+
+def _compile(src):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(BUILD, src + ".o")
+    cmd = [HIPCC] + COMMON + DEVICE
+    if src.endswith(".cpp"):
+        cmd += ["-x", "hip"]
+    cmd += ["-c", path, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s" % (src, msgs))
+    return obj
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(_compile, LIB_SOURCES))
+    tmp = LIB + ".tmp"
+    subprocess.run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs,
+                   check=True, capture_output=True)
+    os.replace(tmp, LIB)
+    main_obj = _compile("main.cpp")
+    subprocess.run([HIPCC, "--offload-arch=" + ARCH, "-o", CLI, main_obj, "-L", os.path.dirname(LIB),
+                    "-lmpdpft", "-Wl,-rpath,$ORIGIN/../lib"], check=True, capture_output=True)
+    if verbose:
+        print("built", LIB, "and", CLI)
+    return LIB
+
+
+if __name__ == "__main__":
+    try:
+        build()
+    except Exception as e:  # noqa: BLE001
+        print(e, file=sys.stderr)
+        sys.exit(1)

@@ -1,0 +1,125 @@
+"""Named workloads and the mpDPFT.input writer.
+
+C0: 2D two-species TF + contact gas (N = 100 each, 128^2 on [-16,16]^2).
+C1: 3D single-species electrons, softened Coulomb Z = 10, a = 0.5, N = 10,
+    Hartree (isolated, zero-padded 256^3) + Dirac + PW92, 128^3 on a 20 bohr cube.
+"""
+from . import default_params
+
+# Names of the 47 positional lines of mpDPFT.input, in order.
+LINE_NAMES = [
+    "System", "Symmetry", "TaskType", "TaskParameterCount", "TaskHyperBox", "DIM", "Units", "edge", "steps",
+    "S", "Abundances", "RelAcc", "InternalAcc", "K", "Resources", "DensityExpression", "mpp", "Mpp", "tauVec",
+    "tVec", "TVec", "Environments", "stretchfactor", "Wall", "Noise", "InterpolVQ", "Interactions",
+    "incrementalV", "muVec", "DeltamuModifier", "Mixer", "thetaVec", "SCcriterion", "maxSCcount", "method",
+    "DelFieldMethod", "MovieQ", "Schedule", "EkinTypes", "alpha", "beta", "gamma", "gammaH", "degeneracy",
+    "regularize", "RegularizationThreshold", "Print",
+]
+
+
+def _fmt(v):
+    if isinstance(v, float):
+        return repr(v)
+    return str(v)
+
+
+def write_input(p, path, extensions=True):
+    """Write params as an mpDPFT.input file (47 positional lines + extension keys)."""
+    S = p.S
+
+    def vec(name, n):
+        return " ".join(_fmt(x) for x in list(getattr(p, name))[:n])
+
+    vals = {
+        "System": p.System, "Symmetry": p.Symmetry, "TaskType": p.TaskType,
+        "TaskParameterCount": p.TaskParameterCount, "TaskHyperBox": "0. 0.", "DIM": p.DIM, "Units": p.Units,
+        "edge": _fmt(p.edge), "steps": p.steps, "S": S, "Abundances": vec("Abundances", S),
+        "RelAcc": _fmt(p.RelAcc), "InternalAcc": _fmt(p.InternalAcc), "K": p.K, "Resources": "0.",
+        "DensityExpression": p.DensityExpression, "mpp": vec("mpp", p.n_mpp), "Mpp": vec("Mpp", p.n_Mpp) or "0.",
+        "tauVec": vec("tauVec", S), "tVec": vec("tVec", S), "TVec": vec("TVec", S),
+        "Environments": vec("Environments", S), "stretchfactor": _fmt(p.stretchfactor), "Wall": _fmt(p.Wall),
+        "Noise": _fmt(p.Noise), "InterpolVQ": p.InterpolVQ, "Interactions": vec("Interactions", p.n_interactions),
+        "incrementalV": _fmt(p.incrementalV), "muVec": vec("muVec", S), "DeltamuModifier": _fmt(p.DeltamuModifier),
+        "Mixer": vec("Mixer", 3), "thetaVec": vec("thetaVec", S), "SCcriterion": _fmt(p.SCcriterion),
+        "maxSCcount": p.maxSCcount, "method": p.method, "DelFieldMethod": p.DelFieldMethod, "MovieQ": p.MovieQ,
+        "Schedule": vec("Schedule", p.n_schedule), "EkinTypes": vec("EkinTypes", p.n_ekin), "alpha": _fmt(p.alpha),
+        "beta": _fmt(p.beta), "gamma": _fmt(p.gamma), "gammaH": _fmt(p.gammaH), "degeneracy": _fmt(p.degeneracy),
+        "regularize": vec("regularize", p.n_regularize), "RegularizationThreshold": _fmt(p.RegularizationThreshold),
+        "Print": p.Print,
+    }
+    lines = ["%s\t\t\t//%s" % (vals[n], n) for n in LINE_NAMES]
+    if extensions:
+        lines += [
+            "%d\t\t\t//HartreeMode" % p.HartreeMode,
+            "%s\t\t\t//SmoothingKWidth" % _fmt(p.SmoothingKWidth),
+            "%d\t\t\t//ConvergenceMode" % p.ConvergenceMode,
+            "%d\t\t\t//Precision" % p.Precision,
+        ]
+        for c in range(p.n_centres):
+            lines.append("%s\t\t\t//Centre" % " ".join(_fmt(x) for x in p.centres[c]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _set(arr, vals):
+    for i, v in enumerate(vals):
+        arr[i] = v
+
+
+def c0(steps=128, edge=32.0, N=100.0, g=1.0, convergence_mode=1, tol=1e-8, maxsc=5000):
+    """2D two-species repulsive Fermi gas, TF + contact (drop-in inputs of the reference)."""
+    p = default_params()
+    p.DIM, p.Units, p.edge, p.steps, p.S = 2, 1, edge, steps, 2
+    _set(p.Abundances, [N, N])
+    p.RelAcc = 1e-11
+    p.DensityExpression = 7
+    p.n_mpp = 2
+    _set(p.mpp, [0.5, 0.5])
+    _set(p.Environments, [1, 1])
+    p.n_interactions = 1
+    p.Interactions[0] = 1
+    p.beta = g
+    p.degeneracy = 1.0
+    _set(p.muVec, [10.0, 10.0])
+    p.DeltamuModifier = -3.0
+    _set(p.Mixer, [1.0, 8.0, 0.0])
+    _set(p.thetaVec, [0.1, 0.1])
+    p.SCcriterion = tol
+    p.ConvergenceMode = convergence_mode
+    p.maxSCcount = maxsc
+    p.DelFieldMethod = 5
+    p.Print = 0
+    return p
+
+
+def c1(steps=128, edge=20.0, Z=10.0, a=0.5, N=10.0, density_expression=7, hartree_mode=1, kwidth=0.3,
+       tol=1e-8, maxsc=2000, T=0.05):
+    """3D electrons in a softened Coulomb well: Hartree + Dirac + PW92."""
+    p = default_params()
+    p.DIM, p.Units, p.edge, p.steps, p.S = 3, 1, edge, steps, 1
+    p.Abundances[0] = N
+    p.RelAcc = 1e-11
+    p.InternalAcc = 1e-6
+    p.DensityExpression = density_expression
+    p.n_mpp = 2
+    _set(p.mpp, [Z, a])
+    p.n_Mpp = 3
+    _set(p.Mpp, [10.0, 20.0, 1e-6])
+    p.TVec[0] = T
+    p.Environments[0] = 40
+    p.n_interactions = 3
+    _set(p.Interactions, [8, 11, -12])
+    p.gammaH = 1.0
+    p.degeneracy = 2.0
+    p.muVec[0] = -1.0
+    p.DeltamuModifier = -3.0
+    _set(p.Mixer, [1.0, 10.0, 0.0])
+    p.thetaVec[0] = 0.05
+    p.SCcriterion = tol
+    p.ConvergenceMode = 1
+    p.maxSCcount = maxsc
+    p.DelFieldMethod = 5
+    p.HartreeMode = hartree_mode
+    p.SmoothingKWidth = kwidth
+    p.Print = 0
+    return p

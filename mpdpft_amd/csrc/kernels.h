@@ -1,0 +1,151 @@
+// kernels.h -- host-callable launchers of the gfx950 kernels (kernels.hip).
+// All launchers enqueue on the given stream and return the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace mk {
+
+// Geometry of one (s+1)^D grid and its s^D periodic cell.
+struct Grid {
+    int D;        // 2 or 3
+    int s;        // steps (FFT length per axis)
+    int EL;       // s + 1
+    size_t G;     // EL^D
+    size_t F;     // s^D
+    double edge, dx;
+    const double *lat;  // device lattice table x_i = -edge/2 + i*edge/s, length EL
+};
+
+// ---- FFT: centred transforms between real grid fields and a packed spectrum
+// (standard DFT order, s^D complex).  spec = DFT(f) (unnormalised, no
+// centring signs); the reference's centred spectrum is
+// dx^D (-1)^{qx+qy+qz} spec[q].
+hipError_t fft_forward_real(const Grid &g, const double *re, const double *im, double2 *spec,
+                            const double2 *tw, double *f0slot, hipStream_t st);
+
+// Backward multipliers applied while loading the spectrum (first pass).
+enum MulKind {
+    MUL_NONE = 0,        // plain
+    MUL_SMOOTH = 1,      // exp(sigma |k|^2)
+    MUL_DERIV_PAIR = 2,  // re <- i k_a e^{sigma k^2}, im <- i k_b e^{sigma k^2} (Nyquist plane of the axis zeroed)
+    MUL_DERIV_LAP = 3,   // re <- i k_z e^{sigma k^2}, im <- -|k|^2 e^{2 sigma k^2}
+    MUL_HARTREE_PER = 4, // reference periodic Hartree with Gaussian compensation
+    MUL_SECOND = 5       // re <- -k_a^2 (method 4 Laplacian pieces)
+};
+struct MulParams {
+    int kind;
+    int axis_a, axis_b;   // axes for MUL_DERIV_PAIR (0..2 in 3D; 1..2 in 2D)
+    double sigma;
+    const double *kq;     // k per standard index (kLattice values), length s
+    const double *kq_h;   // k per standard index (k0 + p dk), Hartree
+    const double *f0slot; // device scalar: dx^D sum n (Hartree)
+    double dxD;           // dx^D
+    double dkzero;        // 1e-6 dk^2
+    int regularize;       // RegularizeCoulombSingularity
+};
+// Backward transform: f = L^-D * IDFT(M * spec) written to re/im real grid
+// fields (either may be null) with periodic-image fill, out = scale*value
+// (+ out if accumulate).  post: 0 none, 1 POS clip of re.
+// post 2 adds the Gaussian compensation f0 erf(r/2)/(4 pi r) and multiplies
+// by c1 = 4 pi gammaH (needs lat, the lattice table, and m.f0slot).
+// The first pass reads 'spec' and writes 'work' (may equal spec), so a
+// spectrum can feed several backward transforms.
+hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, const MulParams &m, double *re,
+                             double *im, double scale, int post, double c1, const double2 *tw, hipStream_t st);
+
+// Isolated-boundary Hartree: zero-padded (2s)^D circular convolution with the
+// precomputed real kernel spectrum khat; out[i] += gammaH dx^D conv (i in [0,s]^D).
+hipError_t hartree_isolated(const Grid &g, const double *n, double2 *pad, const double *khat,
+                            const double2 *tw2, double gamma_dxD_over_M, double *out, int accumulate,
+                            hipStream_t st);
+// Build khat = DFT((2s)^D kernel 1/r, g0 at r=0) into pad then copy real part to khat.
+hipError_t build_isolated_kernel(const Grid &g, double2 *pad, const double2 *tw2, double g0, double *khat,
+                                 hipStream_t st);
+
+// ---- reductions: partial sums per block, then a fixed-order final sum.
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+// out[k] for k < nres; 'partials' holds kRedBlocks*nres doubles.
+hipError_t finish_sums(const double *partials, int nres, double *out, double scale, hipStream_t st);
+hipError_t finish_minmax(const double *partials, int nres, double *out, hipStream_t st);
+
+// TF density n = pref (mu - V)_+^{D/2} with fused trapezoid weights; writes n
+// if n != nullptr; partial sums of w*n into partials[0..kRedBlocks).
+hipError_t tf_density(const Grid &g, const double *V, double mu, double pref, double *n, double *partials,
+                      hipStream_t st);
+// Evaluate N(mu_j) for nmu chemical potentials in one pass over V (no n written).
+hipError_t tf_density_multi(const Grid &g, const double *V, const double *mus_dev, int nmu, double pref,
+                            double *partials, hipStream_t st);
+
+// V_s = inc * (H_s + sum of pointwise interactions) + Env_s, interactions in
+// list order; 'hartree_in_V' says V already holds the Hartree term.
+struct VintParams {
+    int n_inter;
+    int inter[16];
+    int S, s;
+    double beta, gammaH, inc;
+    int units;
+    int hartree_in_V;
+};
+hipError_t update_v(const Grid &g, const VintParams &vp, const double *den_all, const double *env, double *V,
+                    hipStream_t st);
+
+// RelDev dots per species + field stats: partials layout per species s:
+// [<a,a>, <b,b>, <a,b>] sums and [denmin, denmax, vmin, vmax].
+hipError_t conv_dots_S(const Grid &g, int S, const double *a, const double *b, double *partials, hipStream_t st);
+hipError_t field_minmax_S(const Grid &g, int S, const double *den, const double *V, double *partials,
+                          hipStream_t st);
+hipError_t max_abs_diff(size_t n, const double *a, const double *b, double *partials, hipStream_t st);
+
+// Pulay helpers over n = S*G entries.
+hipError_t pulay_push(size_t n, const double *nin, const double *nout, double *slot_nin, double *slot_res,
+                      hipStream_t st);
+// partials[j*kRedBlocks + blk] = partial <res_new, res_j>, j < nh
+hipError_t multi_dot(size_t n, const double *x, const double *const *ys, int nh, double *partials,
+                     hipStream_t st);
+// den[s] = sum_p c_p (nin_p + theta_s res_p)
+hipError_t pulay_mix(const Grid &g, int S, int I, const double *const *nin, const double *const *res,
+                     const double *coeff_dev, const double *theta_dev, double *den, hipStream_t st);
+hipError_t linear_mix(size_t n_per_species, int S, const double *theta_dev, const double *old, double *den,
+                      hipStream_t st);
+
+// Shift by the shell maximum and taper (DelField's TaperOff, mpDPFT.cpp:7251-7264).
+hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials, hipStream_t st);
+hipError_t taper(const Grid &g, const double *in, const double *shift_dev, double rt, double *out,
+                 hipStream_t st);
+hipError_t sum_squares3(const Grid &g, const double *a, const double *b, const double *c, double *out,
+                        hipStream_t st);
+
+// Environment fields.
+struct EnvParams {
+    int type;
+    double c0, c1;           // mpp[s], mpp[S+s]
+    int ncentres;
+    const double *centres;   // device [n][5]
+};
+hipError_t environment(const Grid &g, const EnvParams &e, double *env, hipStream_t st);
+
+// Energy integrands (weighted trapezoid sums into partials[0..kRedBlocks)).
+enum EnergyOp {
+    E_PROD = 0,          // a*b
+    E_DIRAC = 1,         // dirac energy density of a (scale passed)
+    E_XC = 2,            // POS(a) * eps_xc(a)
+    E_TF_DISP = 3,       // (mu - a)_+^{(D+2)/2}
+    E_POW_POS = 4,       // POS(a)^p
+    E_GRADCORR = 5,      // b/|a| if |a| > thr else 0   (a = n, b = grad2)
+    E_VIRIAL = 6,        // a * (x b + y c + z d)
+    E_HALF_PROD = 7      // 0.5 a b
+};
+struct EnergyParams {
+    int op;
+    double p0, p1;       // op parameters (mu, power, threshold, scale)
+    int xc_id, units;
+};
+hipError_t energy_integrand(const Grid &g, const EnergyParams &ep, const double *a, const double *b,
+                            const double *c, const double *d, double *partials, hipStream_t st);
+
+// Fill a device array with a value.
+hipError_t fill(size_t n, double v, double *a, hipStream_t st);
+
+}  // namespace mk

@@ -1,0 +1,880 @@
+// kernels.hip -- gfx950 kernels of the SC loop: fused FFT load/store
+// functors, pointwise stages and deterministic two-level reductions.
+#include <hip/hip_runtime.h>
+
+#include "fft_core.h"
+#include "kernels.h"
+
+namespace mk {
+
+using fftk::launch_pass;
+
+#define PI_D 3.141592653589793
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double d_exp(double x) {  // the reference's EXP (mpDPFT.h:266)
+    if (x < -700.) return 0.;
+    if (x < 700.) return exp(x);
+    return 1.0142320547350045e+304;
+}
+__device__ __forceinline__ double d_pos(double x) { return x > 0. ? x : 0.; }
+
+// trapezoid weight of grid point idx (1/2 at index 0 and s per axis)
+__device__ __forceinline__ double trap_weight(const Grid &g, int idx) {
+    double w = 1.;
+    int rem = idx;
+    for (int d = 0; d < g.D; d++) {
+        int c = rem % g.EL;
+        rem /= g.EL;
+        if (c == 0 || c == g.s) w *= 0.5;
+    }
+    return w;
+}
+
+// physical coordinates of grid point idx from the lattice table
+__device__ __forceinline__ void coords(const Grid &g, int idx, double *r) {
+    const double *lat = g.lat;
+    if (g.D == 2) {
+        int i = idx / g.EL, j = idx - i * g.EL;
+        r[0] = lat[i];
+        r[1] = lat[j];
+        r[2] = 0.;
+    } else {
+        int i = idx / (g.EL * g.EL), rem = idx - i * g.EL * g.EL, j = rem / g.EL, k = rem - j * g.EL;
+        r[0] = lat[i];
+        r[1] = lat[j];
+        r[2] = lat[k];
+    }
+}
+
+// block sum of one double over 256 threads (4 wavefronts of 64)
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    return r;
+}
+__device__ __forceinline__ double block_min(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0) r = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+    return r;
+}
+
+__device__ double pw92_eps(double n, double *vc) {
+    const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+    if (!(n > 0.)) {
+        *vc = 0.;
+        return 0.;
+    }
+    double rs = cbrt(3. / (4. * PI_D * n));
+    double srs = sqrt(rs);
+    double Q0 = -2. * A * (1. + a1 * rs);
+    double Q1 = 2. * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+    double Q1p = A * (b1 / srs + 2. * b2 + 3. * b3 * srs + 4. * b4 * rs);
+    double lg = log(1. + 1. / Q1);
+    double ec = Q0 * lg;
+    double dec = -2. * A * a1 * lg - Q0 * Q1p / (Q1 * Q1 + Q1);
+    *vc = ec - rs / 3. * dec;
+    return ec;
+}
+
+// LDA xc (id -1 exchange, -12 PW92) per particle energy and potential,
+// with the Angstrom/eV conversion of Units 2.
+__device__ void xc_lda(int id, int units, double n, double *e, double *v) {
+    const double Hartree = 27.211386245988, Bohr = 0.529177210903;
+    double cn = units == 2 ? Bohr * Bohr * Bohr : 1., ce = units == 2 ? Hartree : 1.;
+    double rho = n > 0. ? cn * n : 0.;
+    double ee = 0., vv = 0.;
+    if (rho > 0.) {
+        if (id == -1) {
+            double c = cbrt(3. / PI_D);
+            ee = -0.75 * c * cbrt(rho);
+            vv = -c * cbrt(rho);
+        } else {
+            ee = pw92_eps(rho, &vv);
+        }
+    }
+    *e = ce * ee;
+    *v = ce * vv;
+}
+
+// ---------------------------------------------------------------------------
+// FFT functors
+// ---------------------------------------------------------------------------
+struct LdGrid {  // real grid fields -> complex (im may be null)
+    const double *re, *im;
+    int EL;
+    __device__ double2 operator()(int x, int y, int z) const {
+        int i = (x * EL + y) * EL + z;
+        double2 v;
+        v.x = re[i];
+        v.y = im ? im[i] : 0.;
+        return v;
+    }
+};
+struct LdBuf {
+    const double2 *p;
+    int n1, n2;
+    __device__ double2 operator()(int x, int y, int z) const { return p[((size_t)x * n1 + y) * n2 + z]; }
+};
+struct StBuf {
+    double2 *p;
+    int n1, n2;
+    __device__ void operator()(int x, int y, int z, double2 v) const { p[((size_t)x * n1 + y) * n2 + z] = v; }
+};
+struct StBufF0 {  // also captures dx^D * DC term
+    double2 *p;
+    int n1, n2;
+    double *f0slot;
+    double dxD;
+    __device__ void operator()(int x, int y, int z, double2 v) const {
+        p[((size_t)x * n1 + y) * n2 + z] = v;
+        if (f0slot && x == 0 && y == 0 && z == 0) *f0slot = dxD * v.x;
+    }
+};
+
+// multiplier applied on the first backward load
+struct LdMul {
+    const double2 *p;
+    int n1, n2, D, s;
+    MulParams m;
+    __device__ double2 operator()(int x, int y, int z) const {
+        double2 v = p[((size_t)x * n1 + y) * n2 + z];
+        int q3[3] = {x, y, z};
+        int q[3];
+        for (int d = 0; d < D; d++) q[d] = q3[3 - D + d];
+        double k[3] = {0., 0., 0.}, k2 = 0.;
+        for (int d = 0; d < D; d++) {
+            k[d] = m.kq[q[d]];
+            k2 += k[d] * k[d];
+        }
+        double2 o;
+        switch (m.kind) {
+        case MUL_SMOOTH: {
+            double f = d_exp(m.sigma * k2);
+            o.x = v.x * f;
+            o.y = v.y * f;
+            return o;
+        }
+        case MUL_DERIV_PAIR:
+        case MUL_DERIV_LAP: {
+            double f = d_exp(m.sigma * k2);
+            // A = i ka f v  (Nyquist plane of axis a dropped)
+            double2 A = {0., 0.}, B = {0., 0.};
+            if (m.axis_a >= 0 && q[m.axis_a] != s / 2) {
+                double c = k[m.axis_a] * f;
+                A.x = -c * v.y;
+                A.y = c * v.x;
+            }
+            if (m.kind == MUL_DERIV_PAIR) {
+                if (q[m.axis_b] != s / 2) {
+                    double c = k[m.axis_b] * f;
+                    B.x = -c * v.y;
+                    B.y = c * v.x;
+                }
+            } else {
+                double c = -k2 * f * f;
+                B.x = c * v.x;
+                B.y = c * v.y;
+            }
+            // A + i B
+            o.x = A.x - B.y;
+            o.y = A.y + B.x;
+            return o;
+        }
+        case MUL_SECOND: {
+            double c = -k[m.axis_a] * k[m.axis_a];
+            o.x = c * v.x;
+            o.y = c * v.y;
+            return o;
+        }
+        case MUL_HARTREE_PER: {
+            double kk2 = 0.;
+            int par = 0;
+            for (int d = 0; d < D; d++) {
+                double kh = m.kq_h[q[d]];
+                kk2 += kh * kh;
+                par += q[d];
+            }
+            double f0 = *m.f0slot;
+            double re = m.dxD * v.x, im = m.dxD * v.y;
+            if (kk2 > m.dkzero) {
+                double sg = (par & 1) ? -1. : 1.;
+                if (m.regularize) o.x = (re - sg * f0 * d_exp(-kk2)) / kk2;
+                else o.x = re / kk2;
+                o.y = im / kk2;
+            } else {
+                o.x = m.regularize ? f0 : 0.;
+                o.y = 0.;
+            }
+            return o;
+        }
+        default:
+            return v;
+        }
+    }
+};
+
+// store the real/imag parts onto the (s+1)^D grid with the periodic images
+struct StGrid {
+    double *re, *im;
+    double scale;
+    int post;  // 0 plain, 1 POS(re), 2 Hartree compensation on re
+    int EL, s, D;
+    const double *lat;
+    const double *f0slot;
+    double c1;  // 4 pi gammaH for post 2
+    __device__ double hart(int idx, double val) const {
+        double r[3];
+        int i = idx;
+        if (D == 2) {
+            int a = i / EL, b = i - a * EL;
+            r[0] = lat[a];
+            r[1] = lat[b];
+            r[2] = 0.;
+        } else {
+            int a = i / (EL * EL), rem = i - a * EL * EL, b = rem / EL, c = rem - b * EL;
+            r[0] = lat[a];
+            r[1] = lat[b];
+            r[2] = lat[c];
+        }
+        double z = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        double zf = z > 1.0e-10 ? PI_D * erf(z / 2.) / z : sqrt(PI_D);
+        return c1 * (val + (*f0slot) * zf / (4. * PI_D * PI_D));
+    }
+    __device__ void put(int idx, double2 v) const {
+        if (re) {
+            double a = scale * v.x;
+            if (post == 1) a = d_pos(a);
+            else if (post == 2) a = hart(idx, a);
+            re[idx] = a;
+        }
+        if (im) im[idx] = scale * v.y;
+    }
+    __device__ void operator()(int x, int y, int z, double2 v) const {
+        int xs[2] = {x, s}, ys[2] = {y, s}, zs[2] = {z, s};
+        int nx = (D == 3 && x == 0) ? 2 : 1, ny = (y == 0) ? 2 : 1, nz = (z == 0) ? 2 : 1;
+        for (int a = 0; a < nx; a++)
+            for (int b = 0; b < ny; b++)
+                for (int c = 0; c < nz; c++) put((xs[a] * EL + ys[b]) * EL + zs[c], v);
+    }
+};
+
+// zero-padded loads of the isolated Hartree transform
+struct LdGridPadZ {
+    const double *n;
+    int EL, s;
+    __device__ double2 operator()(int x, int y, int z) const {
+        double2 v = {0., 0.};
+        if (z < s) v.x = n[(x * EL + y) * EL + z];
+        return v;
+    }
+};
+struct LdBufPad {  // virtual zeros beyond index s along 'axis'
+    const double2 *p;
+    int n1, n2, axis, s;
+    __device__ double2 operator()(int x, int y, int z) const {
+        int c = axis == 0 ? x : (axis == 1 ? y : z);
+        if (c >= s) return double2{0., 0.};
+        return p[((size_t)x * n1 + y) * n2 + z];
+    }
+};
+struct StBufMulK {  // store v * khat
+    double2 *p;
+    const double *khat;
+    int n1, n2;
+    __device__ void operator()(int x, int y, int z, double2 v) const {
+        size_t i = ((size_t)x * n1 + y) * n2 + z;
+        double k = khat[i];
+        p[i] = double2{v.x * k, v.y * k};
+    }
+};
+struct StGridIso {  // keep [0,s]^D of the circular convolution
+    double *out;
+    double scale;
+    int EL, s, D, accumulate;
+    __device__ void operator()(int x, int y, int z, double2 v) const {
+        if (z > s) return;
+        int idx = (x * EL + y) * EL + z;
+        double a = scale * v.x;
+        if (accumulate) out[idx] += a;
+        else out[idx] = a;
+    }
+};
+struct LdKernel {  // 1/r on the (2s)^D torus, g0 at the origin
+    int M, s;
+    double dx, g0;
+    int D;
+    __device__ double2 operator()(int x, int y, int z) const {
+        int c[3] = {x, y, z};
+        double r2 = 0.;
+        for (int d = 3 - D; d < 3; d++) {
+            int m = c[d] <= s ? c[d] : c[d] - M;
+            double t = (double)m * dx;
+            r2 += t * t;
+        }
+        double2 v;
+        v.x = r2 > 0. ? 1. / sqrt(r2) : g0;
+        v.y = 0.;
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// axis-pass dispatch over the supported lengths
+// ---------------------------------------------------------------------------
+template <int AXIS, class Ld, class St>
+static hipError_t pass(int N, Ld ld, St st, int L0, int L1, int sign, const double2 *tw, hipStream_t stream) {
+    switch (N) {
+    case 16: return launch_pass<16, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    case 32: return launch_pass<32, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    case 64: return launch_pass<64, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    case 96: return launch_pass<96, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    case 128: return launch_pass<128, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    case 192: return launch_pass<192, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    case 256: return launch_pass<256, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    case 512: return launch_pass<512, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+#define CHK(x)                               \
+    do {                                     \
+        hipError_t e_ = (x);                 \
+        if (e_ != hipSuccess) return e_;     \
+    } while (0)
+
+hipError_t fft_forward_real(const Grid &g, const double *re, const double *im, double2 *spec,
+                            const double2 *tw, double *f0slot, hipStream_t st) {
+    int s = g.s;
+    double dxD = g.D == 3 ? g.dx * g.dx * g.dx : g.dx * g.dx;
+    LdGrid lg{re, im, g.EL};
+    StBuf sb{spec, s, s};
+    LdBuf lb{spec, s, s};
+    StBufF0 sf{spec, s, s, f0slot, dxD};
+    if (g.D == 3) {
+        CHK(pass<2>(s, lg, sb, s, s, -1, tw, st));
+        CHK(pass<1>(s, lb, sb, s, s, -1, tw, st));
+        CHK(pass<0>(s, lb, sf, s, s, -1, tw, st));
+    } else {
+        CHK(pass<2>(s, lg, sb, 1, s, -1, tw, st));
+        CHK(pass<1>(s, lb, sf, 1, s, -1, tw, st));
+    }
+    return hipSuccess;
+}
+
+
+hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, const MulParams &m, double *re,
+                             double *im, double scale, int post, double c1, const double2 *tw, hipStream_t st) {
+    int s = g.s;
+    LdMul lm{spec, s, s, g.D, s, m};
+    StBuf sb{work, s, s};
+    LdBuf lb{work, s, s};
+    StGrid sg{re, im, scale, post, g.EL, s, g.D, g.lat, m.f0slot, c1};
+    if (g.D == 3) {
+        CHK(pass<0>(s, lm, sb, s, s, +1, tw, st));
+        CHK(pass<1>(s, lb, sb, s, s, +1, tw, st));
+        CHK(pass<2>(s, lb, sg, s, s, +1, tw, st));
+    } else {
+        CHK(pass<1>(s, lm, sb, 1, s, +1, tw, st));
+        CHK(pass<2>(s, lb, sg, 1, s, +1, tw, st));
+    }
+    return hipSuccess;
+}
+
+hipError_t build_isolated_kernel(const Grid &g, double2 *pad, const double2 *tw2, double g0, double *khat,
+                                 hipStream_t st);
+
+hipError_t hartree_isolated(const Grid &g, const double *n, double2 *pad, const double *khat,
+                            const double2 *tw2, double scale, double *out, int accumulate, hipStream_t st) {
+    int s = g.s, M = 2 * s;
+    StBuf sb{pad, M, M};
+    LdBuf lb{pad, M, M};
+    StBufMulK sk{pad, khat, M, M};
+    StGridIso so{out, scale, g.EL, s, g.D, accumulate};
+    if (g.D == 3) {
+        // forward, pruned: only lines that can hold nonzeros
+        CHK(pass<2>(M, LdGridPadZ{n, g.EL, s}, sb, s, s, -1, tw2, st));
+        CHK(pass<1>(M, LdBufPad{pad, M, M, 1, s}, sb, s, M, -1, tw2, st));
+        CHK(pass<0>(M, LdBufPad{pad, M, M, 0, s}, sk, M, M, -1, tw2, st));
+        // backward, pruned: only lines that reach [0,s]^3
+        CHK(pass<0>(M, lb, sb, M, M, +1, tw2, st));
+        CHK(pass<1>(M, lb, sb, s + 1, M, +1, tw2, st));
+        CHK(pass<2>(M, lb, so, s + 1, s + 1, +1, tw2, st));
+    } else {
+        CHK(pass<2>(M, LdGridPadZ{n, g.EL, s}, sb, 1, s, -1, tw2, st));
+        CHK(pass<1>(M, LdBufPad{pad, M, M, 1, s}, sk, 1, M, -1, tw2, st));
+        CHK(pass<1>(M, lb, sb, 1, M, +1, tw2, st));
+        CHK(pass<2>(M, lb, so, 1, s + 1, +1, tw2, st));
+    }
+    return hipSuccess;
+}
+
+__global__ void k_copy_re(size_t n, const double2 *a, double *b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i].x;
+}
+
+hipError_t build_isolated_kernel(const Grid &g, double2 *pad, const double2 *tw2, double g0, double *khat,
+                                 hipStream_t st) {
+    int s = g.s, M = 2 * s;
+    StBuf sb{pad, M, M};
+    LdBuf lb{pad, M, M};
+    LdKernel lk{M, s, g.dx, g0, g.D};
+    size_t tot = (size_t)M * M * (g.D == 3 ? M : 1);
+    if (g.D == 3) {
+        CHK(pass<2>(M, lk, sb, M, M, -1, tw2, st));
+        CHK(pass<1>(M, lb, sb, M, M, -1, tw2, st));
+        CHK(pass<0>(M, lb, sb, M, M, -1, tw2, st));
+    } else {
+        CHK(pass<2>(M, lk, sb, 1, M, -1, tw2, st));
+        CHK(pass<1>(M, lb, sb, 1, M, -1, tw2, st));
+    }
+    hipLaunchKernelGGL(k_copy_re, dim3(1024), dim3(256), 0, st, tot, (const double2 *)pad, khat);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_finish_sums(const double *partials, int nres, double *out, double scale) {
+    __shared__ double sh[4];
+    for (int r = 0; r < nres; r++) {
+        double v = 0.;
+        for (int i = threadIdx.x; i < kRedBlocks; i += 256) v += partials[(size_t)r * kRedBlocks + i];
+        double t = block_sum(v, sh);
+        if (threadIdx.x == 0) out[r] = t * scale;
+        __syncthreads();
+    }
+}
+hipError_t finish_sums(const double *partials, int nres, double *out, double scale, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(256), 0, st, partials, nres, out, scale);
+    return hipGetLastError();
+}
+__global__ __launch_bounds__(256) void k_finish_min(const double *partials, int nres, double *out) {
+    __shared__ double sh[4];
+    for (int r = 0; r < nres; r++) {
+        double v = 1.0e308;
+        for (int i = threadIdx.x; i < kRedBlocks; i += 256) v = fmin(v, partials[(size_t)r * kRedBlocks + i]);
+        double t = block_min(v, sh);
+        if (threadIdx.x == 0) out[r] = t;
+        __syncthreads();
+    }
+}
+hipError_t finish_minmax(const double *partials, int nres, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish_min, dim3(1), dim3(256), 0, st, partials, nres, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// densities
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tf_density(Grid g, const double *__restrict__ V, double mu, double pref,
+                                                    double *__restrict__ n, double *partials) {
+    __shared__ double sh[4];
+    double acc = 0.;
+    int G = (int)g.G;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < G; i += gridDim.x * 256) {
+        double t = mu - V[i];
+        double val = 0.;
+        if (t > 0.) val = g.D == 2 ? pref * t : pref * t * sqrt(t);
+        if (n) n[i] = val;
+        acc += trap_weight(g, i) * val;
+    }
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+hipError_t tf_density(const Grid &g, const double *V, double mu, double pref, double *n, double *partials,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_tf_density, dim3(kRedBlocks), dim3(256), 0, st, g, V, mu, pref, n, partials);
+    return hipGetLastError();
+}
+
+constexpr int kMaxMu = 32;
+__global__ __launch_bounds__(256) void k_tf_multi(Grid g, const double *__restrict__ V, const double *mus, int nmu,
+                                                  double pref, double *partials) {
+    __shared__ double sh[4];
+    double acc[kMaxMu];
+    double mu[kMaxMu];
+#pragma unroll
+    for (int j = 0; j < kMaxMu; j++) {
+        acc[j] = 0.;
+        mu[j] = j < nmu ? mus[j] : -1.0e300;
+    }
+    int G = (int)g.G;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < G; i += gridDim.x * 256) {
+        double v = V[i], w = trap_weight(g, i);
+#pragma unroll
+        for (int j = 0; j < kMaxMu; j++) {
+            double t = mu[j] - v;
+            double val = 0.;
+            if (t > 0.) val = g.D == 2 ? pref * t : pref * t * sqrt(t);
+            acc[j] += w * val;
+        }
+    }
+    for (int j = 0; j < nmu; j++) {
+        double r = block_sum(acc[j], sh);
+        if (threadIdx.x == 0) partials[(size_t)j * kRedBlocks + blockIdx.x] = r;
+        __syncthreads();
+    }
+}
+hipError_t tf_density_multi(const Grid &g, const double *V, const double *mus_dev, int nmu, double pref,
+                            double *partials, hipStream_t st) {
+    if (nmu > kMaxMu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tf_multi, dim3(kRedBlocks), dim3(256), 0, st, g, V, mus_dev, nmu, pref, partials);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// potentials
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const double *__restrict__ den,
+                                                  const double *__restrict__ env, double *V) {
+    size_t G = g.G, tot = G * (size_t)vp.S;
+    double DD = (double)g.D;
+    for (size_t idx = blockIdx.x * 256 + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * 256) {
+        int s = (int)(idx / G);
+        size_t i = idx - (size_t)s * G;
+        double ns = den[idx];
+        double acc = 0.;
+        for (int a = 0; a < vp.n_inter; a++) {
+            int t = vp.inter[a];
+            double v = 0.;
+            if (t == 1) {
+                for (int sp = 0; sp < vp.S; sp++)
+                    if (sp != s) v += den[(size_t)sp * G + i];
+                v *= vp.beta;
+            } else if (t == 8) {
+                v = vp.hartree_in_V ? V[idx] : 0.;
+            } else if (t == 10) {
+                v = den[i] * (2. * vp.beta);
+            } else if (t == 11) {
+                v = g.D > 1 ? -vp.gammaH * (2. * pow(d_pos(ns) * DD / PI_D, 1. / DD)) / (DD - 1.) : 0.;
+            } else if (t == -1 || t == -12) {
+                double e;
+                xc_lda(t, vp.units, ns, &e, &v);
+            }
+            acc += v;
+        }
+        if (vp.inc < 1.) acc *= vp.inc;
+        V[idx] = acc + env[idx];
+    }
+}
+hipError_t update_v(const Grid &g, const VintParams &vp, const double *den_all, const double *env, double *V,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_update_v, dim3(2048), dim3(256), 0, st, g, vp, den_all, env, V);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// convergence, stats
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_dots(Grid g, const double *a, const double *b, double *partials) {
+    __shared__ double sh[4];
+    int s = blockIdx.y;
+    const double *pa = a + (size_t)s * g.G, *pb = b + (size_t)s * g.G;
+    double aa = 0., bb = 0., ab = 0.;
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < g.G; i += (size_t)gridDim.x * 256) {
+        double x = pa[i], y = pb[i];
+        aa += x * x;
+        bb += y * y;
+        ab += x * y;
+    }
+    double r0 = block_sum(aa, sh);
+    __syncthreads();
+    double r1 = block_sum(bb, sh);
+    __syncthreads();
+    double r2 = block_sum(ab, sh);
+    if (threadIdx.x == 0) {
+        partials[(size_t)(3 * s + 0) * kRedBlocks + blockIdx.x] = r0;
+        partials[(size_t)(3 * s + 1) * kRedBlocks + blockIdx.x] = r1;
+        partials[(size_t)(3 * s + 2) * kRedBlocks + blockIdx.x] = r2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_field_minmax(Grid g, const double *den, const double *V, double *partials) {
+    __shared__ double sh[4];
+    int s = blockIdx.y;
+    const double *pd = den + (size_t)s * g.G, *pv = V + (size_t)s * g.G;
+    double m0 = 1e308, m1 = 1e308, m2 = 1e308, m3 = 1e308;
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < g.G; i += (size_t)gridDim.x * 256) {
+        double x = pd[i], y = pv[i];
+        m0 = fmin(m0, x);
+        m1 = fmin(m1, -x);
+        m2 = fmin(m2, y);
+        m3 = fmin(m3, -y);
+    }
+    double r[4];
+    r[0] = block_min(m0, sh);
+    __syncthreads();
+    r[1] = block_min(m1, sh);
+    __syncthreads();
+    r[2] = block_min(m2, sh);
+    __syncthreads();
+    r[3] = block_min(m3, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) partials[(size_t)(4 * s + k) * kRedBlocks + blockIdx.x] = r[k];
+}
+
+__global__ __launch_bounds__(256) void k_max_abs_diff(size_t n, const double *a, const double *b, double *partials) {
+    __shared__ double sh[4];
+    double m = 1e308;
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) m = fmin(m, -fabs(a[i] - b[i]));
+    double r = block_min(m, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Pulay
+// ---------------------------------------------------------------------------
+__global__ void k_pulay_push(size_t n, const double *nin, const double *nout, double *snin, double *sres) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double a = nin[i], b = nout[i];
+        snin[i] = a;
+        sres[i] = b - a;
+    }
+}
+
+struct PtrList {
+    const double *p[64];
+};
+__global__ __launch_bounds__(256) void k_multi_dot(size_t n, const double *x, PtrList ys, int nh, double *partials) {
+    __shared__ double sh[4];
+    double acc[16];
+    for (int base = 0; base < nh; base += 16) {
+        int cnt = nh - base < 16 ? nh - base : 16;
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[j] = 0.;
+        for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+            double xv = x[i];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (j < cnt) acc[j] += xv * ys.p[base + j][i];
+        }
+        for (int j = 0; j < cnt; j++) {
+            double r = block_sum(acc[j], sh);
+            if (threadIdx.x == 0) partials[(size_t)(base + j) * kRedBlocks + blockIdx.x] = r;
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pulay_mix(size_t G, int S, int I, PtrList nin, PtrList res,
+                                                   const double *coeff, const double *theta, double *den) {
+    size_t tot = G * (size_t)S;
+    for (size_t idx = blockIdx.x * 256 + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * 256) {
+        int s = (int)(idx / G);
+        double th = theta[s];
+        double acc = 0.;
+        for (int p = 0; p < I; p++) acc += coeff[p] * (nin.p[p][idx] + th * res.p[p][idx]);
+        den[idx] = acc;
+    }
+}
+
+__global__ void k_linear_mix(size_t G, int S, const double *theta, const double *old, double *den) {
+    size_t tot = G * (size_t)S;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
+        int s = (int)(idx / G);
+        double th = theta[s];
+        den[idx] = th * den[idx] + (1. - th) * old[idx];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DelField helpers, environments, energies
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_shell_max(Grid g, const double *f, double rt,
+                                                   double *partials) {
+    __shared__ double sh[4];
+    double m = 1e308;
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < g.G; i += (size_t)gridDim.x * 256) {
+        double r[3];
+        coords(g, (int)i, r);
+        double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        if (fabs(rn - rt) < 2. * g.dx) m = fmin(m, -f[i]);
+    }
+    double r = block_min(m, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ void k_taper(Grid g, const double *in, const double *shift, double rt, double *out) {
+    double sft = -(*shift);  // shift slot holds -max
+    if (sft < -1e307 || sft > 1e307) sft = 0.;
+    double a = -2500. / (g.edge * g.edge);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < g.G; i += (size_t)gridDim.x * blockDim.x) {
+        double r[3];
+        coords(g, (int)i, r);
+        double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        double v = in[i] - sft;
+        if (rn > rt) v *= d_exp(a * (rn - rt) * (rn - rt));
+        out[i] = v;
+    }
+}
+
+__global__ void k_sum_squares3(size_t G, const double *a, const double *b, const double *c, double *out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < G; i += (size_t)gridDim.x * blockDim.x) {
+        double v = a[i] * a[i];
+        if (b) v += b[i] * b[i];
+        if (c) v += c[i] * c[i];
+        out[i] = v;
+    }
+}
+
+__global__ void k_env(Grid g, EnvParams e, double *env) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < g.G; i += (size_t)gridDim.x * blockDim.x) {
+        double r[3];
+        coords(g, (int)i, r);
+        double r2 = 0.;
+        for (int d = 0; d < g.D; d++) r2 += r[d] * r[d];
+        double v = 0.;
+        if (e.type == 1) v = e.c0 * pow(r2, 1.);
+        else if (e.type == 3) v = e.c0 * ((1. - e.c1) * r[0] * r[0] + (1. + e.c1) * r[1] * r[1]);
+        else if (e.type == 40) v = -e.c0 / sqrt(r2 + e.c1 * e.c1);
+        else if (e.type == 41) {
+            for (int c = 0; c < e.ncentres; c++) {
+                const double *C = e.centres + 5 * c;
+                double dd = 0.;
+                for (int d = 0; d < g.D; d++) dd += (r[d] - C[d]) * (r[d] - C[d]);
+                v += -C[3] / sqrt(dd + C[4] * C[4]);
+            }
+        }
+        env[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy(Grid g, EnergyParams ep, const double *a,
+                                                const double *b, const double *c, const double *d, double *partials) {
+    __shared__ double sh[4];
+    double acc = 0., DD = (double)g.D;
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < g.G; i += (size_t)gridDim.x * 256) {
+        double v = 0.;
+        switch (ep.op) {
+        case E_PROD: v = a[i] * b[i]; break;
+        case E_HALF_PROD: v = 0.5 * a[i] * b[i]; break;
+        case E_DIRAC: v = ep.p0 * pow(d_pos(a[i]), 1. + 1. / DD); break;
+        case E_XC: {
+            double e, vv;
+            xc_lda(ep.xc_id, ep.units, a[i], &e, &vv);
+            v = d_pos(a[i]) * e;
+            break;
+        }
+        case E_TF_DISP: {
+            double t = ep.p0 - a[i];
+            v = t > 0. ? pow(t, (DD + 2.) / 2.) : 0.;
+            break;
+        }
+        case E_POW_POS: v = pow(d_pos(a[i]), ep.p0); break;
+        case E_GRADCORR: {
+            double rho = fabs(a[i]);
+            v = rho > ep.p0 ? b[i] / rho : 0.;
+            break;
+        }
+        case E_VIRIAL: {
+            double r[3];
+            coords(g, (int)i, r);
+            v = a[i] * (r[0] * b[i] + r[1] * c[i] + r[2] * d[i]);
+            break;
+        }
+        }
+        acc += trap_weight(g, (int)i) * v;
+    }
+    double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ void k_fill(size_t n, double v, double *a) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host wrappers
+// ---------------------------------------------------------------------------
+static inline unsigned gblocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+}
+
+hipError_t conv_dots_S(const Grid &g, int S, const double *a, const double *b, double *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_conv_dots, dim3(kRedBlocks, S), dim3(256), 0, st, g, a, b, partials);
+    return hipGetLastError();
+}
+hipError_t field_minmax_S(const Grid &g, int S, const double *den, const double *V, double *partials,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_field_minmax, dim3(kRedBlocks, S), dim3(256), 0, st, g, den, V, partials);
+    return hipGetLastError();
+}
+hipError_t max_abs_diff(size_t n, const double *a, const double *b, double *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_max_abs_diff, dim3(kRedBlocks), dim3(256), 0, st, n, a, b, partials);
+    return hipGetLastError();
+}
+hipError_t pulay_push(size_t n, const double *nin, const double *nout, double *slot_nin, double *slot_res,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_pulay_push, dim3(gblocks(n)), dim3(256), 0, st, n, nin, nout, slot_nin, slot_res);
+    return hipGetLastError();
+}
+hipError_t multi_dot(size_t n, const double *x, const double *const *ys, int nh, double *partials,
+                     hipStream_t st) {
+    if (nh > 64) return hipErrorInvalidValue;
+    PtrList pl;
+    for (int j = 0; j < 64; j++) pl.p[j] = j < nh ? ys[j] : nullptr;
+    hipLaunchKernelGGL(k_multi_dot, dim3(kRedBlocks), dim3(256), 0, st, n, x, pl, nh, partials);
+    return hipGetLastError();
+}
+hipError_t pulay_mix(const Grid &g, int S, int I, const double *const *nin, const double *const *res,
+                     const double *coeff_dev, const double *theta_dev, double *den, hipStream_t st) {
+    if (I > 64) return hipErrorInvalidValue;
+    PtrList a, b;
+    for (int j = 0; j < 64; j++) {
+        a.p[j] = j < I ? nin[j] : nullptr;
+        b.p[j] = j < I ? res[j] : nullptr;
+    }
+    hipLaunchKernelGGL(k_pulay_mix, dim3(gblocks(g.G * S)), dim3(256), 0, st, g.G, S, I, a, b, coeff_dev,
+                       theta_dev, den);
+    return hipGetLastError();
+}
+hipError_t linear_mix(size_t n_per_species, int S, const double *theta_dev, const double *old, double *den,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_linear_mix, dim3(gblocks(n_per_species * S)), dim3(256), 0, st, n_per_species, S,
+                       theta_dev, old, den);
+    return hipGetLastError();
+}
+hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_shell_max, dim3(kRedBlocks), dim3(256), 0, st, g, f, rt, partials);
+    return hipGetLastError();
+}
+hipError_t taper(const Grid &g, const double *in, const double *shift_dev, double rt, double *out,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(k_taper, dim3(gblocks(g.G)), dim3(256), 0, st, g, in, shift_dev, rt, out);
+    return hipGetLastError();
+}
+hipError_t sum_squares3(const Grid &g, const double *a, const double *b, const double *c, double *out,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_squares3, dim3(gblocks(g.G)), dim3(256), 0, st, g.G, a, b, c, out);
+    return hipGetLastError();
+}
+hipError_t environment(const Grid &g, const EnvParams &e, double *env, hipStream_t st) {
+    hipLaunchKernelGGL(k_env, dim3(gblocks(g.G)), dim3(256), 0, st, g, e, env);
+    return hipGetLastError();
+}
+hipError_t energy_integrand(const Grid &g, const EnergyParams &ep, const double *a, const double *b,
+                            const double *c, const double *d, double *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_energy, dim3(kRedBlocks), dim3(256), 0, st, g, ep, a, b, c, d, partials);
+    return hipGetLastError();
+}
+hipError_t fill(size_t n, double v, double *a, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill, dim3(gblocks(n)), dim3(256), 0, st, n, v, a);
+    return hipGetLastError();
+}
+
+}  // namespace mk

@@ -1,0 +1,75 @@
+// solver.h -- the context behind the C ABI (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "mpdpft.h"
+
+struct NaitTables;  // nait.hip
+
+struct mpdpft_ctx {
+    mpdpft_params p;
+    mk::Grid g;
+    int S = 1, D = 3, s = 0, EL = 0;
+    size_t G = 0, F = 0, SG = 0, CentreIndex = 0;
+    double dx = 0, k0 = 0, dk = 0, kdel = 0, sigma = 0;
+    int device = 0;
+    hipStream_t st = nullptr;
+
+    std::vector<double> lat_h;  // lattice x_i
+    // device buffers
+    double *d_lat = nullptr, *d_kq = nullptr, *d_kqh = nullptr;
+    double2 *d_tw = nullptr, *d_tw2 = nullptr;
+    double2 *d_spec = nullptr, *d_work = nullptr, *d_pad = nullptr;
+    double *d_khat = nullptr;
+    double *d_Env = nullptr, *d_V = nullptr, *d_OldV = nullptr, *d_Den = nullptr, *d_OldDen = nullptr;
+    double *d_Dx = nullptr, *d_Dy = nullptr, *d_Dz = nullptr, *d_Grad2 = nullptr, *d_Lap = nullptr;
+    double *d_tmp1 = nullptr, *d_tmp2 = nullptr, *d_tmp3 = nullptr;
+    double *d_partials = nullptr, *d_scal = nullptr, *d_coeff = nullptr, *d_theta = nullptr;
+    double *d_centres = nullptr, *d_mus = nullptr;
+    double *h_scal = nullptr;  // pinned
+    std::vector<void *> allocations;
+
+    // Pulay history ring (scope+1 slots of S*G)
+    int pulay_mixer = 0, scope = 0, pI = 0;
+    std::vector<double *> nin, res;
+    std::vector<double> ResMat;  // (scope+1)^2
+    double coeff[64];
+
+    // SC state (host scalars)
+    double muVec[MPDPFT_MAX_SPECIES], TmpAb[MPDPFT_MAX_SPECIES], OldAb[MPDPFT_MAX_SPECIES];
+    double OldmuVec[MPDPFT_MAX_SPECIES], PreviousmuVec[MPDPFT_MAX_SPECIES], thetaVec[MPDPFT_MAX_SPECIES];
+    double TVec[MPDPFT_MAX_SPECIES];
+    double DenMin[MPDPFT_MAX_SPECIES], DenMax[MPDPFT_MAX_SPECIES], VMin[MPDPFT_MAX_SPECIES],
+        VMax[MPDPFT_MAX_SPECIES];
+    double DeltamuModifier = 0, RelAcc = 0, RelAccOriginal = 0, incV = 1, incV0 = 1, CC = 0;
+    int SCcount = 0, ExitSCcount = 0, ExitSCloop = 0, LinMix = 0, PulMix = 0, TrappedQ = 0;
+    int warningCount = 0, errorCount = 0, bisection_steps = 0, PrintSC = 0, MaxIteration = 4;
+    double EInt = 0, Etot = 0;
+    double IntE[MPDPFT_MAX_LIST], DispE[MPDPFT_MAX_SPECIES], EnvE[MPDPFT_MAX_SPECIES];
+    double AuxEkin[MPDPFT_MAX_SPECIES][4];
+    double solve_seconds = 0;
+    int has_hartree = 0;
+    int delfield_valid = -1;  // species whose Grad2/Lap hold DelField(V_s)
+
+    NaitTables *nait = nullptr;
+
+    std::string err;
+    std::ostringstream control;
+};
+
+// internal helpers shared between translation units
+int ctx_fail(mpdpft_ctx *c, int code, const std::string &msg);
+int hip_fail(mpdpft_ctx *c, hipError_t e, const char *what);
+int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method);
+int ctx_reduce(mpdpft_ctx *c, int nres, double scale, double *out_host);
+
+// nait.hip
+int nait_init(mpdpft_ctx *c);
+void nait_free(mpdpft_ctx *c);
+int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N);
+int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E);

@@ -1,0 +1,1783 @@
+/*
+ * dpft_oracle.c -- ORACLE: test infrastructure only (never shipped, never
+ * called by the product).
+ *
+ * Plain-C FP64 restatement of the reference's self-consistent DPFT loop
+ * (huehou/mpDPFT).  Each routine cites the reference routine whose behaviour
+ * it follows.  Written from the behaviour, not the text: data structures are
+ * flat arrays, the FFT is this file's own mixed-radix transform, and the
+ * centred-phase twiddles of fftParallel are applied as exact +-1 signs (the
+ * reference evaluates them with sin/cos of large arguments, ~1e-13 noise).
+ *
+ * Pins (tests/test_oracle_*.py):
+ *  - FFT: brute-force DFT, the testFFT Gaussian known answer
+ *    (mpDPFT.cpp:8850-8979), round trip, Parseval, wrap-fill.
+ *  - DelField: analytic derivatives of a centred Gaussian; smoothing closed form.
+ *  - Hartree periodic mode: the Gaussian-density closed form of the
+ *    compensation scheme; isolated mode: direct O(G^2) summation.
+ *  - TF + contact: the 2D continuum closed form mu = sqrt(N(2pi+g)/pi).
+ *  - PW92: published values; Dirac: closed form.
+ *  - Pulay small solve: Eigen 3.4 FullPivHouseholderQR (vendored in the
+ *    reference) through oracle/_ref.
+ *  - mu bisection, convergence, mixing: invariants (monotone N(mu), exit rule,
+ *    linear/Pulay fixed points).
+ */
+#include "dpft_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PI_ 3.141592653589793
+static const double MP_ = 1.0e-12;            /* mpDPFT.cpp:111 */
+static const double MachinePrecision_ = 1.0e-16;
+
+typedef struct { double re, im; } cplx;
+
+static double EXPs(double x) { /* mpDPFT.h:266 */
+    if (x < -700.) return 0.;
+    if (x < 700.) return exp(x);
+    return 1.0142320547350045e+304;
+}
+static double POSs(double x) { return x > 0. ? x : 0.; }
+static double Signs(double x) { return x < 0. ? -1. : 1.; }
+static double sqr(double x) { return x * x; }
+
+/* ------------------------------------------------------------------------ */
+/* 1D mixed-radix FFT (own implementation; recursive decimation in time).    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    cplx *w; /* w[t] = exp(sign*2 pi i t/n) for sign=-1; conj for +1 */
+} fftplan;
+
+static fftplan *plan_make(int n) {
+    fftplan *pl = (fftplan *)malloc(sizeof(fftplan));
+    pl->n = n;
+    pl->w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    for (int t = 0; t < n; t++) {
+        double a = 2. * PI_ * (double)t / (double)n;
+        pl->w[t].re = cos(a);
+        pl->w[t].im = -sin(a);
+    }
+    return pl;
+}
+static void plan_free(fftplan *pl) {
+    if (!pl) return;
+    free(pl->w);
+    free(pl);
+}
+static int smallest_factor(int n) {
+    if (n % 4 == 0 && n > 4) return 4;
+    for (int f = 2; f * f <= n; f++)
+        if (n % f == 0) return f;
+    return n;
+}
+/* out[k] = sum_j in[j*stride] exp(sign 2 pi i jk/n); tw holds the top-level
+ * table of size N, step = N/n. */
+static void fft_rec(int n, const cplx *in, int stride, cplx *out, int sign, const cplx *tw,
+                    int N, cplx *tmp) {
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int r = smallest_factor(n);
+    int m = n / r;
+    if (r == n) { /* prime length: direct DFT */
+        for (int k = 0; k < n; k++) {
+            double sr = 0., si = 0.;
+            for (int j = 0; j < n; j++) {
+                int t = (int)(((long long)j * k % n) * (N / n));
+                double wr = tw[t].re, wi = sign < 0 ? tw[t].im : -tw[t].im;
+                sr += in[j * stride].re * wr - in[j * stride].im * wi;
+                si += in[j * stride].re * wi + in[j * stride].im * wr;
+            }
+            tmp[k].re = sr;
+            tmp[k].im = si;
+        }
+        memcpy(out, tmp, sizeof(cplx) * (size_t)n);
+        return;
+    }
+    for (int q = 0; q < r; q++) fft_rec(m, in + q * stride, stride * r, out + q * m, sign, tw, N, tmp);
+    int step = N / n;
+    for (int k = 0; k < m; k++) {
+        cplx y[16];
+        for (int q = 0; q < r; q++) {
+            int t = (q * k) * step;
+            double wr = tw[t].re, wi = sign < 0 ? tw[t].im : -tw[t].im;
+            cplx a = out[q * m + k];
+            y[q].re = a.re * wr - a.im * wi;
+            y[q].im = a.re * wi + a.im * wr;
+        }
+        for (int p = 0; p < r; p++) {
+            double sr = 0., si = 0.;
+            for (int q = 0; q < r; q++) {
+                int t = ((q * p * m) % n) * step;
+                double wr = tw[t].re, wi = sign < 0 ? tw[t].im : -tw[t].im;
+                sr += y[q].re * wr - y[q].im * wi;
+                si += y[q].re * wi + y[q].im * wr;
+            }
+            tmp[k + p * m].re = sr;
+            tmp[k + p * m].im = si;
+        }
+    }
+    memcpy(out, tmp, sizeof(cplx) * (size_t)n);
+}
+
+static void fft1d_line(const fftplan *pl, cplx *line, int sign, cplx *scratch) {
+    /* scratch holds 2n */
+    int n = pl->n;
+    memcpy(scratch, line, sizeof(cplx) * (size_t)n);
+    fft_rec(n, scratch, 1, line, sign, pl->w, n, scratch + n);
+}
+
+/* In-place D-dimensional standard DFT of a packed array of extent n per axis
+ * (x slowest).  Unnormalised; sign -1 forward. */
+static void fftnd(cplx *a, int D, int n, int sign, const fftplan *pl) {
+    long long tot = 1;
+    for (int d = 0; d < D; d++) tot *= n;
+    for (int axis = 0; axis < D; axis++) {
+        long long stride = 1;
+        for (int d = axis + 1; d < D; d++) stride *= n;
+        long long nlines = tot / n;
+#pragma omp parallel
+        {
+            cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+            cplx *scr = (cplx *)malloc(sizeof(cplx) * (size_t)(3 * n));
+#pragma omp for schedule(static)
+            for (long long l = 0; l < nlines; l++) {
+                long long outer = l / stride, inner = l % stride;
+                long long base = outer * stride * n + inner;
+                for (int t = 0; t < n; t++) line[t] = a[base + (long long)t * stride];
+                fft1d_line(pl, line, sign, scr);
+                for (int t = 0; t < n; t++) a[base + (long long)t * stride] = line[t];
+            }
+            free(line);
+            free(scr);
+        }
+    }
+}
+
+int oracle_fft1d(int n, int sign, const double *in, double *out) {
+    fftplan *pl = plan_make(n);
+    cplx *a = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    memcpy(a, in, sizeof(cplx) * (size_t)n);
+    cplx *scr = (cplx *)malloc(sizeof(cplx) * (size_t)(3 * n));
+    fft1d_line(pl, a, sign, scr);
+    memcpy(out, a, sizeof(cplx) * (size_t)n);
+    free(a);
+    free(scr);
+    plan_free(pl);
+    return 0;
+}
+
+int oracle_dft_brute(int n, int sign, const double *in, double *out) {
+    for (int k = 0; k < n; k++) {
+        long double sr = 0., si = 0.;
+        for (int j = 0; j < n; j++) {
+            long double a = (long double)sign * 2.0L * 3.14159265358979323846264338327950288L *
+                             (long double)((long long)j * k % n) / (long double)n;
+            long double c = cosl(a), s = sinl(a);
+            sr += in[2 * j] * c - in[2 * j + 1] * s;
+            si += in[2 * j] * s + in[2 * j + 1] * c;
+        }
+        out[2 * k] = (double)sr;
+        out[2 * k + 1] = (double)si;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* State                                                                    */
+/* ------------------------------------------------------------------------ */
+struct orc_state {
+    mpdpft_params p;
+    int D, s, EL, S;
+    long long G, F;
+    long long CentreIndex;
+    double dx, k0, dk, sigma;
+    double *lat;  /* [D][EL] positions, rGrid (mpDPFT.cpp:7130) */
+    double *klat; /* [D][EL] k values */
+    double *rnorm; /* |r| at each grid point */
+    fftplan *pl;
+    fftplan *pl2; /* 2s-point plan (isolated Hartree) */
+    cplx *khat_iso; /* FFT of the isolated kernel, (2s)^D */
+    double *Env, *V, *Den, *OldDen, *OldV; /* [S][G] */
+    double *Dx, *Dy, *Dz, *D2x, *D2y, *D2z, *Grad2, *Lap; /* [S][G] */
+    double *Tmp1, *Tmp2, *Tmp3; /* [G] */
+    cplx *CF, *TCF;             /* [G] complex */
+    double muVec[MPDPFT_MAX_SPECIES], TmpAb[MPDPFT_MAX_SPECIES], OldAb[MPDPFT_MAX_SPECIES];
+    double OldmuVec[MPDPFT_MAX_SPECIES], PreviousmuVec[MPDPFT_MAX_SPECIES];
+    double thetaVec[MPDPFT_MAX_SPECIES];
+    double DenMax[MPDPFT_MAX_SPECIES], DenMin[MPDPFT_MAX_SPECIES];
+    double VMin[MPDPFT_MAX_SPECIES], VMax[MPDPFT_MAX_SPECIES];
+    double TVec[MPDPFT_MAX_SPECIES];
+    double DeltamuModifier, RelAcc, RelAccOriginal, incrementalV, incrementalVOriginal, CC;
+    int SCcount, ExitSCcount, ExitSCloop, LinMix, PulMix, TrappedQ;
+    int warningCount, errorCount, bisection_steps;
+    /* Pulay */
+    int pulay_mixer, pulay_scope, pulay_I;
+    double **nin, **nout, **res; /* history entries, each [S][G] */
+    double *ResMat;              /* scope x scope cached Gram */
+    double coeff[64];
+    /* energies */
+    double EInt, Etot;
+    double IntE[MPDPFT_MAX_LIST], DispE[MPDPFT_MAX_SPECIES], EnvE[MPDPFT_MAX_SPECIES];
+    double AuxEkin[MPDPFT_MAX_SPECIES][4];
+};
+
+static double *dalloc(long long n) {
+    double *a = (double *)calloc((size_t)n, sizeof(double));
+    return a;
+}
+
+static void vec_at(const orc_state *o, long long idx, double *r) {
+    int EL = o->EL;
+    if (o->D == 1) {
+        r[0] = o->lat[idx];
+    } else if (o->D == 2) {
+        long long i = idx / EL, j = idx - i * EL;
+        r[0] = o->lat[i];
+        r[1] = o->lat[EL + j];
+    } else {
+        long long i = idx / ((long long)EL * EL), rem = idx - i * EL * EL, j = rem / EL, k = rem - j * EL;
+        r[0] = o->lat[i];
+        r[1] = o->lat[EL + j];
+        r[2] = o->lat[2 * EL + k];
+    }
+}
+static void kvec_at(const orc_state *o, long long idx, double *k) {
+    int EL = o->EL;
+    if (o->D == 1) {
+        k[0] = o->klat[idx];
+    } else if (o->D == 2) {
+        long long i = idx / EL, j = idx - i * EL;
+        k[0] = o->klat[i];
+        k[1] = o->klat[EL + j];
+    } else {
+        long long i = idx / ((long long)EL * EL), rem = idx - i * EL * EL, j = rem / EL, kk = rem - j * EL;
+        k[0] = o->klat[i];
+        k[1] = o->klat[EL + j];
+        k[2] = o->klat[2 * EL + kk];
+    }
+}
+static double knorm2_at(const orc_state *o, long long idx) {
+    double k[3] = {0, 0, 0};
+    kvec_at(o, idx, k);
+    double s = 0.;
+    for (int d = 0; d < o->D; d++) s += k[d] * k[d];
+    return s;
+}
+
+static void build_isolated_kernel(orc_state *o);
+
+/* rGrid and the k-lattice, InitializeHardCodedParameters (mpDPFT.cpp:862-896);
+ * smoothing exponent ConvolutionSigma (:1105-1106). */
+static orc_state *orc_create(const mpdpft_params *p) {
+    orc_state *o = (orc_state *)calloc(1, sizeof(orc_state));
+    o->p = *p;
+    o->D = p->DIM;
+    o->s = p->steps;
+    o->EL = p->steps + 1;
+    o->S = p->S;
+    long long G = 1, F = 1;
+    for (int d = 0; d < o->D; d++) {
+        G *= o->EL;
+        F *= o->s;
+    }
+    o->G = G;
+    o->F = F;
+    o->CentreIndex = (G - 1) / 2;
+    double x1 = -p->edge / 2., x2 = p->edge / 2.;
+    o->dx = p->edge / (double)p->steps;
+    o->k0 = -PI_ / o->dx;
+    o->dk = 2. * PI_ / p->edge;
+    o->lat = dalloc((long long)o->D * o->EL);
+    o->klat = dalloc((long long)o->D * o->EL);
+    for (int d = 0; d < o->D; d++) {
+        double del = (x2 - x1) / (double)p->steps;
+        double kdel = (-o->k0 - o->k0) / (double)p->steps;
+        for (int i = 0; i <= p->steps; i++) {
+            o->lat[d * o->EL + i] = x1 + del * (double)i;
+            o->klat[d * o->EL + i] = o->k0 + kdel * (double)i;
+        }
+    }
+    if (p->SmoothingKWidth > 0.)
+        o->sigma = -0.5 / (p->SmoothingKWidth * p->SmoothingKWidth);
+    else
+        o->sigma = -0.5 * pow(p->SmearingExtentDx * o->dx, 2.);
+    o->rnorm = dalloc(G);
+    for (long long i = 0; i < G; i++) {
+        double r[3] = {0, 0, 0};
+        vec_at(o, i, r);
+        double s2 = 0.;
+        for (int d = 0; d < o->D; d++) s2 += r[d] * r[d];
+        o->rnorm[i] = sqrt(s2);
+    }
+    o->pl = plan_make(o->s);
+    int S = o->S;
+    o->Env = dalloc(S * G);
+    o->V = dalloc(S * G);
+    o->Den = dalloc(S * G);
+    o->OldDen = dalloc(S * G);
+    o->OldV = dalloc(S * G);
+    o->Dx = dalloc(S * G);
+    o->Dy = dalloc(S * G);
+    o->Dz = dalloc(S * G);
+    o->D2x = dalloc(S * G);
+    o->D2y = dalloc(S * G);
+    o->D2z = dalloc(S * G);
+    o->Grad2 = dalloc(S * G);
+    o->Lap = dalloc(S * G);
+    o->Tmp1 = dalloc(G);
+    o->Tmp2 = dalloc(G);
+    o->Tmp3 = dalloc(G);
+    o->CF = (cplx *)calloc((size_t)G, sizeof(cplx));
+    o->TCF = (cplx *)calloc((size_t)G, sizeof(cplx));
+    for (int s = 0; s < S; s++) {
+        o->muVec[s] = p->muVec[s];
+        o->thetaVec[s] = p->thetaVec[s];
+        o->TVec[s] = p->TVec[s];
+    }
+    o->DeltamuModifier = p->DeltamuModifier;
+    o->RelAcc = p->RelAcc;
+    o->RelAccOriginal = p->RelAcc;
+    o->incrementalV = p->incrementalV;
+    o->incrementalVOriginal = p->incrementalV;
+    if (p->HartreeMode == 1) build_isolated_kernel(o);
+    return o;
+}
+
+static void orc_free(orc_state *o) {
+    if (!o) return;
+    free(o->lat);
+    free(o->klat);
+    free(o->rnorm);
+    plan_free(o->pl);
+    plan_free(o->pl2);
+    free(o->khat_iso);
+    free(o->Env);
+    free(o->V);
+    free(o->Den);
+    free(o->OldDen);
+    free(o->OldV);
+    free(o->Dx);
+    free(o->Dy);
+    free(o->Dz);
+    free(o->D2x);
+    free(o->D2y);
+    free(o->D2z);
+    free(o->Grad2);
+    free(o->Lap);
+    free(o->Tmp1);
+    free(o->Tmp2);
+    free(o->Tmp3);
+    free(o->CF);
+    free(o->TCF);
+    if (o->nin) {
+        for (int i = 0; i <= o->pulay_scope; i++) {
+            free(o->nin[i]);
+            free(o->nout[i]);
+            free(o->res[i]);
+        }
+        free(o->nin);
+        free(o->nout);
+        free(o->res);
+    }
+    free(o->ResMat);
+    free(o);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Centred FFT, fftParallel (mpDPFT.cpp:8173-8335)                            */
+/* ------------------------------------------------------------------------ */
+static long long grid_index(const orc_state *o, int i, int j, int k) {
+    long long EL = o->EL;
+    if (o->D == 1) return i;
+    if (o->D == 2) return (long long)i * EL + j;
+    return ((long long)i * EL + j) * EL + k;
+}
+
+/* Forward: f~(k_p) = dx^D sum_n f(x_n) e^{-i k_p x_n}; backward: f(x_n) =
+ * L^-D sum_p f~(k_p) e^{i k_p x_n}.  With x_0 = -L/2 and k_0 = -pi/dx both
+ * pre- and post-twiddles reduce to (-1)^{sum index} and the global phase
+ * exp(-+ i D pi s/2) = (-1)^{D s/2}.  Periodic image filled afterwards. */
+static void fft_centered(orc_state *o, cplx *field, int sign) {
+    int s = o->s, D = o->D;
+    long long F = o->F;
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * (size_t)F);
+#pragma omp parallel for schedule(static)
+    for (long long f = 0; f < F; f++) {
+        int i = 0, j = 0, k = 0;
+        if (D == 1) i = (int)f;
+        else if (D == 2) { i = (int)(f / s); j = (int)(f % s); }
+        else { i = (int)(f / ((long long)s * s)); j = (int)((f / s) % s); k = (int)(f % s); }
+        double sg = ((i + j + k) & 1) ? -1. : 1.;
+        cplx v = field[grid_index(o, i, j, k)];
+        buf[f].re = sg * v.re;
+        buf[f].im = sg * v.im;
+    }
+    fftnd(buf, D, s, sign, o->pl);
+    double factor = sign < 0 ? pow(o->dx, (double)D) : pow(1. / o->p.edge, (double)D);
+    long long half = (long long)D * s / 2; /* s even */
+    double gph = (half & 1) ? -1. : 1.;
+#pragma omp parallel for schedule(static)
+    for (long long f = 0; f < F; f++) {
+        int i = 0, j = 0, k = 0;
+        if (D == 1) i = (int)f;
+        else if (D == 2) { i = (int)(f / s); j = (int)(f % s); }
+        else { i = (int)(f / ((long long)s * s)); j = (int)((f / s) % s); k = (int)(f % s); }
+        double sg = ((i + j + k) & 1) ? -gph : gph;
+        cplx *dst = &field[grid_index(o, i, j, k)];
+        dst->re = factor * sg * buf[f].re;
+        dst->im = factor * sg * buf[f].im;
+    }
+    free(buf);
+    /* wrap-fill: value at index s equals index 0 on every axis */
+    long long G = o->G;
+    for (long long g = 0; g < G; g++) {
+        int idx[3] = {0, 0, 0};
+        long long rem = g;
+        for (int d = D - 1; d >= 0; d--) {
+            idx[d] = (int)(rem % o->EL);
+            rem /= o->EL;
+        }
+        int wrap = 0;
+        for (int d = 0; d < D; d++)
+            if (idx[d] == s) {
+                idx[d] = 0;
+                wrap = 1;
+            }
+        if (wrap) field[g] = field[grid_index(o, idx[0], idx[1], idx[2])];
+    }
+}
+
+int oracle_fft_centered(const mpdpft_params *p, int sign, double *cfield) {
+    mpdpft_params q = *p;
+    q.HartreeMode = 0;
+    orc_state *o = orc_create(&q);
+    fft_centered(o, (cplx *)cfield, sign);
+    orc_free(o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Trapezoid integral, Riemann (mpDPFT.cpp:7981-8026)                        */
+/* ------------------------------------------------------------------------ */
+static double integrate(const orc_state *o, const double *f) {
+    int I = o->EL, D = o->D;
+    double res = 0.;
+    if (D == 1) {
+        for (int i = 0; i < I; i++) res += ((i == 0 || i == I - 1) ? 0.5 : 1.) * f[i];
+    } else if (D == 2) {
+        double *RES = dalloc(I);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < I; i++) {
+            double ip = (i == 0 || i == I - 1) ? 0.5 : 1.;
+            double acc = 0.;
+            for (int j = 0; j < I; j++) {
+                double jp = (j == 0 || j == I - 1) ? 0.5 : 1.;
+                acc += ip * jp * f[(long long)i * I + j];
+            }
+            RES[i] = acc;
+        }
+        for (int i = 0; i < I; i++) res += RES[i];
+        free(RES);
+    } else {
+        double *RES = dalloc(I);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < I; i++) {
+            double ip = (i == 0 || i == I - 1) ? 0.5 : 1.;
+            double acc = 0.;
+            for (int j = 0; j < I; j++) {
+                double jp = (j == 0 || j == I - 1) ? 0.5 : 1.;
+                for (int k = 0; k < I; k++) {
+                    double kp = (k == 0 || k == I - 1) ? 0.5 : 1.;
+                    acc += ip * jp * kp * f[((long long)i * I + j) * I + k];
+                }
+            }
+            RES[i] = acc;
+        }
+        for (int i = 0; i < I; i++) res += RES[i];
+        free(RES);
+    }
+    for (int d = 0; d < D; d++) res *= (o->p.edge / 2. - (-o->p.edge / 2.)) / ((double)I - 1.);
+    return res;
+}
+
+double oracle_integrate(const mpdpft_params *p, const double *f) {
+    mpdpft_params q = *p;
+    q.HartreeMode = 0;
+    orc_state *o = orc_create(&q);
+    double r = integrate(o, f);
+    orc_free(o);
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Environments, GetEnv (mpDPFT.cpp:1385-1698) for types 0, 1, 3 + 40, 41    */
+/* ------------------------------------------------------------------------ */
+static int get_env(orc_state *o, int s) {
+    const mpdpft_params *p = &o->p;
+    double *E = o->Env + (long long)s * o->G;
+    int type = p->Environments[s];
+    for (long long i = 0; i < o->G; i++) {
+        double r[3] = {0, 0, 0};
+        vec_at(o, i, r);
+        double r2 = 0.;
+        for (int d = 0; d < o->D; d++) r2 += r[d] * r[d];
+        if (type == 0) E[i] = 0.;
+        else if (type == 1) E[i] = p->mpp[s] * pow(r2, 1.); /* rpow = 1, :1111 */
+        else if (type == 3) {
+            double e = p->mpp[p->S + s];
+            E[i] = p->mpp[s] * ((1. - e) * r[0] * r[0] + (1. + e) * r[1] * r[1]);
+        } else if (type == MPDPFT_ENV_SOFT_COULOMB) {
+            double a = p->mpp[p->S + s];
+            E[i] = -p->mpp[s] / sqrt(r2 + a * a);
+        } else if (type == MPDPFT_ENV_SOFT_COULOMB_CENTRES) {
+            double acc = 0.;
+            for (int c = 0; c < p->n_centres; c++) {
+                double dd = 0.;
+                for (int d = 0; d < o->D; d++) dd += sqr(r[d] - p->centres[c][d]);
+                acc += -p->centres[c][3] / sqrt(dd + sqr(p->centres[c][4]));
+            }
+            E[i] = acc;
+        } else
+            return MPDPFT_EINVAL;
+    }
+    return 0;
+}
+
+int oracle_env(const mpdpft_params *p, double *env) {
+    mpdpft_params q = *p;
+    q.HartreeMode = 0;
+    orc_state *o = orc_create(&q);
+    int rc = 0;
+    for (int s = 0; s < o->S && rc == 0; s++) rc = get_env(o, s);
+    memcpy(env, o->Env, sizeof(double) * (size_t)(o->S * o->G));
+    orc_free(o);
+    return rc;
+}
+
+/* Regularize (mpDPFT.cpp:12259-12306), procedures 0, 1, 2, 4. */
+static void regularize(orc_state *o, int proc, double *f, double aux) {
+    if (proc == 1) {
+        double a = -30. / (o->p.edge * o->p.edge);
+        for (long long i = 0; i < o->G; i++) f[i] *= EXPs(a * o->rnorm[i] * o->rnorm[i]);
+    } else if (proc == 2) {
+        double thr = aux * o->p.edge, a = -2500. / (o->p.edge * o->p.edge);
+        for (long long i = 0; i < o->G; i++) {
+            double r = o->rnorm[i];
+            if (r > thr) f[i] *= EXPs(a * (r - thr) * (r - thr));
+        }
+    } else if (proc == 4) {
+        for (long long i = 0; i < o->G; i++) f[i] = POSs(f[i]);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* DelField methods 4/5 (mpDPFT.cpp:7243-7582), followed step by step: 23    */
+/* centred FFTs per 3D call.                                                  */
+/* ------------------------------------------------------------------------ */
+static void smooth_inplace(orc_state *o, double *f) {
+    for (long long i = 0; i < o->G; i++) {
+        o->CF[i].re = f[i];
+        o->CF[i].im = 0.;
+    }
+    fft_centered(o, o->CF, -1);
+    for (long long i = 0; i < o->G; i++) {
+        double fac = EXPs(o->sigma * knorm2_at(o, i));
+        o->CF[i].re *= fac;
+        o->CF[i].im *= fac;
+    }
+    fft_centered(o, o->CF, +1);
+    for (long long i = 0; i < o->G; i++) f[i] = o->CF[i].re;
+}
+
+static void delfield(orc_state *o, const double *in, int s, int method) {
+    long long G = o->G;
+    int D = o->D;
+    double *T3 = o->Tmp3;
+    memcpy(T3, in, sizeof(double) * (size_t)G);
+    int reg2 = 0;
+    for (int r = 0; r < o->p.n_regularize; r++)
+        if (o->p.regularize[r] == 2) reg2 = 1;
+    if (reg2 || (o->p.TaperOff && (method == 4 || method == 5))) {
+        double mx = 0.;
+        int first = 1;
+        double rt = o->p.edge * o->p.RegularizationThreshold;
+        for (long long i = 0; i < G; i++) {
+            if (fabs(o->rnorm[i] - rt) < 2. * o->dx) {
+                if (first) {
+                    mx = in[i];
+                    first = 0;
+                }
+                if (in[i] > mx) mx = in[i];
+            }
+        }
+        for (long long i = 0; i < G; i++) T3[i] -= mx;
+        regularize(o, 2, T3, o->p.RegularizationThreshold);
+    }
+    double *Dj[3] = {o->Dx + s * G, o->Dy + s * G, o->Dz + s * G};
+    double *D2j[3] = {o->D2x + s * G, o->D2y + s * G, o->D2z + s * G};
+    for (long long i = 0; i < G; i++) {
+        o->CF[i].re = T3[i];
+        o->CF[i].im = 0.;
+    }
+    fft_centered(o, o->CF, -1);
+    memcpy(o->TCF, o->CF, sizeof(cplx) * (size_t)G);
+    for (int deriv = 1; deriv <= 2; deriv++) {
+        for (int j = 0; j < D; j++) {
+            if (deriv > 1 || j > 0) memcpy(o->CF, o->TCF, sizeof(cplx) * (size_t)G);
+            for (long long i = 0; i < G; i++) {
+                double k[3] = {0, 0, 0};
+                kvec_at(o, i, k);
+                double fac = deriv == 1 ? -k[j] : -k[j] * k[j];
+                o->CF[i].re *= fac;
+                o->CF[i].im *= fac;
+            }
+            fft_centered(o, o->CF, +1);
+            double *dst = deriv == 1 ? Dj[j] : D2j[j];
+            for (long long i = 0; i < G; i++) dst[i] = deriv == 1 ? o->CF[i].im : o->CF[i].re;
+        }
+    }
+    if (method == 5) {
+        /* order of the reference: Dx, D2x, Dy, D2y, Dz, D2z */
+        for (int j = 0; j < D; j++) {
+            smooth_inplace(o, Dj[j]);
+            smooth_inplace(o, D2j[j]);
+        }
+    }
+    double *g2 = o->Grad2 + s * G, *lap = o->Lap + s * G;
+    for (long long i = 0; i < G; i++) {
+        double acc = 0.;
+        for (int j = 0; j < D; j++) acc += Dj[j][i] * Dj[j][i];
+        g2[i] = acc;
+        double l = 0.;
+        for (int j = 0; j < D; j++) l += D2j[j][i];
+        lap[i] = l;
+    }
+    if (method == 5) {
+        smooth_inplace(o, g2);
+        for (long long i = 0; i < G; i++) g2[i] = POSs(g2[i]);
+        smooth_inplace(o, lap);
+    }
+}
+
+int oracle_delfield(const mpdpft_params *p, const double *f, int method, double *dx, double *dy,
+                    double *dz, double *gradsq, double *lap) {
+    mpdpft_params q = *p;
+    q.HartreeMode = 0;
+    q.S = 1;
+    orc_state *o = orc_create(&q);
+    delfield(o, f, 0, method);
+    size_t n = sizeof(double) * (size_t)o->G;
+    if (dx) memcpy(dx, o->Dx, n);
+    if (dy && o->D > 1) memcpy(dy, o->Dy, n);
+    if (dz && o->D > 2) memcpy(dz, o->Dz, n);
+    if (gradsq) memcpy(gradsq, o->Grad2, n);
+    if (lap) memcpy(lap, o->Lap, n);
+    orc_free(o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Interaction potentials, getVint (mpDPFT.cpp:2240-2318)                    */
+/* ------------------------------------------------------------------------ */
+
+/* PW92 unpolarised correlation (Perdew & Wang, PRB 45, 13244 (1992), Eq. 10,
+ * Table I, p = 1).  eps_c per particle and v_c = eps_c - (rs/3) d eps_c/d rs,
+ * in hartree; n in bohr^-3. */
+void oracle_pw92(double n, double *eps_c, double *v_c) {
+    const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+    if (!(n > 0.)) {
+        *eps_c = 0.;
+        *v_c = 0.;
+        return;
+    }
+    double rs = cbrt(3. / (4. * PI_ * n));
+    double srs = sqrt(rs);
+    double Q0 = -2. * A * (1. + a1 * rs);
+    double Q1 = 2. * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+    double Q1p = A * (b1 / srs + 2. * b2 + 3. * b3 * srs + 4. * b4 * rs);
+    double lg = log(1. + 1. / Q1);
+    double ec = Q0 * lg;
+    double dec = -2. * A * a1 * lg - Q0 * Q1p / (Q1 * Q1 + Q1);
+    *eps_c = ec;
+    *v_c = ec - rs / 3. * dec;
+}
+
+static void build_isolated_kernel(orc_state *o) {
+    int D = o->D, s = o->s, M = 2 * s;
+    long long tot = 1;
+    for (int d = 0; d < D; d++) tot *= M;
+    o->pl2 = plan_make(M);
+    o->khat_iso = (cplx *)calloc((size_t)tot, sizeof(cplx));
+    double g0 = (D == 3 ? oracle_cube_inverse_r() : 4. * log(1. + sqrt(2.))) / o->dx;
+    for (long long t = 0; t < tot; t++) {
+        long long rem = t;
+        double r2 = 0.;
+        for (int d = D - 1; d >= 0; d--) {
+            int m = (int)(rem % M);
+            rem /= M;
+            int dm = m <= s ? m : m - M;
+            r2 += sqr((double)dm * o->dx);
+        }
+        o->khat_iso[t].re = r2 > 0. ? 1. / sqrt(r2) : g0;
+        o->khat_iso[t].im = 0.;
+    }
+    fftnd(o->khat_iso, D, M, -1, o->pl2);
+}
+
+/* Isolated-boundary Hartree (extension; the reference only has the periodic
+ * mode): V(r_i) = gammaH * dx^D * sum_{j in [0,s)^D} n_j G(r_i - r_j) with
+ * G = 1/r and the cell average at r = 0, by a zero-padded (2s)^D circular
+ * convolution, for all i in [0,s]^D. */
+static void hartree_isolated(orc_state *o, const double *n, double *out) {
+    int D = o->D, s = o->s, M = 2 * s;
+    long long tot = 1;
+    for (int d = 0; d < D; d++) tot *= M;
+    cplx *buf = (cplx *)calloc((size_t)tot, sizeof(cplx));
+    for (long long g = 0; g < o->G; g++) {
+        int idx[3] = {0, 0, 0};
+        long long rem = g;
+        int ok = 1;
+        for (int d = D - 1; d >= 0; d--) {
+            idx[d] = (int)(rem % o->EL);
+            rem /= o->EL;
+            if (idx[d] == s) ok = 0;
+        }
+        if (!ok) continue;
+        long long t = 0;
+        for (int d = 0; d < D; d++) t = t * M + idx[d];
+        buf[t].re = n[g];
+    }
+    fftnd(buf, D, M, -1, o->pl2);
+    for (long long t = 0; t < tot; t++) {
+        double kr = o->khat_iso[t].re, ki = o->khat_iso[t].im;
+        double a = buf[t].re, b = buf[t].im;
+        buf[t].re = a * kr - b * ki;
+        buf[t].im = a * ki + b * kr;
+    }
+    fftnd(buf, D, M, +1, o->pl2);
+    double scale = o->p.gammaH * pow(o->dx, (double)D) / (double)tot;
+    for (long long g = 0; g < o->G; g++) {
+        long long rem = g, t = 0;
+        int idx[3] = {0, 0, 0};
+        for (int d = D - 1; d >= 0; d--) {
+            idx[d] = (int)(rem % o->EL);
+            rem /= o->EL;
+        }
+        for (int d = 0; d < D; d++) t = t * M + idx[d];
+        out[g] = scale * buf[t].re;
+    }
+    free(buf);
+}
+
+/* HartreeRepulsion, HartreeType 0 (mpDPFT.cpp:2583-2717): periodic transform
+ * with Gaussian compensation of the k = 0 singularity. */
+static int hartree(orc_state *o, int EnergyQ, int ss, double *out) {
+    const double *n = o->Den + (long long)ss * o->G;
+    if (o->p.HartreeMode == 1) {
+        hartree_isolated(o, n, out);
+        if (EnergyQ)
+            for (long long i = 0; i < o->G; i++) out[i] *= 0.5 * n[i];
+        return 0;
+    }
+    if (o->D != 3) return MPDPFT_EINVAL; /* 2D Bessel-I0 variant: not on this path */
+    for (long long i = 0; i < o->G; i++) {
+        o->CF[i].re = n[i];
+        o->CF[i].im = 0.;
+    }
+    fft_centered(o, o->CF, -1);
+    double f0 = o->CF[o->CentreIndex].re;
+    double kzero = 0.000001 * o->dk * o->dk;
+    int EL = o->EL;
+    for (int i = 0; i < EL; i++) {
+        double qi = o->k0 + (double)i * o->dk, qi2 = qi * qi;
+        for (int j = 0; j < EL; j++) {
+            double qj = o->k0 + (double)j * o->dk, qj2 = qj * qj;
+            for (int l = 0; l < EL; l++) {
+                double ql = o->k0 + (double)l * o->dk, ql2 = ql * ql;
+                long long k = ((long long)i * EL + j) * EL + l;
+                double q2 = qi2 + qj2 + ql2;
+                if (q2 > kzero) {
+                    if (o->p.RegularizeCoulombSingularity)
+                        o->CF[k].re = (o->CF[k].re - f0 * EXPs(-q2)) / q2;
+                    else
+                        o->CF[k].re /= q2;
+                    o->CF[k].im /= q2;
+                } else {
+                    o->CF[k].re = o->p.RegularizeCoulombSingularity ? f0 : 0.;
+                    o->CF[k].im = 0.;
+                }
+            }
+        }
+    }
+    fft_centered(o, o->CF, +1);
+    for (long long i = 0; i < o->G; i++) {
+        double v = o->CF[i].re;
+        if (o->p.RegularizeCoulombSingularity) {
+            double z = o->rnorm[i], zf;
+            if (z > 1.0e-10) zf = PI_ * erf(z / 2.) / z;
+            else zf = sqrt(PI_);
+            v += f0 * zf / (4. * PI_ * PI_);
+        }
+        out[i] = 4. * PI_ * o->p.gammaH * v;
+        if (EnergyQ) out[i] *= 0.5 * n[i];
+    }
+    return 0;
+}
+
+/* getLIBXC replacement (mpDPFT.cpp:2830-2889) for LDA_X (-1) and LDA_C_PW
+ * (-12), with the reference's Angstrom/eV conversion applied for Units 2 and
+ * none for Units 1 (atomic units). */
+static void xc_lda(orc_state *o, int EnergyQ, int ss, int id, double *out) {
+    const double *n = o->Den + (long long)ss * o->G;
+    const double Hartree = 27.211386245988, Bohr = 0.529177210903;
+    double conv_n = o->p.Units == 2 ? Bohr * Bohr * Bohr : 1.;
+    double conv_e = o->p.Units == 2 ? Hartree : 1.;
+    for (long long i = 0; i < o->G; i++) {
+        double rho = n[i] > 0. ? conv_n * n[i] : 0.;
+        double e = 0., v = 0.;
+        if (rho > 0.) {
+            if (id == -1) {
+                double c = cbrt(3. / PI_);
+                e = -0.75 * c * cbrt(rho);
+                v = -c * cbrt(rho);
+            } else {
+                oracle_pw92(rho, &e, &v);
+            }
+        }
+        if (EnergyQ) out[i] = POSs(n[i]) * conv_e * e;
+        else out[i] = conv_e * v;
+    }
+}
+
+/* DiracExchange (mpDPFT.cpp:2759-2772). */
+static void dirac(orc_state *o, int EnergyQ, int ss, double *out) {
+    const double *n = o->Den + (long long)ss * o->G;
+    double DD = (double)o->D;
+    if (o->D < 2) {
+        memset(out, 0, sizeof(double) * (size_t)o->G);
+        return;
+    }
+    if (EnergyQ) {
+        double cXD = 2. * sqrt(4. * PI_) / ((DD - 1.) * PI_) * pow(0.5 * pow(0.75 * sqrt(PI_), DD - 2.), 1. / DD);
+        for (long long i = 0; i < o->G; i++)
+            out[i] = -o->p.gammaH * cXD * DD / (1. + DD) * pow(POSs(n[i]), 1. + 1. / DD);
+    } else {
+        for (long long i = 0; i < o->G; i++)
+            out[i] = -o->p.gammaH * (2. * pow(POSs(n[i]) * DD / PI_, 1. / DD)) / (DD - 1.);
+    }
+}
+
+static int getvint(orc_state *o, int type, int ss, double *out) {
+    long long G = o->G;
+    if (type == 0) {
+        memset(out, 0, sizeof(double) * (size_t)G);
+    } else if (type == -1 || type == -12) {
+        xc_lda(o, 0, ss, type, out);
+    } else if (type == 1) {
+        memset(out, 0, sizeof(double) * (size_t)G);
+        for (int s = 0; s < o->S; s++)
+            if (s != ss)
+                for (long long i = 0; i < G; i++) out[i] += o->Den[s * G + i];
+        for (long long i = 0; i < G; i++) out[i] *= o->p.beta;
+    } else if (type == 8) {
+        return hartree(o, 0, ss, out);
+    } else if (type == 10) {
+        for (long long i = 0; i < G; i++) out[i] = o->Den[i] * (2. * o->p.beta);
+    } else if (type == 11) {
+        dirac(o, 0, ss, out);
+    } else
+        return MPDPFT_EINVAL;
+    return 0;
+}
+
+int oracle_vint(const mpdpft_params *p, int type, int s, const double *den, double *vint) {
+    orc_state *o = orc_create(p);
+    memcpy(o->Den, den, sizeof(double) * (size_t)(o->S * o->G));
+    int rc = getvint(o, type, s, vint);
+    orc_free(o);
+    return rc;
+}
+
+double oracle_cube_inverse_r(void) {
+    /* (1/a^2) * integral of 1/|r| over a cube of side a centred at 0:
+     * 3 ln((sqrt3+1)/(sqrt3-1)) - pi/2 (closed form; tests pin it against
+     * numerical quadrature). */
+    return 3. * log((sqrt(3.) + 1.) / (sqrt(3.) - 1.)) - PI_ / 2.;
+}
+
+int oracle_hartree_isolated_direct(const mpdpft_params *p, const double *den, double *vh) {
+    mpdpft_params q = *p;
+    q.HartreeMode = 0;
+    orc_state *o = orc_create(&q);
+    int D = o->D, s = o->s;
+    double g0 = (D == 3 ? oracle_cube_inverse_r() : 4. * log(1. + sqrt(2.))) / o->dx;
+    double w = pow(o->dx, (double)D);
+#pragma omp parallel for schedule(dynamic)
+    for (long long i = 0; i < o->G; i++) {
+        double ri[3] = {0, 0, 0};
+        vec_at(o, i, ri);
+        double acc = 0.;
+        for (long long j = 0; j < o->G; j++) {
+            long long rem = j;
+            int skip = 0;
+            for (int d = 0; d < D; d++) {
+                if (rem % o->EL == s) skip = 1;
+                rem /= o->EL;
+            }
+            if (skip) continue;
+            double rj[3] = {0, 0, 0};
+            vec_at(o, j, rj);
+            double r2 = 0.;
+            for (int d = 0; d < D; d++) r2 += sqr(ri[d] - rj[d]);
+            acc += den[j] * (r2 > 0. ? 1. / sqrt(r2) : g0);
+        }
+        vh[i] = p->gammaH * w * acc;
+    }
+    orc_free(o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Densities                                                                 */
+/* ------------------------------------------------------------------------ */
+/* GetTFdensity, DensityExpression 7 (mpDPFT.cpp:4415-4456). */
+static int tf_density(orc_state *o, int s) {
+    double pref;
+    int D = o->D;
+    double g = o->p.degeneracy;
+    if (D == 1) pref = sqrt(2.) * g / PI_;
+    else if (D == 2) pref = g / (2. * PI_);
+    else pref = (o->p.Units == 2 ? 0.0475412512 : 1.) * sqrt(2.) * g / (3. * PI_ * PI_);
+    const double *V = o->V + (long long)s * o->G;
+    double *n = o->Den + (long long)s * o->G;
+    double mu = o->muVec[s];
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < o->G; i++) {
+        double t = mu - V[i];
+        if (t > 0.) {
+            if (D == 1) n[i] = pref * sqrt(t);
+            else if (D == 2) n[i] = pref * t;
+            else n[i] = pref * t * sqrt(t);
+        } else
+            n[i] = 0.;
+    }
+    return 0;
+}
+
+int oracle_nait_density(orc_state *o, int s); /* dpft_oracle_nait.c */
+
+/* GetDensity (mpDPFT.cpp:2142-2238). */
+static int get_density(orc_state *o, int s) {
+    int rc;
+    int de = o->p.DensityExpression;
+    if (de == 7) rc = tf_density(o, s);
+    else if (de == 5) rc = oracle_nait_density(o, s);
+    else return MPDPFT_EINVAL;
+    double *n = o->Den + (long long)s * o->G;
+    for (int r = 0; r < o->p.n_regularize; r++) regularize(o, o->p.regularize[r], n, o->p.RegularizationThreshold);
+    return rc;
+}
+
+int oracle_density(const mpdpft_params *p, int s, double mu, const double *v, double *den, double *N) {
+    mpdpft_params q = *p;
+    q.HartreeMode = 0;
+    orc_state *o = orc_create(&q);
+    memcpy(o->V + (long long)s * o->G, v, sizeof(double) * (size_t)o->G);
+    o->muVec[s] = mu;
+    int rc = get_density(o, s);
+    memcpy(den, o->Den + (long long)s * o->G, sizeof(double) * (size_t)o->G);
+    if (N) *N = integrate(o, den);
+    orc_free(o);
+    return rc;
+}
+
+/* EnforceConstraints (mpDPFT.cpp:4569-4743): mu bracketing then bisection. */
+static int enforce_constraints(orc_state *o) {
+    int S = o->S;
+    const mpdpft_params *p = &o->p;
+    o->TrappedQ = 0;
+    int abortQ = 0;
+    if (!(o->RelAcc > MP_ && p->TaskType < 100)) goto finite;
+    {
+        double nrm = 0.;
+        for (int s = 0; s < S; s++) {
+            o->PreviousmuVec[s] = o->muVec[s];
+            nrm += sqr(o->muVec[s]);
+        }
+        if (sqrt(nrm) < MP_)
+            for (int s = 0; s < S; s++) o->PreviousmuVec[s] = Signs(o->PreviousmuVec[s]) * MP_;
+        double MinDeltamu = fabs(o->muVec[0]);
+        const int MAXRC = 20;
+        double muMIN[MPDPFT_MAX_SPECIES], muMAX[MPDPFT_MAX_SPECIES], Oldmu[MPDPFT_MAX_SPECIES],
+            Deltamu[MPDPFT_MAX_SPECIES];
+        int MINfound[MPDPFT_MAX_SPECIES], MAXfound[MPDPFT_MAX_SPECIES];
+        for (int s = 0; s < S; s++) {
+            muMIN[s] = muMAX[s] = o->muVec[s];
+            Oldmu[s] = 0.;
+            Deltamu[s] = o->muVec[s];
+            MINfound[s] = MAXfound[s] = 0;
+        }
+        for (int s = 0; s < S; s++) {
+            double Ab = p->Abundances[s];
+            double diff0 = fabs(o->TmpAb[s] - Ab);
+            if (diff0 / fabs(2. * Ab) > o->RelAcc) {
+                int rc = 0;
+                while (o->TmpAb[s] > Ab && rc < MAXRC) {
+                    MAXfound[s] = 1;
+                    if (o->muVec[s] < muMAX[s]) muMAX[s] = o->muVec[s];
+                    Deltamu[s] = EXPs(o->DeltamuModifier) * (1. + (double)rc) * fabs(o->PreviousmuVec[s]);
+                    Oldmu[s] = o->muVec[s];
+                    o->muVec[s] -= Deltamu[s];
+                    if (fabs(o->muVec[s]) > 1.0e+20) {
+                        o->warningCount++;
+                        o->muVec[s] = o->PreviousmuVec[s];
+                        break;
+                    }
+                    get_density(o, s);
+                    o->bisection_steps++;
+                    o->TmpAb[s] = integrate(o, o->Den + (long long)s * o->G);
+                    rc++;
+                    if (rc == 1) {
+                        double diff1 = fabs(o->TmpAb[s] - Ab);
+                        o->DeltamuModifier -= Signs(diff1 - diff0) * 0.1 * fabs(o->DeltamuModifier);
+                    }
+                    muMIN[s] = o->muVec[s];
+                    if (o->RelAcc < MP_) o->RelAcc = o->RelAccOriginal;
+                }
+                if (o->TmpAb[s] <= Ab) MINfound[s] = 1;
+                rc = 0;
+                if (!MAXfound[s]) {
+                    while (o->TmpAb[s] < Ab && rc < MAXRC) {
+                        MINfound[s] = 1;
+                        if (fabs(o->muVec[s]) > 1.0e+100) {
+                            o->errorCount++;
+                            o->muVec[s] = o->PreviousmuVec[s];
+                            break;
+                        }
+                        if (o->muVec[s] > muMIN[s]) muMIN[s] = o->muVec[s];
+                        Deltamu[s] = EXPs(o->DeltamuModifier) * (1. + (double)rc) * fabs(o->PreviousmuVec[s]);
+                        Oldmu[s] = o->muVec[s];
+                        o->muVec[s] += Deltamu[s];
+                        if (fabs(o->muVec[s]) > 1.0e+20) {
+                            o->warningCount++;
+                            o->muVec[s] = o->PreviousmuVec[s];
+                            break;
+                        }
+                        get_density(o, s);
+                        o->bisection_steps++;
+                        o->TmpAb[s] = integrate(o, o->Den + (long long)s * o->G);
+                        rc++;
+                        if (rc == 1) {
+                            double diff1 = fabs(o->TmpAb[s] - Ab);
+                            o->DeltamuModifier -= Signs(diff1 - diff0) * 0.1 * fabs(o->DeltamuModifier);
+                        }
+                        muMAX[s] = o->muVec[s];
+                        if (o->RelAcc < MP_) o->RelAcc = o->RelAccOriginal;
+                    }
+                }
+                if (o->TmpAb[s] >= Ab) MAXfound[s] = 1;
+                if (!MAXfound[s] || !MINfound[s]) {
+                    o->muVec[s] = o->PreviousmuVec[s];
+                    abortQ = 1;
+                    o->warningCount++;
+                    break;
+                }
+            }
+        }
+        if (!abortQ) {
+            for (int s = 0; s < S; s++) {
+                Deltamu[s] = (muMAX[s] - muMIN[s]) * 0.5;
+                o->muVec[s] = (muMIN[s] + muMAX[s]) * 0.5;
+            }
+            for (int s = 0; s < S; s++) {
+                double Ab = p->Abundances[s];
+                double diff0 = fabs(o->TmpAb[s] - Ab);
+                if (!(diff0 / fabs(2. * Ab) > o->RelAccOriginal)) continue;
+                double hist[4096];
+                int nh = 0;
+                int success = 0, tooSmall = 0;
+                while (!success && !tooSmall) {
+                    success = 1;
+                    get_density(o, s);
+                    o->bisection_steps++;
+                    o->TmpAb[s] = integrate(o, o->Den + (long long)s * o->G);
+                    if (o->TmpAb[s] > 0. && nh < 4096) hist[nh++] = o->TmpAb[s];
+                    int trap = 0;
+                    if (nh > 5) {
+                        for (int h = nh - 5; h < nh - 1 && !trap; h++)
+                            for (int hp = nh - 5; hp < nh - 1; hp++)
+                                if (h >= 0 && hp > h && fabs(hist[h] - hist[hp]) < o->RelAccOriginal * Ab) {
+                                    trap++;
+                                    o->TrappedQ = 1;
+                                    break;
+                                }
+                    }
+                    if (trap) break;
+                    for (int sign = -1; sign < 2; sign += 2) {
+                        if ((double)sign * (o->TmpAb[s] - Ab) / fabs(2. * Ab) > o->RelAccOriginal) {
+                            Oldmu[s] = o->muVec[s];
+                            o->muVec[s] -= (double)sign * 0.5 * Deltamu[s];
+                            if (fabs(o->muVec[s]) > 1.0e+20) {
+                                o->warningCount++;
+                                o->muVec[s] = o->PreviousmuVec[s];
+                                break;
+                            }
+                            Deltamu[s] = fabs(o->muVec[s] - Oldmu[s]);
+                            if (Deltamu[s] < MinDeltamu) MinDeltamu = Deltamu[s];
+                            if (fabs(Deltamu[s]) < MP_) {
+                                o->warningCount++;
+                                tooSmall = 1;
+                                break;
+                            }
+                            success = 0;
+                        }
+                    }
+                }
+            }
+            o->RelAcc = o->RelAccOriginal;
+        }
+    }
+finite:
+    for (int s = 0; s < S; s++)
+        if (!isfinite(o->muVec[s])) o->muVec[s] = o->PreviousmuVec[s];
+    return abortQ ? MPDPFT_ENOBRACKET : 0;
+}
+
+/* GetDensities (mpDPFT.cpp:3306-3316). */
+static int get_densities(orc_state *o) {
+    for (int s = 0; s < o->S; s++) {
+        int rc = get_density(o, s);
+        if (rc) return rc;
+        o->TmpAb[s] = integrate(o, o->Den + (long long)s * o->G);
+    }
+    enforce_constraints(o);
+    return 0;
+}
+
+/* GetFieldStats (mpDPFT.cpp:12096-12140), the min/max part. */
+static void field_stats(orc_state *o) {
+    for (int s = 0; s < o->S; s++) {
+        const double *n = o->Den + s * o->G, *V = o->V + s * o->G;
+        double dmin = n[o->CentreIndex], dmax = dmin, vmin = V[o->CentreIndex], vmax = vmin;
+        for (long long i = 0; i < o->G; i++) {
+            if (n[i] < dmin) dmin = n[i];
+            else if (n[i] > dmax) dmax = n[i];
+            if (V[i] < vmin) vmin = V[i];
+            else if (V[i] > vmax) vmax = V[i];
+        }
+        o->DenMin[s] = dmin;
+        o->DenMax[s] = dmax;
+        o->VMin[s] = vmin;
+        o->VMax[s] = vmax;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Loop stages                                                               */
+/* ------------------------------------------------------------------------ */
+static void store_current_iteration(orc_state *o) { /* :4804-4813 */
+    size_t n = sizeof(double) * (size_t)(o->S * o->G);
+    memcpy(o->OldDen, o->Den, n);
+    memcpy(o->OldV, o->V, n);
+    for (int s = 0; s < o->S; s++) {
+        o->OldmuVec[s] = o->muVec[s];
+        o->OldAb[s] = o->TmpAb[s];
+    }
+}
+
+static int update_v(orc_state *o) { /* UpdateV + GetVint, :4815-4850 */
+    long long G = o->G;
+    for (int s = 0; s < o->S; s++) {
+        double *V = o->V + s * G;
+        memset(V, 0, sizeof(double) * (size_t)G);
+        for (int a = 0; a < o->p.n_interactions; a++) {
+            int rc = getvint(o, o->p.Interactions[a], s, o->Tmp2);
+            if (rc) return rc;
+            for (long long i = 0; i < G; i++) V[i] += o->Tmp2[i];
+        }
+        if (o->incrementalV < 1.)
+            for (long long i = 0; i < G; i++) V[i] *= o->incrementalV;
+        const double *E = o->Env + s * G;
+        for (long long i = 0; i < G; i++) V[i] += E[i];
+    }
+    if (o->SCcount > 0) o->incrementalV += o->incrementalVOriginal;
+    o->incrementalV = fmin(1., o->incrementalV);
+    return 0;
+}
+
+static void preparations_for_sc_density(orc_state *o) { /* :4506-4529 */
+    for (int s = 0; s < o->S; s++)
+        if (o->TVec[s] < 0.)
+            o->TVec[s] = fmax(fabs(o->muVec[s] - o->VMin[s]), fabs(o->muVec[s] - o->VMax[s])) / 50.;
+}
+
+static double vecmult(const double *a, const double *b, long long n) {
+    double r = 0.;
+    for (long long i = 0; i < n; i++) r += a[i] * b[i];
+    return r;
+}
+
+static void check_convergence(orc_state *o) { /* :4759-4782 */
+    double mp = 100. * MachinePrecision_;
+    long long G = o->G;
+    if (o->SCcount > 0) {
+        field_stats(o);
+        o->CC = 0.;
+        double tot = 0.;
+        for (int s = 0; s < o->S; s++) tot += o->TmpAb[s];
+        if (o->p.ConvergenceMode == 1) {
+            double mx = 0.;
+            for (long long i = 0; i < o->S * G; i++) {
+                double d = fabs(o->V[i] - o->OldV[i]);
+                if (d > mx) mx = d;
+            }
+            o->CC = mx;
+        } else {
+            double maxtest = 0.;
+            for (int s = 0; s < o->S; s++) {
+                const double *a = o->Den + s * G, *b = o->OldDen + s * G;
+                double aa = vecmult(a, a, G), bb = vecmult(b, b, G);
+                double rd = (aa + bb > MP_) ? 1. - 2. * vecmult(a, b, G) / (aa + bb) : 0.;
+                double test = rd * o->TmpAb[s] / tot;
+                if (test > maxtest) {
+                    maxtest = test;
+                    o->CC = maxtest;
+                }
+            }
+        }
+        double dab = 0., dmu = 0.;
+        for (int s = 0; s < o->S; s++) {
+            dab += sqr(o->TmpAb[s] - o->OldAb[s]);
+            dmu += sqr(o->muVec[s] - o->OldmuVec[s]);
+        }
+        int steadyQ = (o->CC < mp && sqrt(dab) < mp && sqrt(dmu) < mp);
+        double thav = 0.;
+        for (int s = 0; s < o->S; s++) thav += o->thetaVec[s];
+        thav /= (double)o->S;
+        if ((tot > mp || (tot <= mp && (double)o->SCcount * thav > 1.)) && o->p.SCcriterion >= 0. &&
+            (o->CC < o->p.SCcriterion || steadyQ))
+            o->ExitSCcount++;
+        else
+            o->ExitSCcount = 0;
+        if (o->ExitSCcount >= 3) o->ExitSCloop = 1;
+    } else
+        o->CC = 1.2345678901234567890;
+}
+
+/* Householder reflector of x (length m): x -> beta e1, v = (1, essential). */
+static void make_householder(double *x, int m, int stride, double *tau, double *beta) {
+    double c0 = x[0], tail = 0.;
+    for (int i = 1; i < m; i++) tail += x[i * stride] * x[i * stride];
+    const double tiny = 2.2250738585072014e-308;
+    if (tail <= tiny) {
+        *tau = 0.;
+        *beta = c0;
+        for (int i = 1; i < m; i++) x[i * stride] = 0.;
+    } else {
+        double b = sqrt(c0 * c0 + tail);
+        if (c0 >= 0.) b = -b;
+        for (int i = 1; i < m; i++) x[i * stride] /= (c0 - b);
+        *tau = (b - c0) / b;
+        *beta = b;
+    }
+}
+
+/* Full-pivoting Householder QR (Businger-Golub with complete pivoting, the
+ * algorithm of Eigen's FullPivHouseholderQR used at mpDPFT.cpp:5089):
+ * returns 1 if invertible by the same rank rule, and A^{-1} in Ainv. */
+static int fullpiv_qr_inverse(int n, const double *A, double *Ainv) {
+    double qr[64 * 64], hc[64];
+    int rowt[64], colt[64];
+    memcpy(qr, A, sizeof(double) * (size_t)(n * n));
+    double prec = 2.220446049250313e-16 * (double)n;
+    double biggest = 0., maxpivot = 0.;
+    int nonzero = n;
+    for (int k = 0; k < n; k++) {
+        int rb = k, cb = k;
+        double best = -1.;
+        for (int j = k; j < n; j++) /* column-major scan as Eigen's maxCoeff */
+            for (int i = k; i < n; i++) {
+                double v = fabs(qr[i * n + j]);
+                if (v > best) {
+                    best = v;
+                    rb = i;
+                    cb = j;
+                }
+            }
+        if (k == 0) biggest = best;
+        if (best <= biggest * prec) {
+            nonzero = k;
+            for (int i = k; i < n; i++) {
+                rowt[i] = i;
+                colt[i] = i;
+                hc[i] = 0.;
+            }
+            break;
+        }
+        rowt[k] = rb;
+        colt[k] = cb;
+        if (k != rb)
+            for (int j = k; j < n; j++) {
+                double t = qr[k * n + j];
+                qr[k * n + j] = qr[rb * n + j];
+                qr[rb * n + j] = t;
+            }
+        if (k != cb)
+            for (int i = 0; i < n; i++) {
+                double t = qr[i * n + k];
+                qr[i * n + k] = qr[i * n + cb];
+                qr[i * n + cb] = t;
+            }
+        double tau, beta;
+        make_householder(&qr[k * n + k], n - k, n, &tau, &beta);
+        qr[k * n + k] = beta;
+        hc[k] = tau;
+        if (fabs(beta) > maxpivot) maxpivot = fabs(beta);
+        /* apply H = I - tau v v^T to the trailing columns */
+        for (int j = k + 1; j < n; j++) {
+            double dot = qr[k * n + j];
+            for (int i = k + 1; i < n; i++) dot += qr[i * n + k] * qr[i * n + j];
+            dot *= tau;
+            qr[k * n + j] -= dot;
+            for (int i = k + 1; i < n; i++) qr[i * n + j] -= dot * qr[i * n + k];
+        }
+    }
+    double thr = maxpivot * 2.220446049250313e-16 * (double)n;
+    int rank = 0;
+    for (int i = 0; i < nonzero; i++) rank += fabs(qr[i * n + i]) > thr;
+    if (rank != n) return 0;
+    /* solve A X = I: apply row transpositions + Householders to each column
+     * of I, back-substitute R, undo column permutation. */
+    int colperm[64];
+    for (int i = 0; i < n; i++) colperm[i] = i;
+    for (int k = 0; k < n; k++) {
+        int t = colperm[k];
+        colperm[k] = colperm[colt[k]];
+        colperm[colt[k]] = t;
+    }
+    for (int c = 0; c < n; c++) {
+        double b[64], y[64];
+        for (int i = 0; i < n; i++) b[i] = (i == c) ? 1. : 0.;
+        for (int k = 0; k < n; k++) {
+            if (rowt[k] != k) {
+                double t = b[k];
+                b[k] = b[rowt[k]];
+                b[rowt[k]] = t;
+            }
+            double dot = b[k];
+            for (int i = k + 1; i < n; i++) dot += qr[i * n + k] * b[i];
+            dot *= hc[k];
+            b[k] -= dot;
+            for (int i = k + 1; i < n; i++) b[i] -= dot * qr[i * n + k];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double acc = b[i];
+            for (int j = i + 1; j < n; j++) acc -= qr[i * n + j] * y[j];
+            y[i] = acc / qr[i * n + i];
+        }
+        for (int i = 0; i < n; i++) Ainv[colperm[i] * n + c] = y[i];
+    }
+    return 1;
+}
+
+int oracle_pulay_coefficients(int I, const double *A, double *coeff, double *testval) {
+    double inv[64 * 64];
+    *testval = -1.;
+    if (!fullpiv_qr_inverse(I, A, inv)) return 0;
+    double tv = 0.;
+    for (int i = 0; i < I; i++)
+        for (int j = 0; j < I; j++) {
+            double acc = 0.;
+            for (int k = 0; k < I; k++) acc += inv[i * I + k] * A[k * I + j];
+            acc -= (i == j) ? 1. : 0.;
+            tv += acc * acc;
+        }
+    *testval = tv;
+    if (tv > 1.0e-8) return 0;
+    double x[64], Asum = 0.;
+    for (int i = 0; i < I; i++) {
+        double acc = 0.;
+        for (int k = 0; k < I; k++) acc += inv[i * I + k];
+        x[i] = acc;
+    }
+    for (int i = 0; i < I; i++) Asum += x[i];
+    for (int i = 0; i < I; i++) coeff[i] = x[i] / Asum;
+    return 1;
+}
+
+/* PulayMixer, mixer 1 with the Euclidean metric (mpDPFT.cpp:5005-5167). */
+static int pulay_mixer(orc_state *o) {
+    long long SG = (long long)o->S * o->G;
+    int scope = o->pulay_scope;
+    /* push (the slot at index I is free: arrays hold scope+1 entries) */
+    int I = o->pulay_I;
+    memcpy(o->nin[I], o->OldDen, sizeof(double) * (size_t)SG);
+    memcpy(o->nout[I], o->Den, sizeof(double) * (size_t)SG);
+    for (long long i = 0; i < SG; i++) o->res[I][i] = o->nout[I][i] - o->nin[I][i];
+    I++;
+    int shift = 0;
+    if (I > scope) {
+        double *a = o->nin[0], *b = o->nout[0], *c = o->res[0];
+        for (int k = 0; k < I - 1; k++) {
+            o->nin[k] = o->nin[k + 1];
+            o->nout[k] = o->nout[k + 1];
+            o->res[k] = o->res[k + 1];
+        }
+        o->nin[I - 1] = a;
+        o->nout[I - 1] = b;
+        o->res[I - 1] = c;
+        I--;
+        shift = 1;
+    }
+    o->pulay_I = I;
+    double old[64 * 64];
+    int oldn = scope; /* cached matrix is stored with leading dimension scope+1 */
+    memcpy(old, o->ResMat, sizeof(double) * (size_t)((scope + 1) * (scope + 1)));
+    double M[64 * 64];
+    for (int i = 0; i < I; i++)
+        for (int j = 0; j < I; j++) {
+            double v;
+            if (i == I - 1 || j == I - 1) {
+                v = 0.;
+                for (int s = 0; s < o->S; s++)
+                    v += vecmult(o->res[i] + s * o->G, o->res[j] + s * o->G, o->G);
+            } else
+                v = old[(i + shift) * (oldn + 1) + (j + shift)];
+            M[i * I + j] = v;
+            o->ResMat[i * (scope + 1) + j] = v;
+        }
+    double tv;
+    int ok = oracle_pulay_coefficients(I, M, o->coeff, &tv);
+    return ok;
+}
+
+static void admix_densities(orc_state *o) { /* :4921-4992 */
+    long long G = o->G;
+    int S = o->S;
+    double nA = 0., nD = 0.;
+    for (int s = 0; s < S; s++) {
+        nD += sqr(o->p.Abundances[s] - o->TmpAb[s]);
+        nA += sqr(o->p.Abundances[s]);
+    }
+    double RelDiff = sqrt(nD) / sqrt(nA), mod = 1.;
+    if (RelDiff > MP_) mod = fmin(1., pow(fabs(1. - RelDiff), 30.));
+    if (o->TrappedQ)
+        for (int s = 0; s < S; s++) o->thetaVec[s] *= mod;
+    int PulayQ = 0;
+    if (o->pulay_mixer > 0 && o->SCcount > 0 && o->CC < 1.0e+300) PulayQ = pulay_mixer(o);
+    if (PulayQ > 0) {
+        o->PulMix++;
+        int I = o->pulay_I;
+        for (int s = 0; s < S; s++) {
+            double th = o->thetaVec[s];
+            double *n = o->Den + s * G;
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < G; i++) {
+                double acc = 0.;
+                for (int pp = 0; pp < I; pp++)
+                    acc += o->coeff[pp] * ((1. - th) * o->nin[pp][s * G + i] + th * o->nout[pp][s * G + i]);
+                n[i] = acc;
+            }
+        }
+    } else {
+        o->LinMix++;
+        for (int s = 0; s < S; s++) {
+            double th = o->thetaVec[s];
+            for (long long i = 0; i < G; i++)
+                o->Den[s * G + i] = th * o->Den[s * G + i] + (1. - th) * o->OldDen[s * G + i];
+        }
+    }
+    if (o->TrappedQ)
+        for (int s = 0; s < S; s++) o->thetaVec[s] *= 1. / mod;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Energies, GetEnergy (mpDPFT.cpp:5414-5918)                                */
+/* ------------------------------------------------------------------------ */
+double oracle_nait_dispersal_energy(orc_state *o, int s); /* dpft_oracle_nait.c */
+
+static double angular_factor(int D) {
+    return D == 1 ? 2. : (D == 2 ? 2. * PI_ : 4. * PI_);
+}
+
+static double interaction_energy(orc_state *o, int type) { /* :5805-5918 */
+    long long G = o->G;
+    if (type == 0) return 0.;
+    if (type < 0) {
+        memset(o->Tmp1, 0, sizeof(double) * (size_t)G);
+        for (int s = 0; s < o->S; s++) {
+            xc_lda(o, 1, s, type, o->Tmp2);
+            for (long long i = 0; i < G; i++) o->Tmp1[i] += o->Tmp2[i];
+        }
+        return integrate(o, o->Tmp1);
+    }
+    if (type == 1) {
+        double E = 0.;
+        for (int s = 0; s < o->S; s++)
+            for (int sp = s + 1; sp < o->S; sp++) {
+                for (long long i = 0; i < G; i++) o->Tmp1[i] = o->Den[s * G + i] * o->Den[sp * G + i];
+                E += integrate(o, o->Tmp1);
+            }
+        return o->p.beta * E;
+    }
+    if (type == 8) {
+        memset(o->Tmp1, 0, sizeof(double) * (size_t)G);
+        for (int ss = 0; ss < o->S; ss++) {
+            hartree(o, 1, ss, o->Tmp2);
+            for (long long i = 0; i < G; i++) o->Tmp1[i] += o->Tmp2[i];
+        }
+        return integrate(o, o->Tmp1);
+    }
+    if (type == 10) {
+        for (long long i = 0; i < G; i++) o->Tmp1[i] = o->Den[i] * o->Den[i];
+        return o->p.beta * integrate(o, o->Tmp1);
+    }
+    if (o->D == 3 && o->S == 1 && type == 11) {
+        dirac(o, 1, 0, o->Tmp2);
+        return integrate(o, o->Tmp2);
+    }
+    return 0.;
+}
+
+static void aux_ekin(orc_state *o) { /* GetAuxEkin, :5461-5516 */
+    long long G = o->G;
+    int D = o->D;
+    double DD = (double)D, g = o->p.degeneracy;
+    for (int s = 0; s < o->S; s++) {
+        const double *n = o->Den + s * G;
+        if (o->p.Units > 0) {
+            double pref = D == 1 ? PI_ * PI_ / (6. * g * g) : (D == 2 ? PI_ / g : pow(243. * pow(PI_, 4.) / (250. * g * g), 1. / 3.));
+            if (o->p.Units == 2) pref *= 1. / 0.131163;
+            for (long long i = 0; i < G; i++) o->Tmp1[i] = pow(POSs(n[i]), (DD + 2.) / DD);
+            o->AuxEkin[s][0] = pref * integrate(o, o->Tmp1);
+        } else
+            o->AuxEkin[s][0] = 0.;
+        if (D == 3) {
+            delfield(o, n, s, o->p.DelFieldMethod);
+            const double *g2 = o->Grad2 + s * G;
+            double *gc = dalloc(G);
+            for (long long i = 0; i < G; i++) {
+                double rho = fabs(n[i]);
+                gc[i] = rho > 1.0e-6 * o->DenMax[s] ? g2[i] / rho : 0.;
+            }
+            double e2 = integrate(o, gc) / 72.;
+            free(gc);
+            if (o->p.Units == 2) e2 *= 1. / 0.131163;
+            o->AuxEkin[s][1] = e2;
+            o->AuxEkin[s][2] = 9. * e2;
+        } else {
+            o->AuxEkin[s][1] = 0.;
+            o->AuxEkin[s][2] = 0.;
+        }
+        if (o->S == 1 && D == 3) {
+            delfield(o, o->Env + s * G, s, o->p.DelFieldMethod);
+            for (long long i = 0; i < G; i++) {
+                double r[3];
+                vec_at(o, i, r);
+                o->Tmp1[i] = n[i] * (r[0] * o->Dx[s * G + i] + r[1] * o->Dy[s * G + i] + r[2] * o->Dz[s * G + i]);
+            }
+            o->AuxEkin[s][3] = 0.5 * (integrate(o, o->Tmp1) - o->EInt);
+        } else
+            o->AuxEkin[s][3] = 0.;
+    }
+}
+
+static double dispersal_energy(orc_state *o, int s) { /* :5551-5762 */
+    double E = 0.;
+    long long G = o->G;
+    int D = o->D;
+    double DD = (double)D;
+    for (int e = 0; e < o->p.n_ekin; e++) {
+        int t = o->p.EkinTypes[e];
+        if (t == 0) {
+            int de = o->p.DensityExpression;
+            if (de >= 6 && de <= 10) {
+                const double *V = o->V + s * G;
+                for (long long i = 0; i < G; i++) {
+                    double tt = o->muVec[s] - V[i];
+                    o->Tmp1[i] = tt > 0. ? pow(tt, (DD + 2.) / 2.) : 0.;
+                }
+                double U = o->p.Units == 2 ? 7.619964158443410 : 1.;
+                E += o->p.degeneracy * angular_factor(D) * pow(2., (DD + 2.) / 2.) /
+                     (pow(2. * PI_ * sqrt(U), DD) * (2. * DD + 4.)) * integrate(o, o->Tmp1);
+            } else if (de == 5 && D == 3) {
+                E += oracle_nait_dispersal_energy(o, s);
+            }
+        } else if (t > 0 && t <= 4)
+            E += o->AuxEkin[s][t - 1];
+    }
+    return E;
+}
+
+static void get_energy(orc_state *o) {
+    o->EInt = 0.;
+    o->Etot = 0.;
+    for (int a = 0; a < o->p.n_interactions; a++) {
+        o->IntE[a] = interaction_energy(o, o->p.Interactions[a]);
+        o->EInt += o->IntE[a];
+    }
+    o->Etot += o->EInt;
+    aux_ekin(o);
+    for (int s = 0; s < o->S; s++) {
+        long long G = o->G;
+        for (long long i = 0; i < G; i++) o->Tmp1[i] = o->Env[s * G + i] * o->Den[s * G + i];
+        o->EnvE[s] = integrate(o, o->Tmp1);
+        o->DispE[s] = dispersal_energy(o, s);
+        o->Etot += o->DispE[s] + o->EnvE[s];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Driver, GetData (mpDPFT.cpp:135-244)                                      */
+/* ------------------------------------------------------------------------ */
+static int orc_init_loop(orc_state *o) {
+    for (int s = 0; s < o->S; s++) {
+        int rc = get_env(o, s);
+        if (rc) return rc;
+    }
+    memcpy(o->V, o->Env, sizeof(double) * (size_t)(o->S * o->G));
+    int rc = get_densities(o);
+    if (rc) return rc;
+    field_stats(o);
+    /* InitializePulayMixer (mpDPFT.cpp:7029-7062); RAM guard not applied,
+     * metric weight 0 (Euclidean) when Mixer[2] == 0. */
+    o->pulay_mixer = (int)(o->p.Mixer[0] + 0.5);
+    o->pulay_scope = (int)o->p.Mixer[1];
+    if (o->pulay_scope < 1) o->pulay_scope = 1;
+    if (o->pulay_scope > 60) return MPDPFT_EINVAL;
+    if (o->pulay_mixer > 1) return MPDPFT_EINVAL;
+    if (fabs(o->p.Mixer[2]) > MP_ && o->D > 1) return MPDPFT_EINVAL; /* momentum metric: not on this path */
+    int n = o->pulay_scope + 1;
+    o->nin = (double **)calloc((size_t)n, sizeof(double *));
+    o->nout = (double **)calloc((size_t)n, sizeof(double *));
+    o->res = (double **)calloc((size_t)n, sizeof(double *));
+    for (int i = 0; i < n; i++) {
+        o->nin[i] = dalloc(o->S * o->G);
+        o->nout[i] = dalloc(o->S * o->G);
+        o->res[i] = dalloc(o->S * o->G);
+    }
+    o->ResMat = dalloc((long long)n * n);
+    return 0;
+}
+
+static int orc_iteration(orc_state *o) {
+    store_current_iteration(o);
+    int rc = update_v(o);
+    if (rc) return rc;
+    preparations_for_sc_density(o);
+    rc = get_densities(o);
+    if (rc) return rc;
+    check_convergence(o);
+    /* Intercept (:4852-4879): no break-file poll in the oracle */
+    if (o->SCcount == o->p.maxSCcount) o->ExitSCloop = 1;
+    if (getenv("ORACLE_VERBOSE"))
+        fprintf(stderr, "SC %d CC %.6e mu %.12f N %.12f DM %.4f\n", o->SCcount, o->CC, o->muVec[0], o->TmpAb[0],
+                o->DeltamuModifier);
+    admix_densities(o);
+    o->SCcount++; /* PrepareNextIteration (:5317) */
+    return 0;
+}
+
+int oracle_nait_before_energy(orc_state *o); /* raises MaxIteration like :4870-4874 */
+
+int oracle_solve(const mpdpft_params *p, mpdpft_result *r, double *den_out, double *v_out) {
+    orc_state *o = orc_create(p);
+    memset(r, 0, sizeof(*r));
+    double t0 = 0.;
+#ifdef _OPENMP
+    t0 = omp_get_wtime();
+#endif
+    int rc = orc_init_loop(o);
+    if (rc) {
+        orc_free(o);
+        return rc;
+    }
+    if (p->maxSCcount > 0) {
+        while (!o->ExitSCloop) {
+            rc = orc_iteration(o);
+            if (rc) {
+                orc_free(o);
+                return rc;
+            }
+            if (o->ExitSCloop && p->DensityExpression == 5) oracle_nait_before_energy(o);
+        }
+    }
+    get_energy(o);
+#ifdef _OPENMP
+    r->solve_seconds = omp_get_wtime() - t0;
+#endif
+    r->S = o->S;
+    r->SCcount = o->SCcount;
+    r->converged = o->ExitSCcount >= 3;
+    r->LinMix = o->LinMix;
+    r->PulMix = o->PulMix;
+    r->errorCount = o->errorCount;
+    r->warningCount = o->warningCount;
+    r->CC = o->CC;
+    r->DeltamuModifier = o->DeltamuModifier;
+    r->TotalEnergy = o->Etot;
+    r->bisection_steps = o->bisection_steps;
+    for (int s = 0; s < o->S; s++) {
+        r->muVec[s] = o->muVec[s];
+        r->FinalAbundances[s] = o->TmpAb[s];
+        r->DispersalEnergies[s] = o->DispE[s];
+        r->EnvironmentalEnergies[s] = o->EnvE[s];
+        for (int k = 0; k < 4; k++) r->AuxEkin[s][k] = o->AuxEkin[s][k];
+    }
+    for (int a = 0; a < p->n_interactions; a++) r->InteractionEnergies[a] = o->IntE[a];
+    size_t n = sizeof(double) * (size_t)(o->S * o->G);
+    if (den_out) memcpy(den_out, o->Den, n);
+    if (v_out) memcpy(v_out, o->V, n);
+    int conv = o->ExitSCcount >= 3 || p->maxSCcount == 0;
+    orc_free(o);
+    return conv ? 0 : MPDPFT_ENOTCONV;
+}
+
+double oracle_time_iterations(const mpdpft_params *p, int n_iter, int *bisection_steps) {
+    orc_state *o = orc_create(p);
+    double t0 = 0., t1 = 0.;
+#ifdef _OPENMP
+    t0 = omp_get_wtime();
+#else
+    t0 = (double)clock() / CLOCKS_PER_SEC;
+#endif
+    if (orc_init_loop(o) == 0)
+        for (int it = 0; it < n_iter && !o->ExitSCloop; it++)
+            if (orc_iteration(o)) break;
+#ifdef _OPENMP
+    t1 = omp_get_wtime();
+#else
+    t1 = (double)clock() / CLOCKS_PER_SEC;
+#endif
+    if (bisection_steps) *bisection_steps = o->bisection_steps;
+    orc_free(o);
+    return t1 - t0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* nAiT (DensityExpression 5) -- not yet restated                            */
+/* ------------------------------------------------------------------------ */
+int oracle_nait_density(orc_state *o, int s) {
+    (void)o;
+    (void)s;
+    return MPDPFT_EINVAL;
+}
+double oracle_nait_dispersal_energy(orc_state *o, int s) {
+    (void)o;
+    (void)s;
+    return 0.;
+}
+int oracle_nait_before_energy(orc_state *o) {
+    (void)o;
+    return 0;
+}

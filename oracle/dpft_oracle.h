@@ -1,0 +1,53 @@
+/*
+ * dpft_oracle.h -- ORACLE: test infrastructure only.
+ *
+ * A plain-C FP64 CPU restatement of the reference's self-consistent DPFT loop
+ * (huehou/mpDPFT, mpDPFT.cpp GetData :135-244 and the routines it calls).  It
+ * exists to check the HIP product and to time a CPU baseline; only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product never links or calls it.
+ */
+#ifndef DPFT_ORACLE_H
+#define DPFT_ORACLE_H
+#include "../include/mpdpft.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_state orc_state;
+
+/* Full solve: GetData with TaskType 0.  den_out/v_out are [S][G] (may be NULL). */
+int oracle_solve(const mpdpft_params *p, mpdpft_result *r, double *den_out, double *v_out);
+
+/* Stage entry points on a fresh state built from p. */
+int oracle_fft_centered(const mpdpft_params *p, int sign, double *cfield /* [(s+1)^D][2] */);
+int oracle_dft_brute(int n, int sign, const double *in, double *out); /* 1D, unnormalised */
+int oracle_fft1d(int n, int sign, const double *in, double *out);     /* 1D, unnormalised */
+int oracle_delfield(const mpdpft_params *p, const double *f, int method, double *dx, double *dy,
+                    double *dz, double *gradsq, double *lap);
+int oracle_vint(const mpdpft_params *p, int type, int s, const double *den, double *vint);
+int oracle_density(const mpdpft_params *p, int s, double mu, const double *v, double *den,
+                   double *N);
+int oracle_env(const mpdpft_params *p, double *env /* [S][G] */);
+double oracle_integrate(const mpdpft_params *p, const double *f);
+void oracle_pw92(double n, double *eps_c, double *v_c);
+/* Pulay small solve: full-pivoting Householder QR inverse of the I x I Gram
+ * matrix A (row-major), validation and normalised coefficients.  Returns 1 on
+ * success, 0 when the reference would fall back to linear mixing. */
+int oracle_pulay_coefficients(int I, const double *A, double *coeff, double *testval);
+/* Hartree by direct O(G^2) summation of the isolated-boundary convolution
+ * (checks the zero-padded FFT path on small grids). */
+int oracle_hartree_isolated_direct(const mpdpft_params *p, const double *den, double *vh);
+/* Cell-averaged 1/r at the origin of the isolated Hartree kernel, in units of
+ * 1/dx: (1/dx^3) * integral over the cube [-dx/2,dx/2]^3 of 1/|r|. */
+double oracle_cube_inverse_r(void);
+
+/* Timed partial workload for the CPU baseline: n_iter SC iterations (no
+ * convergence test) starting from the initial density; returns seconds. */
+double oracle_time_iterations(const mpdpft_params *p, int n_iter, int *bisection_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

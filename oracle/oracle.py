@@ -1,0 +1,161 @@
+"""ORACLE loader -- test infrastructure only.
+
+ctypes access to oracle/liboracle.so, the plain-C FP64 restatement of the
+reference's SC loop.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module; the product never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+SRC = [os.path.join(HERE, "dpft_oracle.c")]
+
+
+def build():
+    """gcc -O2 -fopenmp the oracle into oracle/liboracle.so."""
+    cmd = ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-o", LIB + ".tmp"] + SRC + ["-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        from mpdpft_amd._abi import Params, Result  # shared ABI structs only
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER(Params)
+        R = ctypes.POINTER(Result)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_solve.argtypes = [P, R, dp, dp]
+        L.oracle_fft_centered.argtypes = [P, ctypes.c_int, dp]
+        L.oracle_dft_brute.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+        L.oracle_fft1d.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+        L.oracle_delfield.argtypes = [P, dp, ctypes.c_int, dp, dp, dp, dp, dp]
+        L.oracle_vint.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.oracle_density.argtypes = [P, ctypes.c_int, ctypes.c_double, dp, dp, dp]
+        L.oracle_env.argtypes = [P, dp]
+        L.oracle_integrate.argtypes = [P, dp]
+        L.oracle_integrate.restype = ctypes.c_double
+        L.oracle_pw92.argtypes = [ctypes.c_double, dp, dp]
+        L.oracle_pw92.restype = None
+        L.oracle_pulay_coefficients.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.oracle_hartree_isolated_direct.argtypes = [P, dp, dp]
+        L.oracle_cube_inverse_r.restype = ctypes.c_double
+        L.oracle_time_iterations.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_time_iterations.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def grid_size(p):
+    return (p.steps + 1) ** p.DIM
+
+
+def solve(p):
+    from mpdpft_amd._abi import Result
+    G = grid_size(p)
+    r = Result()
+    den = np.zeros(p.S * G)
+    V = np.zeros(p.S * G)
+    rc = lib().oracle_solve(ctypes.byref(p), ctypes.byref(r), _dp(den), _dp(V))
+    return rc, r, den.reshape(p.S, G), V.reshape(p.S, G)
+
+
+def fft_centered(p, field, sign):
+    a = np.ascontiguousarray(field, dtype=np.complex128).copy()
+    rc = lib().oracle_fft_centered(ctypes.byref(p), sign, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0
+    return a
+
+
+def fft1d(x, sign, brute=False):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    f = lib().oracle_dft_brute if brute else lib().oracle_fft1d
+    f(len(x), sign, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def delfield(p, f, method=5):
+    G = grid_size(p)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    outs = [np.zeros(G) for _ in range(5)]
+    lib().oracle_delfield(ctypes.byref(p), _dp(f), method, *[_dp(o) for o in outs])
+    return dict(zip(("dx", "dy", "dz", "gradsq", "laplacian"), outs))
+
+
+def vint(p, type_, s, den):
+    G = grid_size(p)
+    den = np.ascontiguousarray(den, dtype=np.float64).reshape(-1)
+    out = np.zeros(G)
+    rc = lib().oracle_vint(ctypes.byref(p), type_, s, _dp(den), _dp(out))
+    assert rc == 0, rc
+    return out
+
+
+def density(p, s, mu, v):
+    G = grid_size(p)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros(G)
+    N = ctypes.c_double()
+    rc = lib().oracle_density(ctypes.byref(p), s, mu, _dp(v), _dp(out), ctypes.byref(N))
+    assert rc == 0, rc
+    return out, N.value
+
+
+def env(p):
+    out = np.zeros(p.S * grid_size(p))
+    assert lib().oracle_env(ctypes.byref(p), _dp(out)) == 0
+    return out.reshape(p.S, -1)
+
+
+def integrate(p, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().oracle_integrate(ctypes.byref(p), _dp(f))
+
+
+def pw92(n):
+    e, v = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_pw92(n, ctypes.byref(e), ctypes.byref(v))
+    return e.value, v.value
+
+
+def pulay_coefficients(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    c = np.zeros(n)
+    tv = ctypes.c_double()
+    ok = lib().oracle_pulay_coefficients(n, _dp(A), _dp(c), ctypes.byref(tv))
+    return ok, c, tv.value
+
+
+def hartree_isolated_direct(p, den):
+    out = np.zeros(grid_size(p))
+    den = np.ascontiguousarray(den, dtype=np.float64)
+    lib().oracle_hartree_isolated_direct(ctypes.byref(p), _dp(den), _dp(out))
+    return out
+
+
+def cube_inverse_r():
+    return lib().oracle_cube_inverse_r()
+
+
+def time_iterations(p, n_iter):
+    nb = ctypes.c_int()
+    t = lib().oracle_time_iterations(ctypes.byref(p), n_iter, ctypes.byref(nb))
+    return t, nb.value

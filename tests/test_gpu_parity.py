@@ -1,0 +1,215 @@
+"""HIP path vs the oracle on the same inputs (the parity gate).
+
+Integer/count results must agree exactly (iteration counts within +-1 for a
+converged solve); floating point within 1e-10 relative (the FP64 tolerance of
+the drop-in: density, potential, mu and total energy).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mpdpft_amd
+from mpdpft_amd import Context, configs
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-10
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def rand_field(G, seed, cplx=False):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=G)
+    if cplx:
+        f = f + 1j * rng.normal(size=G)
+    return f
+
+
+# ------------------------------------------------------------------ stages
+@pytest.mark.parametrize("D,steps", [(2, 16), (2, 64), (2, 128), (3, 16), (3, 32), (3, 96), (3, 128)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_centred(oracle, D, steps, sign):
+    print("case fft D=%d steps=%d sign=%d" % (D, steps, sign), flush=True)
+    p = configs.c0(steps=steps) if D == 2 else configs.c1(steps=steps)
+    p.HartreeMode = 0
+    p.Interactions[0] = 0
+    p.n_interactions = 1
+    G = (steps + 1) ** D
+    f = rand_field(G, steps, cplx=True)
+    # a centred field must be periodic for the backward test to be meaningful
+    ctx = Context(p)
+    a = ctx.fft(f, sign)
+    b = oracle.fft_centered(p, f, sign)
+    assert rel(a, b) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("D,steps,method", [(3, 32, 5), (3, 64, 5), (3, 32, 4), (2, 64, 5), (3, 96, 5)])
+def test_delfield(oracle, D, steps, method):
+    print("case delfield D=%d steps=%d method=%d" % (D, steps, method), flush=True)
+    p = configs.c1(steps=steps) if D == 3 else configs.c0(steps=steps, edge=16.0)
+    p.DelFieldMethod = method
+    ctx = Context(p)
+    env = oracle.env(p)[0]
+    f = env + 0.3 * np.sin(np.arange(env.size) * 0.001)
+    a = ctx.delfield(f)
+    b = oracle.delfield(p, f, method=method)
+    for k in ("dx", "dy", "gradsq", "laplacian") + (("dz",) if D == 3 else ()):
+        assert rel(a[k], b[k]) < 1e-10, k
+    ctx.close()
+
+
+@pytest.mark.parametrize("case", ["contact2d", "contact3", "dirac", "ldax", "pw92", "hartree_per", "hartree_iso",
+                                  "hartree_iso2d", "single_contact"])
+def test_interaction_potentials(oracle, case):
+    print("case vint", case, flush=True)
+    if case in ("contact2d", "hartree_iso2d"):
+        p = configs.c0(steps=64, edge=12.0)
+    elif case == "contact3":
+        p = configs.c0(steps=32, edge=12.0)
+        p.S = 3
+        p.Abundances[2] = 50.0
+        p.Environments[2] = 1
+        p.mpp[2] = 0.5
+        p.n_mpp = 3
+    else:
+        p = configs.c1(steps=32, edge=12.0, hartree_mode=1 if case.startswith("hartree_iso") else 0)
+    typ = {"contact2d": 1, "contact3": 1, "dirac": 11, "ldax": -1, "pw92": -12, "hartree_per": 8,
+           "hartree_iso": 8, "hartree_iso2d": 8, "single_contact": 10}[case]
+    p.n_interactions = 1
+    p.Interactions[0] = typ
+    if case == "hartree_iso2d":
+        p.HartreeMode = 1
+    G = (p.steps + 1) ** p.DIM
+    den = np.abs(rand_field(p.S * G, 11)).reshape(p.S, G)
+    ctx = Context(p)
+    for s in range(p.S):
+        a = ctx.interaction_potential(typ, s, den)
+        b = oracle.vint(p, typ, s, den)
+        assert rel(a, b) < 1e-12, s
+    ctx.close()
+
+
+@pytest.mark.parametrize("D,steps", [(2, 64), (3, 32)])
+def test_tf_density_and_integral(oracle, D, steps):
+    p = configs.c0(steps=steps) if D == 2 else configs.c1(steps=steps)
+    ctx = Context(p)
+    v = oracle.env(p)[0]
+    mu = 4.0 if D == 2 else -0.3
+    a, Na = ctx.density_at(0, mu, v)
+    b, Nb = oracle.density(p, 0, mu, v)
+    assert rel(a, b) < 1e-14
+    assert abs(Na - Nb) < 1e-13 * abs(Nb)
+    ctx.close()
+
+
+def test_environments(oracle):
+    p = configs.c0(steps=32)
+    p.Environments[1] = 3
+    p.n_mpp = 4
+    p.mpp[2], p.mpp[3] = 0.2, -0.3
+    ctx = Context(p)
+    assert rel(ctx.environment(), oracle.env(p)) < 1e-15
+    ctx.close()
+    q = configs.c1(steps=32)
+    q.Environments[0] = 41
+    q.n_centres = 3
+    for c, row in enumerate([(1.0, 2.0, -1.0, 1.0, 0.6), (0.0, 0.0, 0.0, 2.0, 0.4), (-3.0, 1.0, 2.0, 1.0, 0.6)]):
+        for k, v in enumerate(row):
+            q.centres[c][k] = v
+    ctx = Context(q)
+    assert rel(ctx.environment(), oracle.env(q)) < 1e-15
+    ctx.close()
+
+
+# ------------------------------------------------------------------ full loop
+def solve_both(oracle, p):
+    rc, ro, dO, VO = oracle.solve(p)
+    ctx = Context(p)
+    rg = ctx.solve()
+    dG, VG = ctx.density(), ctx.potential()
+    ctx.close()
+    return ro, dO, VO, rg, dG, VG
+
+
+def check_loop(ro, dO, VO, rg, dG, VG, S, exact_count=True):
+    if exact_count:
+        assert rg.SCcount == ro.SCcount
+    else:
+        assert abs(rg.SCcount - ro.SCcount) <= 1
+    assert rel(dG, dO) < RTOL
+    assert rel(VG, VO) < RTOL
+    for s in range(S):
+        assert abs(rg.muVec[s] - ro.muVec[s]) <= RTOL * abs(ro.muVec[s])
+        assert abs(rg.FinalAbundances[s] - ro.FinalAbundances[s]) <= RTOL * abs(ro.FinalAbundances[s])
+    assert abs(rg.TotalEnergy - ro.TotalEnergy) <= RTOL * abs(ro.TotalEnergy)
+
+
+@pytest.mark.parametrize("steps,mode", [(32, 1), (64, 1), (128, 1), (128, 0)])
+def test_c0_converged_solve(oracle, steps, mode):
+    print("case c0 steps=%d mode=%d" % (steps, mode), flush=True)
+    p = configs.c0(steps=steps, convergence_mode=mode, tol=1e-8 if mode == 1 else 1e-15)
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    assert ro.converged and rg.converged
+    check_loop(ro, dO, VO, rg, dG, VG, p.S, exact_count=False)
+    assert (rg.LinMix, rg.PulMix) == (ro.LinMix, ro.PulMix)
+
+
+@pytest.mark.parametrize("hmode,iters", [(1, 8), (0, 8), (1, 25)])
+def test_3d_tf_hartree_xc_trajectory(oracle, hmode, iters):
+    """C1 physics with the TF density: the same K iterations on both paths."""
+    print("case 3d traj hmode=%d iters=%d" % (hmode, iters), flush=True)
+    p = configs.c1(steps=32, density_expression=7, hartree_mode=hmode, maxsc=iters)
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    assert ro.SCcount == iters + 1 and rg.SCcount == iters + 1
+    check_loop(ro, dO, VO, rg, dG, VG, 1)
+    assert rg.bisection_steps == ro.bisection_steps
+
+
+def test_3d_tf_ion_converges(oracle):
+    """A positive ion (N < Z) converges in both implementations."""
+    p = configs.c1(steps=32, density_expression=7, hartree_mode=1, N=6.0, maxsc=600)
+    p.Interactions[1] = 0
+    p.Interactions[2] = 0
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    assert ro.converged and rg.converged
+    check_loop(ro, dO, VO, rg, dG, VG, 1, exact_count=False)
+
+
+# ------------------------------------------------------------------ drop-in
+def test_drop_in_directory_run(oracle, tmp_path):
+    p = configs.c0(steps=32)
+    configs.write_input(p, tmp_path / "mpDPFT.input")
+    cli = os.path.join(os.path.dirname(mpdpft_amd.__file__), "bin", "mpdpft_hip")
+    out = subprocess.run([cli, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    G = 33 * 33
+    den_lines = (tmp_path / "mpDPFT_Den.dat").read_text().splitlines()
+    assert len(den_lines) == G
+    assert all(len(l.split(" ")) == 2 and not l.endswith(" ") for l in den_lines)
+    v_lines = (tmp_path / "mpDPFT_V.dat").read_text().splitlines()
+    assert len(v_lines) == G and all(l.endswith(" ") and len(l.split()) == 4 for l in v_lines)
+    assert v_lines[0].split()[:2] == ["-16", "-16"]
+    ctl = (tmp_path / "mpDPFT_Control.dat").read_text()
+    for key in ("FinalAbundances", "muVec", "TotalEnergy", "SCiterations", "ConvergenceCheck"):
+        assert key in ctl
+    rc, ro, dO, VO = oracle.solve(p)
+    den = np.array([[float(x) for x in l.split()] for l in den_lines]).T
+    assert rel(den, dO) < RTOL
+    V = np.array([[float(x) for x in l.split()[2:]] for l in v_lines]).T
+    assert rel(V, VO) < RTOL
+    it = [l for l in ctl.split("\\\\") if l.startswith("SCiterations")][0]
+    assert it.split("=")[1].strip() == "%d/%d" % (ro.SCcount - 1, p.maxSCcount)
+
+
+def test_not_converged_is_reported(oracle):
+    p = configs.c0(steps=32, maxsc=5)
+    ctx = Context(p)
+    r = ctx.solve()
+    assert r.converged == 0 and r.SCcount == 6
+    ctx.close()

@@ -1,0 +1,273 @@
+"""Pins of the oracle's physics stages against closed forms, published
+values, direct summation and Eigen (the reference's own QR)."""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import integrate as sint
+
+from mpdpft_amd import configs
+
+
+def grid(p):
+    x = -p.edge / 2 + (p.edge / p.steps) * np.arange(p.steps + 1)
+    mesh = np.meshgrid(*([x] * p.DIM), indexing="ij")
+    return [m.reshape(-1) for m in mesh]
+
+
+# ---------------------------------------------------------------- integration
+def test_trapezoid_exact_on_bilinear(oracle):
+    """Riemann (mpDPFT.cpp:7981) is the trapezoid rule: exact for bilinear f."""
+    p = configs.c0(steps=16, edge=4.0)
+    x, y = grid(p)
+    f = 1.0 + 2 * x + 3 * y + 0.5 * x * y
+    assert abs(oracle.integrate(p, f) - 16.0) < 1e-12
+    # and for a periodic smooth function the sum is spectrally accurate
+    f2 = np.cos(2 * math.pi * x / 4.0) ** 2
+    assert abs(oracle.integrate(p, f2) - 8.0) < 1e-12
+
+
+# ---------------------------------------------------------------- TF + contact
+def test_tf_density_closed_form(oracle):
+    p = configs.c0(steps=32, edge=20.0)
+    x, y = grid(p)
+    V = 0.5 * (x * x + y * y)
+    n, N = oracle.density(p, 0, 3.0, V)
+    exact = np.maximum(3.0 - V, 0) / (2 * math.pi)
+    assert np.abs(n - exact).max() < 1e-15
+    # N = pi mu^2 / (2 pi) * ... continuum: integral (mu - r^2/2)_+/(2pi) = mu^2/2
+    assert abs(N - 4.5) < 2e-2
+
+
+def test_c0_converges_to_continuum_mu(oracle):
+    """Symmetric TF + contact: n_s = (mu - r^2/2)_+/(2 pi + g), N = pi mu^2/(2 pi + g)."""
+    p = configs.c0(steps=128, edge=32.0)
+    rc, r, den, V = oracle.solve(p)
+    assert rc == 0 and r.converged
+    mu_exact = math.sqrt(100 * (2 * math.pi + 1) / math.pi)
+    assert abs(r.muVec[0] - mu_exact) < 2e-3 and abs(r.muVec[1] - mu_exact) < 2e-3
+    assert abs(r.FinalAbundances[0] - 100) < 1e-8
+    # species symmetry survives the whole loop
+    assert np.abs(den[0] - den[1]).max() < 1e-9 * den.max()
+    # continuum energy: with mu - V_s = 2 pi n, E = 2 pi int n^2 + int r^2 n + g int n^2
+    # = 4 pi mu^3 / (3 (2 pi + g))
+    g = 1.0
+    E_exact = 4 * math.pi * mu_exact ** 3 / (3 * (2 * math.pi + g))
+    assert abs(r.TotalEnergy - E_exact) < 1e-3 * E_exact
+
+
+def test_c0_energy_virial(oracle):
+    """Noninteracting 2D TF in a harmonic trap: E_kin = E_pot (virial), E = 2 E_pot."""
+    p = configs.c0(steps=128, edge=32.0, g=0.0)
+    rc, r, den, V = oracle.solve(p)
+    assert rc == 0
+    for s in range(2):
+        assert abs(r.DispersalEnergies[s] - r.EnvironmentalEnergies[s]) < 2e-3 * r.EnvironmentalEnergies[s]
+    mu = math.sqrt(2 * 100)  # N = mu^2/2 for 2D TF in V = r^2/2 with g = 1
+    assert abs(r.muVec[0] - mu) < 2e-3
+
+
+# ---------------------------------------------------------------- xc
+@pytest.mark.parametrize("rs,ec", [(1.0, -0.0598), (2.0, -0.0448), (5.0, -0.0282), (10.0, -0.0186)])
+def test_pw92_published_values(oracle, rs, ec):
+    """PW92 unpolarised correlation energies (PRB 45, 13244), to 0.2 mHa."""
+    n = 3 / (4 * math.pi * rs ** 3)
+    e, v = oracle.pw92(n)
+    assert abs(e - ec) < 2e-4
+
+
+def test_pw92_potential_is_derivative(oracle):
+    for n in [1e-4, 1e-2, 0.3, 5.0]:
+        h = n * 1e-5
+        ep, _ = oracle.pw92(n + h)
+        em, _ = oracle.pw92(n - h)
+        _, v = oracle.pw92(n)
+        dfd = ((n + h) * ep - (n - h) * em) / (2 * h)
+        assert abs(v - dfd) < 1e-8 * max(1, abs(v))
+
+
+def test_pw92_high_density_limit(oracle):
+    """eps_c -> A ln rs + const as rs -> 0, A = 0.031091."""
+    r1, r2 = 1e-4, 2e-4
+    e1, _ = oracle.pw92(3 / (4 * math.pi * r1 ** 3))
+    e2, _ = oracle.pw92(3 / (4 * math.pi * r2 ** 3))
+    assert abs((e2 - e1) / math.log(2) - 0.031091) < 2e-4
+
+
+def test_dirac_and_lda_exchange(oracle):
+    p = configs.c1(steps=8, edge=4.0)
+    p.n_interactions = 1
+    n = np.linspace(0, 2, (p.steps + 1) ** 3)
+    vx = oracle.vint(p, 11, 0, n)
+    assert np.abs(vx + (3 * n / math.pi) ** (1 / 3)).max() < 1e-13
+    vl = oracle.vint(p, -1, 0, n)
+    assert np.abs(vl - vx).max() < 1e-13
+
+
+# ---------------------------------------------------------------- Hartree
+def test_cube_inverse_r_constant(oracle):
+    val, err = sint.tplquad(lambda z, y, x: 1 / math.sqrt(x * x + y * y + z * z), 0, 0.5, 0, 0.5, 0, 0.5,
+                            epsabs=1e-12, epsrel=1e-12)
+    assert abs(8 * val - oracle.cube_inverse_r()) < 1e-8
+
+
+def test_hartree_periodic_gaussian_closed_form(oracle):
+    """n = f0 exp(-r^2/4)/(4 pi)^{3/2} has n~ = f0 exp(-k^2): the compensated
+    periodic solve (mpDPFT.cpp:2677-2712) gives V = gH (4 pi f0/L^3 + f0 erf(r/2)/r)."""
+    p = configs.c1(steps=48, edge=32.0, hartree_mode=0)
+    p.gammaH = 1.3
+    x, y, z = grid(p)
+    r = np.sqrt(x * x + y * y + z * z)
+    f0 = 2.0
+    n = f0 * np.exp(-r * r / 4) / (4 * math.pi) ** 1.5
+    V = oracle.vint(p, 8, 0, n)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        comp = np.where(r > 1e-10, f0 * np.array([math.erf(t / 2) for t in r]) / r, f0 / math.sqrt(math.pi))
+    exact = p.gammaH * (4 * math.pi * f0 / p.edge ** 3 + comp)
+    assert np.abs(V - exact).max() < 1e-6 * np.abs(exact).max()
+
+
+@pytest.mark.parametrize("D,steps", [(3, 12), (2, 24)])
+def test_hartree_isolated_fft_equals_direct_sum(oracle, D, steps):
+    p = configs.c1(steps=steps, edge=8.0, hartree_mode=1) if D == 3 else configs.c0(steps=steps, edge=8.0)
+    p.HartreeMode = 1
+    p.gammaH = 0.7
+    p.S = 1
+    rng = np.random.default_rng(D)
+    G = (steps + 1) ** D
+    n = rng.random(G)
+    a = oracle.vint(p, 8, 0, n)
+    b = oracle.hartree_isolated_direct(p, n)
+    assert np.abs(a - b).max() < 1e-11 * np.abs(b).max()
+
+
+def test_hartree_isolated_gaussian_potential(oracle):
+    """Isolated solve of a resolved Gaussian charge tends to Q erf(r/sqrt2)/r;
+    the error of the sampled 1/r kernel shrinks with the grid spacing."""
+    errs = []
+    for steps in (32, 48):
+        p = configs.c1(steps=steps, edge=16.0, hartree_mode=1)
+        x, y, z = grid(p)
+        r = np.sqrt(x * x + y * y + z * z)
+        n = np.exp(-r * r / 2.0) / (2 * math.pi) ** 1.5
+        V = oracle.vint(p, 8, 0, n)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            exact = np.where(r > 1e-12, np.array([math.erf(t / math.sqrt(2)) for t in r]) / r,
+                             math.sqrt(2 / math.pi))
+        errs.append(np.abs(V - exact).max())
+    assert errs[0] < 1e-2
+    assert errs[1] < 0.6 * errs[0]
+
+
+# ---------------------------------------------------------------- DelField
+def _gauss_field(p, w):
+    xs = grid(p)
+    r2 = sum(c * c for c in xs)
+    return xs, r2, np.exp(-r2 / (2 * w * w))
+
+
+def test_delfield_method4_gaussian(oracle):
+    p = configs.c1(steps=48, edge=16.0)
+    p.TaperOff = 0
+    (x, y, z), r2, f = _gauss_field(p, 1.0)
+    out = oracle.delfield(p, f, method=4)
+    assert np.abs(out["dx"] + x * f).max() < 1e-10
+    assert np.abs(out["dz"] + z * f).max() < 1e-10
+    assert np.abs(out["laplacian"] - (r2 - 3) * f).max() < 1e-9
+    assert np.abs(out["gradsq"] - r2 * f * f).max() < 1e-10
+
+
+def test_delfield_method5_smoothing_closed_form(oracle):
+    """Smoothing by exp(sigma k^2) is convolution with a Gaussian of variance
+    2|sigma|: the smoothed derivative of a width-w Gaussian is the derivative of a
+    Gaussian of variance w^2 + 2|sigma|, the Laplacian (smoothed twice) of
+    variance w^2 + 4|sigma|."""
+    p = configs.c1(steps=64, edge=24.0, kwidth=1.0)
+    p.TaperOff = 0
+    w = 1.0
+    (x, y, z), r2, f = _gauss_field(p, w)
+    sig = 0.5 / p.SmoothingKWidth ** 2
+    out = oracle.delfield(p, f, method=5)
+    v1 = w * w + 2 * sig
+    g1 = (w * w / v1) ** 1.5 * np.exp(-r2 / (2 * v1))
+    assert np.abs(out["dx"] + x / v1 * g1).max() < 1e-10
+    v2 = w * w + 4 * sig
+    g2 = (w * w / v2) ** 1.5 * np.exp(-r2 / (2 * v2))
+    assert np.abs(out["laplacian"] - (r2 / v2 - 3) / v2 * g2).max() < 1e-9
+
+
+def test_delfield_taper_shift(oracle):
+    """TaperOff: the field is shifted by its maximum on the shell |r| ~ 0.375 L
+    and damped beyond it (mpDPFT.cpp:7251-7264); derivatives of a constant vanish."""
+    p = configs.c1(steps=32, edge=16.0)
+    f = np.full((p.steps + 1) ** 3, 3.7)
+    out = oracle.delfield(p, f, method=4)
+    assert np.abs(out["dx"]).max() < 1e-12 and np.abs(out["laplacian"]).max() < 1e-12
+
+
+# ---------------------------------------------------------------- Pulay
+def _eigen_ref():
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                        "libref_eigen_qr.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (needs the reference tree's Eigen headers)")
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.ref_pulay_coefficients.argtypes = [ctypes.c_int, dp, dp, dp]
+    return L
+
+
+def _gram(rng, I, G, rank=None):
+    R = rng.normal(size=(I, G))
+    if rank is not None:
+        R = rng.normal(size=(I, rank)) @ rng.normal(size=(rank, G))
+    return R @ R.T
+
+
+@pytest.mark.parametrize("I,rank,scale", [(1, None, 1.0), (3, None, 1.0), (8, None, 1e-6), (10, None, 1e3),
+                                          (6, 3, 1.0), (10, 9, 1.0), (5, None, 1e-30)])
+def test_pulay_qr_matches_eigen(oracle, I, rank, scale):
+    L = _eigen_ref()
+    rng = np.random.default_rng(I * 7 + (rank or 0))
+    A = np.ascontiguousarray(_gram(rng, I, 40, rank) * scale)
+    ok, c, tv = oracle.pulay_coefficients(A)
+    ce = np.zeros(I)
+    tve = ctypes.c_double()
+    oke = L.ref_pulay_coefficients(I, A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   ce.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(tve))
+    assert ok == oke
+    if ok:
+        assert np.abs(c - ce).max() < 1e-9 * max(1, np.abs(ce).max())
+        assert abs(c.sum() - 1) < 1e-12
+
+
+def test_pulay_coefficients_minimise_residual(oracle):
+    """c = A^-1 1 / 1^T A^-1 1 minimises |sum c_i R_i|^2 subject to sum c_i = 1."""
+    rng = np.random.default_rng(1)
+    R = rng.normal(size=(4, 30))
+    A = R @ R.T
+    ok, c, tv = oracle.pulay_coefficients(np.ascontiguousarray(A))
+    assert ok
+    best = np.linalg.norm(c @ R)
+    for _ in range(50):
+        d = rng.normal(size=4)
+        d -= d.mean()
+        assert np.linalg.norm((c + 1e-3 * d) @ R) >= best - 1e-12
+
+
+def test_linear_fixed_point_is_pulay_fixed_point(oracle):
+    """At a fixed point n_out = n_in, residuals vanish and both mixers return n."""
+    p = configs.c0(steps=32, edge=32.0)
+    p.maxSCcount = 400
+    p.ConvergenceMode = 0
+    p.SCcriterion = 1e-15
+    rc, r, den, V = oracle.solve(p)
+    q = configs.c0(steps=32, edge=32.0)
+    q.Mixer[0] = 0.0
+    q.ConvergenceMode = 0
+    q.SCcriterion = 1e-15
+    q.maxSCcount = 4000
+    rc2, r2, den2, V2 = oracle.solve(q)
+    assert np.abs(den - den2).max() < 1e-6 * den.max()
