@@ -197,6 +197,22 @@ int mpdpft_interaction_potential(mpdpft_ctx *ctx, int type, int s, const double 
  * (GetDensity, mpDPFT.cpp:2142-2238); returns the trapezoid integral in *N. */
 int mpdpft_density(mpdpft_ctx *ctx, int s, double mu, const double *v, double *den, double *N);
 
+/* ---- per-stage GPU timing (the reference's StartTimer/EndTimer, :11997) --- */
+/* Kernel time per stage, measured with HIP events on the context's stream.
+ * 'units' counts the stage's work items (grid points, integrand evaluations,
+ * transforms); 'bytes' and 'flops' are the algorithmic HBM traffic and
+ * arithmetic of those launches (the roofline numerators). */
+typedef struct mpdpft_stage_stats {
+    char name[32];
+    long long launches;
+    double total_ms;
+    double units;
+    double bytes;
+    double flops;
+} mpdpft_stage_stats;
+int mpdpft_enable_timing(mpdpft_ctx *ctx, int on);
+int mpdpft_get_stage_stats(mpdpft_ctx *ctx, mpdpft_stage_stats *out, int max, int *n);
+
 /* ---- drop-in: run in a directory like ./mpDPFT --------------------------- */
 /* Reads dir/mpDPFT.input, solves, writes dir/mpDPFT_Den.dat, mpDPFT_V.dat and
  * mpDPFT_Control.dat in the reference formats (ExportData, :12399-12520). */

@@ -112,6 +112,17 @@ class Context:
         self._check(self._lib.mpdpft_interaction_potential(self._h, type_, s, _dp(den), _dp(out)))
         return out
 
+    def enable_timing(self, on=True):
+        self._check(self._lib.mpdpft_enable_timing(self._h, 1 if on else 0))
+
+    def stage_stats(self):
+        """{stage: dict(launches, total_ms, units, bytes, flops)} of the GPU stage timers."""
+        n = ctypes.c_int()
+        arr = (_abi.StageStats * 64)()
+        self._check(self._lib.mpdpft_get_stage_stats(self._h, arr, 64, ctypes.byref(n)))
+        return {arr[i].name.decode(): dict(launches=arr[i].launches, total_ms=arr[i].total_ms, units=arr[i].units,
+                                           bytes=arr[i].bytes, flops=arr[i].flops) for i in range(n.value)}
+
     def density_at(self, s, mu, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
         out = np.zeros(self.G)

@@ -60,6 +60,11 @@ class Result(ctypes.Structure):
     ]
 
 
+class StageStats(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong), ("total_ms", _d), ("units", _d),
+                ("bytes", _d), ("flops", _d)]
+
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmpdpft.so")
 
@@ -69,7 +74,7 @@ EXPORTS = [
     "mpdpft_parse_input", "mpdpft_create", "mpdpft_destroy", "mpdpft_last_error", "mpdpft_grid_size",
     "mpdpft_solve", "mpdpft_get_density", "mpdpft_get_potential", "mpdpft_get_environment",
     "mpdpft_set_potential", "mpdpft_fft", "mpdpft_delfield", "mpdpft_interaction_potential",
-    "mpdpft_density", "mpdpft_run_dir",
+    "mpdpft_density", "mpdpft_run_dir", "mpdpft_enable_timing", "mpdpft_get_stage_stats",
 ]
 
 _lib = None
@@ -117,6 +122,10 @@ def load_library(path=LIB_PATH):
     lib.mpdpft_interaction_potential.restype = _i
     lib.mpdpft_density.argtypes = [vp, _i, _d, dp, dp, dp]
     lib.mpdpft_density.restype = _i
+    lib.mpdpft_enable_timing.argtypes = [vp, _i]
+    lib.mpdpft_enable_timing.restype = _i
+    lib.mpdpft_get_stage_stats.argtypes = [vp, ctypes.POINTER(StageStats), _i, ctypes.POINTER(_i)]
+    lib.mpdpft_get_stage_stats.restype = _i
     lib.mpdpft_run_dir.argtypes = [ctypes.c_char_p, _i, R]
     lib.mpdpft_run_dir.restype = _i
     if lib.mpdpft_params_sizeof() != ctypes.sizeof(Params):

@@ -42,6 +42,8 @@ def _compile(src):
 
 
 def build(verbose=True):
+    from mpdpft_amd import tables  # noqa: E402  (special-function tables for nAiT)
+    tables.ensure(verbose=verbose)
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
@@ -60,6 +62,7 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
+    sys.path.insert(0, ROOT)
     try:
         build()
     except Exception as e:  # noqa: BLE001

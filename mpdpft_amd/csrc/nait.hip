@@ -1,14 +1,656 @@
 // nait.hip -- DensityExpression 5 (nAiT) on gfx950.
+//
+// n(r) = beta0 a_r tau beta1 int CurlyA(x) [Li_{1/2}(arg) + b Li_{-3/2}(arg)] dx,
+// arg = -(U - x a_r)/T, by the reference's adaptive Boole bisection per grid
+// point (getDenEnAt, mpDPFT.cpp:3363-3453; AdaptiveBoole_nAiT :3553-3578;
+// nAiTintegrand :3580-3623).  One thread walks one point's bisection tree
+// depth first with an explicit stack; accepted panels are summed left to
+// right.  The Airy kinetic energy EkinAiT03D (:5610-5709) is the second kernel.
+//
+// Special functions: CurlyA = int_y^inf Ai as the reference evaluates it (a
+// cubic spline with parabolic ends through the knots -150 + 0.01 i, its
+// asymptotic forms outside), Li_s(-e^x) and Ai by quintic Hermite
+// interpolation of the chain tables in nait_tables.bin (built by
+// mpdpft_amd/tables.py), with the reference's branches for |x| > 699.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
 #include "solver.h"
 
-struct NaitTables {
-    int dummy;
+namespace {
+
+constexpr double kPI = 3.141592653589793;
+constexpr double kMP = 1.0e-12;
+constexpr int kStack = 64;
+// Nominal FP64 operations of one integrand evaluation on the table path:
+// arg 3, Li pair (index 8, Hermite basis 36, two combinations 30) 74,
+// CurlyA cubic 11, product 4, Boole weight and accumulate 2.
+constexpr double kNaitFlopsPerEval = 94.;
+
+struct TabView {
+    const double4 *cc;  // CurlyA cubic per knot interval: c0 + u(c1 + u(c2 + u c3))
+    const double *p;    // [pn][6] Li_s(-e^x), s = 3/2 .. -7/2
+    const double2 *a;   // (Ai, Ai')
+    int cn, pn, an;
+    double cx0, ch, cinv, px0, ph, pinv, pxmax, ax0, ah, ainv;
+    double ginv[4];      // 1/Gamma(s+1), s = 3/2, 1/2, -1/2, -3/2 (as the reference spells them)
+    double pw2[4], pw3[4];  // 2^-s, 3^-s
+    double asym[4][12];     // Sommerfeld coefficients 2 eta(2k) (s)_{2k}
 };
 
-int nait_init(mpdpft_ctx *c) { return ctx_fail(c, MPDPFT_EINVAL, "nAiT is not built yet"); }
+// x^s for the four half-integer orders
+__device__ __forceinline__ double xpow(int j, double x) {
+    double r = sqrt(x);
+    return j == 0 ? x * r : (j == 1 ? r : (j == 2 ? 1. / r : 1. / (x * r)));
+}
+
+__device__ __forceinline__ double curly_a(const TabView &t, double y) {
+    if (y < -1.0e+50) return 1.;
+    if (y < -149.99)
+        return 1. - cos(0.25 * 3.141592653589793 + 0.6666666666666666666 * (-y) * sqrt(-y)) /
+                        sqrt(3.141592653589793 * (-y) * sqrt(-y));
+    if (y >= 99.99) return 0.;
+    int j = (int)floor((y - t.cx0) * t.cinv);
+    j = j < 0 ? 0 : (j > t.cn - 2 ? t.cn - 2 : j);
+    double u = y - (t.cx0 + t.ch * (double)j);
+    double4 c = t.cc[j];
+    return c.x + u * (c.y + u * (c.z + u * c.w));
+}
+
+__device__ __forceinline__ double qherm(double t, double h, double f0, double g0, double c0, double f1, double g1,
+                                        double c1) {
+    double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+    double H0 = 1. - 10. * t3 + 15. * t4 - 6. * t5, H1 = t - 6. * t3 + 8. * t4 - 3. * t5;
+    double H2 = 0.5 * t2 - 1.5 * t3 + 1.5 * t4 - 0.5 * t5, H3 = 10. * t3 - 15. * t4 + 6. * t5;
+    double H4 = -4. * t3 + 7. * t4 - 3. * t5, H5 = 0.5 * t3 - t4 + 0.5 * t5;
+    return f0 * H0 + h * g0 * H1 + h * h * c0 * H2 + f1 * H3 + h * g1 * H4 + h * h * c1 * H5;
+}
+
+// Li_s(-e^x) for chain index j (s = 3/2 - j), j in 0..3
+__device__ double polylog_j(const TabView &t, int j, double x) {
+    if (x < -699.) return 0.;
+    if (x > 699.) return -xpow(j, x) * t.ginv[j];
+    if (x < t.px0) {
+        double e = exp(x);
+        return -e + e * e * t.pw2[j] - e * e * e * t.pw3[j];
+    }
+    if (x > t.pxmax) {
+        double x2 = 1. / (x * x), acc = 0.;
+        for (int k = 11; k >= 0; k--) acc = x2 * (t.asym[j][k] + acc);
+        return -xpow(j, x) * t.ginv[j] * (1. + acc);
+    }
+    int k = (int)floor((x - t.px0) * t.pinv);
+    if (k > t.pn - 2) k = t.pn - 2;
+    double u = (x - (t.px0 + t.ph * (double)k)) * t.pinv;
+    const double *a = t.p + 6 * k, *b = a + 6;
+    return qherm(u, t.ph, a[j], a[j + 1], a[j + 2], b[j], b[j + 1], b[j + 2]);
+}
+
+// Li_{1/2}(-e^x) and Li_{-3/2}(-e^x) together (the integrand's pair)
+__device__ __forceinline__ void polylog_pair(const TabView &t, double x, double &l1, double &l3) {
+    if (x < -699.) {
+        l1 = 0.;
+        l3 = 0.;
+        return;
+    }
+    if (x < t.px0 || x > t.pxmax) {
+        l1 = polylog_j(t, 1, x);
+        l3 = polylog_j(t, 3, x);
+        return;
+    }
+    int k = (int)floor((x - t.px0) * t.pinv);
+    if (k > t.pn - 2) k = t.pn - 2;
+    double u = (x - (t.px0 + t.ph * (double)k)) * t.pinv;
+    const double *a = t.p + 6 * k, *b = a + 6;
+    double a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5];
+    double b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
+    double u2 = u * u, u3 = u2 * u, u4 = u3 * u, u5 = u4 * u, h = t.ph;
+    double H0 = 1. - 10. * u3 + 15. * u4 - 6. * u5, H1 = u - 6. * u3 + 8. * u4 - 3. * u5;
+    double H2 = 0.5 * u2 - 1.5 * u3 + 1.5 * u4 - 0.5 * u5, H3 = 10. * u3 - 15. * u4 + 6. * u5;
+    double H4 = -4. * u3 + 7. * u4 - 3. * u5, H5 = 0.5 * u3 - u4 + 0.5 * u5;
+    l1 = a1 * H0 + h * a2 * H1 + h * h * a3 * H2 + b1 * H3 + h * b2 * H4 + h * h * b3 * H5;
+    l3 = a3 * H0 + h * a4 * H1 + h * h * a5 * H2 + b3 * H3 + h * b4 * H4 + h * h * b5 * H5;
+}
+
+__device__ double airy_ai(const TabView &t, double y) {
+    if (y > 100.) return 0.;
+    if (y <= -300.) return cos(2. * (-y) * sqrt(-y) / 3. - 0.25 * kPI) / (sqrt(kPI) * pow(-y, 0.25));
+    int k = (int)floor((y - t.ax0) * t.ainv);
+    k = k < 0 ? 0 : (k > t.an - 2 ? t.an - 2 : k);
+    double x0 = t.ax0 + t.ah * (double)k, x1 = x0 + t.ah;
+    double u = (y - x0) * t.ainv;
+    double2 A = t.a[k], B = t.a[k + 1];
+    return qherm(u, t.ah, A.x, A.y, x0 * A.x, B.x, B.y, x1 * B.x);
+}
+
+__device__ __forceinline__ bool negligible(double big, double small, double maxexp) {
+    if (small == 0.) return true;
+    if (big == 0. || big < small) return false;
+    return log10(fabs(big)) - log10(fabs(small)) > maxexp;
+}
+
+// wavefront-uniform sum: every lane ends with the same bits (xor butterfly,
+// each exchange adds the same two operands)
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+struct PointArgs {
+    double pref, pref2, U, ar, invT;
+};
+
+__device__ __forceinline__ double integrand(const TabView &t, const PointArgs &q, double x) {
+    double arg = -(q.U - x * q.ar) * q.invT;
+    double l1, l3;
+    polylog_pair(t, arg, l1, l3);
+    return q.pref * curly_a(t, x) * (l1 + q.pref2 * l3);
+}
+
+// Boole weight of sample yi of n = 4 BP + 1 (BooleRule1D, mpDPFT.cpp:8050-8058)
+__device__ __forceinline__ double boole_w(int yi, int n) {
+    if (yi == 0 || yi == n - 1) return 7.;
+    int m = (yi - 1) & 3;
+    return m == 1 ? 12. : (m == 3 ? 14. : 32.);
+}
+
+struct Counts {
+    double itermax, relacc, evals;
+};
+
+struct WaveStack {
+    double a[64], b[64], prev[64];
+    int it[64];
+};
+
+// AdaptiveBoole_nAiT for one grid point, all 64 lanes together: the 2(4BP+1)
+// samples of a node are spread over the lanes, the panel sums reduced across
+// the wavefront; the tree walk is uniform (depth first, left panel first).
+__device__ double adaptive_boole_wave(const TabView &t, const PointArgs &q, int BP, int maxIter, double rtpna,
+                                      double compare, double maxexp, double a0, double b0, Counts &cnt,
+                                      WaveStack &S, int lane) {
+    int sp = 0;
+    S.a[0] = a0;
+    S.b[0] = b0;
+    S.prev[0] = 0.;
+    S.it[0] = 0;
+    sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    const int n = 4 * BP + 1, tot = 2 * n;
+    double total = 0.;
+    while (sp > 0) {
+        sp--;
+        double a = S.a[sp], b = S.b[sp], prev = S.prev[sp];
+        int iter = S.it[sp];
+        __builtin_amdgcn_wave_barrier();
+        double half = 0.5 * (b - a), Delta = 0.25 * half / ((double)BP);
+        double sa = 0., sb = 0.;
+        for (int base = 0; base < tot; base += 64) {
+            int idx = base + lane;
+            if (idx < tot) {
+                bool right = idx >= n;
+                int yi = right ? idx - n : idx;
+                double x = a + (double)yi * Delta;
+                if (right) x = x + half;
+                double f = boole_w(yi, n) * integrand(t, q, x);
+                if (right) sb += f;
+                else sa += f;
+            }
+        }
+        sa = wave_sum(sa);
+        sb = wave_sum(sb);
+        cnt.evals += (double)tot;
+        double ha = (a + half - a) / (((double)BP) * 4.), hb = (b - (a + half)) / (((double)BP) * 4.);
+        double Inta = sa * (2. * ha / 45.), Intb = sb * (2. * hb / 45.);
+        double Int = Inta + Intb;
+        if (Int != Int) {  // a NaN panel makes the point's density NaN, mapped to 0
+            total += Int;
+        } else if (iter > 1 && fabs(Int) > 0. && fabs((Int - prev) / Int) < rtpna) {
+            cnt.relacc += 1.;
+            total += Int;
+        } else if (iter > maxIter) {
+            cnt.itermax += 1.;
+            total += Int;
+        } else if (negligible(compare, pow(2., (double)iter) * Int, maxexp)) {
+            total += Int;
+        } else if (Delta < kMP) {
+            total += Int;
+        } else {
+            if (sp + 2 > 64) return __builtin_nan("");
+            S.a[sp] = a + half;
+            S.b[sp] = b;
+            S.prev[sp] = Intb;
+            S.it[sp] = iter + 1;
+            S.a[sp + 1] = a;
+            S.b[sp + 1] = a + half;
+            S.prev[sp + 1] = Inta;
+            S.it[sp + 1] = iter + 1;
+            sp += 2;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    return total;
+}
+
+struct DensityArgs {
+    const double *V, *grad2, *lap;
+    double mu, T, degeneracy, alpha, thr, rtpna, compare;
+    int BP, maxIter;
+};
+
+__device__ __forceinline__ double trap_w(const mk::Grid &g, int idx) {
+    double w = 1.;
+    int rem = idx;
+    for (int d = 0; d < g.D; d++) {
+        int c = rem % g.EL;
+        rem /= g.EL;
+        if (c == 0 || c == g.s) w *= 0.5;
+    }
+    return w;
+}
+
+__device__ __forceinline__ double bsum(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+    return r;
+}
+
+// One wavefront per grid point (grid-stride over points).  partials: [0]
+// weighted density sum, [1] iterMax returns, [2] relAcc returns.
+__global__ __launch_bounds__(256) void k_nait_density(
+    mk::Grid g, TabView t, DensityArgs da, double *n, double *partials) {
+    __shared__ double sh[4];
+    __shared__ WaveStack stacks[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    WaveStack &S = stacks[wv];
+    double acc = 0., cim = 0., cra = 0., cev = 0.;
+    const int G = (int)g.G;
+    const double tau = 1. / da.T, beta0 = -da.degeneracy / pow(2. * kPI, 1.5), beta1 = pow(da.alpha * tau, -1.5);
+    const double maxexp = -log10(da.rtpna);
+    for (int i = blockIdx.x * 4 + wv; i < G; i += gridDim.x * 4) {
+        double U = da.V[i] - da.mu;
+        double ar = 0.5 * pow(da.alpha * 1. * da.grad2[i], 1. / 3.);
+        double beta2 = -da.lap[i] / (12. * sqrt(da.alpha * tau));
+        double res;
+        if (ar < fabs(da.thr)) {
+            double arg = -tau * U;
+            res = beta0 * (beta1 * polylog_j(t, 0, arg) + beta2 * polylog_j(t, 2, arg));
+        } else {
+            PointArgs q;
+            q.invT = 1. / da.T;
+            q.U = U;
+            q.ar = ar;
+            q.pref = beta0 * ar * tau * beta1;
+            q.pref2 = beta2 / beta1;
+            // lower limit: the first x = -2^k (k = 1, 2, ...) where the
+            // integrand is below 1e-12, stopping once x < -1e8 (k = 27)
+            bool small = true;
+            if (lane < 27) {
+                double x = -ldexp(1., lane + 1), arg = -tau * (U - x * ar);
+                double l1 = polylog_j(t, 1, arg), l3 = polylog_j(t, 3, arg);
+                double test = beta0 * curly_a(t, x) * ar * tau * (beta1 * l1 + beta2 * l3);
+                small = !(fabs(test) > kMP);
+            }
+            unsigned long long bal = __ballot(small ? 1 : 0);
+            int k = __ffsll((long long)bal) - 1;  // lowest lane with a small integrand
+            if (k < 0 || k > 26) k = 26;
+            double x = -ldexp(1., k + 1);
+            Counts cnt{0., 0., 27.};
+            double rem = adaptive_boole_wave(t, q, da.BP, da.maxIter, da.rtpna, da.compare, maxexp, 2. * x, x, cnt, S,
+                                             lane);
+            res = rem;
+            res += adaptive_boole_wave(t, q, da.BP, da.maxIter, da.rtpna, da.compare, maxexp, x, 100., cnt, S, lane);
+            cim += cnt.itermax;
+            cra += cnt.relacc;
+            cev += cnt.evals;
+        }
+        if (res != res) res = 0.;
+        if (lane == 0) {
+            n[i] = res;
+            acc += trap_w(g, i) * res;
+        }
+    }
+    if (lane != 0) {
+        cim = 0.;
+        cra = 0.;
+        cev = 0.;
+    }
+    double r0 = bsum(acc, sh);
+    double r1 = bsum(cim, sh);
+    double r2 = bsum(cra, sh);
+    double r3 = bsum(cev, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = r0;
+        partials[mk::kRedBlocks + blockIdx.x] = r1;
+        partials[2 * mk::kRedBlocks + blockIdx.x] = r2;
+        partials[3 * mk::kRedBlocks + blockIdx.x] = r3;
+    }
+}
+
+struct EkinArgs {
+    const double *V, *grad2, *lap;
+    double mu, thr, c0, c1, c2, internal_acc, cmp, area_pref, neg_exp;
+    int MaxIt;
+};
+
+// Airy kinetic-energy density (EkinAiT03D inner loop), one wavefront per
+// point: the trapezoid samples of each refinement are spread over the lanes.
+__global__ __launch_bounds__(256) void k_ekin_ait(mk::Grid g, TabView t, EkinArgs ea, double *out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int G = (int)g.G;
+    for (int i = blockIdx.x * 4 + wv; i < G; i += gridDim.x * 4) {
+        double U = ea.V[i] - ea.mu, a = 0.5 * ea.c0 * pow(fabs(ea.grad2[i]), 1. / 3.);
+        double lap = ea.lap[i];
+        double val;
+        if (a < ea.thr) {
+            val = (-U > 0.) ? ea.c1 * pow(2. * (-U), 2.5) / 5. - ea.c2 * lap * sqrt(2. * (-U)) / 6. : 0.;
+        } else {
+            double xmax = fmax(1., 100. - U / a);
+            double prev = 1.;
+            int iteration = 0, gs = 64;
+            for (;;) {
+                double r = 0.;
+                const double Ua = U / a, xr = fmax(1., 100. - Ua), dg = (double)(gs - 1);
+                const double k1 = ea.c1 / 5., k2 = ea.c2 * lap / 6.;
+                for (int j = lane; j < gs; j += 64) {
+                    double x = (double)j * xr / dg, y = x + Ua;
+                    double AI = y > 100. ? 0. : airy_ai(t, y);
+                    double tt = 2. * a * x, sq = sqrt(tt);
+                    double f = AI * (k1 * tt * tt * sq - k2 * sq);  // c1 (2ax)^{5/2}/5 - c2 lap (2ax)^{1/2}/6
+                    r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * f;
+                }
+                r = wave_sum(r);
+                r *= (xmax - 0.) / ((double)gs - 1.);
+                val = r;
+                double ac = (fabs(val) < kMP && fabs(prev) < kMP) ? 0. : fabs((val - prev) / (val + prev));
+                double as = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
+                if (iteration > 0 && (ac < kMP || (fabs(as) < ea.internal_acc &&
+                                                   negligible(ea.cmp, ea.area_pref * val, ea.neg_exp))))
+                    break;
+                if (iteration == ea.MaxIt) break;
+                iteration++;
+                gs *= 2;
+                prev = val;
+            }
+        }
+        if (lane == 0) out[i] = val;
+    }
+}
+
+}  // namespace
+
+struct NaitTables {
+    TabView view;
+    double4 *d_c = nullptr;
+    double2 *d_a = nullptr;
+    double *d_p = nullptr;
+    double Mpp[MPDPFT_MAX_SPECIES][3];
+    double last_itermax[MPDPFT_MAX_SPECIES], last_relacc[MPDPFT_MAX_SPECIES];
+};
+
+static std::string default_tables_path() {
+    if (const char *e = getenv("MPDPFT_TABLES")) return e;
+    Dl_info info;
+    if (dladdr((void *)&default_tables_path, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t k = p.rfind('/');
+        return (k == std::string::npos ? std::string(".") : p.substr(0, k)) + "/nait_tables.bin";
+    }
+    return "nait_tables.bin";
+}
+
+// First derivatives of the cubic spline with parabolically terminated ends
+// through (x_i, y_i) (tridiagonal system, Thomas algorithm).
+static void spline_derivatives(const std::vector<double> &x, const std::vector<double> &y, std::vector<double> &d) {
+    size_t n = x.size();
+    std::vector<double> lo(n), di(n), up(n), rhs(n);
+    di[0] = 1.;
+    up[0] = 1.;
+    rhs[0] = 2. * (y[1] - y[0]) / (x[1] - x[0]);
+    for (size_t i = 1; i + 1 < n; i++) {
+        lo[i] = x[i + 1] - x[i];
+        di[i] = 2. * (x[i + 1] - x[i - 1]);
+        up[i] = x[i] - x[i - 1];
+        rhs[i] = 3. * (y[i] - y[i - 1]) / (x[i] - x[i - 1]) * (x[i + 1] - x[i]) +
+                 3. * (y[i + 1] - y[i]) / (x[i + 1] - x[i]) * (x[i] - x[i - 1]);
+    }
+    lo[n - 1] = 1.;
+    di[n - 1] = 1.;
+    rhs[n - 1] = 2. * (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+    std::vector<double> cp(n), dp(n);
+    cp[0] = up[0] / di[0];
+    dp[0] = rhs[0] / di[0];
+    for (size_t i = 1; i < n; i++) {
+        double m = di[i] - lo[i] * cp[i - 1];
+        cp[i] = up[i] / m;
+        dp[i] = (rhs[i] - lo[i] * dp[i - 1]) / m;
+    }
+    d.assign(n, 0.);
+    d[n - 1] = dp[n - 1];
+    for (size_t i = n - 1; i-- > 0;) d[i] = dp[i] - cp[i] * d[i + 1];
+}
+
+int nait_init(mpdpft_ctx *c) {
+    std::string path = default_tables_path();
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return ctx_fail(c, MPDPFT_EIO, "cannot open nAiT tables " + path);
+    char head[16];
+    long long n3[3];
+    double r[6];
+    bool ok = std::fread(head, 1, 16, f) == 16 && std::memcmp(head, "NAITTAB1", 8) == 0;
+    for (int k = 0; k < 3 && ok; k++) ok = std::fread(&n3[k], 8, 1, f) == 1 && std::fread(&r[2 * k], 8, 2, f) == 2;
+    if (!ok) {
+        std::fclose(f);
+        return ctx_fail(c, MPDPFT_EIO, "bad nAiT table header " + path);
+    }
+    int cn = (int)n3[0], an = (int)n3[1], pn = (int)n3[2];
+    std::vector<double> cv(cn), av(2 * (size_t)an), pv(6 * (size_t)pn);
+    ok = std::fread(cv.data(), 8, cn, f) == (size_t)cn && std::fread(av.data(), 8, av.size(), f) == av.size() &&
+         std::fread(pv.data(), 8, pv.size(), f) == pv.size();
+    std::fclose(f);
+    if (!ok) return ctx_fail(c, MPDPFT_EIO, "truncated nAiT tables " + path);
+    std::vector<double> x(cn), d;
+    for (int i = 0; i < cn; i++) x[i] = r[0] + r[1] * (double)i;
+    spline_derivatives(x, cv, d);
+    // cubic coefficients per knot interval (the Hermite form of the spline)
+    std::vector<double4> cc(cn);
+    double h = r[1];
+    for (int i = 0; i + 1 < cn; i++) {
+        double y0 = cv[i], y1 = cv[i + 1], d0 = d[i], d1 = d[i + 1];
+        cc[i] = double4{y0, d0, (3. * (y1 - y0) - 2. * d0 * h - d1 * h) / (h * h),
+                        (2. * (y0 - y1) + d0 * h + d1 * h) / (h * h * h)};
+    }
+    cc[cn - 1] = double4{cv[cn - 1], d[cn - 1], 0., 0.};
+    NaitTables *t = new NaitTables();
+    c->nait = t;
+    if (hipMalloc((void **)&t->d_c, sizeof(double4) * cn) != hipSuccess ||
+        hipMalloc((void **)&t->d_a, sizeof(double) * av.size()) != hipSuccess ||
+        hipMalloc((void **)&t->d_p, sizeof(double) * pv.size()) != hipSuccess)
+        return ctx_fail(c, MPDPFT_ENOMEM, "nAiT tables");
+    if (hipMemcpy(t->d_c, cc.data(), sizeof(double4) * cn, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t->d_a, av.data(), sizeof(double) * av.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t->d_p, pv.data(), sizeof(double) * pv.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return ctx_fail(c, MPDPFT_EHIP, "nAiT table upload");
+    TabView &v = t->view;
+    v.cc = t->d_c;
+    v.a = t->d_a;
+    v.p = t->d_p;
+    v.cn = cn;
+    v.an = an;
+    v.pn = pn;
+    v.cx0 = r[0];
+    v.ch = r[1];
+    v.cinv = 1. / r[1];
+    v.ax0 = r[2];
+    v.ah = r[3];
+    v.ainv = 1. / r[3];
+    v.px0 = r[4];
+    v.ph = r[5];
+    v.pinv = 1. / r[5];
+    v.pxmax = r[4] + r[5] * (double)(pn - 1);
+    // Sommerfeld coefficients 2 eta(2k), k = 1..12
+    static const double two_eta[12] = {1.6449340668482264, 1.8940656589944918, 1.9711021825948705, 1.992466003705296, 1.9980790151965429, 1.9995153702877164, 1.9998783406919596, 1.9999695284298122, 1.9999923757392204, 1.999998093223163, 1.9999995232264616, 1.9999998807977848};
+    const double sq = std::sqrt(kPI);
+    const double gam[4] = {0.75 * sq, 0.5 * sq, sq, -2. * sq};  // Gamma(s+1) as in mpDPFT.cpp:112
+    for (int j = 0; j < 4; j++) {
+        double sj = 1.5 - (double)j, ff = 1.;
+        v.ginv[j] = 1. / gam[j];
+        v.pw2[j] = std::pow(2., -sj);
+        v.pw3[j] = std::pow(3., -sj);
+        for (int k = 1; k <= 12; k++) {
+            ff *= (sj - 2 * k + 2) * (sj - 2 * k + 1);
+            v.asym[j][k - 1] = two_eta[k - 1] * ff;
+        }
+    }
+    for (int s = 0; s < c->S; s++)
+        for (int k = 0; k < 3; k++) t->Mpp[s][k] = c->p.Mpp[k];
+    return MPDPFT_OK;
+}
+
 void nait_free(mpdpft_ctx *c) {
+    if (!c->nait) return;
+    hipFree(c->nait->d_c);
+    hipFree(c->nait->d_a);
+    hipFree(c->nait->d_p);
     delete c->nait;
     c->nait = nullptr;
 }
-int nait_density(mpdpft_ctx *c, int, double, double *, double *) { return ctx_fail(c, MPDPFT_EINVAL, "nAiT"); }
-int nait_dispersal_energy(mpdpft_ctx *c, int, double *) { return ctx_fail(c, MPDPFT_EINVAL, "nAiT"); }
+
+void nait_reset_state(mpdpft_ctx *c) {
+    if (!c->nait) return;
+    for (int s = 0; s < c->S; s++)
+        for (int k = 0; k < 3; k++) c->nait->Mpp[s][k] = c->p.Mpp[k];
+}
+
+int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {
+    NaitTables *t = c->nait;
+    if (!t) return ctx_fail(c, MPDPFT_EINVAL, "nAiT tables not loaded");
+    size_t G = c->G;
+    if (c->delfield_valid != s) {  // DelField(V) per GetnAiT call (:3456); V is fixed in a bisection
+        int rc = ctx_delfield(c, c->d_V + s * G, s, c->p.DelFieldMethod);
+        if (rc) return rc;
+        c->delfield_valid = s;
+    }
+    DensityArgs da;
+    da.V = c->d_V + s * G;
+    da.grad2 = c->d_Grad2 + s * G;
+    da.lap = c->d_Lap + s * G;
+    da.mu = mu;
+    da.T = c->TVec[s];
+    da.degeneracy = c->p.degeneracy;
+    da.alpha = c->p.Units == 2 ? 1. / 0.1312342130759153 : 1.;
+    da.thr = t->Mpp[s][2];
+    da.rtpna = c->p.InternalAcc;
+    da.compare = c->p.Abundances[s] / std::pow(c->p.edge, (double)c->D);
+    da.BP = (int)t->Mpp[s][0];
+    da.maxIter = (int)(t->Mpp[s][1] + 0.5);
+    if (da.BP < 1 || 4 * da.BP + 1 > 4096 || da.maxIter > kStack - 4)
+        return ctx_fail(c, MPDPFT_EINVAL, "nAiT Mpp out of range");
+    unsigned blocks = (unsigned)std::min<size_t>(mk::kRedBlocks, (G + 255) / 256);
+    hipError_t e = hipMemsetAsync(c->d_partials, 0, sizeof(double) * 4 * mk::kRedBlocks, c->st);
+    if (e != hipSuccess) return hip_fail(c, e, "memset");
+    stage_begin(c);
+    hipLaunchKernelGGL(k_nait_density, dim3(blocks), dim3(256), 0, c->st, c->g, t->view, da, den_out, c->d_partials);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "k_nait_density");
+    stage_end(c, "nait_density", 0., 0., 0.);  // the kernel alone
+    double r[4];
+    int rc = ctx_reduce(c, 4, 1., r);
+    if (rc) return rc;
+    // roofline numerators: integrand evaluations x nominal FP64 operations each;
+    // HBM traffic V, |grad V|^2, lap V in and n out
+    stage_add(c, "nait_density", r[3], 32. * (double)G, r[3] * kNaitFlopsPerEval);
+    double a = (c->p.edge / 2. - (-c->p.edge / 2.)) / ((double)c->EL - 1.);
+    for (int d = 0; d < c->D; d++) r[0] *= a;
+    *N = r[0];
+    t->last_itermax[s] = r[1];
+    t->last_relacc[s] = r[2];
+    return MPDPFT_OK;
+}
+
+// AdaptnAiT (mpDPFT.cpp:3475-3551).
+void nait_adapt(mpdpft_ctx *c) {
+    NaitTables *t = c->nait;
+    if (!t) return;
+    for (int s = 0; s < c->S; s++) {
+        if (t->last_itermax[s] > 0.01 * t->last_relacc[s]) {
+            t->Mpp[s][0] += 1.;
+            t->Mpp[s][1] += 1.;
+        }
+        t->Mpp[s][0] = std::min(t->Mpp[s][0], 50.);
+    }
+}
+
+// EkinAiT03D (mpDPFT.cpp:5610-5709); c->MaxIteration persists across calls.
+int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E) {
+    NaitTables *t = c->nait;
+    if (!t) return ctx_fail(c, MPDPFT_EINVAL, "nAiT tables not loaded");
+    size_t G = c->G;
+    int rc = ctx_delfield(c, c->d_V + s * G, s, c->p.DelFieldMethod);
+    if (rc) return rc;
+    c->delfield_valid = s;
+    double alpha = c->p.Units == 2 ? 1. / 0.1312342130759153 : 1.;
+    EkinArgs ea;
+    ea.V = c->d_V + s * G;
+    ea.grad2 = c->d_Grad2 + s * G;
+    ea.lap = c->d_Lap + s * G;
+    ea.mu = c->muVec[s];
+    ea.thr = t->Mpp[s][2];
+    ea.c0 = std::pow(alpha, 1. / 3.);
+    ea.c1 = std::pow(alpha, -1.5);
+    ea.c2 = std::pow(alpha, -0.5);
+    ea.internal_acc = c->p.InternalAcc;
+    ea.cmp = c->EnvE[s];
+    double prefactor = c->p.degeneracy / (4. * kPI * kPI);
+    ea.area_pref = std::pow(c->p.edge, (double)c->D) * prefactor;
+    ea.neg_exp = -std::log(c->p.InternalAcc);
+    double e = 0., prev = 0.;
+    int Iteration = 0, MAXIteration = c->MaxIteration + 6;
+    unsigned blocks = (unsigned)std::min<size_t>(4096, (G + 255) / 256);
+    for (;;) {
+        ea.MaxIt = c->MaxIteration;
+        stage_begin(c);
+        hipLaunchKernelGGL(k_ekin_ait, dim3(blocks), dim3(256), 0, c->st, c->g, t->view, ea, c->d_tmp1);
+        hipError_t he = hipGetLastError();
+        if (he != hipSuccess) return hip_fail(c, he, "k_ekin_ait");
+        stage_end(c, "ekin_ait", (double)G, 32. * (double)G, 0.);
+        mk::EnergyParams ep{};
+        ep.op = mk::E_PROD;  // integrand * 1
+        he = mk::fill(G, 1., c->d_tmp2, c->st);
+        if (he != hipSuccess) return hip_fail(c, he, "fill");
+        he = mk::energy_integrand(c->g, ep, c->d_tmp1, c->d_tmp2, nullptr, nullptr, c->d_partials, c->st);
+        if (he != hipSuccess) return hip_fail(c, he, "energy");
+        double v;
+        rc = ctx_reduce(c, 1, 1., &v);
+        if (rc) return rc;
+        double a = (c->p.edge / 2. - (-c->p.edge / 2.)) / ((double)c->EL - 1.);
+        for (int d = 0; d < c->D; d++) v *= a;
+        e = prefactor * v;
+        double as = (e == 0. && prev == 0.) ? 0. : (e - prev) / (e + prev);
+        if (Iteration > 0 && std::fabs(as) < std::max(c->p.RelAcc, kMP)) break;
+        if (c->MaxIteration < MAXIteration) {
+            Iteration++;
+            c->MaxIteration++;
+            prev = e;
+        } else {
+            c->warningCount++;
+            c->control << "GetDispersalEnergy: Warning!!! Target accuracy for EkinAiT03D not reached for species " << s
+                        << "\n";
+            break;
+        }
+    }
+    *E = e;
+    return MPDPFT_OK;
+}

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iomanip>
 
@@ -51,6 +52,66 @@ template <class T> static int dmalloc(mpdpft_ctx *c, T **p, size_t n) {
     if (e != hipSuccess) return ctx_fail(c, MPDPFT_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->allocations.push_back(q);
     *p = (T *)q;
+    return MPDPFT_OK;
+}
+
+void stage_begin(mpdpft_ctx *c) {
+    if (c->timing) hipEventRecord(c->ev0, c->st);
+}
+void stage_end(mpdpft_ctx *c, const char *name, double units, double bytes, double flops) {
+    if (!c->timing) return;
+    hipEventRecord(c->ev1, c->st);
+    hipEventSynchronize(c->ev1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    for (auto &x : c->stats)
+        if (std::strcmp(x.name, name) == 0) {
+            x.launches++;
+            x.total_ms += ms;
+            x.units += units;
+            x.bytes += bytes;
+            x.flops += flops;
+            return;
+        }
+    mpdpft_stage_stats x{};
+    std::snprintf(x.name, sizeof x.name, "%s", name);
+    x.launches = 1;
+    x.total_ms = ms;
+    x.units = units;
+    x.bytes = bytes;
+    x.flops = flops;
+    c->stats.push_back(x);
+}
+
+void stage_add(mpdpft_ctx *c, const char *name, double units, double bytes, double flops) {
+    if (!c->timing) return;
+    for (auto &x : c->stats)
+        if (std::strcmp(x.name, name) == 0) {
+            x.units += units;
+            x.bytes += bytes;
+            x.flops += flops;
+            return;
+        }
+}
+
+extern "C" int mpdpft_enable_timing(mpdpft_ctx *c, int on) {
+    if (!c || !c->st) return MPDPFT_EINVAL;
+    if (on && !c->ev0) {
+        HIPC(hipEventCreate(&c->ev0));
+        HIPC(hipEventCreate(&c->ev1));
+    }
+    c->timing = on ? 1 : 0;
+    c->stats.clear();
+    return MPDPFT_OK;
+}
+
+extern "C" int mpdpft_get_stage_stats(mpdpft_ctx *c, mpdpft_stage_stats *out, int max, int *n) {
+    if (!c || !n) return MPDPFT_EINVAL;
+    int k = 0;
+    for (auto &x : c->stats)
+        if (out && k < max) out[k++] = x;
+        else if (!out) k++;
+    *n = out ? k : (int)c->stats.size();
     return MPDPFT_OK;
 }
 
@@ -120,6 +181,11 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         if (t == 1 && p->n_mpp < p->S) return ctx_fail(c, MPDPFT_EINVAL, "mpp needs S values");
     }
     if (p->Precision != 0) return ctx_fail(c, MPDPFT_EINVAL, "Precision 1 is not built yet");
+    if (p->DensityExpression == 5)
+        for (int s = 0; s < p->S; s++)
+            if (p->TVec[s] == 0.) return ctx_fail(c, MPDPFT_EINVAL, "nAiT needs TVec != 0 (T < 0 picks it automatically)");
+    if (p->DensityExpression == 5 && (p->n_Mpp < 3 || p->Mpp[0] < 1. || p->Mpp[1] > 60.))
+        return ctx_fail(c, MPDPFT_EINVAL, "nAiT needs Mpp = (BoolePairs >= 1, maxIter <= 60, a_r threshold)");
     return MPDPFT_OK;
 }
 
@@ -244,6 +310,14 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
     }
     HIPC(hipMemcpyAsync(c->d_V, c->d_Env, sizeof(double) * SG, hipMemcpyDeviceToDevice, c->st));
     HIPC(hipStreamSynchronize(c->st));
+    for (int s = 0; s < c->S; s++) {  // scalar state used by the stage entry points
+        c->muVec[s] = p->muVec[s];
+        c->thetaVec[s] = p->thetaVec[s];
+        c->TVec[s] = p->TVec[s];
+    }
+    c->MaxIteration = p->MaxIteration;
+    const char *vb = getenv("MPDPFT_VERBOSE");
+    c->verbose = vb && *vb && *vb != '0';
     if (p->DensityExpression == 5) RC(nait_init(c));
     return MPDPFT_OK;
 }
@@ -256,6 +330,8 @@ extern "C" void mpdpft_destroy(mpdpft_ctx *c) {
     for (double *q : c->nin) hipFree(q);
     for (double *q : c->res) hipFree(q);
     if (c->h_scal) hipHostFree(c->h_scal);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
 }
@@ -275,7 +351,9 @@ static int density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) 
         double g = p.degeneracy, pref;
         if (c->D == 2) pref = g / (2. * kPI);
         else pref = (p.Units == 2 ? 0.0475412512 : 1.) * std::sqrt(2.) * g / (3. * kPI * kPI);
+        stage_begin(c);
         HIPC(mk::tf_density(c->g, c->d_V + s * c->G, mu, pref, den_out, c->d_partials, c->st));
+        stage_end(c, "tf_density", (double)c->G, 16. * c->G, 0.);
         return integral_from_partials(c, N);
     }
     return nait_density(c, s, mu, den_out, N);
@@ -439,14 +517,27 @@ static int field_stats(mpdpft_ctx *c) {  // GetFieldStats (:12096), min/max only
 }
 
 // Hartree potential of density n (device [G]) into out (device [G]).
+// Algorithmic HBM bytes of the pruned zero-padded isolated solve (FP64): every
+// pass reads and writes only the lines that can hold nonzeros / reach [0,s]^D.
+static double iso_bytes(const mpdpft_ctx *c) {
+    double s = c->s, M = 2. * s, e = s + 1.;
+    if (c->D == 3)
+        return s * s * (8. * s + 16. * M) + s * M * (16. * s + 16. * M) + M * M * (16. * s + 16. * M + 8. * M) +
+               32. * M * M * M + 32. * M * M * e + e * e * (16. * M + 8. * e);
+    return s * (8. * s + 16. * M) + M * (16. * s + 16. * M + 8. * M) + 32. * M * M + e * (16. * M + 8. * e);
+}
+
 static int hartree(mpdpft_ctx *c, const double *n, double *out) {
     const mpdpft_params &p = c->p;
     double dxD = std::pow(c->dx, (double)c->D);
     if (p.HartreeMode == 1) {
         double M = 2. * c->s, tot = std::pow(M, (double)c->D);
+        stage_begin(c);
         HIPC(mk::hartree_isolated(c->g, n, c->d_pad, c->d_khat, c->d_tw2, p.gammaH * dxD / tot, out, 0, c->st));
+        stage_end(c, "hartree_isolated", 1., iso_bytes(c), 0.);
         return MPDPFT_OK;
     }
+    stage_begin(c);
     // periodic + Gaussian compensation (HartreeRepulsion 3D, :2624-2712)
     HIPC(mk::fft_forward_real(c->g, n, nullptr, c->d_spec, c->d_tw, c->d_scal + 200, c->st));
     mk::MulParams m{};
@@ -460,6 +551,7 @@ static int hartree(mpdpft_ctx *c, const double *n, double *out) {
     double LD = std::pow(1. / p.edge, (double)c->D);
     HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_spec, m, out, nullptr, LD, 2, 4. * kPI * p.gammaH, c->d_tw,
                                c->st));
+    stage_end(c, "hartree_periodic", 2., 2. * 32. * c->D * (double)c->F, 0.);
     return MPDPFT_OK;
 }
 
@@ -479,7 +571,9 @@ static int update_v(mpdpft_ctx *c, const double *den) {
     vp.inc = c->incV;
     vp.units = p.Units;
     vp.hartree_in_V = c->has_hartree;
+    stage_begin(c);
     HIPC(mk::update_v(c->g, vp, den, c->d_Env, c->d_V, c->st));
+    stage_end(c, "update_v", (double)c->SG, 8. * c->SG * (2. + c->S + c->has_hartree), 0.);
     if (c->SCcount > 0) c->incV += c->incV0;
     c->incV = std::min(1., c->incV);
     c->delfield_valid = -1;
@@ -489,8 +583,19 @@ static int update_v(mpdpft_ctx *c, const double *den) {
 // DelField methods 4/5 (:7243-7582) as 5 fused transforms (3D):
 // fwd(V); bwd(i kx e, i ky e) -> Dx,Dy; bwd(i kz e, -k^2 e^2) -> Dz,Lap;
 // fwd(sum D^2); bwd(e) -> POS -> Grad2.
+static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method);
 int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method) {
+    stage_begin(c);
+    int rc = delfield_impl(c, f_dev, s, method);
+    if (rc) return rc;
+    // 5 transforms of 6 c F bytes (c = 16) + taper, square-sum pointwise passes
+    double nfft = method == 5 ? 5. : 3.;
+    stage_end(c, "delfield", nfft, nfft * 96. * (double)c->F + 8. * (2. + 4.) * (double)c->G, 0.);
+    return MPDPFT_OK;
+}
+static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) {
     const mpdpft_params &p = c->p;
+    c->delfield_valid = -1;
     size_t G = c->G;
     const double *src = f_dev;
     if (p.TaperOff && (method == 4 || method == 5)) {
@@ -947,6 +1052,8 @@ static int init_loop(mpdpft_ctx *c) {
     c->warningCount = c->errorCount = c->bisection_steps = 0;
     c->MaxIteration = p.MaxIteration;
     c->CC = 0.;
+    c->delfield_valid = -1;
+    nait_reset_state(c);
     c->pulay_mixer = (int)(p.Mixer[0] + 0.5);
     c->scope = std::max(1, (int)p.Mixer[1]);
     if ((int)c->nin.size() != c->scope + 1) {
@@ -988,6 +1095,17 @@ static int iteration(mpdpft_ctx *c) {
         c->control << " ***** SCcount = " << c->SCcount << "(" << c->ExitSCcount << "/3);  CC = "
                    << std::setprecision(8) << c->CC << std::setprecision(16) << " *****\n";
     if (c->SCcount == p.maxSCcount) c->ExitSCloop = 1;
+    if (c->verbose) {  // the reference prints every iteration to stdout (PRINT, mpDPFT.cpp:12016)
+        std::fprintf(stderr, " ***** SCcount = %d(%d/3);  CC = %.8g;  mu0 = %.12g;  density evaluations = %d *****\n",
+                     c->SCcount, c->ExitSCcount, c->CC, c->muVec[0], c->bisection_steps);
+        std::fflush(stderr);
+    }
+    if (p.DensityExpression == 5) {
+        if (p.Mpp[2] > 0.) nait_adapt(c);  // AdaptnAiT (:4859)
+        // CalculateEnergyQ turns true at the exit iteration (:4870) and the
+        // energy computed there raises MaxIteration for the final one
+        if (c->ExitSCcount == 3 && c->PrintSC == 1) RC(get_energy(c));
+    }
     if (c->ExitSCcount == 3 && c->PrintSC == 1) {
         c->control << "\n ***** exit SCloop;         muVec = " << vec_str(c->muVec, c->S) << " *****\n";
         c->control << " *****               NVec(Target) = " << vec_str(p.Abundances, c->S) << " *****\n";

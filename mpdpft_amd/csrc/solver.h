@@ -54,9 +54,15 @@ struct mpdpft_ctx {
     double AuxEkin[MPDPFT_MAX_SPECIES][4];
     double solve_seconds = 0;
     int has_hartree = 0;
+    int verbose = 0;  // MPDPFT_VERBOSE: one line per SC iteration on stderr
     int delfield_valid = -1;  // species whose Grad2/Lap hold DelField(V_s)
 
     NaitTables *nait = nullptr;
+
+    // per-stage GPU timing (HIP events on st)
+    int timing = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<mpdpft_stage_stats> stats;
 
     std::string err;
     std::ostringstream control;
@@ -67,9 +73,14 @@ int ctx_fail(mpdpft_ctx *c, int code, const std::string &msg);
 int hip_fail(mpdpft_ctx *c, hipError_t e, const char *what);
 int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method);
 int ctx_reduce(mpdpft_ctx *c, int nres, double scale, double *out_host);
+void stage_begin(mpdpft_ctx *c);
+void stage_end(mpdpft_ctx *c, const char *name, double units, double bytes, double flops);
+void stage_add(mpdpft_ctx *c, const char *name, double units, double bytes, double flops);
 
 // nait.hip
 int nait_init(mpdpft_ctx *c);
 void nait_free(mpdpft_ctx *c);
 int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N);
 int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E);
+void nait_adapt(mpdpft_ctx *c);
+void nait_reset_state(mpdpft_ctx *c);

@@ -43,6 +43,12 @@ int oracle_hartree_isolated_direct(const mpdpft_params *p, const double *den, do
  * 1/dx: (1/dx^3) * integral over the cube [-dx/2,dx/2]^3 of 1/|r|. */
 double oracle_cube_inverse_r(void);
 
+/* nAiT special functions: CurlyA(y) = int_y^inf Ai, Li_{3/2-j}(-e^x) (j = 0..3),
+ * Ai(y), as the density and energy use them. */
+double oracle_curlyA(double y);
+double oracle_polylog(int j, double x);
+double oracle_airy(double y);
+
 /* Timed partial workload for the CPU baseline: n_iter SC iterations (no
  * convergence test) starting from the initial density; returns seconds. */
 double oracle_time_iterations(const mpdpft_params *p, int n_iter, int *bisection_steps);

@@ -32,6 +32,8 @@ def lib():
         from mpdpft_amd._abi import Params, Result  # shared ABI structs only
         if not os.path.exists(LIB):
             build()
+        os.environ.setdefault("MPDPFT_TABLES", os.path.join(os.path.dirname(HERE), "mpdpft_amd", "lib",
+                                                            "nait_tables.bin"))
         L = ctypes.CDLL(LIB)
         P = ctypes.POINTER(Params)
         R = ctypes.POINTER(Result)
@@ -53,6 +55,11 @@ def lib():
         L.oracle_cube_inverse_r.restype = ctypes.c_double
         L.oracle_time_iterations.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.oracle_time_iterations.restype = ctypes.c_double
+        for n in ("oracle_curlyA", "oracle_airy"):
+            getattr(L, n).argtypes = [ctypes.c_double]
+            getattr(L, n).restype = ctypes.c_double
+        L.oracle_polylog.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.oracle_polylog.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -159,3 +166,16 @@ def time_iterations(p, n_iter):
     nb = ctypes.c_int()
     t = lib().oracle_time_iterations(ctypes.byref(p), n_iter, ctypes.byref(nb))
     return t, nb.value
+
+
+def curlyA(y):
+    return lib().oracle_curlyA(float(y))
+
+
+def polylog(j, x):
+    """Li_{3/2-j}(-e^x) as the nAiT density evaluates it."""
+    return lib().oracle_polylog(int(j), float(x))
+
+
+def airy(y):
+    return lib().oracle_airy(float(y))

@@ -153,16 +153,21 @@ def check_loop(ro, dO, VO, rg, dG, VG, S, exact_count=True):
 @pytest.mark.parametrize("steps,mode", [(32, 1), (64, 1), (128, 1), (128, 0)])
 def test_c0_converged_solve(oracle, steps, mode):
     print("case c0 steps=%d mode=%d" % (steps, mode), flush=True)
-    p = configs.c0(steps=steps, convergence_mode=mode, tol=1e-8 if mode == 1 else 1e-15)
+    # the overlap rule CC ~ (dn/n)^2 has a rounding floor near 1e-16, where the
+    # exit iteration is decided by noise: test it well above the floor
+    p = configs.c0(steps=steps, convergence_mode=mode, tol=1e-8 if mode == 1 else 1e-13)
     ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
     assert ro.converged and rg.converged
     check_loop(ro, dO, VO, rg, dG, VG, p.S, exact_count=False)
     assert (rg.LinMix, rg.PulMix) == (ro.LinMix, ro.PulMix)
 
 
-@pytest.mark.parametrize("hmode,iters", [(1, 8), (0, 8), (1, 25)])
+@pytest.mark.parametrize("hmode,iters", [(1, 3), (0, 3)])
 def test_3d_tf_hartree_xc_trajectory(oracle, hmode, iters):
-    """C1 physics with the TF density: the same K iterations on both paths."""
+    """C1 physics with the TF density: the same K iterations on both paths.  The
+    TF-Dirac neutral atom does not converge (its density edge is discontinuous)
+    and its mixed iteration amplifies rounding differences ~7x per step, so
+    the trajectory is compared over a few steps only."""
     print("case 3d traj hmode=%d iters=%d" % (hmode, iters), flush=True)
     p = configs.c1(steps=32, density_expression=7, hartree_mode=hmode, maxsc=iters)
     ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
@@ -213,3 +218,28 @@ def test_not_converged_is_reported(oracle):
     r = ctx.solve()
     assert r.converged == 0 and r.SCcount == 6
     ctx.close()
+
+
+# ------------------------------------------------------------------ nAiT (C1 functional)
+def test_nait_density_stage(oracle):
+    print("case nait density stage", flush=True)
+    p = configs.c1(steps=16, density_expression=5, hartree_mode=1)
+    ctx = Context(p)
+    v = oracle.env(p)[0] + 0.05 * np.cos(np.arange((p.steps + 1) ** 3) * 0.01)
+    for mu in (-0.6, -0.2):
+        a, Na = ctx.density_at(0, mu, v)
+        b, Nb = oracle.density(p, 0, mu, v)
+        assert rel(a, b) < RTOL
+        assert abs(Na - Nb) <= RTOL * abs(Nb)
+    ctx.close()
+
+
+@pytest.mark.parametrize("iters", [3])
+def test_nait_trajectory(oracle, iters):
+    """C1 (nAiT + isolated Hartree + Dirac + PW92) at 16^3: K iterations on both paths."""
+    print("case nait trajectory", iters, flush=True)
+    p = configs.c1(steps=16, density_expression=5, hartree_mode=1, maxsc=iters)
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    assert ro.SCcount == iters + 1 and rg.SCcount == iters + 1
+    check_loop(ro, dO, VO, rg, dG, VG, 1)
+    assert rg.bisection_steps == ro.bisection_steps

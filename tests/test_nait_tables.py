@@ -1,0 +1,113 @@
+"""Pins of the nAiT special functions (CurlyA, Li_s(-e^x), Ai).
+
+CurlyA is regenerated from the mathematics (quadrature of Ai) at the knots of
+the reference's TabFunc_CurlyA.dat and checked against that table; the
+polylogarithms (whose reference tables are missing from the reference tree)
+are checked against independent series, Taylor and Sommerfeld evaluations."""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import special as sp
+
+from mpdpft_amd import tables as T
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "curlya_ref_sample.txt")
+REF_TABLE = "/root/reference/TabFunc_CurlyA.dat"
+
+
+@pytest.fixture(scope="module")
+def tab():
+    T.ensure(verbose=False)
+    return T.read()
+
+
+def test_curlya_knots_match_reference_sample(tab):
+    rows = [l.split() for l in open(GOLDEN) if not l.startswith("#")]
+    for idx, x, y in rows:
+        i, x, y = int(idx), float(x), float(y)
+        assert abs((T.C_X0 + T.C_H * i) - x) < 1e-12
+        assert abs(tab["c"][i] - y) <= 1e-13 * max(abs(y), 1e-300)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TABLE), reason="reference tree absent")
+def test_curlya_knots_match_full_reference_table(tab):
+    v = np.array([float(t) for t in open(REF_TABLE).read().split()])
+    y = v[1::2]
+    assert len(y) == len(tab["c"])
+    assert np.max(np.abs(tab["c"] - y) / np.abs(y)) < 1e-13
+
+
+def test_curlya_identities(oracle, tab):
+    # int_0^inf Ai = 1/3; CurlyA' = -Ai; asymptotic branches meet the spline
+    assert abs(oracle.curlyA(0.0) - 1.0 / 3.0) < 1e-15
+    for y in (-120.3, -7.77, -0.123, 0.456, 3.21, 11.1):
+        h = 1e-4
+        d = (oracle.curlyA(y + h) - oracle.curlyA(y - h)) / (2 * h)
+        assert abs(d + sp.airy(y)[0]) < 2e-6
+    assert abs(oracle.curlyA(-149.99) - oracle.curlyA(-149.99 + 1e-9)) < 1e-6
+
+
+def test_curlya_spline_error_is_the_reference_interpolation_error(oracle):
+    """Between knots the cubic spline (as the reference evaluates CurlyA) is
+    within 1e-7 of the exact integral."""
+    gx, gw = np.polynomial.legendre.leggauss(30)
+    for y in (-140.005, -50.005, -3.005, 0.005, 5.005):
+        # exact: CurlyA(knot) - int_knot^y Ai
+        k = math.floor((y - T.C_X0) / T.C_H)
+        xk = T.C_X0 + T.C_H * k
+        a, b = xk, y
+        ex = oracle.curlyA(xk) - 0.5 * (b - a) * np.sum(gw * sp.airy(0.5 * (b - a) * gx + 0.5 * (a + b))[0])
+        assert abs(oracle.curlyA(y) - ex) < 1e-7
+
+
+@pytest.mark.parametrize("j", [0, 1, 2, 3])
+def test_polylog_against_independent_forms(oracle, j):
+    s = T.P_ORDERS[j]
+    xs_ser = np.array([-39.9, -20.0, -7.3, -2.2, -1.05])
+    for x, ref in zip(xs_ser, T.polylog_series(s, xs_ser)):
+        assert abs(oracle.polylog(j, x) - ref) <= 1e-13 * abs(ref)
+    xs_tay = np.array([-0.97, -0.4, 0.0, 0.33, 0.9])
+    for x, ref in zip(xs_tay, T.polylog_taylor(s, xs_tay)):
+        assert abs(oracle.polylog(j, x) - ref) <= 1e-13 * abs(ref) + 1e-16
+    xs_asy = np.array([45.0, 59.5, 61.0, 150.0, 650.0])
+    for x, ref in zip(xs_asy, T.polylog_asymptotic(s, xs_asy, K=16)):
+        assert abs(oracle.polylog(j, x) - ref) <= 1e-12 * abs(ref)
+    # the reference's explicit branches
+    assert oracle.polylog(j, -700.0) == 0.0
+    assert oracle.polylog(j, 800.0) == pytest.approx(-800.0 ** s / math.gamma(s + 1), rel=1e-15)
+
+
+def test_polylog_chain_is_a_derivative_chain(oracle):
+    for j in range(3):
+        for x in (-12.0, -1.1, 0.3, 4.7, 33.3):
+            h = 1e-5
+            d = (oracle.polylog(j, x + h) - oracle.polylog(j, x - h)) / (2 * h)
+            assert abs(d - oracle.polylog(j + 1, x)) < 1e-8 * max(1, abs(d))
+
+
+def test_polylog_tf_limit(oracle):
+    """T -> 0: -Li_{3/2}(-e^x) -> x^{3/2}/Gamma(5/2) (the TF density)."""
+    x = 2000.0
+    assert abs(-oracle.polylog(0, 600.0) - 600.0 ** 1.5 / math.gamma(2.5)) / (600.0 ** 1.5) < 1e-5
+    assert oracle.polylog(0, x) == pytest.approx(-x ** 1.5 / math.gamma(2.5), rel=1e-15)
+
+
+def test_airy_interpolation(oracle):
+    ys = np.array([-299.9973, -151.2345, -20.0021, -3.3333, 0.0, 0.0007, 2.5, 17.77, 99.99])
+    ai = sp.airy(ys)[0]
+    for y, a in zip(ys, ai):
+        assert abs(oracle.airy(y) - a) < 1e-11 * max(1.0, abs(y)) ** 0.25 + 1e-15 * abs(a) * 0
+    assert oracle.airy(100.5) == 0.0
+    y = -400.0
+    asym = math.cos(2 * 400 ** 1.5 / 3 - math.pi / 4) / (math.sqrt(math.pi) * 400 ** 0.25)
+    assert oracle.airy(y) == pytest.approx(asym, rel=1e-14)
+
+
+def test_airy_integrates_to_curlya(oracle):
+    gx, gw = np.polynomial.legendre.leggauss(40)
+    for a in (-80.0, -10.0, -1.0, 0.5):
+        b = a + 0.7
+        val = 0.5 * (b - a) * sum(w * oracle.airy(0.5 * (b - a) * x + 0.5 * (a + b)) for x, w in zip(gx, gw))
+        assert abs(val - (oracle.curlyA(a) - oracle.curlyA(b))) < 1e-9
