@@ -68,8 +68,8 @@ def main():
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes/launch of the nAiT kernel")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the batched and sharded "
-              "layouts are described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported (the batched and sharded "
+              "layouts are not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     os.environ.setdefault("MPDPFT_VERBOSE", "1")  # one line per SC iteration on stderr

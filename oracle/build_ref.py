@@ -4,7 +4,7 @@ Only one piece of the reference's path compiles here: the vendored Eigen 3.4.0
 headers (Eigen_Headers/), used for the Pulay Gram inverse.  The rest of the
 reference (mpDPFT.cpp) needs FFTW, GSL, libxc, Boost and the missing
 optimization.cpp, none of which exist in this image, so it is not built.
-Outputs go to oracle/_ref/ (git-ignored; travels to the GPU box).
+Outputs go to oracle/_ref/ (git-ignored, kept beside the built library).
 """
 import os
 import subprocess
