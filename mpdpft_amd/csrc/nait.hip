@@ -41,8 +41,28 @@ struct TabView {
     double cx0, ch, cinv, px0, ph, pinv, pxmax, ax0, ah, ainv;
     double ginv[4];      // 1/Gamma(s+1), s = 3/2, 1/2, -1/2, -3/2 (as the reference spells them)
     double pw2[4], pw3[4];  // 2^-s, 3^-s
-    double asym[4][12];     // Sommerfeld coefficients 2 eta(2k) (s)_{2k}
+    const double *asym;     // [4][12] Sommerfeld coefficients (device copy of kAsym)
 };
+
+// Sommerfeld coefficients 2 eta(2k) (s)_{2k}, k = 1..12, for s = 3/2 - j
+constexpr double kTwoEta[12] = {1.6449340668482264, 1.8940656589944918, 1.9711021825948705, 1.992466003705296,
+                                1.9980790151965429, 1.9995153702877164, 1.9998783406919596, 1.9999695284298122,
+                                1.9999923757392204, 1.999998093223163,  1.9999995232264616, 1.9999998807977848};
+struct AsymTable {
+    double v[4][12];
+};
+constexpr AsymTable make_asym() {
+    AsymTable r{};
+    for (int j = 0; j < 4; j++) {
+        double sj = 1.5 - (double)j, ff = 1.;
+        for (int k = 1; k <= 12; k++) {
+            ff *= (sj - 2 * k + 2) * (sj - 2 * k + 1);
+            r.v[j][k - 1] = kTwoEta[k - 1] * ff;
+        }
+    }
+    return r;
+}
+constexpr AsymTable kAsym = make_asym();
 
 // x^s for the four half-integer orders
 __device__ __forceinline__ double xpow(int j, double x) {
@@ -82,7 +102,7 @@ __device__ double polylog_j(const TabView &t, int j, double x) {
     }
     if (x > t.pxmax) {
         double x2 = 1. / (x * x), acc = 0.;
-        for (int k = 11; k >= 0; k--) acc = x2 * (t.asym[j][k] + acc);
+        for (int k = 11; k >= 0; k--) acc = x2 * (t.asym[12 * j + k] + acc);
         return -xpow(j, x) * t.ginv[j] * (1. + acc);
     }
     int k = (int)floor((x - t.px0) * t.pinv);
@@ -339,6 +359,210 @@ __global__ __launch_bounds__(256) void k_nait_density(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Panel-reuse variant of the same quadrature.  A node [a, b] is sampled at
+// k = 0..8BP (left half a + k Delta, right half (a + k' Delta) + half with
+// k = 4BP + k'); the shared midpoint is evaluated once.  When a node is
+// bisected, each child's even samples are its parent's half, so a child
+// evaluates only its 4BP odd samples.  The accepted panels, the acceptance
+// tests and their order are those of AdaptiveBoole_nAiT (:3553-3578); a reused
+// abscissa can differ from the reference's recomputed one in the last bit.
+// One wavefront per grid point; LDS holds the node's samples and, per tree
+// level, the sample half of the pending right child.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double uniform_d(double v) {
+    int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes: quad, half-row and row butterflies (DPP), then the
+// four row sums combined in a fixed order; the result is wave-uniform.
+__device__ __forceinline__ double wave_total(double v) {
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    v += dpp_mov<0x140>(v);  // row_mirror
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+struct PointScalars {
+    const double *V, *grad2, *lap;
+    double mu, invT, tau, beta0, beta1, sqrt_at, alpha, thr, rtpna, compare, log10cmp, maxexp;
+    int BP, maxIter, depth;
+    long long G;
+};
+
+constexpr int kReuseMaxPer = 4;  // 4BP + 1 <= 256 samples per half
+
+__global__ __launch_bounds__(64) void k_nait_reuse(TabView t, PointScalars ps, double *__restrict__ n,
+                                                  int *__restrict__ counts) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const long long i = blockIdx.x;
+    const int BP = ps.BP, h4 = 4 * BP, full = 8 * BP;
+    double *cur = lds;                            // [8BP + 1] samples of the current node
+    double *stk = lds + full + 1;                 // [depth][4BP + 1] pending right halves
+    double *stn = stk + (size_t)ps.depth * (h4 + 1);  // [depth][4]: a, b, prev, iter
+
+    const double U = ps.V[i] - ps.mu;
+    const double ar = 0.5 * pow(ps.alpha * 1. * ps.grad2[i], 1. / 3.);
+    const double beta2 = -ps.lap[i] / (12. * ps.sqrt_at);
+    double res = 0.;
+    int evals = 0, nim = 0, nra = 0;
+    if (ar < fabs(ps.thr)) {
+        double arg = -ps.tau * U;
+        res = ps.beta0 * (ps.beta1 * polylog_j(t, 0, arg) + beta2 * polylog_j(t, 2, arg));
+    } else {
+        PointArgs q;
+        q.invT = ps.invT;
+        q.U = U;
+        q.ar = ar;
+        q.pref = ps.beta0 * ar * ps.tau * ps.beta1;
+        q.pref2 = beta2 / ps.beta1;
+        // lower limit: the first x = -2^k (k = 1, 2, ...) where the integrand
+        // is below 1e-12, stopping once x < -1e8 (k = 27)
+        bool small = true;
+        if (lane < 27) {
+            double x = -ldexp(1., lane + 1), arg = -ps.tau * (U - x * ar);
+            double l1 = polylog_j(t, 1, arg), l3 = polylog_j(t, 3, arg);
+            double test = ps.beta0 * curly_a(t, x) * ar * ps.tau * (ps.beta1 * l1 + beta2 * l3);
+            small = !(fabs(test) > kMP);
+        }
+        unsigned long long bal = __ballot(small ? 1 : 0);
+        int kx = __ffsll((long long)bal) - 1;
+        if (kx < 0 || kx > 26) kx = 26;
+        const double xlow = -ldexp(1., kx + 1);
+        evals = 27;
+        for (int root = 0; root < 2; root++) {
+            double a = root ? xlow : 2. * xlow, b = root ? 100. : xlow, prev = 0.;
+            int iter = 0, sp = 0, mode = 0;  // 0 all samples, 1 left child in place, 2 popped right child
+            for (;;) {
+                const double half = 0.5 * (b - a), Delta = 0.25 * half / ((double)BP);
+                if (mode != 0) {  // even samples from the parent's half
+                    const double *src = mode == 1 ? cur : stk + (size_t)sp * (h4 + 1);
+                    double v[kReuseMaxPer];
+#pragma unroll
+                    for (int r = 0; r < kReuseMaxPer; r++) {
+                        int j = lane + 64 * r;
+                        v[r] = j <= h4 ? src[j] : 0.;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int r = 0; r < kReuseMaxPer; r++) {
+                        int j = lane + 64 * r;
+                        if (j <= h4) cur[2 * j] = v[r];
+                    }
+                }
+                const int nslots = mode == 0 ? full + 1 : h4;
+                for (int sl = lane; sl < nslots; sl += 64) {
+                    int k = mode == 0 ? sl : 2 * sl + 1;
+                    double x = k <= h4 ? a + (double)k * Delta : (a + (double)(k - h4) * Delta) + half;
+                    cur[k] = integrand(t, q, x);
+                }
+                evals += nslots;
+                __syncthreads();
+                double sa = 0., sb = 0.;
+                for (int k = lane; k <= full; k += 64) {
+                    double f = cur[k];
+                    if (k <= h4) sa += boole_w(k, h4 + 1) * f;
+                    if (k >= h4) sb += boole_w(k - h4, h4 + 1) * f;
+                }
+                sa = wave_total(sa);
+                sb = wave_total(sb);
+                double ha = (a + half - a) / (((double)BP) * 4.), hb = (b - (a + half)) / (((double)BP) * 4.);
+                double Inta = sa * (2. * ha / 45.), Intb = sb * (2. * hb / 45.);
+                double Int = Inta + Intb;
+                bool accept = true;
+                if (Int != Int) {
+                } else if (iter > 1 && fabs(Int) > 0. && fabs((Int - prev) / Int) < ps.rtpna) {
+                    nra++;
+                } else if (iter > ps.maxIter) {
+                    nim++;
+                } else {
+                    double sm = ldexp(Int, iter);  // 2^iter Int
+                    bool negl;
+                    if (sm == 0.) negl = true;
+                    else if (ps.compare == 0. || ps.compare < sm) negl = false;
+                    else negl = ps.log10cmp - log10(fabs(sm)) > ps.maxexp;
+                    accept = negl || Delta < kMP;
+                }
+                if (accept) {
+                    res += Int;
+                    if (sp == 0) break;
+                    sp--;
+                    a = uniform_d(stn[4 * sp]);
+                    b = uniform_d(stn[4 * sp + 1]);
+                    prev = uniform_d(stn[4 * sp + 2]);
+                    iter = (int)uniform_d(stn[4 * sp + 3]);
+                    mode = 2;
+                } else {
+                    if (sp >= ps.depth) {  // deeper than maxIter allows; cannot happen
+                        res = __builtin_nan("");
+                        break;
+                    }
+                    double *dst = stk + (size_t)sp * (h4 + 1);
+                    for (int j = lane; j <= h4; j += 64) dst[j] = cur[h4 + j];
+                    if (lane == 0) {
+                        stn[4 * sp] = a + half;
+                        stn[4 * sp + 1] = b;
+                        stn[4 * sp + 2] = Intb;
+                        stn[4 * sp + 3] = (double)(iter + 1);
+                    }
+                    sp++;
+                    b = a + half;
+                    prev = Inta;
+                    iter++;
+                    mode = 1;
+                    __syncthreads();
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        n[i] = res != res ? 0. : res;
+        counts[i] = evals;
+        counts[ps.G + i] = nim;
+        counts[2 * ps.G + i] = nra;
+    }
+}
+
+// Trapezoid-weighted density sum and the three per-point counters, as
+// per-block partials (fixed order).
+__global__ __launch_bounds__(256) void k_nait_sums(mk::Grid g, const double *__restrict__ n,
+                                                  const int *__restrict__ counts, double *partials) {
+    __shared__ double sh[4];
+    double acc = 0., ev = 0., im = 0., ra = 0.;
+    const long long G = g.G;
+    for (long long i = blockIdx.x * 256 + threadIdx.x; i < G; i += (long long)gridDim.x * 256) {
+        acc += trap_w(g, (int)i) * n[i];
+        ev += (double)counts[i];
+        im += (double)counts[G + i];
+        ra += (double)counts[2 * G + i];
+    }
+    double r0 = bsum(acc, sh), r1 = bsum(im, sh), r2 = bsum(ra, sh), r3 = bsum(ev, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = r0;
+        partials[mk::kRedBlocks + blockIdx.x] = r1;
+        partials[2 * mk::kRedBlocks + blockIdx.x] = r2;
+        partials[3 * mk::kRedBlocks + blockIdx.x] = r3;
+    }
+}
+
 struct EkinArgs {
     const double *V, *grad2, *lap;
     double mu, thr, c0, c1, c2, internal_acc, cmp, area_pref, neg_exp;
@@ -396,6 +620,8 @@ struct NaitTables {
     double4 *d_c = nullptr;
     double2 *d_a = nullptr;
     double *d_p = nullptr;
+    double *d_asym = nullptr;   // [4][12]
+    int *d_counts = nullptr;    // [3][G] per-point evaluations, iterMax and relAcc returns
     double Mpp[MPDPFT_MAX_SPECIES][3];
     double last_itermax[MPDPFT_MAX_SPECIES], last_relacc[MPDPFT_MAX_SPECIES];
 };
@@ -500,22 +726,22 @@ int nait_init(mpdpft_ctx *c) {
     v.ph = r[5];
     v.pinv = 1. / r[5];
     v.pxmax = r[4] + r[5] * (double)(pn - 1);
-    // Sommerfeld coefficients 2 eta(2k), k = 1..12
-    static const double two_eta[12] = {1.6449340668482264, 1.8940656589944918, 1.9711021825948705, 1.992466003705296, 1.9980790151965429, 1.9995153702877164, 1.9998783406919596, 1.9999695284298122, 1.9999923757392204, 1.999998093223163, 1.9999995232264616, 1.9999998807977848};
     const double sq = std::sqrt(kPI);
     const double gam[4] = {0.75 * sq, 0.5 * sq, sq, -2. * sq};  // Gamma(s+1) as in mpDPFT.cpp:112
     for (int j = 0; j < 4; j++) {
-        double sj = 1.5 - (double)j, ff = 1.;
+        double sj = 1.5 - (double)j;
         v.ginv[j] = 1. / gam[j];
         v.pw2[j] = std::pow(2., -sj);
         v.pw3[j] = std::pow(3., -sj);
-        for (int k = 1; k <= 12; k++) {
-            ff *= (sj - 2 * k + 2) * (sj - 2 * k + 1);
-            v.asym[j][k - 1] = two_eta[k - 1] * ff;
-        }
     }
     for (int s = 0; s < c->S; s++)
         for (int k = 0; k < 3; k++) t->Mpp[s][k] = c->p.Mpp[k];
+    if (hipMalloc((void **)&t->d_asym, sizeof(kAsym.v)) != hipSuccess ||
+        hipMalloc((void **)&t->d_counts, sizeof(int) * 3 * c->G) != hipSuccess)
+        return ctx_fail(c, MPDPFT_ENOMEM, "nAiT coefficients/counters");
+    if (hipMemcpy(t->d_asym, &kAsym.v[0][0], sizeof(kAsym.v), hipMemcpyHostToDevice) != hipSuccess)
+        return ctx_fail(c, MPDPFT_EHIP, "nAiT coefficient upload");
+    v.asym = t->d_asym;
     return MPDPFT_OK;
 }
 
@@ -524,6 +750,8 @@ void nait_free(mpdpft_ctx *c) {
     hipFree(c->nait->d_c);
     hipFree(c->nait->d_a);
     hipFree(c->nait->d_p);
+    hipFree(c->nait->d_asym);
+    hipFree(c->nait->d_counts);
     delete c->nait;
     c->nait = nullptr;
 }
@@ -561,11 +789,49 @@ int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {
     unsigned blocks = (unsigned)std::min<size_t>(mk::kRedBlocks, (G + 255) / 256);
     hipError_t e = hipMemsetAsync(c->d_partials, 0, sizeof(double) * 4 * mk::kRedBlocks, c->st);
     if (e != hipSuccess) return hip_fail(c, e, "memset");
+    // panel-reuse kernel when its LDS (node samples + one pending half per
+    // tree level) fits; otherwise the plain per-node kernel
+    PointScalars ps;
+    ps.V = da.V;
+    ps.grad2 = da.grad2;
+    ps.lap = da.lap;
+    ps.mu = mu;
+    ps.invT = 1. / da.T;
+    ps.tau = 1. / da.T;
+    ps.beta0 = -da.degeneracy / std::pow(2. * kPI, 1.5);
+    ps.beta1 = std::pow(da.alpha * ps.tau, -1.5);
+    ps.sqrt_at = std::sqrt(da.alpha * ps.tau);
+    ps.alpha = da.alpha;
+    ps.thr = da.thr;
+    ps.rtpna = da.rtpna;
+    ps.compare = da.compare;
+    ps.log10cmp = da.compare != 0. ? std::log10(std::fabs(da.compare)) : 0.;
+    ps.maxexp = -std::log10(da.rtpna);
+    ps.BP = da.BP;
+    ps.maxIter = da.maxIter;
+    ps.depth = da.maxIter + 2;
+    ps.G = (long long)G;
+    size_t lds = sizeof(double) * ((size_t)(8 * da.BP + 1) + (size_t)ps.depth * (4 * da.BP + 1 + 4));
+    const bool reuse = !c->nait_plain && 4 * da.BP + 1 <= 64 * kReuseMaxPer && lds <= 65536;
     stage_begin(c);
-    hipLaunchKernelGGL(k_nait_density, dim3(blocks), dim3(256), 0, c->st, c->g, t->view, da, den_out, c->d_partials);
-    e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(c, e, "k_nait_density");
+    if (reuse) {
+        hipLaunchKernelGGL(k_nait_reuse, dim3((unsigned)G), dim3(64), lds, c->st, t->view, ps, den_out,
+                           t->d_counts);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(c, e, "k_nait_reuse");
+    } else {
+        hipLaunchKernelGGL(k_nait_density, dim3(blocks), dim3(256), 0, c->st, c->g, t->view, da, den_out,
+                           c->d_partials);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(c, e, "k_nait_density");
+    }
     stage_end(c, "nait_density", 0., 0., 0.);  // the kernel alone
+    if (reuse) {
+        hipLaunchKernelGGL(k_nait_sums, dim3(mk::kRedBlocks), dim3(256), 0, c->st, c->g, den_out, t->d_counts,
+                           c->d_partials);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(c, e, "k_nait_sums");
+    }
     double r[4];
     int rc = ctx_reduce(c, 4, 1., r);
     if (rc) return rc;
@@ -591,6 +857,11 @@ void nait_adapt(mpdpft_ctx *c) {
         }
         t->Mpp[s][0] = std::min(t->Mpp[s][0], 50.);
     }
+}
+
+void nait_state(const mpdpft_ctx *c, int s, double *BP, double *maxIter) {
+    *BP = c->nait ? c->nait->Mpp[s][0] : 0.;
+    *maxIter = c->nait ? c->nait->Mpp[s][1] : 0.;
 }
 
 // EkinAiT03D (mpDPFT.cpp:5610-5709); c->MaxIteration persists across calls.

@@ -318,6 +318,8 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
     c->MaxIteration = p->MaxIteration;
     const char *vb = getenv("MPDPFT_VERBOSE");
     c->verbose = vb && *vb && *vb != '0';
+    const char *nk = std::getenv("MPDPFT_NAIT_KERNEL");
+    c->nait_plain = nk && std::strcmp(nk, "plain") == 0;
     if (p->DensityExpression == 5) RC(nait_init(c));
     return MPDPFT_OK;
 }
@@ -1096,8 +1098,12 @@ static int iteration(mpdpft_ctx *c) {
                    << std::setprecision(8) << c->CC << std::setprecision(16) << " *****\n";
     if (c->SCcount == p.maxSCcount) c->ExitSCloop = 1;
     if (c->verbose) {  // the reference prints every iteration to stdout (PRINT, mpDPFT.cpp:12016)
-        std::fprintf(stderr, " ***** SCcount = %d(%d/3);  CC = %.8g;  mu0 = %.12g;  density evaluations = %d *****\n",
-                     c->SCcount, c->ExitSCcount, c->CC, c->muVec[0], c->bisection_steps);
+        double bp = 0., mi = 0.;
+        if (p.DensityExpression == 5) nait_state(c, 0, &bp, &mi);
+        std::fprintf(stderr,
+                     " ***** SCcount = %d(%d/3);  CC = %.8g;  mu0 = %.12g;  density evaluations = %d;  nAiT BP %g "
+                     "maxIter %g *****\n",
+                     c->SCcount, c->ExitSCcount, c->CC, c->muVec[0], c->bisection_steps, bp, mi);
         std::fflush(stderr);
     }
     if (p.DensityExpression == 5) {

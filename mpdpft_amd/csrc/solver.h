@@ -55,6 +55,7 @@ struct mpdpft_ctx {
     double solve_seconds = 0;
     int has_hartree = 0;
     int verbose = 0;  // MPDPFT_VERBOSE: one line per SC iteration on stderr
+    int nait_plain = 0;  // MPDPFT_NAIT_KERNEL=plain: per-node nAiT kernel without panel reuse
     int delfield_valid = -1;  // species whose Grad2/Lap hold DelField(V_s)
 
     NaitTables *nait = nullptr;
@@ -83,4 +84,5 @@ void nait_free(mpdpft_ctx *c);
 int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N);
 int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E);
 void nait_adapt(mpdpft_ctx *c);
+void nait_state(const mpdpft_ctx *c, int s, double *BP, double *maxIter);
 void nait_reset_state(mpdpft_ctx *c);
