@@ -42,6 +42,10 @@ struct TabView {
     double ginv[4];      // 1/Gamma(s+1), s = 3/2, 1/2, -1/2, -3/2 (as the reference spells them)
     double pw2[4], pw3[4];  // 2^-s, 3^-s
     const double *asym;     // [4][12] Sommerfeld coefficients (device copy of kAsym)
+    const double2 *pp;      // [pn - 1 + nb][6]: per interval, the monomial coefficients in u of the
+                            // quintic Hermite interpolants of Li_{1/2} (0..5) and Li_{-3/2} (6..11);
+                            // intervals pn - 1 ... cover (pxmax, pxmax + nb] with unit step
+    int nb;
 };
 
 // Sommerfeld coefficients 2 eta(2k) (s)_{2k}, k = 1..12, for s = 3/2 - j
@@ -70,11 +74,16 @@ __device__ __forceinline__ double xpow(int j, double x) {
     return j == 0 ? x * r : (j == 1 ? r : (j == 2 ? 1. / r : 1. / (x * r)));
 }
 
+// CurlyA's oscillatory asymptotic form (y < -149.99), kept out of line: only
+// panels below the knot range reach it
+__device__ __noinline__ double curly_a_asym(double y) {
+    return 1. - cos(0.25 * 3.141592653589793 + 0.6666666666666666666 * (-y) * sqrt(-y)) /
+                    sqrt(3.141592653589793 * (-y) * sqrt(-y));
+}
+
 __device__ __forceinline__ double curly_a(const TabView &t, double y) {
     if (y < -1.0e+50) return 1.;
-    if (y < -149.99)
-        return 1. - cos(0.25 * 3.141592653589793 + 0.6666666666666666666 * (-y) * sqrt(-y)) /
-                        sqrt(3.141592653589793 * (-y) * sqrt(-y));
+    if (y < -149.99) return curly_a_asym(y);
     if (y >= 99.99) return 0.;
     int j = (int)floor((y - t.cx0) * t.cinv);
     j = j < 0 ? 0 : (j > t.cn - 2 ? t.cn - 2 : j);
@@ -119,9 +128,27 @@ __device__ __forceinline__ void polylog_pair(const TabView &t, double x, double 
         l3 = 0.;
         return;
     }
-    if (x < t.px0 || x > t.pxmax) {
-        l1 = polylog_j(t, 1, x);
-        l3 = polylog_j(t, 3, x);
+    if (x < t.px0) {  // the series branch of polylog_j for both orders, one exp
+        double e = exp(x), e2 = e * e, e3 = e2 * e;
+        l1 = -e + e2 * t.pw2[1] - e3 * t.pw3[1];
+        l3 = -e + e2 * t.pw2[3] - e3 * t.pw3[3];
+        return;
+    }
+    if (x > t.pxmax) {  // the asymptotic branches of polylog_j for both orders
+        double r = sqrt(x), p1 = r, p3 = 1. / (x * r);
+        if (x > 699.) {
+            l1 = -p1 * t.ginv[1];
+            l3 = -p3 * t.ginv[3];
+            return;
+        }
+        double x2 = 1. / (x * x), c1 = 0., c3 = 0.;
+#pragma unroll 1
+        for (int k = 11; k >= 0; k--) {
+            c1 = x2 * (t.asym[12 + k] + c1);
+            c3 = x2 * (t.asym[36 + k] + c3);
+        }
+        l1 = -p1 * t.ginv[1] * (1. + c1);
+        l3 = -p3 * t.ginv[3] * (1. + c3);
         return;
     }
     int k = (int)floor((x - t.px0) * t.pinv);
@@ -136,6 +163,72 @@ __device__ __forceinline__ void polylog_pair(const TabView &t, double x, double 
     double H4 = -4. * u3 + 7. * u4 - 3. * u5, H5 = 0.5 * u3 - u4 + 0.5 * u5;
     l1 = a1 * H0 + h * a2 * H1 + h * h * a3 * H2 + b1 * H3 + h * b2 * H4 + h * h * b3 * H5;
     l3 = a3 * H0 + h * a4 * H1 + h * h * a5 * H2 + b3 * H3 + h * b4 * H4 + h * h * b5 * H5;
+}
+
+// Table geometry of nait_tables.bin as written by mpdpft_amd/tables.py; the
+// host checks the loaded file against these before using the panel-reuse
+// kernel, whose lookups then need no runtime parameters.
+constexpr double kCX0 = -150., kCH = 0.01, kCINV = 100.;
+constexpr int kCN = 25001;
+constexpr double kPX0 = -40., kPH = 0.01, kPINV = 100., kPXMAX = 200.;
+constexpr int kPN = 24001, kNB = 500;
+
+// What the panel-reuse kernel reads in its inner loop.
+struct ReuseTab {
+    const double2 *pp;  // [kPN - 1 + kNB][6] monomial coefficients (see TabView::pp)
+    const double4 *cc;  // CurlyA cubic per knot interval
+    double pw2_1, pw3_1, pw2_3, pw3_3, ginv1, ginv3;
+};
+
+__device__ __forceinline__ double curly_a_fast(const ReuseTab &t, double y) {
+    if (y < -1.0e+50) return 1.;
+    if (y < -149.99) return curly_a_asym(y);
+    if (y >= 99.99) return 0.;
+    int j = (int)floor((y - kCX0) * kCINV);
+    j = j < 0 ? 0 : (j > kCN - 2 ? kCN - 2 : j);
+    double u = y - (kCX0 + kCH * (double)j);
+    double4 c = t.cc[j];
+    return c.x + u * (c.y + u * (c.z + u * c.w));
+}
+
+// The integrand's Li pair for the panel-reuse kernel.  Below px0 and above
+// 699 the branches of polylog_j; on [px0, pxmax] the quintic Hermite
+// interpolants of the chain table in monomial form; on (pxmax, 699] a second
+// quintic Hermite table with unit step built from the Sommerfeld expansion
+// that polylog_j sums there (agreement ~1e-17 relative).
+__device__ __forceinline__ void polylog_pair_fast(const ReuseTab &t, double x, double &l1, double &l3) {
+    if (x < kPX0) {
+        if (x < -699.) {
+            l1 = l3 = 0.;
+        } else {
+            double e = exp(x), e2 = e * e, e3 = e2 * e;
+            l1 = -e + e2 * t.pw2_1 - e3 * t.pw3_1;
+            l3 = -e + e2 * t.pw2_3 - e3 * t.pw3_3;
+        }
+        return;
+    }
+    if (x > 699.) {
+        double r = sqrt(x);
+        l1 = -r * t.ginv1;
+        l3 = -(1. / (x * r)) * t.ginv3;
+        return;
+    }
+    int k;
+    double u;
+    if (x <= kPXMAX) {
+        k = (int)floor((x - kPX0) * kPINV);
+        if (k > kPN - 2) k = kPN - 2;
+        u = (x - (kPX0 + kPH * (double)k)) * kPINV;
+    } else {
+        int m = (int)floor(x - kPXMAX);
+        if (m > kNB - 1) m = kNB - 1;
+        u = x - (kPXMAX + (double)m);
+        k = kPN - 1 + m;
+    }
+    const double2 *c = t.pp + 6 * k;
+    double2 c01 = c[0], c23 = c[1], c45 = c[2], d01 = c[3], d23 = c[4], d45 = c[5];
+    l1 = ((((c45.y * u + c45.x) * u + c23.y) * u + c23.x) * u + c01.y) * u + c01.x;
+    l3 = ((((d45.y * u + d45.x) * u + d23.y) * u + d23.x) * u + d01.y) * u + d01.x;
 }
 
 __device__ double airy_ai(const TabView &t, double y) {
@@ -171,6 +264,13 @@ __device__ __forceinline__ double integrand(const TabView &t, const PointArgs &q
     double l1, l3;
     polylog_pair(t, arg, l1, l3);
     return q.pref * curly_a(t, x) * (l1 + q.pref2 * l3);
+}
+
+__device__ __forceinline__ double integrand_fast(const ReuseTab &t, const PointArgs &q, double x) {
+    double arg = -(q.U - x * q.ar) * q.invT;
+    double l1, l3;
+    polylog_pair_fast(t, arg, l1, l3);
+    return q.pref * curly_a_fast(t, x) * (l1 + q.pref2 * l3);
 }
 
 // Boole weight of sample yi of n = 4 BP + 1 (BooleRule1D, mpDPFT.cpp:8050-8058)
@@ -402,18 +502,37 @@ __device__ __forceinline__ double wave_total(double v) {
 struct PointScalars {
     const double *V, *grad2, *lap;
     double mu, invT, tau, beta0, beta1, sqrt_at, alpha, thr, rtpna, compare, log10cmp, maxexp;
+    double negl_lo, negl_hi;  // |2^iter Int| below / above: NegligibleMagnitudeQ decided without log10
     int BP, maxIter, depth;
     long long G;
 };
 
 constexpr int kReuseMaxPer = 4;  // 4BP + 1 <= 256 samples per half
 
-__global__ __launch_bounds__(64) void k_nait_reuse(TabView t, PointScalars ps, double *__restrict__ n,
-                                                  int *__restrict__ counts) {
+// Boole weight of sample k of a node with 8BP + 1 samples, in its left (k <=
+// 4BP) and right (k >= 4BP) half; 0 outside the half.
+__device__ __forceinline__ double wleft(int k, int h4) { return k <= h4 ? boole_w(k, h4 + 1) : 0.; }
+__device__ __forceinline__ double wright(int k, int h4) { return k >= h4 ? boole_w(k - h4, h4 + 1) : 0.; }
+
+// fabs((Int - prev) / Int) < rtpna, decided by a product unless it lies
+// within 1e-12 of the threshold (then by the division itself)
+__device__ __forceinline__ bool relacc_ok(double Int, double prev, double rtpna) {
+    double d = fabs(Int - prev), m = fabs(Int) * rtpna;
+    if (d < m * (1. - 1e-12)) return true;
+    if (d > m * (1. + 1e-12)) return false;
+    return fabs((Int - prev) / Int) < rtpna;
+}
+
+// Four waves per SIMD (at most 128 VGPRs): the walk is latency bound, and the
+// few spills this costs are cheaper than the lost occupancy.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nait_reuse(
+    ReuseTab rt, const TabView *__restrict__ tv, PointScalars ps, double *__restrict__ n, int *__restrict__ counts) {
     extern __shared__ double lds[];
+    const TabView &t = *tv;  // chain tables and constants for the lower limit and the a_r = 0 form
     const int lane = threadIdx.x;
     const long long i = blockIdx.x;
     const int BP = ps.BP, h4 = 4 * BP, full = 8 * BP;
+    const int R = (h4 + 64) / 64;                 // rounds over the 4BP + 1 samples of a half
     double *cur = lds;                            // [8BP + 1] samples of the current node
     double *stk = lds + full + 1;                 // [depth][4BP + 1] pending right halves
     double *stn = stk + (size_t)ps.depth * (h4 + 1);  // [depth][4]: a, b, prev, iter
@@ -438,7 +557,8 @@ __global__ __launch_bounds__(64) void k_nait_reuse(TabView t, PointScalars ps, d
         bool small = true;
         if (lane < 27) {
             double x = -ldexp(1., lane + 1), arg = -ps.tau * (U - x * ar);
-            double l1 = polylog_j(t, 1, arg), l3 = polylog_j(t, 3, arg);
+            double l1, l3;
+            polylog_pair(t, arg, l1, l3);  // the values of polylog_j(1, .) and polylog_j(3, .)
             double test = ps.beta0 * curly_a(t, x) * ar * ps.tau * (ps.beta1 * l1 + beta2 * l3);
             small = !(fabs(test) > kMP);
         }
@@ -447,59 +567,67 @@ __global__ __launch_bounds__(64) void k_nait_reuse(TabView t, PointScalars ps, d
         if (kx < 0 || kx > 26) kx = 26;
         const double xlow = -ldexp(1., kx + 1);
         evals = 27;
+#pragma unroll 1
         for (int root = 0; root < 2; root++) {
             double a = root ? xlow : 2. * xlow, b = root ? 100. : xlow, prev = 0.;
-            int iter = 0, sp = 0, mode = 0;  // 0 all samples, 1 left child in place, 2 popped right child
+            // per-root constants: at level iter an exactly bisected node has
+            // half = half0 2^-iter, Delta = Delta0 2^-iter, 2h/45 = c0 2^-iter
+            const double half0 = 0.5 * (b - a), Delta0 = 0.25 * half0 / ((double)BP);
+            const double c0 = 2. * (((a + half0) - a) / (((double)BP) * 4.)) / 45.;
+            int iter = 0, sp = 0;
+            bool fresh = true;  // the root evaluates all 8BP + 1 samples
+            double sa = 0., sb = 0.;
             for (;;) {
-                const double half = 0.5 * (b - a), Delta = 0.25 * half / ((double)BP);
-                if (mode != 0) {  // even samples from the parent's half
-                    const double *src = mode == 1 ? cur : stk + (size_t)sp * (h4 + 1);
-                    double v[kReuseMaxPer];
-#pragma unroll
-                    for (int r = 0; r < kReuseMaxPer; r++) {
-                        int j = lane + 64 * r;
-                        v[r] = j <= h4 ? src[j] : 0.;
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (int r = 0; r < kReuseMaxPer; r++) {
-                        int j = lane + 64 * r;
-                        if (j <= h4) cur[2 * j] = v[r];
-                    }
+                // the walk is wave-uniform; say so to the compiler (scalar registers, scalar branches)
+                a = uniform_d(a);
+                b = uniform_d(b);
+                prev = uniform_d(prev);
+                iter = __builtin_amdgcn_readfirstlane(iter);
+                sp = __builtin_amdgcn_readfirstlane(sp);
+                const double half = 0.5 * (b - a);
+                double Delta, ca, cb;
+                if (half == ldexp(half0, -iter) && (a + half) - a == half && b - (a + half) == half) {
+                    Delta = ldexp(Delta0, -iter);
+                    ca = cb = ldexp(c0, -iter);
+                } else {
+                    Delta = 0.25 * half / ((double)BP);
+                    ca = 2. * (((a + half) - a) / (((double)BP) * 4.)) / 45.;
+                    cb = 2. * ((b - (a + half)) / (((double)BP) * 4.)) / 45.;
                 }
-                const int nslots = mode == 0 ? full + 1 : h4;
+                // samples to evaluate: all (root) or the 4BP odd ones (the even
+                // ones and their Boole sums were placed by the parent step)
+                const int nslots = fresh ? full + 1 : h4;
                 for (int sl = lane; sl < nslots; sl += 64) {
-                    int k = mode == 0 ? sl : 2 * sl + 1;
+                    int k = fresh ? sl : 2 * sl + 1;
                     double x = k <= h4 ? a + (double)k * Delta : (a + (double)(k - h4) * Delta) + half;
-                    cur[k] = integrand(t, q, x);
+                    double f = integrand_fast(rt, q, x);
+                    cur[k] = f;
+                    sa += wleft(k, h4) * f;
+                    sb += wright(k, h4) * f;
                 }
                 evals += nslots;
-                __syncthreads();
-                double sa = 0., sb = 0.;
-                for (int k = lane; k <= full; k += 64) {
-                    double f = cur[k];
-                    if (k <= h4) sa += boole_w(k, h4 + 1) * f;
-                    if (k >= h4) sb += boole_w(k - h4, h4 + 1) * f;
-                }
-                sa = wave_total(sa);
-                sb = wave_total(sb);
-                double ha = (a + half - a) / (((double)BP) * 4.), hb = (b - (a + half)) / (((double)BP) * 4.);
-                double Inta = sa * (2. * ha / 45.), Intb = sb * (2. * hb / 45.);
-                double Int = Inta + Intb;
+                fresh = false;
+                const double Sa = wave_total(sa), Sb = wave_total(sb);
+                const double Inta = Sa * ca, Intb = Sb * cb, Int = Inta + Intb;
                 bool accept = true;
-                if (Int != Int) {
-                } else if (iter > 1 && fabs(Int) > 0. && fabs((Int - prev) / Int) < ps.rtpna) {
+                if (Int != Int) {  // (Int, Inta, Intb are wave-uniform: wave_total)
+                } else if (iter > 1 && fabs(Int) > 0. && relacc_ok(Int, prev, ps.rtpna)) {
                     nra++;
                 } else if (iter > ps.maxIter) {
                     nim++;
                 } else {
-                    double sm = ldexp(Int, iter);  // 2^iter Int
+                    double sm = ldexp(Int, iter);  // 2^iter Int (NegligibleMagnitudeQ)
                     bool negl;
                     if (sm == 0.) negl = true;
                     else if (ps.compare == 0. || ps.compare < sm) negl = false;
+                    else if (fabs(sm) < ps.negl_lo) negl = true;
+                    else if (fabs(sm) > ps.negl_hi) negl = false;
                     else negl = ps.log10cmp - log10(fabs(sm)) > ps.maxexp;
                     accept = negl || Delta < kMP;
                 }
+                accept = __builtin_amdgcn_readfirstlane((int)accept) != 0;
+                __syncthreads();  // cur complete
+                const double *src;
                 if (accept) {
                     res += Int;
                     if (sp == 0) break;
@@ -508,12 +636,13 @@ __global__ __launch_bounds__(64) void k_nait_reuse(TabView t, PointScalars ps, d
                     b = uniform_d(stn[4 * sp + 1]);
                     prev = uniform_d(stn[4 * sp + 2]);
                     iter = (int)uniform_d(stn[4 * sp + 3]);
-                    mode = 2;
+                    src = stk + (size_t)sp * (h4 + 1);
                 } else {
                     if (sp >= ps.depth) {  // deeper than maxIter allows; cannot happen
                         res = __builtin_nan("");
                         break;
                     }
+                    // the right child waits on the stack with the right half of cur
                     double *dst = stk + (size_t)sp * (h4 + 1);
                     for (int j = lane; j <= h4; j += 64) dst[j] = cur[h4 + j];
                     if (lane == 0) {
@@ -526,8 +655,27 @@ __global__ __launch_bounds__(64) void k_nait_reuse(TabView t, PointScalars ps, d
                     b = a + half;
                     prev = Inta;
                     iter++;
-                    mode = 1;
-                    __syncthreads();
+                    src = cur;  // the left child: the left half of cur, in place
+                }
+                // the next node's even samples k = 2j are src[j]; their Boole
+                // sums are formed here, the odd samples are added above
+                double v[kReuseMaxPer];
+                sa = 0.;
+                sb = 0.;
+#pragma unroll
+                for (int r = 0; r < kReuseMaxPer; r++) {
+                    int j = lane + 64 * r;
+                    v[r] = (r < R && j <= h4) ? src[j] : 0.;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < kReuseMaxPer; r++) {
+                    int j = lane + 64 * r;
+                    if (r < R && j <= h4) {
+                        cur[2 * j] = v[r];
+                        sa += wleft(2 * j, h4) * v[r];
+                        sb += wright(2 * j, h4) * v[r];
+                    }
                 }
             }
             __syncthreads();
@@ -621,6 +769,9 @@ struct NaitTables {
     double2 *d_a = nullptr;
     double *d_p = nullptr;
     double *d_asym = nullptr;   // [4][12]
+    double2 *d_pp = nullptr;    // monomial form of the Li pair interpolants
+    TabView *d_view = nullptr;  // device copy of view
+    bool reuse_ok = false;      // the file's geometry is the one the panel-reuse kernel assumes
     int *d_counts = nullptr;    // [3][G] per-point evaluations, iterMax and relAcc returns
     double Mpp[MPDPFT_MAX_SPECIES][3];
     double last_itermax[MPDPFT_MAX_SPECIES], last_relacc[MPDPFT_MAX_SPECIES];
@@ -742,16 +893,78 @@ int nait_init(mpdpft_ctx *c) {
     if (hipMemcpy(t->d_asym, &kAsym.v[0][0], sizeof(kAsym.v), hipMemcpyHostToDevice) != hipSuccess)
         return ctx_fail(c, MPDPFT_EHIP, "nAiT coefficient upload");
     v.asym = t->d_asym;
+    {  // monomial coefficients of the quintic Hermite interpolants of L_{1/2}, L_{-3/2}
+        const int nb = 500;  // (pxmax, pxmax + 500], unit step
+        std::vector<double2> pp(6 * (size_t)(pn - 1 + nb));
+        const long double hh = r[5];
+        for (int k = 0; k + 1 < pn; k++) {
+            for (int o = 0; o < 2; o++) {
+                int j = o == 0 ? 1 : 3;
+                const double *A = pv.data() + 6 * (size_t)k, *B = A + 6;
+                long double F0 = A[j], G0 = hh * A[j + 1], C0 = hh * hh * A[j + 2];
+                long double F1 = B[j], G1 = hh * B[j + 1], C1 = hh * hh * B[j + 2];
+                long double m[6] = {F0, G0, 0.5L * C0,
+                                    -10 * F0 - 6 * G0 - 1.5L * C0 + 10 * F1 - 4 * G1 + 0.5L * C1,
+                                    15 * F0 + 8 * G0 + 1.5L * C0 - 15 * F1 + 7 * G1 - C1,
+                                    -6 * F0 - 3 * G0 - 0.5L * C0 + 6 * F1 - 3 * G1 + 0.5L * C1};
+                for (int q2 = 0; q2 < 3; q2++)
+                    pp[6 * (size_t)k + 3 * o + q2] = double2{(double)m[2 * q2], (double)m[2 * q2 + 1]};
+            }
+        }
+        // second segment: L_s(x) = -x^s/Gamma(s+1) (1 + sum_k 2 eta(2k) (s)_{2k} x^-2k),
+        // s = 1/2 - m (m = 0..4 covers both orders and two derivatives each)
+        auto Lasym = [](int m, long double x) {
+            const long double s = 0.5L - m;
+            const long double sqpi = 1.772453850905516027298167483341145183L;
+            const long double gam[5] = {0.5L * sqpi, sqpi, -2.L * sqpi, 4.L * sqpi / 3.L, -8.L * sqpi / 15.L};
+            long double x2 = 1.L / (x * x), ff = 1.L, acc = 1.L, xp = 1.L;
+            for (int k = 1; k <= 12; k++) {
+                ff *= (s - 2 * k + 2) * (s - 2 * k + 1);
+                xp *= x2;
+                acc += (long double)kTwoEta[k - 1] * ff * xp;
+            }
+            return -powl(x, s) / gam[m] * acc;
+        };
+        for (int mb = 0; mb < nb; mb++) {
+            long double x0 = (long double)v.pxmax + mb, x1 = x0 + 1.L;
+            for (int o = 0; o < 2; o++) {
+                int m0 = o == 0 ? 0 : 2;  // L_{1/2} chain from m = 0, L_{-3/2} chain from m = 2
+                long double F0 = Lasym(m0, x0), G0 = Lasym(m0 + 1, x0), C0 = Lasym(m0 + 2, x0);
+                long double F1 = Lasym(m0, x1), G1 = Lasym(m0 + 1, x1), C1 = Lasym(m0 + 2, x1);
+                long double m[6] = {F0, G0, 0.5L * C0,
+                                    -10 * F0 - 6 * G0 - 1.5L * C0 + 10 * F1 - 4 * G1 + 0.5L * C1,
+                                    15 * F0 + 8 * G0 + 1.5L * C0 - 15 * F1 + 7 * G1 - C1,
+                                    -6 * F0 - 3 * G0 - 0.5L * C0 + 6 * F1 - 3 * G1 + 0.5L * C1};
+                size_t row = (size_t)(pn - 1 + mb);
+                for (int q2 = 0; q2 < 3; q2++)
+                    pp[6 * row + 3 * o + q2] = double2{(double)m[2 * q2], (double)m[2 * q2 + 1]};
+            }
+        }
+        v.nb = nb;
+        if (hipMalloc((void **)&t->d_pp, sizeof(double2) * pp.size()) != hipSuccess)
+            return ctx_fail(c, MPDPFT_ENOMEM, "nAiT polynomial table");
+        if (hipMemcpy(t->d_pp, pp.data(), sizeof(double2) * pp.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return ctx_fail(c, MPDPFT_EHIP, "nAiT polynomial table upload");
+        v.pp = t->d_pp;
+    }
+    if (hipMalloc((void **)&t->d_view, sizeof(TabView)) != hipSuccess)
+        return ctx_fail(c, MPDPFT_ENOMEM, "nAiT view");
+    if (hipMemcpy(t->d_view, &v, sizeof(TabView), hipMemcpyHostToDevice) != hipSuccess)
+        return ctx_fail(c, MPDPFT_EHIP, "nAiT view upload");
+    t->reuse_ok = v.cx0 == kCX0 && v.ch == kCH && v.cinv == kCINV && v.cn == kCN && v.px0 == kPX0 &&
+                  v.ph == kPH && v.pinv == kPINV && v.pxmax == kPXMAX && v.pn == kPN && v.nb == kNB;
     return MPDPFT_OK;
 }
 
 void nait_free(mpdpft_ctx *c) {
     if (!c->nait) return;
-    hipFree(c->nait->d_c);
-    hipFree(c->nait->d_a);
-    hipFree(c->nait->d_p);
-    hipFree(c->nait->d_asym);
-    hipFree(c->nait->d_counts);
+    (void)hipFree(c->nait->d_c);
+    (void)hipFree(c->nait->d_a);
+    (void)hipFree(c->nait->d_p);
+    (void)hipFree(c->nait->d_asym);
+    (void)hipFree(c->nait->d_pp);
+    (void)hipFree(c->nait->d_view);
+    (void)hipFree(c->nait->d_counts);
     delete c->nait;
     c->nait = nullptr;
 }
@@ -807,15 +1020,29 @@ int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {
     ps.compare = da.compare;
     ps.log10cmp = da.compare != 0. ? std::log10(std::fabs(da.compare)) : 0.;
     ps.maxexp = -std::log10(da.rtpna);
+    {  // NegligibleMagnitudeQ: log10(compare) - log10|sm| > maxexp <=> |sm| < compare 10^-maxexp
+        double thr10 = std::pow(10., ps.log10cmp - ps.maxexp);
+        ps.negl_lo = thr10 * (1. - 1e-9);
+        ps.negl_hi = thr10 * (1. + 1e-9);
+    }
     ps.BP = da.BP;
     ps.maxIter = da.maxIter;
     ps.depth = da.maxIter + 2;
     ps.G = (long long)G;
     size_t lds = sizeof(double) * ((size_t)(8 * da.BP + 1) + (size_t)ps.depth * (4 * da.BP + 1 + 4));
-    const bool reuse = !c->nait_plain && 4 * da.BP + 1 <= 64 * kReuseMaxPer && lds <= 65536;
+    const bool reuse = !c->nait_plain && t->reuse_ok && 4 * da.BP + 1 <= 64 * kReuseMaxPer && lds <= 65536;
+    ReuseTab rt;
+    rt.pp = t->view.pp;
+    rt.cc = t->view.cc;
+    rt.pw2_1 = t->view.pw2[1];
+    rt.pw3_1 = t->view.pw3[1];
+    rt.pw2_3 = t->view.pw2[3];
+    rt.pw3_3 = t->view.pw3[3];
+    rt.ginv1 = t->view.ginv[1];
+    rt.ginv3 = t->view.ginv[3];
     stage_begin(c);
     if (reuse) {
-        hipLaunchKernelGGL(k_nait_reuse, dim3((unsigned)G), dim3(64), lds, c->st, t->view, ps, den_out,
+        hipLaunchKernelGGL(k_nait_reuse, dim3((unsigned)G), dim3(64), lds, c->st, rt, t->d_view, ps, den_out,
                            t->d_counts);
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(c, e, "k_nait_reuse");

@@ -221,9 +221,15 @@ def test_not_converged_is_reported(oracle):
 
 
 # ------------------------------------------------------------------ nAiT (C1 functional)
-def test_nait_density_stage(oracle):
-    print("case nait density stage", flush=True)
-    p = configs.c1(steps=16, density_expression=5, hartree_mode=1)
+@pytest.mark.parametrize("kernel,steps,bp", [("reuse", 16, 10), ("plain", 16, 10), ("reuse", 16, 17),
+                                              ("reuse", 32, 10)])
+def test_nait_density_stage(oracle, monkeypatch, kernel, steps, bp):
+    """nAiT density at fixed V and mu: the panel-reuse kernel (one or two
+    samples per lane and half) and the per-node kernel against the oracle."""
+    print("case nait density stage", kernel, steps, bp, flush=True)
+    monkeypatch.setenv("MPDPFT_NAIT_KERNEL", kernel)
+    p = configs.c1(steps=steps, density_expression=5, hartree_mode=1)
+    p.Mpp[0] = bp
     ctx = Context(p)
     v = oracle.env(p)[0] + 0.05 * np.cos(np.arange((p.steps + 1) ** 3) * 0.01)
     for mu in (-0.6, -0.2):
