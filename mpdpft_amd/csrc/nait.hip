@@ -715,49 +715,98 @@ struct EkinArgs {
     const double *V, *grad2, *lap;
     double mu, thr, c0, c1, c2, internal_acc, cmp, area_pref, neg_exp;
     int MaxIt;
+    int resume;  // continue each point from the refinement the previous launch stopped at
 };
 
-// Airy kinetic-energy density (EkinAiT03D inner loop), one wavefront per
-// point: the trapezoid samples of each refinement are spread over the lanes.
-__global__ __launch_bounds__(256) void k_ekin_ait(mk::Grid g, TabView t, EkinArgs ea, double *out) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int G = (int)g.G;
-    for (int i = blockIdx.x * 4 + wv; i < G; i += gridDim.x * 4) {
-        double U = ea.V[i] - ea.mu, a = 0.5 * ea.c0 * pow(fabs(ea.grad2[i]), 1. / 3.);
-        double lap = ea.lap[i];
-        double val;
-        if (a < ea.thr) {
-            val = (-U > 0.) ? ea.c1 * pow(2. * (-U), 2.5) / 5. - ea.c2 * lap * sqrt(2. * (-U)) / 6. : 0.;
-        } else {
-            double xmax = fmax(1., 100. - U / a);
-            double prev = 1.;
-            int iteration = 0, gs = 64;
-            for (;;) {
-                double r = 0.;
-                const double Ua = U / a, xr = fmax(1., 100. - Ua), dg = (double)(gs - 1);
-                const double k1 = ea.c1 / 5., k2 = ea.c2 * lap / 6.;
-                for (int j = lane; j < gs; j += 64) {
-                    double x = (double)j * xr / dg, y = x + Ua;
-                    double AI = y > 100. ? 0. : airy_ai(t, y);
-                    double tt = 2. * a * x, sq = sqrt(tt);
-                    double f = AI * (k1 * tt * tt * sq - k2 * sq);  // c1 (2ax)^{5/2}/5 - c2 lap (2ax)^{1/2}/6
-                    r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * f;
-                }
-                r = wave_sum(r);
-                r *= (xmax - 0.) / ((double)gs - 1.);
-                val = r;
-                double ac = (fabs(val) < kMP && fabs(prev) < kMP) ? 0. : fabs((val - prev) / (val + prev));
-                double as = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
-                if (iteration > 0 && (ac < kMP || (fabs(as) < ea.internal_acc &&
-                                                   negligible(ea.cmp, ea.area_pref * val, ea.neg_exp))))
-                    break;
-                if (iteration == ea.MaxIt) break;
-                iteration++;
-                gs *= 2;
-                prev = val;
-            }
+// Ai on [-300, 100]: the quintic Hermite interpolants of the (Ai, Ai') table
+// (Ai'' = y Ai) in monomial form per interval; the asymptotic form below.
+constexpr double kAX0 = -300., kAH = 0.005, kAINV = 200.;
+constexpr int kAN = 80001;
+
+__device__ __noinline__ double airy_asym(double y) {
+    return cos(2. * (-y) * sqrt(-y) / 3. - 0.25 * kPI) / (sqrt(kPI) * pow(-y, 0.25));
+}
+
+__device__ __forceinline__ double airy_fast(const double2 *__restrict__ ap, double y) {
+    if (y <= -300.) return airy_asym(y);
+    int k = (int)floor((y - kAX0) * kAINV);
+    k = k < 0 ? 0 : (k > kAN - 2 ? kAN - 2 : k);
+    double u = (y - (kAX0 + kAH * (double)k)) * kAINV;
+    const double2 *c = ap + 3 * k;
+    double2 c01 = c[0], c23 = c[1], c45 = c[2];
+    return ((((c45.y * u + c45.x) * u + c23.y) * u + c23.x) * u + c01.y) * u + c01.x;
+}
+
+// Airy kinetic-energy density (EkinAiT03D inner loop, mpDPFT.cpp:5624-5709),
+// one wavefront per grid point: the trapezoid samples of each refinement are
+// spread over the lanes.  The host raises MaxIt by one per launch within one
+// energy evaluation; a point's refinements up to the previous cap are the same
+// in every launch, so each launch resumes from the stored last refinement
+// (st_m: its index, -1 once the convergence test stopped the point; st_v: its
+// value) and evaluates only the new one.
+__global__ __launch_bounds__(64) void k_ekin_ait(const double2 *__restrict__ ap, EkinArgs ea, double *out,
+                                                double *st_v, int *st_m) {
+    const int lane = threadIdx.x;
+    const long long i = blockIdx.x;
+    double U = ea.V[i] - ea.mu, a = 0.5 * ea.c0 * pow(fabs(ea.grad2[i]), 1. / 3.);
+    double lap = ea.lap[i];
+    double val;
+    if (a < ea.thr) {
+        val = (-U > 0.) ? ea.c1 * pow(2. * (-U), 2.5) / 5. - ea.c2 * lap * sqrt(2. * (-U)) / 6. : 0.;
+        if (lane == 0) {
+            out[i] = val;
+            st_v[i] = val;
+            st_m[i] = -1;
         }
-        if (lane == 0) out[i] = val;
+        return;
+    }
+    double prev = 1.;
+    int iteration = 0;
+    if (ea.resume) {
+        int m0 = st_m[i];
+        if (m0 < 0) {
+            if (lane == 0) out[i] = st_v[i];
+            return;
+        }
+        iteration = m0 + 1;
+        prev = st_v[i];
+    }
+    const double xmax = fmax(1., 100. - U / a);
+    const double Ua = U / a, xr = fmax(1., 100. - Ua);
+    const double k1 = ea.c1 / 5., k2 = ea.c2 * lap / 6., twoa = 2. * a;
+    bool done = false;
+    for (;;) {
+        const int gs = 64 << iteration;
+        const double dg = (double)(gs - 1), inv = 1. / dg;
+        double r = 0.;
+        for (int j = lane; j < gs; j += 64) {
+            // x = j xr / dg, correctly rounded: product by the reciprocal plus one
+            // residual correction (exact residual by FMA)
+            double num = (double)j * xr, q = num * inv;
+            double x = fma(fma(-q, dg, num), inv, q), y = x + Ua;
+            double AI = y > 100. ? 0. : airy_fast(ap, y);
+            double tt = twoa * x, sq = sqrt(tt);
+            double f = AI * (k1 * tt * tt * sq - k2 * sq);  // c1 (2ax)^{5/2}/5 - c2 lap (2ax)^{1/2}/6
+            r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * f;
+        }
+        r = wave_total(r);
+        r *= (xmax - 0.) / ((double)gs - 1.);
+        val = r;
+        double ac = (fabs(val) < kMP && fabs(prev) < kMP) ? 0. : fabs((val - prev) / (val + prev));
+        double as = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
+        if (iteration > 0 &&
+            (ac < kMP || (fabs(as) < ea.internal_acc && negligible(ea.cmp, ea.area_pref * val, ea.neg_exp)))) {
+            done = true;
+            break;
+        }
+        if (iteration >= ea.MaxIt) break;
+        iteration++;
+        prev = val;
+    }
+    if (lane == 0) {
+        out[i] = val;
+        st_v[i] = val;
+        st_m[i] = done ? -1 : iteration;
     }
 }
 
@@ -771,6 +820,9 @@ struct NaitTables {
     double *d_asym = nullptr;   // [4][12]
     double2 *d_pp = nullptr;    // monomial form of the Li pair interpolants
     TabView *d_view = nullptr;  // device copy of view
+    double2 *d_ap = nullptr;    // Ai in monomial form, [kAN - 1][3]
+    double *d_ekin_v = nullptr; // [G] last refinement value per point (EkinAiT03D)
+    int *d_ekin_m = nullptr;    // [G] its index, -1 when converged
     bool reuse_ok = false;      // the file's geometry is the one the panel-reuse kernel assumes
     int *d_counts = nullptr;    // [3][G] per-point evaluations, iterMax and relAcc returns
     double Mpp[MPDPFT_MAX_SPECIES][3];
@@ -947,6 +999,28 @@ int nait_init(mpdpft_ctx *c) {
             return ctx_fail(c, MPDPFT_EHIP, "nAiT polynomial table upload");
         v.pp = t->d_pp;
     }
+    if (v.ax0 != kAX0 || v.ah != kAH || v.ainv != kAINV || v.an != kAN)
+        return ctx_fail(c, MPDPFT_EIO, "nAiT Airy table geometry differs from the one compiled in");
+    {  // Ai in monomial form per interval (Ai'' = y Ai)
+        std::vector<double2> ap(3 * (size_t)(an - 1));
+        const long double hh = v.ah;
+        for (int k = 0; k + 1 < an; k++) {
+            long double x0 = (long double)(v.ax0 + v.ah * (double)k), x1 = x0 + hh;
+            long double F0 = av[2 * (size_t)k], G0 = hh * av[2 * (size_t)k + 1], C0 = hh * hh * x0 * F0;
+            long double F1 = av[2 * (size_t)k + 2], G1 = hh * av[2 * (size_t)k + 3], C1 = hh * hh * x1 * F1;
+            long double m[6] = {F0, G0, 0.5L * C0,
+                                -10 * F0 - 6 * G0 - 1.5L * C0 + 10 * F1 - 4 * G1 + 0.5L * C1,
+                                15 * F0 + 8 * G0 + 1.5L * C0 - 15 * F1 + 7 * G1 - C1,
+                                -6 * F0 - 3 * G0 - 0.5L * C0 + 6 * F1 - 3 * G1 + 0.5L * C1};
+            for (int q2 = 0; q2 < 3; q2++) ap[3 * (size_t)k + q2] = double2{(double)m[2 * q2], (double)m[2 * q2 + 1]};
+        }
+        if (hipMalloc((void **)&t->d_ap, sizeof(double2) * ap.size()) != hipSuccess ||
+            hipMalloc((void **)&t->d_ekin_v, sizeof(double) * c->G) != hipSuccess ||
+            hipMalloc((void **)&t->d_ekin_m, sizeof(int) * c->G) != hipSuccess)
+            return ctx_fail(c, MPDPFT_ENOMEM, "nAiT Airy table/state");
+        if (hipMemcpy(t->d_ap, ap.data(), sizeof(double2) * ap.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return ctx_fail(c, MPDPFT_EHIP, "nAiT Airy table upload");
+    }
     if (hipMalloc((void **)&t->d_view, sizeof(TabView)) != hipSuccess)
         return ctx_fail(c, MPDPFT_ENOMEM, "nAiT view");
     if (hipMemcpy(t->d_view, &v, sizeof(TabView), hipMemcpyHostToDevice) != hipSuccess)
@@ -964,6 +1038,9 @@ void nait_free(mpdpft_ctx *c) {
     (void)hipFree(c->nait->d_asym);
     (void)hipFree(c->nait->d_pp);
     (void)hipFree(c->nait->d_view);
+    (void)hipFree(c->nait->d_ap);
+    (void)hipFree(c->nait->d_ekin_v);
+    (void)hipFree(c->nait->d_ekin_m);
     (void)hipFree(c->nait->d_counts);
     delete c->nait;
     c->nait = nullptr;
@@ -1116,11 +1193,13 @@ int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E) {
     ea.neg_exp = -std::log(c->p.InternalAcc);
     double e = 0., prev = 0.;
     int Iteration = 0, MAXIteration = c->MaxIteration + 6;
-    unsigned blocks = (unsigned)std::min<size_t>(4096, (G + 255) / 256);
+    if (MAXIteration > 24) return ctx_fail(c, MPDPFT_EINVAL, "EkinAiT03D: MaxIteration above 18");
     for (;;) {
         ea.MaxIt = c->MaxIteration;
+        ea.resume = Iteration > 0;  // MaxIt is one above the previous launch's
         stage_begin(c);
-        hipLaunchKernelGGL(k_ekin_ait, dim3(blocks), dim3(256), 0, c->st, c->g, t->view, ea, c->d_tmp1);
+        hipLaunchKernelGGL(k_ekin_ait, dim3((unsigned)G), dim3(64), 0, c->st, t->d_ap, ea, c->d_tmp1, t->d_ekin_v,
+                           t->d_ekin_m);
         hipError_t he = hipGetLastError();
         if (he != hipSuccess) return hip_fail(c, he, "k_ekin_ait");
         stage_end(c, "ekin_ait", (double)G, 32. * (double)G, 0.);
