@@ -179,8 +179,25 @@ int mpdpft_solve(mpdpft_ctx *ctx, mpdpft_result *res);
 int mpdpft_get_density(mpdpft_ctx *ctx, double *host);
 int mpdpft_get_potential(mpdpft_ctx *ctx, double *host);
 int mpdpft_get_environment(mpdpft_ctx *ctx, double *host);
-/* Replace V (and the initial density is recomputed from it at solve time). */
+/* Replace V on the device (input of the stage entry points below; a solve
+ * starts from Env, or from the initial potential set next). */
 int mpdpft_set_potential(mpdpft_ctx *ctx, const double *host);
+
+/* ---- warm start and loop control ----------------------------------------- */
+/* LoadV/ImportV (mpDPFT.cpp:6430-6502): read a potential file in the
+ * mpDPFT_V.dat format ("x [y [z]] V_0 .. V_{S-1}" per line, n^DIM lines) and
+ * interpolate it multilinearly (linear extrapolation outside its grid) onto
+ * the lattice of p; out: S*G doubles.  Host code only, no GPU.  *warnings
+ * (may be NULL) counts the reference's cross-check warnings: column count,
+ * NaN entries, integrals of the two fields differing by more than 5%. */
+int mpdpft_import_v(const char *path, const mpdpft_params *p, double *out, int *warnings);
+/* Potential the next mpdpft_solve starts from instead of V = Env (the
+ * InterpolVQ = 1 path, mpDPFT.cpp:184); host [S][G]; NULL restores V = Env. */
+int mpdpft_set_initial_potential(mpdpft_ctx *ctx, const double *host);
+/* ManualSCloopBreakQ (mpDPFT.cpp:11959-11976): file polled once per SC
+ * iteration while Print >= 0; a first integer 1 (or 2) ends the loop after
+ * that iteration and the file is rewritten as 0.  NULL or "" turns it off. */
+int mpdpft_set_break_file(mpdpft_ctx *ctx, const char *path);
 
 /* ---- stage entry points (one call = one reference routine) ---------------- */
 /* Centred continuous-FT approximation on a complex [(s+1)^D][2] field,
@@ -214,8 +231,10 @@ int mpdpft_enable_timing(mpdpft_ctx *ctx, int on);
 int mpdpft_get_stage_stats(mpdpft_ctx *ctx, mpdpft_stage_stats *out, int max, int *n);
 
 /* ---- drop-in: run in a directory like ./mpDPFT --------------------------- */
-/* Reads dir/mpDPFT.input, solves, writes dir/mpDPFT_Den.dat, mpDPFT_V.dat and
- * mpDPFT_Control.dat in the reference formats (ExportData, :12399-12520). */
+/* Reads dir/mpDPFT.input (and dir/mpDPFT_V.dat when InterpolVQ = 1), polls
+ * dir/mpDPFTmanualSCloopBreakQ.dat, solves, writes dir/mpDPFT_Den.dat,
+ * mpDPFT_V.dat and mpDPFT_Control.dat in the reference formats (ExportData,
+ * :12399-12520). */
 int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res);
 
 #ifdef __cplusplus

@@ -123,12 +123,37 @@ class Context:
         return {arr[i].name.decode(): dict(launches=arr[i].launches, total_ms=arr[i].total_ms, units=arr[i].units,
                                            bytes=arr[i].bytes, flops=arr[i].flops) for i in range(n.value)}
 
+    def set_initial_potential(self, v):
+        """Start the next solve from v ([S][G]) instead of V = Env; None resets."""
+        if v is None:
+            self._check(self._lib.mpdpft_set_initial_potential(self._h, None))
+            return
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        self._check(self._lib.mpdpft_set_initial_potential(self._h, _dp(v)))
+
+    def set_break_file(self, path):
+        self._check(self._lib.mpdpft_set_break_file(self._h, str(path).encode() if path else None))
+
     def density_at(self, s, mu, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
         out = np.zeros(self.G)
         N = ctypes.c_double()
         self._check(self._lib.mpdpft_density(self._h, s, mu, _dp(v), _dp(out), ctypes.byref(N)))
         return out, N.value
+
+
+def import_v(path, params):
+    """LoadV/ImportV: a potential file interpolated onto the lattice of params
+    (host code, no GPU).  Returns ([S][G] array, warning count)."""
+    lib = load_library()
+    S = params.S
+    G = (params.steps + 1) ** params.DIM
+    out = np.zeros(S * G)
+    w = ctypes.c_int()
+    rc = lib.mpdpft_import_v(str(path).encode(), ctypes.byref(params), _dp(out), ctypes.byref(w))
+    if rc != _abi.OK:
+        raise MpdpftError(rc, "import_v(%s)" % path)
+    return out.reshape(S, G), w.value
 
 
 def run_dir(path, device=-1):

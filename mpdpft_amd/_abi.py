@@ -75,6 +75,7 @@ EXPORTS = [
     "mpdpft_solve", "mpdpft_get_density", "mpdpft_get_potential", "mpdpft_get_environment",
     "mpdpft_set_potential", "mpdpft_fft", "mpdpft_delfield", "mpdpft_interaction_potential",
     "mpdpft_density", "mpdpft_run_dir", "mpdpft_enable_timing", "mpdpft_get_stage_stats",
+    "mpdpft_import_v", "mpdpft_set_initial_potential", "mpdpft_set_break_file",
 ]
 
 _lib = None
@@ -128,6 +129,12 @@ def load_library(path=LIB_PATH):
     lib.mpdpft_get_stage_stats.restype = _i
     lib.mpdpft_run_dir.argtypes = [ctypes.c_char_p, _i, R]
     lib.mpdpft_run_dir.restype = _i
+    lib.mpdpft_import_v.argtypes = [ctypes.c_char_p, P, dp, ctypes.POINTER(_i)]
+    lib.mpdpft_import_v.restype = _i
+    lib.mpdpft_set_initial_potential.argtypes = [vp, dp]
+    lib.mpdpft_set_initial_potential.restype = _i
+    lib.mpdpft_set_break_file.argtypes = [vp, ctypes.c_char_p]
+    lib.mpdpft_set_break_file.restype = _i
     if lib.mpdpft_params_sizeof() != ctypes.sizeof(Params):
         raise OSError("mpdpft_params layout mismatch: C %d vs ctypes %d"
                       % (lib.mpdpft_params_sizeof(), ctypes.sizeof(Params)))

@@ -184,8 +184,7 @@ __device__ __forceinline__ double curly_a_fast(const ReuseTab &t, double y) {
     if (y < -1.0e+50) return 1.;
     if (y < -149.99) return curly_a_asym(y);
     if (y >= 99.99) return 0.;
-    int j = (int)floor((y - kCX0) * kCINV);
-    j = j < 0 ? 0 : (j > kCN - 2 ? kCN - 2 : j);
+    int j = (int)floor((y - kCX0) * kCINV);  // 0 < j < kCN - 2 for -149.99 <= y < 99.99
     double u = y - (kCX0 + kCH * (double)j);
     double4 c = t.cc[j];
     return c.x + u * (c.y + u * (c.z + u * c.w));
@@ -201,9 +200,9 @@ __device__ __forceinline__ void polylog_pair_fast(const ReuseTab &t, double x, d
         if (x < -699.) {
             l1 = l3 = 0.;
         } else {
-            double e = exp(x), e2 = e * e, e3 = e2 * e;
-            l1 = -e + e2 * t.pw2_1 - e3 * t.pw3_1;
-            l3 = -e + e2 * t.pw2_3 - e3 * t.pw3_3;
+            // -e + e^2 2^-s - e^3 3^-s rounds to -e: e <= e^-40, so the two
+            // corrections lie below half an ulp of e
+            l1 = l3 = -exp(x);
         }
         return;
     }
@@ -472,8 +471,8 @@ __global__ __launch_bounds__(256) void k_nait_density(
 // ---------------------------------------------------------------------------
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v) {
-    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 
@@ -602,8 +601,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                     double x = k <= h4 ? a + (double)k * Delta : (a + (double)(k - h4) * Delta) + half;
                     double f = integrand_fast(rt, q, x);
                     cur[k] = f;
-                    sa += wleft(k, h4) * f;
-                    sb += wright(k, h4) * f;
+                    double wl, wr;
+                    if (fresh) {
+                        wl = wleft(k, h4);
+                        wr = wright(k, h4);
+                    } else {  // odd samples: weight 32 in the half that holds them
+                        wl = k < h4 ? 32. : 0.;
+                        wr = k < h4 ? 0. : 32.;
+                    }
+                    sa += wl * f;
+                    sb += wr * f;
                 }
                 evals += nslots;
                 fresh = false;

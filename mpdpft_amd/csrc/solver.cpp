@@ -181,6 +181,13 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         if (t == 1 && p->n_mpp < p->S) return ctx_fail(c, MPDPFT_EINVAL, "mpp needs S values");
     }
     if (p->Precision != 0) return ctx_fail(c, MPDPFT_EINVAL, "Precision 1 is not built yet");
+    if (p->InterpolVQ != 0 && p->InterpolVQ != 1)
+        return ctx_fail(c, MPDPFT_EINVAL, "InterpolVQ must be 0 or 1 (2, 3 come from task loops)");
+    for (int i = 0; i < p->n_schedule; i++) {
+        int sc = p->Schedule[i];
+        if (!(sc == 0 || sc == 5 || sc == 9))
+            return ctx_fail(c, MPDPFT_EINVAL, "Schedule " + std::to_string(sc) + " is not built (0, 5, 9 are)");
+    }
     if (p->DensityExpression == 5)
         for (int s = 0; s < p->S; s++)
             if (p->TVec[s] == 0.) return ctx_fail(c, MPDPFT_EINVAL, "nAiT needs TVec != 0 (T < 0 picks it automatically)");
@@ -1051,9 +1058,12 @@ static int init_loop(mpdpft_ctx *c) {
     c->incV = c->incV0 = p.incrementalV;
     c->SCcount = c->ExitSCcount = c->ExitSCloop = 0;
     c->LinMix = c->PulMix = c->TrappedQ = 0;
-    c->warningCount = c->errorCount = c->bisection_steps = 0;
+    c->warningCount = c->importWarnings;
+    c->importWarnings = 0;
+    c->errorCount = c->bisection_steps = 0;
     c->MaxIteration = p.MaxIteration;
-    c->CC = 0.;
+    c->CC = c->OldCC = 0.;
+    for (int s = 0; s < c->S; s++) c->thetaVec0[s] = p.thetaVec[s];
     c->delfield_valid = -1;
     nait_reset_state(c);
     c->pulay_mixer = (int)(p.Mixer[0] + 0.5);
@@ -1077,12 +1087,64 @@ static int init_loop(mpdpft_ctx *c) {
     return MPDPFT_OK;
 }
 
+// ManualSCloopBreakQ (mpDPFT.cpp:11959-11976): the first integer of the break
+// file; a nonzero value is reported, the file reset to 0 and Print set to 2.
+static int manual_break(mpdpft_ctx *c) {
+    if (c->break_file.empty()) return 0;
+    FILE *f = std::fopen(c->break_file.c_str(), "r");
+    if (!f) return 0;
+    int Q = 0;
+    if (std::fscanf(f, "%d", &Q) != 1) Q = 0;
+    std::fclose(f);
+    if (Q) {
+        if (Q == 1)
+            c->control << "---------------------------------------SCloop--Manually--Terminated----------------------"
+                          "-----------------\n";
+        if (Q == 2)
+            c->control << "---------------------------------------TASKloop--Manually--Terminated--------------------"
+                          "-------------------\n";
+        FILE *w = std::fopen(c->break_file.c_str(), "w");
+        if (w) {
+            std::fprintf(w, "0\n");
+            std::fclose(w);
+        }
+        c->p.Print = 2;
+    }
+    return Q;
+}
+
+// InjectSchedule (mpDPFT.cpp:5229-5315) for the schedules that only retune
+// the mixing: 5 (theta up by 1.1 while CC falls, else halved, clipped to
+// [10/maxSCcount, 0.8]) and 9 (theta ramped linearly towards 0.2).
+static void inject_schedule(mpdpft_ctx *c) {
+    const mpdpft_params &p = c->p;
+    for (int i = 0; i < p.n_schedule; i++) {
+        int sc = p.Schedule[i];
+        if (sc == 5) {
+            for (int s = 0; s < c->S; s++) {
+                if (c->CC < c->OldCC) c->thetaVec[s] *= 1.1;
+                else c->thetaVec[s] *= 0.5;
+                c->thetaVec[s] = std::max(10. / ((double)p.maxSCcount), std::min(0.8, c->thetaVec[s]));
+            }
+            if (p.Print >= 0 && c->PrintSC)
+                c->control << "InjectSchedule: Update thetaVec -> " << vec_str(c->thetaVec, c->S) << "\n";
+        } else if (sc == 9) {
+            const double maxTheta = 0.2;
+            for (int s = 0; s < c->S; s++)
+                if (c->thetaVec[s] < maxTheta)
+                    c->thetaVec[s] = c->thetaVec0[s] + (maxTheta - c->thetaVec0[s]) * (double)c->SCcount /
+                                                           ((double)p.maxSCcount);
+        }
+    }
+}
+
 static int iteration(mpdpft_ctx *c) {
     const mpdpft_params &p = c->p;
     // StoreCurrentIteration (:4804): swap instead of copying; UpdateV then
     // reads the mixed density from OldDen and GetDensities overwrites Den.
     std::swap(c->d_Den, c->d_OldDen);
     std::swap(c->d_V, c->d_OldV);
+    c->OldCC = c->CC;
     for (int s = 0; s < c->S; s++) {
         c->OldmuVec[s] = c->muVec[s];
         c->OldAb[s] = c->TmpAb[s];
@@ -1096,6 +1158,7 @@ static int iteration(mpdpft_ctx *c) {
     if (c->PrintSC == 1)
         c->control << " ***** SCcount = " << c->SCcount << "(" << c->ExitSCcount << "/3);  CC = "
                    << std::setprecision(8) << c->CC << std::setprecision(16) << " *****\n";
+    if (p.Print >= 0 && manual_break(c) > 0) c->ExitSCloop = 1;  // ManualSCloopBreakQ (:4864)
     if (c->SCcount == p.maxSCcount) c->ExitSCloop = 1;
     if (c->verbose) {  // the reference prints every iteration to stdout (PRINT, mpDPFT.cpp:12016)
         double bp = 0., mi = 0.;
@@ -1118,6 +1181,7 @@ static int iteration(mpdpft_ctx *c) {
         c->control << " *****                       NVec = " << vec_str(c->TmpAb, c->S) << " *****\n";
     }
     RC(admix(c));
+    inject_schedule(c);
     // PrepareNextIteration (:5317-5389): counter and print cadence
     c->SCcount++;
     if (p.Print > 0) c->PrintSC = 1;
@@ -1138,8 +1202,13 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
     c->control.str("");
     c->control.precision(16);
     c->control << " ***** mpdpft_hip: self-consistent DPFT loop on " << "gfx950 *****\n";
+    c->control << c->notes.str();
+    c->notes.str("");
     auto t0 = std::chrono::steady_clock::now();
-    HIPC(hipMemcpyAsync(c->d_V, c->d_Env, sizeof(double) * c->SG, hipMemcpyDeviceToDevice, c->st));
+    if (c->V0.empty())  // InitializeEnvironments: V <- Env (:6388)
+        HIPC(hipMemcpyAsync(c->d_V, c->d_Env, sizeof(double) * c->SG, hipMemcpyDeviceToDevice, c->st));
+    else  // LoadV (InterpolVQ = 1, :184)
+        HIPC(hipMemcpyAsync(c->d_V, c->V0.data(), sizeof(double) * c->SG, hipMemcpyHostToDevice, c->st));
     RC(init_loop(c));
     c->PrintSC = p.Print > 0 ? 1 : 0;
     if (p.maxSCcount > 0) {
@@ -1192,6 +1261,19 @@ static int copy_out(mpdpft_ctx *c, const double *d, double *h) {
 extern "C" int mpdpft_get_density(mpdpft_ctx *c, double *h) { return c ? copy_out(c, c->d_Den, h) : MPDPFT_EINVAL; }
 extern "C" int mpdpft_get_potential(mpdpft_ctx *c, double *h) { return c ? copy_out(c, c->d_V, h) : MPDPFT_EINVAL; }
 extern "C" int mpdpft_get_environment(mpdpft_ctx *c, double *h) { return c ? copy_out(c, c->d_Env, h) : MPDPFT_EINVAL; }
+extern "C" int mpdpft_set_initial_potential(mpdpft_ctx *c, const double *h) {
+    if (!c || !c->st) return MPDPFT_EINVAL;
+    if (h) c->V0.assign(h, h + c->SG);
+    else c->V0.clear();
+    return MPDPFT_OK;
+}
+
+extern "C" int mpdpft_set_break_file(mpdpft_ctx *c, const char *path) {
+    if (!c) return MPDPFT_EINVAL;
+    c->break_file = path ? path : "";
+    return MPDPFT_OK;
+}
+
 extern "C" int mpdpft_set_potential(mpdpft_ctx *c, const double *h) {
     if (!c || !h) return MPDPFT_EINVAL;
     HIPC(hipMemcpyAsync(c->d_V, h, sizeof(double) * c->SG, hipMemcpyHostToDevice, c->st));

@@ -56,6 +56,12 @@ struct mpdpft_ctx {
     int has_hartree = 0;
     int verbose = 0;  // MPDPFT_VERBOSE: one line per SC iteration on stderr
     int nait_plain = 0;  // MPDPFT_NAIT_KERNEL=plain: per-node nAiT kernel without panel reuse
+    std::vector<double> V0;   // initial potential of a solve (empty: V = Env)
+    std::ostringstream notes; // lines for the next solve's control stream (LoadV)
+    int importWarnings = 0;   // warnings of that import, added to the solve's count
+    std::string break_file;   // ManualSCloopBreakQ file ("" = not polled)
+    double OldCC = 0.;        // CC of the previous iteration (InjectSchedule 5)
+    double thetaVec0[MPDPFT_MAX_SPECIES] = {};  // thetaVecOriginal (InjectSchedule 9)
     int delfield_valid = -1;  // species whose Grad2/Lap hold DelField(V_s)
 
     NaitTables *nait = nullptr;

@@ -217,6 +217,9 @@ struct orc_state {
     double muVec[MPDPFT_MAX_SPECIES], TmpAb[MPDPFT_MAX_SPECIES], OldAb[MPDPFT_MAX_SPECIES];
     double OldmuVec[MPDPFT_MAX_SPECIES], PreviousmuVec[MPDPFT_MAX_SPECIES];
     double thetaVec[MPDPFT_MAX_SPECIES];
+    double thetaVec0[MPDPFT_MAX_SPECIES]; /* thetaVecOriginal (:900) */
+    double OldCC;                         /* CC of the previous iteration (:4810) */
+    const double *V0;                     /* initial potential (LoadV) or NULL */
     double DenMax[MPDPFT_MAX_SPECIES], DenMin[MPDPFT_MAX_SPECIES];
     double VMin[MPDPFT_MAX_SPECIES], VMax[MPDPFT_MAX_SPECIES];
     double TVec[MPDPFT_MAX_SPECIES];
@@ -346,6 +349,7 @@ static orc_state *orc_create(const mpdpft_params *p) {
     for (int s = 0; s < S; s++) {
         o->muVec[s] = p->muVec[s];
         o->thetaVec[s] = p->thetaVec[s];
+        o->thetaVec0[s] = p->thetaVec[s];
         o->TVec[s] = p->TVec[s];
     }
     o->DeltamuModifier = p->DeltamuModifier;
@@ -1652,7 +1656,8 @@ static int orc_init_loop(orc_state *o) {
         int rc = get_env(o, s);
         if (rc) return rc;
     }
-    memcpy(o->V, o->Env, sizeof(double) * (size_t)(o->S * o->G));
+    /* V <- Env (InitializeEnvironments :6388), or the loaded V (LoadV :184) */
+    memcpy(o->V, o->V0 ? o->V0 : o->Env, sizeof(double) * (size_t)(o->S * o->G));
     int rc = get_densities(o);
     if (rc) return rc;
     field_stats(o);
@@ -1677,7 +1682,27 @@ static int orc_init_loop(orc_state *o) {
     return 0;
 }
 
+/* InjectSchedule (mpDPFT.cpp:5229-5315), schedules 5 and 9: theta retuning. */
+static void inject_schedule(orc_state *o) {
+    for (int i = 0; i < o->p.n_schedule; i++) {
+        int sc = o->p.Schedule[i];
+        if (sc == 5) { /* :5261-5264 */
+            for (int s = 0; s < o->S; s++) {
+                o->thetaVec[s] *= o->CC < o->OldCC ? 1.1 : 0.5;
+                double lo = 10. / (double)o->p.maxSCcount;
+                o->thetaVec[s] = fmax(lo, fmin(0.8, o->thetaVec[s]));
+            }
+        } else if (sc == 9) { /* :5281-5285 */
+            for (int s = 0; s < o->S; s++)
+                if (o->thetaVec[s] < 0.2)
+                    o->thetaVec[s] =
+                        o->thetaVec0[s] + (0.2 - o->thetaVec0[s]) * (double)o->SCcount / (double)o->p.maxSCcount;
+        }
+    }
+}
+
 static int orc_iteration(orc_state *o) {
+    o->OldCC = o->CC; /* StoreCurrentIteration (:4810) */
     store_current_iteration(o);
     int rc = update_v(o);
     if (rc) return rc;
@@ -1692,14 +1717,23 @@ static int orc_iteration(orc_state *o) {
         fprintf(stderr, "SC %d CC %.6e mu %.12f N %.12f DM %.4f\n", o->SCcount, o->CC, o->muVec[0], o->TmpAb[0],
                 o->DeltamuModifier);
     admix_densities(o);
+    inject_schedule(o);
     o->SCcount++; /* PrepareNextIteration (:5317) */
     return 0;
 }
 
 int oracle_nait_before_energy(orc_state *o);
 
+int oracle_solve_v0(const mpdpft_params *p, const double *v0, mpdpft_result *r, double *den_out, double *v_out);
+
 int oracle_solve(const mpdpft_params *p, mpdpft_result *r, double *den_out, double *v_out) {
+    return oracle_solve_v0(p, NULL, r, den_out, v_out);
+}
+
+/* The loop started from v0 ([S][G]) instead of V = Env (InterpolVQ = 1). */
+int oracle_solve_v0(const mpdpft_params *p, const double *v0, mpdpft_result *r, double *den_out, double *v_out) {
     orc_state *o = orc_create(p);
+    o->V0 = v0;
     memset(r, 0, sizeof(*r));
     double t0 = 0.;
 #ifdef _OPENMP

@@ -72,13 +72,18 @@ def grid_size(p):
     return (p.steps + 1) ** p.DIM
 
 
-def solve(p):
+def solve(p, v0=None):
+    """The SC loop from V = Env, or from v0 ([S][G]) when given."""
     from mpdpft_amd._abi import Result
     G = grid_size(p)
     r = Result()
     den = np.zeros(p.S * G)
     V = np.zeros(p.S * G)
-    rc = lib().oracle_solve(ctypes.byref(p), ctypes.byref(r), _dp(den), _dp(V))
+    if v0 is None:
+        rc = lib().oracle_solve(ctypes.byref(p), ctypes.byref(r), _dp(den), _dp(V))
+    else:
+        v0 = np.ascontiguousarray(v0, dtype=np.float64).reshape(-1)
+        rc = lib().oracle_solve_v0(ctypes.byref(p), _dp(v0), ctypes.byref(r), _dp(den), _dp(V))
     return rc, r, den.reshape(p.S, G), V.reshape(p.S, G)
 
 

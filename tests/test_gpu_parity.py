@@ -5,6 +5,7 @@ converged solve); floating point within 1e-10 relative (the FP64 tolerance of
 the drop-in: density, potential, mu and total energy).
 """
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -210,6 +211,62 @@ def test_drop_in_directory_run(oracle, tmp_path):
     assert rel(V, VO) < RTOL
     it = [l for l in ctl.split("\\\\") if l.startswith("SCiterations")][0]
     assert it.split("=")[1].strip() == "%d/%d" % (ro.SCcount - 1, p.maxSCcount)
+
+
+def test_warm_start_from_v_file(oracle, tmp_path):
+    """InterpolVQ = 1: the CLI reads mpDPFT_V.dat of a coarser run, interpolates
+    it onto its own lattice and starts the loop from it (LoadV, mpDPFT.cpp:184,
+    :6445-6502); the oracle runs the same loop from the same imported field."""
+    print("case warm start", flush=True)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    cli = os.path.join(os.path.dirname(mpdpft_amd.__file__), "bin", "mpdpft_hip")
+    p = configs.c0(steps=32)
+    configs.write_input(p, a / "mpDPFT.input")
+    assert subprocess.run([cli, str(a)], capture_output=True, timeout=120).returncode == 0
+    q = configs.c0(steps=64)
+    q.InterpolVQ = 1
+    configs.write_input(q, b / "mpDPFT.input")
+    shutil.copy(a / "mpDPFT_V.dat", b / "mpDPFT_V.dat")
+    v0, warn = mpdpft_amd.import_v(b / "mpDPFT_V.dat", q)
+    out = subprocess.run([cli, str(b)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    ctl = (b / "mpDPFT_Control.dat").read_text()
+    assert "LoadV: Interpolate V from mpDPFT_V.dat" in ctl
+    rc, ro, dO, VO = oracle.solve(q, v0=v0)
+    den = np.array([[float(x) for x in l.split()] for l in (b / "mpDPFT_Den.dat").read_text().splitlines()]).T
+    assert rel(den, dO) < RTOL
+    it = [l for l in ctl.split("\\\\") if l.startswith("SCiterations")][0]
+    assert it.split("=")[1].strip() == "%d/%d" % (ro.SCcount - 1, q.maxSCcount)
+
+
+def test_manual_break_file(tmp_path):
+    """ManualSCloopBreakQ (mpDPFT.cpp:11959-11976): a 1 in the break file ends
+    the loop after the current iteration and the file is reset to 0."""
+    print("case break file", flush=True)
+    p = configs.c0(steps=32)
+    configs.write_input(p, tmp_path / "mpDPFT.input")
+    (tmp_path / "mpDPFTmanualSCloopBreakQ.dat").write_text("1\n")
+    cli = os.path.join(os.path.dirname(mpdpft_amd.__file__), "bin", "mpdpft_hip")
+    out = subprocess.run([cli, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 3, out.stderr  # not converged
+    ctl = (tmp_path / "mpDPFT_Control.dat").read_text()
+    assert "SCloop--Manually--Terminated" in ctl
+    it = [l for l in ctl.split("\\\\") if l.startswith("SCiterations")][0]
+    assert it.split("=")[1].strip() == "0/%d" % p.maxSCcount
+    assert (tmp_path / "mpDPFTmanualSCloopBreakQ.dat").read_text().split()[0] == "0"
+
+
+@pytest.mark.parametrize("schedule", [5, 9])
+def test_schedule_retunes_theta(oracle, schedule):
+    """InjectSchedule 5 and 9 (mpDPFT.cpp:5261-5285) on both paths."""
+    print("case schedule", schedule, flush=True)
+    p = configs.c0(steps=32, maxsc=40)
+    p.n_schedule = 1
+    p.Schedule[0] = schedule
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    check_loop(ro, dO, VO, rg, dG, VG, 2)
 
 
 def test_not_converged_is_reported(oracle):
