@@ -121,10 +121,40 @@ template <typename T2> struct SmallDFT<8, T2> {
 
 constexpr int kThreads = 256;
 
-// One Stockham stage (Govindaraju et al., SC'08 formulation) on B lines of
-// length N held in LDS with row pitch P.  Ns = product of earlier radices.
-template <int N, int B, int P, int R, typename T2>
-__device__ __forceinline__ void stockham_stage(T2 *lds, const T2 *tw, int Ns, int sign) {
+// Radix plan per length: first radix R1 (evaluated straight from the load
+// functor), middle radices in LDS, last radix written straight through the
+// store functor.
+template <int N> struct Plan;
+template <> struct Plan<16> { static constexpr int n = 2, r[3] = {4, 4, 1}; };
+template <> struct Plan<32> { static constexpr int n = 2, r[3] = {8, 4, 1}; };
+template <> struct Plan<64> { static constexpr int n = 2, r[3] = {8, 8, 1}; };
+template <> struct Plan<96> { static constexpr int n = 3, r[3] = {8, 4, 3}; };
+template <> struct Plan<128> { static constexpr int n = 3, r[3] = {8, 8, 2}; };
+template <> struct Plan<192> { static constexpr int n = 3, r[3] = {8, 8, 3}; };
+template <> struct Plan<256> { static constexpr int n = 3, r[3] = {8, 8, 4}; };
+template <> struct Plan<512> { static constexpr int n = 3, r[3] = {8, 8, 8}; };
+
+// Butterfly index j of a stage with N/R butterflies per line -> (line b,
+// butterfly jj).  For lines along the contiguous axis (AXIS 2) consecutive
+// threads take consecutive butterflies of one line; otherwise consecutive
+// lines (contiguous in memory) -- either way global accesses are runs.
+template <int N, int R, int B, int AXIS>
+__device__ __forceinline__ void split(int j, int &b, int &jj) {
+    if constexpr (AXIS == 2) {
+        b = j / (N / R);
+        jj = j - b * (N / R);
+    } else {
+        jj = j / B;
+        b = j - jj * B;
+    }
+}
+
+// One Stockham stage (Govindaraju et al., SC'08 formulation) over B lines.
+// SRC/DST 0: LDS (row pitch P); SRC 1: the load functor; DST 1: the store
+// functor.  Ns = product of the earlier radices.
+template <int N, int B, int P, int R, int AXIS, int SRC, int DST, typename T2, class Ld, class St>
+__device__ __forceinline__ void stage(T2 *lds, const T2 *tw, int Ns, int sign, const Ld &ld, const St &st, int slow,
+                                      int fast0, int L1) {
     constexpr int NB = B * (N / R);
     constexpr int PER = (NB + kThreads - 1) / kThreads;
     const int tid = threadIdx.x;
@@ -135,66 +165,65 @@ __device__ __forceinline__ void stockham_stage(T2 *lds, const T2 *tw, int Ns, in
         int j = tid + q * kThreads;
         dst[q] = -1;
         if (j < NB) {
-            int b = j / (N / R), jj = j - b * (N / R);
-            int k = jj % Ns;
-            const T2 *line = lds + b * P;
+            int b, jj;
+            split<N, R, B, AXIS>(j, b, jj);
+            const int k = jj % Ns, f = fast0 + b;
+            const bool live = f < L1;
+            if constexpr (SRC == 1) {
 #pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = line[jj + r * (N / R)];
-            int step = N / (Ns * R);
+                for (int r = 0; r < R; r++) {
+                    int t = jj + r * (N / R);
+                    T2 z;
+                    z.x = 0;
+                    z.y = 0;
+                    if (live) {
+                        if constexpr (AXIS == 2) z = ld(slow, f, t);
+                        else if constexpr (AXIS == 1) z = ld(slow, t, f);
+                        else z = ld(t, slow, f);
+                    }
+                    v[q][r] = z;
+                }
+            } else {
+                const T2 *line = lds + b * P;
 #pragma unroll
-            for (int r = 1; r < R; r++) {
-                T2 w = tw[(r * k * step) % N];
-                if (sign > 0) w.y = -w.y;
-                v[q][r] = cmul(v[q][r], w);
+                for (int r = 0; r < R; r++) v[q][r] = line[jj + r * (N / R)];
+            }
+            if (Ns > 1) {
+                const int step = N / (Ns * R);
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    T2 w = tw[(r * k * step) % N];
+                    if (sign > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
             }
             SmallDFT<R, T2>::run(v[q], sign);
-            dst[q] = b * P + (jj / Ns) * Ns * R + k;
+            const int o = (jj / Ns) * Ns * R + k;
+            if constexpr (DST == 1) {
+                if (live) {
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        int t = o + r * Ns;
+                        if constexpr (AXIS == 2) st(slow, f, t, v[q][r]);
+                        else if constexpr (AXIS == 1) st(slow, t, f, v[q][r]);
+                        else st(t, slow, f, v[q][r]);
+                    }
+                }
+            } else {
+                dst[q] = b * P + o;
+            }
         }
     }
-    __syncthreads();
+    if constexpr (DST == 0) {
+        if constexpr (SRC == 0) __syncthreads();  // every read of this stage is done
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        if (dst[q] >= 0) {
+        for (int q = 0; q < PER; q++) {
+            if (dst[q] >= 0) {
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[dst[q] + r * Ns] = v[q][r];
+                for (int r = 0; r < R; r++) lds[dst[q] + r * Ns] = v[q][r];
+            }
         }
-    }
-    __syncthreads();
-}
-
-template <int N, int B, int P, typename T2>
-__device__ __forceinline__ void line_fft(T2 *lds, const T2 *tw, int sign) {
-    if constexpr (N == 16) {
-        stockham_stage<N, B, P, 4>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 4>(lds, tw, 4, sign);
-    } else if constexpr (N == 32) {
-        stockham_stage<N, B, P, 8>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 4>(lds, tw, 8, sign);
-    } else if constexpr (N == 64) {
-        stockham_stage<N, B, P, 8>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 8>(lds, tw, 8, sign);
-    } else if constexpr (N == 96) {
-        stockham_stage<N, B, P, 8>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 4>(lds, tw, 8, sign);
-        stockham_stage<N, B, P, 3>(lds, tw, 32, sign);
-    } else if constexpr (N == 128) {
-        stockham_stage<N, B, P, 8>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 8>(lds, tw, 8, sign);
-        stockham_stage<N, B, P, 2>(lds, tw, 64, sign);
-    } else if constexpr (N == 192) {
-        stockham_stage<N, B, P, 8>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 8>(lds, tw, 8, sign);
-        stockham_stage<N, B, P, 3>(lds, tw, 64, sign);
-    } else if constexpr (N == 256) {
-        stockham_stage<N, B, P, 8>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 8>(lds, tw, 8, sign);
-        stockham_stage<N, B, P, 4>(lds, tw, 64, sign);
-    } else if constexpr (N == 512) {
-        stockham_stage<N, B, P, 8>(lds, tw, 1, sign);
-        stockham_stage<N, B, P, 8>(lds, tw, 8, sign);
-        stockham_stage<N, B, P, 8>(lds, tw, 64, sign);
-    } else {
-        static_assert(N == 16, "unsupported FFT length");
+        __syncthreads();
     }
 }
 
@@ -208,9 +237,13 @@ struct PassGeom {
 
 // Load functor: T2 operator()(int x, int y, int z) const
 // Store functor: void operator()(int x, int y, int z, T2 v) const
+// The first radix stage reads through the load functor and the last writes
+// through the store functor, so the B lines make one round trip through LDS
+// per middle stage plus one.
 template <int N, int B, int AXIS, typename T2, class Ld, class St>
 __global__ __launch_bounds__(kThreads) void fft_pass(Ld ld, St st, PassGeom g, const T2 *__restrict__ twg) {
     constexpr int P = N + 1;  // pad: keep column accesses off one bank
+    constexpr int R1 = Plan<N>::r[0], R2 = Plan<N>::r[1], R3 = Plan<N>::r[2];
     __shared__ T2 lds[B * P];
     __shared__ T2 tw[N];
     const int tid = threadIdx.x;
@@ -218,41 +251,13 @@ __global__ __launch_bounds__(kThreads) void fft_pass(Ld ld, St st, PassGeom g, c
     const int tilesFast = (g.L1 + B - 1) / B;
     const int slow = blockIdx.x / tilesFast;
     const int fast0 = (blockIdx.x - slow * tilesFast) * B;
-    // load
-    if constexpr (AXIS == 2) {
-        for (int e = tid; e < B * N; e += kThreads) {
-            int b = e / N, t = e - b * N;
-            int y = fast0 + b;
-            if (y < g.L1) lds[b * P + t] = ld(slow, y, t);
-        }
+    if constexpr (Plan<N>::n == 2) {
+        stage<N, B, P, R1, AXIS, 1, 0>(lds, tw, 1, g.sign, ld, st, slow, fast0, g.L1);
+        stage<N, B, P, R2, AXIS, 0, 1>(lds, tw, R1, g.sign, ld, st, slow, fast0, g.L1);
     } else {
-        for (int e = tid; e < B * N; e += kThreads) {
-            int t = e / B, b = e - t * B;
-            int f = fast0 + b;
-            if (f < g.L1) {
-                if constexpr (AXIS == 1) lds[b * P + t] = ld(slow, t, f);
-                else lds[b * P + t] = ld(t, slow, f);
-            }
-        }
-    }
-    __syncthreads();
-    line_fft<N, B, P>(lds, tw, g.sign);
-    // store
-    if constexpr (AXIS == 2) {
-        for (int e = tid; e < B * N; e += kThreads) {
-            int b = e / N, t = e - b * N;
-            int y = fast0 + b;
-            if (y < g.L1) st(slow, y, t, lds[b * P + t]);
-        }
-    } else {
-        for (int e = tid; e < B * N; e += kThreads) {
-            int t = e / B, b = e - t * B;
-            int f = fast0 + b;
-            if (f < g.L1) {
-                if constexpr (AXIS == 1) st(slow, t, f, lds[b * P + t]);
-                else st(t, slow, f, lds[b * P + t]);
-            }
-        }
+        stage<N, B, P, R1, AXIS, 1, 0>(lds, tw, 1, g.sign, ld, st, slow, fast0, g.L1);
+        stage<N, B, P, R2, AXIS, 0, 0>(lds, tw, R1, g.sign, ld, st, slow, fast0, g.L1);
+        stage<N, B, P, R3, AXIS, 0, 1>(lds, tw, R1 * R2, g.sign, ld, st, slow, fast0, g.L1);
     }
 }
 
