@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass(Ld ld, St st, PassGeom g, c
 }
 
 template <int N> constexpr int lines_per_block() {
-    return N <= 64 ? 32 : (N <= 256 ? 16 : 8);
+    return N <= 64 ? 32 : (N <= 192 ? 16 : 8);
 }
 
 template <int N, int AXIS, typename T2, class Ld, class St>

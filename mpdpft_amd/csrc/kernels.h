@@ -37,6 +37,8 @@ struct MulParams {
     int kind;
     int axis_a, axis_b;   // axes for MUL_DERIV_PAIR (0..2 in 3D; 1..2 in 2D)
     double sigma;
+    const double *ek;     // EXP(sigma k^2) per standard index (the smoothing factor is separable:
+                          // exp(sigma |k|^2) = prod_d ek[q_d]); null: sigma = 0
     const double *kq;     // k per standard index (kLattice values), length s
     const double *kq_h;   // k per standard index (k0 + p dk), Hartree
     const double *f0slot; // device scalar: dx^D sum n (Hartree)

@@ -149,6 +149,12 @@ struct LdMul {
     const double2 *p;
     int n1, n2, D, s;
     MulParams m;
+    __device__ __forceinline__ double smooth(const int *q) const {
+        if (!m.ek) return 1.;
+        double f = m.ek[q[0]];
+        for (int d = 1; d < D; d++) f *= m.ek[q[d]];
+        return f;
+    }
     __device__ double2 operator()(int x, int y, int z) const {
         double2 v = p[((size_t)x * n1 + y) * n2 + z];
         int q3[3] = {x, y, z};
@@ -162,14 +168,14 @@ struct LdMul {
         double2 o;
         switch (m.kind) {
         case MUL_SMOOTH: {
-            double f = d_exp(m.sigma * k2);
+            double f = smooth(q);
             o.x = v.x * f;
             o.y = v.y * f;
             return o;
         }
         case MUL_DERIV_PAIR:
         case MUL_DERIV_LAP: {
-            double f = d_exp(m.sigma * k2);
+            double f = smooth(q);
             // A = i ka f v  (Nyquist plane of axis a dropped)
             double2 A = {0., 0.}, B = {0., 0.};
             if (m.axis_a >= 0 && q[m.axis_a] != s / 2) {

@@ -32,6 +32,10 @@ constexpr int kStack = 64;
 // arg 3, Li pair (index 8, Hermite basis 36, two combinations 30) 74,
 // CurlyA cubic 11, product 4, Boole weight and accumulate 2.
 constexpr double kNaitFlopsPerEval = 94.;
+// The panel-reuse kernel's evaluation on its table path: abscissa 2, argument
+// 3, Li index and offset 6, two degree-5 Horner chains 20, CurlyA index,
+// offset and cubic 11, product 4, Boole weights into both half sums 4.
+constexpr double kNaitReuseFlopsPerEval = 50.;
 
 struct TabView {
     const double4 *cc;  // CurlyA cubic per knot interval: c0 + u(c1 + u(c2 + u c3))
@@ -1148,7 +1152,9 @@ int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {
     if (rc) return rc;
     // roofline numerators: integrand evaluations x nominal FP64 operations each;
     // HBM traffic V, |grad V|^2, lap V in and n out
-    stage_add(c, "nait_density", r[3], 32. * (double)G, r[3] * kNaitFlopsPerEval);
+    // HBM: V, |grad V|^2, lap V in, n out (+ three int counters per point for the reuse kernel)
+    stage_add(c, "nait_density", r[3], (reuse ? 44. : 32.) * (double)G,
+              r[3] * (reuse ? kNaitReuseFlopsPerEval : kNaitFlopsPerEval));
     double a = (c->p.edge / 2. - (-c->p.edge / 2.)) / ((double)c->EL - 1.);
     for (int d = 0; d < c->D; d++) r[0] *= a;
     *N = r[0];

@@ -259,6 +259,7 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
     size_t SG = c->SG, G = c->G;
     RC(dmalloc(c, &c->d_lat, c->EL));
     RC(dmalloc(c, &c->d_kq, c->s));
+    RC(dmalloc(c, &c->d_ek, c->s));
     RC(dmalloc(c, &c->d_kqh, c->s));
     RC(dmalloc(c, &c->d_tw, c->s));
     RC(dmalloc(c, &c->d_spec, c->F));
@@ -285,6 +286,14 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
     HIPC(hipHostMalloc((void **)&c->h_scal, 256 * sizeof(double), hipHostMallocDefault));
     HIPC(hipMemcpy(c->d_lat, c->lat_h.data(), sizeof(double) * c->EL, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d_kq, kq.data(), sizeof(double) * c->s, hipMemcpyHostToDevice));
+    {  // smoothing factor per axis index: EXP(sigma k^2) with the reference's EXP cut-offs (mpDPFT.h:266)
+        std::vector<double> ek(c->s);
+        for (int q = 0; q < c->s; q++) {
+            double x = c->sigma * kq[q] * kq[q];
+            ek[q] = x < -700. ? 0. : (x < 700. ? std::exp(x) : 1.0142320547350045e+304);
+        }
+        HIPC(hipMemcpy(c->d_ek, ek.data(), sizeof(double) * c->s, hipMemcpyHostToDevice));
+    }
     HIPC(hipMemcpy(c->d_kqh, kqh.data(), sizeof(double) * c->s, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d_tw, tw.data(), sizeof(double2) * c->s, hipMemcpyHostToDevice));
     if (p->n_centres > 0)
@@ -597,9 +606,17 @@ int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method) {
     stage_begin(c);
     int rc = delfield_impl(c, f_dev, s, method);
     if (rc) return rc;
-    // 5 transforms of 6 c F bytes (c = 16) + taper, square-sum pointwise passes
-    double nfft = method == 5 ? 5. : 3.;
-    stage_end(c, "delfield", nfft, nfft * 96. * (double)c->F + 8. * (2. + 4.) * (double)c->G, 0.);
+    // HBM bytes the kernels move: a forward transform from a real grid reads
+    // 8F and writes 16F in its first pass, then 32F per pass; a backward one
+    // to k real grids moves 32F per pass and 16F + 8kG in its last; plus the
+    // taper (shell max 8G, taper 16G) and the square sum (8(D+1)G)
+    const double F = (double)c->F, Gd = (double)c->G, D = (double)c->D;
+    auto fwd = [&]() { return 24. * F + 32. * F * (D - 1.); };
+    auto bwd = [&](double k) { return 32. * F * (D - 1.) + 16. * F + 8. * k * Gd; };
+    double bytes = fwd() + bwd(2.) + bwd(c->D == 3 ? 2. : 1.) + 8. * (D + 1.) * Gd;
+    if (method == 5) bytes += fwd() + bwd(1.);
+    if (c->p.TaperOff && (method == 4 || method == 5)) bytes += 24. * Gd;
+    stage_end(c, "delfield", method == 5 ? 5. : 3., bytes, 0.);
     return MPDPFT_OK;
 }
 static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) {
@@ -622,6 +639,7 @@ static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) 
     mk::MulParams m{};
     m.kq = c->d_kq;
     m.sigma = sig;
+    m.ek = method == 5 ? c->d_ek : nullptr;
     m.kind = mk::MUL_DERIV_PAIR;
     m.axis_a = 0;
     m.axis_b = 1;
@@ -637,6 +655,7 @@ static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) 
         ms.kind = mk::MUL_SMOOTH;
         ms.kq = c->d_kq;
         ms.sigma = sig;
+        ms.ek = c->d_ek;
         HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_spec, ms, g2, nullptr, scale, 1, 0., c->d_tw, c->st));
     }
     return MPDPFT_OK;

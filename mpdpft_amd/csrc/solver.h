@@ -23,6 +23,7 @@ struct mpdpft_ctx {
     std::vector<double> lat_h;  // lattice x_i
     // device buffers
     double *d_lat = nullptr, *d_kq = nullptr, *d_kqh = nullptr;
+    double *d_ek = nullptr;  // EXP(sigma kq^2) per standard index (DelField smoothing)
     double2 *d_tw = nullptr, *d_tw2 = nullptr;
     double2 *d_spec = nullptr, *d_work = nullptr, *d_pad = nullptr;
     double *d_khat = nullptr;
