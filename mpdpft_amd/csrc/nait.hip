@@ -181,7 +181,7 @@ constexpr int kPN = 24001, kNB = 500;
 struct ReuseTab {
     const double2 *pp;  // [kPN - 1 + kNB][6] monomial coefficients (see TabView::pp)
     const double4 *cc;  // CurlyA cubic per knot interval
-    double pw2_1, pw3_1, pw2_3, pw3_3, ginv1, ginv3;
+    double ginv1, ginv3;
 };
 
 __device__ __forceinline__ double curly_a_fast(const ReuseTab &t, double y) {
@@ -517,13 +517,22 @@ constexpr int kReuseMaxPer = 4;  // 4BP + 1 <= 256 samples per half
 __device__ __forceinline__ double wleft(int k, int h4) { return k <= h4 ? boole_w(k, h4 + 1) : 0.; }
 __device__ __forceinline__ double wright(int k, int h4) { return k >= h4 ? boole_w(k - h4, h4 + 1) : 0.; }
 
+// The exact forms of the two acceptance tests near their thresholds, out of
+// line: their constants then stay out of the walk's registers.
+__device__ __noinline__ bool negl_exact(const double *consts, double sm) {
+    return consts[0] - log10(fabs(sm)) > consts[1];  // log10(compare) - log10|sm| > -log10(rtpna)
+}
+__device__ __noinline__ bool relacc_exact(double Int, double prev, double rtpna) {
+    return fabs((Int - prev) / Int) < rtpna;
+}
+
 // fabs((Int - prev) / Int) < rtpna, decided by a product unless it lies
 // within 1e-12 of the threshold (then by the division itself)
 __device__ __forceinline__ bool relacc_ok(double Int, double prev, double rtpna) {
     double d = fabs(Int - prev), m = fabs(Int) * rtpna;
     if (d < m * (1. - 1e-12)) return true;
     if (d > m * (1. + 1e-12)) return false;
-    return fabs((Int - prev) / Int) < rtpna;
+    return relacc_exact(Int, prev, rtpna);
 }
 
 // Four waves per SIMD (at most 128 VGPRs): the walk is latency bound, and the
@@ -539,6 +548,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     double *cur = lds;                            // [8BP + 1] samples of the current node
     double *stk = lds + full + 1;                 // [depth][4BP + 1] pending right halves
     double *stn = stk + (size_t)ps.depth * (h4 + 1);  // [depth][4]: a, b, prev, iter
+    double *cst = stn + 4 * (size_t)ps.depth;         // [2]: log10(compare), -log10(rtpna)
+    if (lane == 0) {
+        cst[0] = ps.log10cmp;
+        cst[1] = ps.maxexp;
+    }
 
     const double U = ps.V[i] - ps.mu;
     const double ar = 0.5 * pow(ps.alpha * 1. * ps.grad2[i], 1. / 3.);
@@ -633,7 +647,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                     else if (ps.compare == 0. || ps.compare < sm) negl = false;
                     else if (fabs(sm) < ps.negl_lo) negl = true;
                     else if (fabs(sm) > ps.negl_hi) negl = false;
-                    else negl = ps.log10cmp - log10(fabs(sm)) > ps.maxexp;
+                    else negl = negl_exact(cst, sm);
                     accept = negl || Delta < kMP;
                 }
                 accept = __builtin_amdgcn_readfirstlane((int)accept) != 0;
@@ -693,10 +707,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         }
     }
     if (lane == 0) {
+        const long long G = gridDim.x;  // one workgroup per grid point
         n[i] = res != res ? 0. : res;
         counts[i] = evals;
-        counts[ps.G + i] = nim;
-        counts[2 * ps.G + i] = nra;
+        counts[G + i] = nim;
+        counts[2 * G + i] = nra;
     }
 }
 
@@ -1117,15 +1132,11 @@ int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {
     ps.maxIter = da.maxIter;
     ps.depth = da.maxIter + 2;
     ps.G = (long long)G;
-    size_t lds = sizeof(double) * ((size_t)(8 * da.BP + 1) + (size_t)ps.depth * (4 * da.BP + 1 + 4));
+    size_t lds = sizeof(double) * ((size_t)(8 * da.BP + 1) + (size_t)ps.depth * (4 * da.BP + 1 + 4) + 2);
     const bool reuse = !c->nait_plain && t->reuse_ok && 4 * da.BP + 1 <= 64 * kReuseMaxPer && lds <= 65536;
     ReuseTab rt;
     rt.pp = t->view.pp;
     rt.cc = t->view.cc;
-    rt.pw2_1 = t->view.pw2[1];
-    rt.pw3_1 = t->view.pw3[1];
-    rt.pw2_3 = t->view.pw2[3];
-    rt.pw3_3 = t->view.pw3[3];
     rt.ginv1 = t->view.ginv[1];
     rt.ginv3 = t->view.ginv[3];
     stage_begin(c);
