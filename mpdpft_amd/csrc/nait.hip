@@ -194,6 +194,10 @@ __device__ __forceinline__ double curly_a_fast(const ReuseTab &t, double y) {
     return c.x + u * (c.y + u * (c.z + u * c.w));
 }
 
+// exp for the far branch (arg < -40), out of line: inlined, its polynomial
+// constants are hoisted into registers the walk needs.
+__device__ __noinline__ double exp_far(double x) { return exp(x); }
+
 // The integrand's Li pair for the panel-reuse kernel.  Below px0 and above
 // 699 the branches of polylog_j; on [px0, pxmax] the quintic Hermite
 // interpolants of the chain table in monomial form; on (pxmax, 699] a second
@@ -206,7 +210,7 @@ __device__ __forceinline__ void polylog_pair_fast(const ReuseTab &t, double x, d
         } else {
             // -e + e^2 2^-s - e^3 3^-s rounds to -e: e <= e^-40, so the two
             // corrections lie below half an ulp of e
-            l1 = l3 = -exp(x);
+            l1 = l3 = -exp_far(x);
         }
         return;
     }
