@@ -65,7 +65,8 @@ def main():
     ap.add_argument("--maxsc", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
-    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes/launch of the nAiT kernel")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "nait_reuse_128_traffic.md"),
+                    help="rocprof summary holding 'traffic_bytes_per_launch = N' for the nAiT kernel at this grid")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU; --gpus %d is not supported (the batched and sharded "
@@ -96,8 +97,10 @@ def main():
         flops_per_launch = nait["flops"] / nait["launches"]
         achieved = flops_per_launch / avg_s / 1e12
         traffic = None
-        if args.traffic and os.path.exists(args.traffic):
-            traffic = json.load(open(args.traffic)).get("nait_density_bytes_per_launch")
+        if args.grid == 128 and args.traffic and os.path.exists(args.traffic):
+            for ln in open(args.traffic):
+                if ln.startswith("traffic_bytes_per_launch"):
+                    traffic = float(ln.split("=")[1])
         roof = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
                 "kernel": "k_nait_reuse", "avg_launch_ms": round(avg_s * 1e3, 4),

@@ -516,6 +516,21 @@ struct PointScalars {
 
 constexpr int kReuseMaxPer = 4;  // 4BP + 1 <= 256 samples per half
 
+// One workgroup per grid point, dispatched round-robin over the 8 XCDs
+// (workgroup w runs on XCD w mod 8).  The 16 points of one 128-byte line of a
+// field go to the same XCD, lines alternate between XCDs: the 8-byte
+// per-point reads and writes then fill whole L2 lines once instead of pulling
+// each line into every L2, and the work stays interleaved across the chip.
+constexpr int kXcds = 8, kLinePts = 16;
+__device__ __forceinline__ long long xcd_point(long long) {
+    const long long w = blockIdx.x, j = w / kXcds;
+    return ((j / kLinePts) * kXcds + w % kXcds) * kLinePts + j % kLinePts;
+}
+inline unsigned xcd_blocks(long long G) {
+    long long lines = (G + kLinePts - 1) / kLinePts, per = (lines + kXcds - 1) / kXcds;
+    return (unsigned)(per * kXcds * kLinePts);
+}
+
 // Boole weight of sample k of a node with 8BP + 1 samples, in its left (k <=
 // 4BP) and right (k >= 4BP) half; 0 outside the half.
 __device__ __forceinline__ double wleft(int k, int h4) { return k <= h4 ? boole_w(k, h4 + 1) : 0.; }
@@ -546,7 +561,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     extern __shared__ double lds[];
     const TabView &t = *tv;  // chain tables and constants for the lower limit and the a_r = 0 form
     const int lane = threadIdx.x;
-    const long long i = blockIdx.x;
+    const long long i = xcd_point(ps.G);
+    if (i >= ps.G) return;
     const int BP = ps.BP, h4 = 4 * BP, full = 8 * BP;
     const int R = (h4 + 64) / 64;                 // rounds over the 4BP + 1 samples of a half
     double *cur = lds;                            // [8BP + 1] samples of the current node
@@ -711,7 +727,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         }
     }
     if (lane == 0) {
-        const long long G = gridDim.x;  // one workgroup per grid point
+        const long long G = ps.G;
         n[i] = res != res ? 0. : res;
         counts[i] = evals;
         counts[G + i] = nim;
@@ -746,6 +762,7 @@ struct EkinArgs {
     double mu, thr, c0, c1, c2, internal_acc, cmp, area_pref, neg_exp;
     int MaxIt;
     int resume;  // continue each point from the refinement the previous launch stopped at
+    long long G;
 };
 
 // Ai on [-300, 100]: the quintic Hermite interpolants of the (Ai, Ai') table
@@ -777,7 +794,8 @@ __device__ __forceinline__ double airy_fast(const double2 *__restrict__ ap, doub
 __global__ __launch_bounds__(64) void k_ekin_ait(const double2 *__restrict__ ap, EkinArgs ea, double *out,
                                                 double *st_v, int *st_m) {
     const int lane = threadIdx.x;
-    const long long i = blockIdx.x;
+    const long long i = xcd_point(ea.G);
+    if (i >= ea.G) return;
     double U = ea.V[i] - ea.mu, a = 0.5 * ea.c0 * pow(fabs(ea.grad2[i]), 1. / 3.);
     double lap = ea.lap[i];
     double val;
@@ -1145,7 +1163,7 @@ int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {
     rt.ginv3 = t->view.ginv[3];
     stage_begin(c);
     if (reuse) {
-        hipLaunchKernelGGL(k_nait_reuse, dim3((unsigned)G), dim3(64), lds, c->st, rt, t->d_view, ps, den_out,
+        hipLaunchKernelGGL(k_nait_reuse, dim3(xcd_blocks((long long)G)), dim3(64), lds, c->st, rt, t->d_view, ps, den_out,
                            t->d_counts);
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(c, e, "k_nait_reuse");
@@ -1219,6 +1237,7 @@ int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E) {
     double prefactor = c->p.degeneracy / (4. * kPI * kPI);
     ea.area_pref = std::pow(c->p.edge, (double)c->D) * prefactor;
     ea.neg_exp = -std::log(c->p.InternalAcc);
+    ea.G = (long long)G;
     double e = 0., prev = 0.;
     int Iteration = 0, MAXIteration = c->MaxIteration + 6;
     if (MAXIteration > 24) return ctx_fail(c, MPDPFT_EINVAL, "EkinAiT03D: MaxIteration above 18");
@@ -1226,7 +1245,7 @@ int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E) {
         ea.MaxIt = c->MaxIteration;
         ea.resume = Iteration > 0;  // MaxIt is one above the previous launch's
         stage_begin(c);
-        hipLaunchKernelGGL(k_ekin_ait, dim3((unsigned)G), dim3(64), 0, c->st, t->d_ap, ea, c->d_tmp1, t->d_ekin_v,
+        hipLaunchKernelGGL(k_ekin_ait, dim3(xcd_blocks((long long)G)), dim3(64), 0, c->st, t->d_ap, ea, c->d_tmp1, t->d_ekin_v,
                            t->d_ekin_m);
         hipError_t he = hipGetLastError();
         if (he != hipSuccess) return hip_fail(c, he, "k_ekin_ait");
