@@ -230,6 +230,32 @@ typedef struct mpdpft_stage_stats {
 int mpdpft_enable_timing(mpdpft_ctx *ctx, int on);
 int mpdpft_get_stage_stats(mpdpft_ctx *ctx, mpdpft_stage_stats *out, int max, int *n);
 
+/* ---- batched solves: many independent instances in one launch ------------ */
+/* The jobs the reference runs as separate runY directories (mpDPFT.sh,
+ * mpDPFTmain.cpp:46-86 once per job), here in one persistent launch: one
+ * workgroup per running instance executes the whole SC loop of GetData
+ * (:210-225) and takes the next instance from the queue when it retires one.
+ * Instances: DensityExpression 7 (TF), DIM 2 or 3, S <= 2, interactions 0/1
+ * (contact), environments 0/1/3, Pulay mixer 0/1 with scope <= 16.  Results are
+ * what mpdpft_solve reports for the same instance (muVec, FinalAbundances,
+ * energies, SCcount, mixing counts). */
+#define MPDPFT_BATCH_FP32 1 /* FP32 fields; FP64 integrals, mu, Pulay solve, energies; then an FP64 polish */
+typedef struct mpdpft_batch mpdpft_batch;
+int mpdpft_batch_create(const mpdpft_params *instances, int n, int device, int flags, mpdpft_batch **out);
+int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *results);
+/* Converged density and potential of instance i ([S][G] host arrays; either may be NULL). */
+int mpdpft_batch_get_fields(mpdpft_batch *b, int i, double *den, double *V);
+double mpdpft_batch_seconds(const mpdpft_batch *b);
+/* Wall seconds and summed iteration counts of the last solve's two phases:
+ * [0] the FP32 loop (zero without MPDPFT_BATCH_FP32), [1] the FP64 loop or polish.
+ * The FP32 loop stops at max|dV| < max(criterion, 1e-6 max|V|) (density mode:
+ * 1e-12) or after 16 iterations without a new smallest CC; the polish then
+ * runs the instance's own criterion, so results[i].SCcount counts polish
+ * iterations only. */
+int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, long long *iterations);
+const char *mpdpft_batch_last_error(const mpdpft_batch *b);
+void mpdpft_batch_destroy(mpdpft_batch *b);
+
 /* ---- drop-in: run in a directory like ./mpDPFT --------------------------- */
 /* Reads dir/mpDPFT.input (and dir/mpDPFT_V.dat when InterpolVQ = 1), polls
  * dir/mpDPFTmanualSCloopBreakQ.dat, solves, writes dir/mpDPFT_Den.dat,

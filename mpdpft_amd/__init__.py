@@ -10,7 +10,7 @@ import numpy as np
 from . import _abi
 from ._abi import Params, Result, load_library
 
-__all__ = ["Params", "Result", "default_params", "parse_input", "Context", "run_dir", "load_library",
+__all__ = ["Params", "Result", "default_params", "parse_input", "Context", "Batch", "run_dir", "load_library",
            "MpdpftError"]
 
 
@@ -140,6 +140,65 @@ class Context:
         N = ctypes.c_double()
         self._check(self._lib.mpdpft_density(self._h, s, mu, _dp(v), _dp(out), ctypes.byref(N)))
         return out, N.value
+
+
+class Batch:
+    """Many independent TF-family solves in one persistent launch
+    (mpdpft_batch_*).  instances: a list of Params; fp32: FP32 field storage
+    with an FP64 polish."""
+
+    def __init__(self, instances, device=-1, fp32=False):
+        self._lib = load_library()
+        self.n = len(instances)
+        self.params = list(instances)
+        arr = (Params * self.n)(*instances)
+        self._h = ctypes.c_void_p()
+        rc = self._lib.mpdpft_batch_create(arr, self.n, device, _abi.BATCH_FP32 if fp32 else 0, ctypes.byref(self._h))
+        if rc != _abi.OK:
+            msg = self._lib.mpdpft_batch_last_error(self._h).decode() if self._h else ""
+            self.close()
+            raise MpdpftError(rc, "batch_create: " + msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mpdpft_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def solve(self):
+        res = (Result * self.n)()
+        rc = self._lib.mpdpft_batch_solve(self._h, res)
+        if rc != _abi.OK:
+            raise MpdpftError(rc, "batch_solve: " + self._lib.mpdpft_batch_last_error(self._h).decode())
+        return list(res)
+
+    def seconds(self):
+        return self._lib.mpdpft_batch_seconds(self._h)
+
+    def phase_stats(self):
+        """((fp32_seconds, fp64_seconds), (fp32_iterations, fp64_iterations)) of the last solve."""
+        sec = (ctypes.c_double * 2)()
+        its = (ctypes.c_longlong * 2)()
+        self._check(self._lib.mpdpft_batch_phase_stats(self._h, sec, its))
+        return (sec[0], sec[1]), (its[0], its[1])
+
+    def _check(self, rc):
+        if rc != _abi.OK:
+            raise MpdpftError(rc, self._lib.mpdpft_batch_last_error(self._h).decode())
+
+    def fields(self, i):
+        p = self.params[i]
+        SG = p.S * (p.steps + 1) ** p.DIM
+        den, V = np.zeros(SG), np.zeros(SG)
+        rc = self._lib.mpdpft_batch_get_fields(self._h, i, _dp(den), _dp(V))
+        if rc != _abi.OK:
+            raise MpdpftError(rc, "batch_get_fields")
+        return den.reshape(p.S, -1), V.reshape(p.S, -1)
 
 
 def import_v(path, params):

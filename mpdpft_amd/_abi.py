@@ -76,7 +76,10 @@ EXPORTS = [
     "mpdpft_set_potential", "mpdpft_fft", "mpdpft_delfield", "mpdpft_interaction_potential",
     "mpdpft_density", "mpdpft_run_dir", "mpdpft_enable_timing", "mpdpft_get_stage_stats",
     "mpdpft_import_v", "mpdpft_set_initial_potential", "mpdpft_set_break_file",
+    "mpdpft_batch_create", "mpdpft_batch_solve", "mpdpft_batch_get_fields", "mpdpft_batch_seconds",
+    "mpdpft_batch_phase_stats", "mpdpft_batch_last_error", "mpdpft_batch_destroy",
 ]
+BATCH_FP32 = 1
 
 _lib = None
 
@@ -135,6 +138,20 @@ def load_library(path=LIB_PATH):
     lib.mpdpft_set_initial_potential.restype = _i
     lib.mpdpft_set_break_file.argtypes = [vp, ctypes.c_char_p]
     lib.mpdpft_set_break_file.restype = _i
+    lib.mpdpft_batch_create.argtypes = [P, _i, _i, _i, ctypes.POINTER(vp)]
+    lib.mpdpft_batch_create.restype = _i
+    lib.mpdpft_batch_solve.argtypes = [vp, R]
+    lib.mpdpft_batch_solve.restype = _i
+    lib.mpdpft_batch_get_fields.argtypes = [vp, _i, dp, dp]
+    lib.mpdpft_batch_get_fields.restype = _i
+    lib.mpdpft_batch_seconds.argtypes = [vp]
+    lib.mpdpft_batch_seconds.restype = ctypes.c_double
+    lib.mpdpft_batch_phase_stats.argtypes = [vp, vp, vp]
+    lib.mpdpft_batch_phase_stats.restype = ctypes.c_int
+    lib.mpdpft_batch_last_error.argtypes = [vp]
+    lib.mpdpft_batch_last_error.restype = ctypes.c_char_p
+    lib.mpdpft_batch_destroy.argtypes = [vp]
+    lib.mpdpft_batch_destroy.restype = None
     if lib.mpdpft_params_sizeof() != ctypes.sizeof(Params):
         raise OSError("mpdpft_params layout mismatch: C %d vs ctypes %d"
                       % (lib.mpdpft_params_sizeof(), ctypes.sizeof(Params)))

@@ -3,7 +3,10 @@
 C0: 2D two-species TF + contact gas (N = 100 each, 128^2 on [-16,16]^2).
 C1: 3D single-species electrons, softened Coulomb Z = 10, a = 0.5, N = 10,
     Hartree (isolated, zero-padded 256^3) + Dirac + PW92, 128^3 on a 20 bohr cube.
+C3: seeded batches of 2D two-species TF + contact gases in anisotropic traps.
 """
+import math
+
 from . import default_params
 
 # Names of the 47 positional lines of mpDPFT.input, in order.
@@ -123,3 +126,49 @@ def c1(steps=128, edge=20.0, Z=10.0, a=0.5, N=10.0, density_expression=7, hartre
     p.SmoothingKWidth = kwidth
     p.Print = 0
     return p
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea & Flood 2014): u = (x >> 11) 2^-53."""
+
+    def __init__(self, seed):
+        self.state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * 2.0 ** -53
+
+    def integer(self, a, b):
+        return a + int(self.uniform() * (b - a + 1))
+
+
+def c3(n=512, seed=20260101, grids=(64, 128, 256), tol=1e-8, maxsc=5000):
+    """Batched 2D two-species gases: per instance, in this order, N ~ U{20..400}
+    (both species), lambda ~ U[1,2], g ~ U[0.2,2.0], grid 64/128/256 with
+    probabilities 0.5/0.3/0.2 (grids[0..2]), box 2.5 sqrt(2N); trap
+    (x^2 + lambda^2 y^2)/2 as environment 3; TF + contact g; Pulay history 8,
+    theta 0.1, converged when max|dV| < tol."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        N = rng.integer(20, 400)
+        lam = 1.0 + rng.uniform()
+        g = 0.2 + 1.8 * rng.uniform()
+        u = rng.uniform()
+        steps = grids[0] if u < 0.5 else (grids[1] if u < 0.8 else grids[2])
+        p = c0(steps=steps, edge=2.5 * math.sqrt(2.0 * N), N=float(N), g=g, tol=tol, maxsc=maxsc)
+        e = (lam * lam - 1.0) / (lam * lam + 1.0)
+        c = (1.0 + lam * lam) / 4.0
+        p.n_mpp = 4
+        _set(p.mpp, [c, c, e, e])
+        _set(p.Environments, [3, 3])
+        _set(p.Mixer, [1.0, 8.0, 0.0])
+        _set(p.thetaVec, [0.1, 0.1])
+        out.append(p)
+    return out
