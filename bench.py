@@ -15,6 +15,13 @@ roofline of the dominant kernel (the nAiT density kernel, FP64) timed with
 HIP events on the solver's stream, the HBM rates of the FFT stages, and a
 CPU baseline from the oracle on a bounded sample.
 
+The "batch" object adds the batched workload: 512 seeded 2D two-species TF +
+contact gases (grids 64^2/128^2/256^2, seed 20260101) solved in one persistent
+launch, in FP64 and as FP32 with an FP64 polish, against the plain FP64 GPU
+loop solving the same instances one after another; the polish results are
+compared with the FP64 results (energy, density, mu) and, on a sample, with
+the oracle, which is also timed there as the CPU baseline of that workload.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128]
 """
 import argparse
@@ -56,6 +63,83 @@ def cpu_baseline(p, mu, V, density_evals, budget_s):
     }
 
 
+def batch_leg(n, plain_n, cpu_n, cpu_on):
+    """C3: the batched solver (FP64; FP32 + FP64 polish) against the plain FP64
+    loop on the same instances; errors of the polished results."""
+    import numpy as np
+    from mpdpft_amd import Batch, Context, configs
+    insts = configs.c3(n=n)
+    Batch(insts[:4]).solve()  # loads the kernels
+    b64 = Batch(insts)
+    r64 = b64.solve()
+    t64 = b64.seconds()
+    f64 = [b64.fields(i) for i in range(n)]
+    b64.close()
+    b32 = Batch(insts, fp32=True)
+    r32 = b32.solve()
+    t32 = b32.seconds()
+    (ph_s, ph_it, ph_bis) = b32.phase_stats()
+    f32 = [b32.fields(i) for i in range(n)]
+    b32.close()
+    dE = max(abs(a.TotalEnergy - b.TotalEnergy) / abs(b.TotalEnergy) for a, b in zip(r32, r64))
+    dmu = max(abs(a.muVec[s] - b.muVec[s]) / abs(b.muVec[s]) for a, b in zip(r32, r64) for s in range(a.S))
+    dn = max(float(np.max(np.abs(a[0] - b[0])) / np.max(np.abs(b[0]))) for a, b in zip(f32, f64))
+    plain_n = min(plain_n, n)
+    t_plain = 0.0
+    plain_match = 0
+    for i in range(plain_n):
+        ctx = Context(insts[i])
+        t0 = time.perf_counter()
+        r = ctx.solve()
+        t_plain += time.perf_counter() - t0
+        ctx.close()
+        plain_match += int(abs(r.TotalEnergy - r64[i].TotalEnergy) <= 1e-10 * abs(r.TotalEnergy)
+                           and r.SCcount == r64[i].SCcount)
+    plain_rate = plain_n / t_plain
+    out = {
+        "workload": "C3: %d 2D two-species TF + contact gases, seed 20260101, grids 64/128/256 (p 0.5/0.3/0.2), "
+                    "box 2.5 sqrt(2N), Pulay 8, theta 0.1, max|dV|<1e-8" % n,
+        "instances": n,
+        "grid_points": int(sum(p.S * (p.steps + 1) ** 2 for p in insts)),
+        "fp64_batch": {"seconds": round(t64, 4), "solves_per_s": round(n / t64, 2),
+                       "converged": sum(r.converged for r in r64),
+                       "sc_iterations": int(sum(r.SCcount for r in r64))},
+        "fp32_polish_batch": {"seconds": round(t32, 4), "solves_per_s": round(n / t32, 2),
+                              "converged": sum(r.converged for r in r32),
+                              "fp32_seconds": round(ph_s[0], 4), "polish_seconds": round(ph_s[1], 4),
+                              "fp32_iterations": int(ph_it[0]), "polish_iterations": int(ph_it[1]),
+                              "fp32_bisection_steps": int(ph_bis[0]), "polish_bisection_steps": int(ph_bis[1]),
+                              "max_rel_dE_vs_fp64": dE, "max_rel_dmu_vs_fp64": dmu,
+                              "max_dn_over_max_n_vs_fp64": dn},
+        "plain_fp64_loop": {"instances": plain_n, "seconds": round(t_plain, 4),
+                            "solves_per_s": round(plain_rate, 2),
+                            "identical_to_fp64_batch": plain_match},
+        "speedup_fp32_polish_vs_plain_fp64": round((n / t32) / plain_rate, 2),
+        "speedup_fp64_batch_vs_plain_fp64": round((n / t64) / plain_rate, 2),
+    }
+    if cpu_on:
+        from oracle import oracle as O
+        O.lib()
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t_cpu = 0.0
+        eE = en = emu = 0.0
+        cpu_n = min(cpu_n, n)
+        for i in range(cpu_n):
+            t0 = time.perf_counter()
+            _, r, den, _ = O.solve(insts[i])
+            t_cpu += time.perf_counter() - t0
+            eE = max(eE, abs(r32[i].TotalEnergy - r.TotalEnergy) / abs(r.TotalEnergy))
+            emu = max(emu, max(abs(r32[i].muVec[s] - r.muVec[s]) / abs(r.muVec[s]) for s in range(r.S)))
+            en = max(en, float(np.max(np.abs(f32[i][0] - den)) / np.max(np.abs(den))))
+        out["fp32_polish_batch"].update({"max_rel_dE_vs_oracle": eE, "max_rel_dmu_vs_oracle": emu,
+                                         "max_dn_over_max_n_vs_oracle": en, "oracle_instances": cpu_n})
+        out["cpu_baseline"] = {"value": round(cpu_n / t_cpu, 3), "unit": "solves/s", "cores": cores,
+                               "kind": "port",
+                               "sample": "the oracle's SC loop on the first %d instances, one after another "
+                                         "(OpenMP over %d threads): %.2f s" % (cpu_n, cores, t_cpu)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +149,10 @@ def main():
     ap.add_argument("--maxsc", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--batch", type=int, default=512, help="instances of the batched workload (0: skip)")
+    ap.add_argument("--batch-plain", type=int, default=512,
+                    help="instances the plain FP64 loop solves for the batch comparison")
+    ap.add_argument("--batch-cpu", type=int, default=16, help="instances the oracle solves for the batch baseline")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "nait_reuse_128_traffic.md"),
                     help="rocprof summary holding 'traffic_bytes_per_launch = N' for the nAiT kernel at this grid")
     args = ap.parse_args()
@@ -126,6 +214,12 @@ def main():
         except Exception as e:  # noqa: BLE001
             cpu = {"value": None, "error": str(e)}
 
+    batch = None
+    if args.batch > 0:
+        os.environ["MPDPFT_VERBOSE"] = "0"  # 512 plain solves: no per-iteration lines
+        print("bench: batched workload, %d instances" % args.batch, file=sys.stderr, flush=True)
+        batch = batch_leg(args.batch, args.batch_plain, args.batch_cpu, not args.no_cpu_baseline)
+
     line = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -148,6 +242,7 @@ def main():
         "cpu_baseline": cpu,
         "stages_hbm": hbm,
         "stage_ms": stages,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

@@ -246,13 +246,16 @@ int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *results);
 /* Converged density and potential of instance i ([S][G] host arrays; either may be NULL). */
 int mpdpft_batch_get_fields(mpdpft_batch *b, int i, double *den, double *V);
 double mpdpft_batch_seconds(const mpdpft_batch *b);
-/* Wall seconds and summed iteration counts of the last solve's two phases:
+/* Wall seconds, summed iteration counts and summed mu-bisection density
+ * evaluations of the last solve's two phases (each argument may be NULL):
  * [0] the FP32 loop (zero without MPDPFT_BATCH_FP32), [1] the FP64 loop or polish.
  * The FP32 loop stops at max|dV| < max(criterion, 1e-6 max|V|) (density mode:
- * 1e-12) or after 16 iterations without a new smallest CC; the polish then
+ * 1e-12) or after 16 iterations without a new smallest CC, and its mu search
+ * at |N - N0| / 2N0 <= max(RelAcc, 1e-8); the polish then
  * runs the instance's own criterion, so results[i].SCcount counts polish
  * iterations only. */
-int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, long long *iterations);
+int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, long long *iterations,
+                             long long *bisection_steps);
 const char *mpdpft_batch_last_error(const mpdpft_batch *b);
 void mpdpft_batch_destroy(mpdpft_batch *b);
 

@@ -181,11 +181,13 @@ class Batch:
         return self._lib.mpdpft_batch_seconds(self._h)
 
     def phase_stats(self):
-        """((fp32_seconds, fp64_seconds), (fp32_iterations, fp64_iterations)) of the last solve."""
+        """((fp32_seconds, fp64_seconds), (fp32_iterations, fp64_iterations),
+        (fp32_bisection_steps, fp64_bisection_steps)) of the last solve."""
         sec = (ctypes.c_double * 2)()
         its = (ctypes.c_longlong * 2)()
-        self._check(self._lib.mpdpft_batch_phase_stats(self._h, sec, its))
-        return (sec[0], sec[1]), (its[0], its[1])
+        bis = (ctypes.c_longlong * 2)()
+        self._check(self._lib.mpdpft_batch_phase_stats(self._h, sec, its, bis))
+        return (sec[0], sec[1]), (its[0], its[1]), (bis[0], bis[1])
 
     def _check(self, rc):
         if rc != _abi.OK:

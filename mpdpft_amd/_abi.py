@@ -146,7 +146,7 @@ def load_library(path=LIB_PATH):
     lib.mpdpft_batch_get_fields.restype = _i
     lib.mpdpft_batch_seconds.argtypes = [vp]
     lib.mpdpft_batch_seconds.restype = ctypes.c_double
-    lib.mpdpft_batch_phase_stats.argtypes = [vp, vp, vp]
+    lib.mpdpft_batch_phase_stats.argtypes = [vp, vp, vp, vp]
     lib.mpdpft_batch_phase_stats.restype = ctypes.c_int
     lib.mpdpft_batch_last_error.argtypes = [vp]
     lib.mpdpft_batch_last_error.restype = ctypes.c_char_p

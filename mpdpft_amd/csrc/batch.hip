@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kBT = 256;        // threads per instance workgroup
 constexpr int kMaxScope = 16;   // Pulay history (entries) in the batch
-constexpr int kBS = 2;          // species per instance
+constexpr int kBS = 2;          // species per instance (the species of a field index is idx >= G)
 constexpr double kMPb = 1.0e-12;
 constexpr int kWork = 5 * (kMaxScope + 1) * (kMaxScope + 1);  // >= row + A + qr + inv + vectors + coef + flag
 
@@ -66,6 +66,7 @@ template <typename T> struct BArgs {
     const double *start;  // output arena holding the start densities/potentials (polish)
     double floor_rel;     // > 0: stop criterion floor (max|dV| relative to max|V|; its square for density mode)
     int stall_max;        // > 0: leave the loop after this many iterations without a new smallest CC
+    double relacc_floor;  // lower bound of the mu search's relative particle-number accuracy
 };
 
 // ---- block primitives (deterministic, result broadcast to all threads) ----
@@ -92,6 +93,28 @@ __device__ __forceinline__ double bmax(double v, double *sh) {
     return sh[0];
 }
 
+// Visit the points i = threadIdx.x + k kBT < n in increasing k, kU at a time:
+// the kU loads of a step are issued before their values are used, so a lane
+// keeps kU loads in flight (one workgroup per instance is latency bound
+// otherwise).  Per-lane accumulation order is that of the plain strided loop.
+constexpr int kU = 8;
+template <class Ld, class Use>
+__device__ __forceinline__ void stream(long long n, Ld ld, Use use) {
+    long long i = threadIdx.x;
+    for (; i + (kU - 1) * kBT < n; i += kU * kBT) {
+        decltype(ld(0LL)) x[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) x[u] = ld(i + u * kBT);
+#pragma unroll
+        for (int u = 0; u < kU; u++) use(i + u * kBT, x[u]);
+    }
+    for (; i < n; i += kBT) use(i, ld(i));
+}
+
+struct Pair {
+    double a, b;
+};
+
 __device__ __forceinline__ double dEXP(double x) {  // EXP, mpDPFT.h:266
     if (x < -700.) return 0.;
     if (x < 700.) return exp(x);
@@ -99,26 +122,28 @@ __device__ __forceinline__ double dEXP(double x) {  // EXP, mpDPFT.h:266
 }
 __device__ __forceinline__ double dSign(double x) { return x < 0. ? -1. : 1.; }
 
+// Grid indices fit 32 bits (G <= 257^3 < 2^32): 32-bit division only.
 __device__ __forceinline__ double trapw(const BInst &in, long long i) {
     double w = 1.;
-    long long rem = i;
+    unsigned rem = (unsigned)i;
+    const unsigned EL = (unsigned)in.EL;
     for (int d = 0; d < in.D; d++) {
-        long long c = rem % in.EL;
-        rem /= in.EL;
-        if (c == 0 || c == in.EL - 1) w *= 0.5;
+        unsigned q = rem / EL, c = rem - q * EL;
+        rem = q;
+        if (c == 0 || c == EL - 1) w *= 0.5;
     }
     return w;
 }
 
 __device__ __forceinline__ void coords(const BInst &in, long long i, double *r) {
+    const unsigned EL = (unsigned)in.EL, u = (unsigned)i;
     if (in.D == 2) {
-        long long a = i / in.EL, b = i - a * in.EL;
+        unsigned a = u / EL, b = u - a * EL;
         r[0] = in.lat0 + in.del * (double)a;
         r[1] = in.lat0 + in.del * (double)b;
         r[2] = 0.;
     } else {
-        long long a = i / ((long long)in.EL * in.EL), rem = i - a * (long long)in.EL * in.EL, b = rem / in.EL,
-                  c = rem - b * in.EL;
+        unsigned a = u / (EL * EL), rem = u - a * EL * EL, b = rem / EL, c = rem - b * EL;
         r[0] = in.lat0 + in.del * (double)a;
         r[1] = in.lat0 + in.del * (double)b;
         r[2] = in.lat0 + in.del * (double)c;
@@ -140,26 +165,35 @@ template <typename T> struct Inst {
     double *W;                                     // LDS work area [kWork] (Gram row, matrix, QR, coefficients)
     double floor_rel;                              // reduced-precision stop floor (0: the criterion as given)
     int stall_max;                                 // reduced-precision stall exit (0: none)
+    double relacc;                                 // particle-number accuracy of the mu search
     __device__ long long SG() const { return (long long)in.S * in.G; }
 
     // N_s(mu) of the TF density (GetTFdensity :4415-4456 + Riemann :7981-8026)
     __device__ double evalN(int s, double mu) {
         const T *v = V + (long long)s * in.G;
+        const double pref = in.pref;
+        const bool d2 = in.D == 2;
         double acc = 0.;
-        for (long long i = threadIdx.x; i < in.G; i += kBT) {
-            double t = mu - (double)v[i];
-            double val = t > 0. ? (in.D == 2 ? in.pref * t : in.pref * t * sqrt(t)) : 0.;
-            acc += trapw(in, i) * val;
-        }
+        stream(
+            in.G, [&](long long i) { return (double)v[i]; },
+            [&](long long i, double vi) {
+                double t = mu - vi;
+                double val = t > 0. ? (d2 ? pref * t : pref * t * sqrt(t)) : 0.;
+                acc += trapw(in, i) * val;
+            });
         return bsum(acc, sh) * in.trapD;
     }
     __device__ void writeDen(int s, double mu) {
         T *n = Den + (long long)s * in.G;
         const T *v = V + (long long)s * in.G;
-        for (long long i = threadIdx.x; i < in.G; i += kBT) {
-            double t = mu - (double)v[i];
-            n[i] = (T)(t > 0. ? (in.D == 2 ? in.pref * t : in.pref * t * sqrt(t)) : 0.);
-        }
+        const double pref = in.pref;
+        const bool d2 = in.D == 2;
+        stream(
+            in.G, [&](long long i) { return (double)v[i]; },
+            [&](long long i, double vi) {
+                double t = mu - vi;
+                n[i] = (T)(t > 0. ? (d2 ? pref * t : pref * t * sqrt(t)) : 0.);
+            });
     }
     __device__ double N_at(Loop &L, int s, double mu) {
         L.muEval[s] = mu;
@@ -211,7 +245,7 @@ template <typename T> struct Inst {
                     n++;
                     if (n == 1) L.DMM -= dSign(fabs(L.TmpAb[s] - target) - d0) * 0.1 * fabs(L.DMM);
                     lo[s] = L.mu[s];
-                    if (L.RelAcc < kMPb) L.RelAcc = in.RelAcc;
+                    if (L.RelAcc < kMPb) L.RelAcc = relacc;
                 }
                 if (L.TmpAb[s] <= target) loF[s] = true;
                 n = 0;
@@ -236,7 +270,7 @@ template <typename T> struct Inst {
                         n++;
                         if (n == 1) L.DMM -= dSign(fabs(L.TmpAb[s] - target) - d0) * 0.1 * fabs(L.DMM);
                         hi[s] = L.mu[s];
-                        if (L.RelAcc < kMPb) L.RelAcc = in.RelAcc;
+                        if (L.RelAcc < kMPb) L.RelAcc = relacc;
                     }
                 }
                 if (L.TmpAb[s] >= target) hiF[s] = true;
@@ -253,7 +287,7 @@ template <typename T> struct Inst {
                 }
                 for (int s = 0; s < S; s++) {
                     const double target = in.Ab[s];
-                    if (!(fabs(L.TmpAb[s] - target) / fabs(2. * target) > in.RelAcc)) continue;
+                    if (!(fabs(L.TmpAb[s] - target) / fabs(2. * target) > relacc)) continue;
                     // the last five positive N values (hist5[4] the newest) and their count
                     double hist5[5] = {0., 0., 0., 0., 0.};
                     int nh = 0;
@@ -274,14 +308,14 @@ template <typename T> struct Inst {
                         if (nh > 5)
                             for (int h = 0; h < 3 && !trapped; h++)
                                 for (int hp = h + 1; hp < 4; hp++)
-                                    if (fabs(hist5[h] - hist5[hp]) < in.RelAcc * target) {
+                                    if (fabs(hist5[h] - hist5[hp]) < relacc * target) {
                                         trapped = true;
                                         L.TrappedQ = 1;
                                         break;
                                     }
                         if (trapped) break;
                         for (int sg = -1; sg < 2; sg += 2) {
-                            if ((double)sg * (L.TmpAb[s] - target) / fabs(2. * target) > in.RelAcc) {
+                            if ((double)sg * (L.TmpAb[s] - target) / fabs(2. * target) > relacc) {
                                 prev[s] = L.mu[s];
                                 L.mu[s] -= (double)sg * 0.5 * step[s];
                                 if (fabs(L.mu[s]) > 1.0e+20) {
@@ -300,7 +334,7 @@ template <typename T> struct Inst {
                         }
                     }
                 }
-                L.RelAcc = in.RelAcc;
+                L.RelAcc = relacc;
             }
         }
         for (int s = 0; s < S; s++)
@@ -319,20 +353,23 @@ template <typename T> struct Inst {
     // UpdateV (:4815-4838): V_s = incV (beta sum_{s' != s} n_s') + Env_s from
     // the mixed density (OldDen after the swap)
     __device__ void updateV(Loop &L) {
-        const long long SGn = SG();
-        for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-            int s = (int)(idx / in.G);
-            long long i = idx - (long long)s * in.G;
-            double acc = 0.;
-            if (in.contact) {
-                double v = 0.;
-                for (int sp = 0; sp < in.S; sp++)
-                    if (sp != s) v += (double)OldDen[(long long)sp * in.G + i];
-                acc += v * in.beta;
-            }
-            if (L.incV < 1.) acc *= L.incV;
-            V[idx] = (T)(acc + (double)Env[idx]);
-        }
+        const long long SGn = SG(), G = in.G;
+        const bool pair = in.contact && in.S == 2;  // S <= 2: the other species' density
+        const double beta = in.beta, incV = L.incV;
+        stream(
+            SGn,
+            [&](long long idx) {
+                Pair x;
+                x.a = (double)Env[idx];
+                x.b = pair ? (double)OldDen[idx >= G ? idx - G : idx + G] : 0.;
+                return x;
+            },
+            [&](long long idx, Pair x) {
+                double acc = 0.;
+                if (in.contact) acc += x.b * beta;
+                if (incV < 1.) acc *= incV;
+                V[idx] = (T)(acc + x.a);
+            });
         __syncthreads();
         if (L.SCcount > 0) L.incV += in.incV0;
         L.incV = fmin(1., L.incV);
@@ -348,10 +385,18 @@ template <typename T> struct Inst {
             double crit = in.sccrit;
             if (in.convmode == 1) {
                 double m = 0., vm = 0.;
-                for (long long idx = threadIdx.x; idx < SG(); idx += kBT) {
-                    m = fmax(m, fabs((double)V[idx] - (double)OldV[idx]));
-                    vm = fmax(vm, fabs((double)V[idx]));
-                }
+                stream(
+                    SG(),
+                    [&](long long idx) {
+                        Pair x;
+                        x.a = (double)V[idx];
+                        x.b = (double)OldV[idx];
+                        return x;
+                    },
+                    [&](long long, Pair x) {
+                        m = fmax(m, fabs(x.a - x.b));
+                        vm = fmax(vm, fabs(x.a));
+                    });
                 L.CC = bmax(m, sh);
                 if (floor_rel > 0.) crit = fmax(crit, floor_rel * bmax(vm, sh));
             } else {
@@ -359,12 +404,19 @@ template <typename T> struct Inst {
                 for (int s = 0; s < in.S; s++) {
                     const T *a = Den + (long long)s * in.G, *b = OldDen + (long long)s * in.G;
                     double aa = 0., bb = 0., ab = 0.;
-                    for (long long i = threadIdx.x; i < in.G; i += kBT) {
-                        double x = (double)a[i], y = (double)b[i];
-                        aa += x * x;
-                        bb += y * y;
-                        ab += x * y;
-                    }
+                    stream(
+                        in.G,
+                        [&](long long i) {
+                            Pair x;
+                            x.a = (double)a[i];
+                            x.b = (double)b[i];
+                            return x;
+                        },
+                        [&](long long, Pair p) {
+                            aa += p.a * p.a;
+                            bb += p.b * p.b;
+                            ab += p.a * p.b;
+                        });
                     aa = bsum(aa, sh);
                     bb = bsum(bb, sh);
                     ab = bsum(ab, sh);
@@ -412,11 +464,18 @@ template <typename T> struct Inst {
         {  // push n_in = OldDen, R = Den - OldDen into logical entry I
             int ps = (head + I) % nslot;
             T *a = nin + (long long)ps * SGn, *r = res + (long long)ps * SGn;
-            for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-                double x = (double)OldDen[idx], y = (double)Den[idx];
-                a[idx] = (T)x;
-                r[idx] = (T)(y - x);
-            }
+            stream(
+                SGn,
+                [&](long long idx) {
+                    Pair x;
+                    x.a = (double)OldDen[idx];
+                    x.b = (double)Den[idx];
+                    return x;
+                },
+                [&](long long idx, Pair x) {
+                    a[idx] = (T)x.a;
+                    r[idx] = (T)(x.b - x.a);
+                });
         }
         I++;
         int shift = 0;
@@ -433,7 +492,15 @@ template <typename T> struct Inst {
         for (int j = 0; j < I; j++) {
             const T *rj = res + (long long)((head + j) % nslot) * SGn;
             double acc = 0.;
-            for (long long idx = threadIdx.x; idx < SGn; idx += kBT) acc += (double)rn[idx] * (double)rj[idx];
+            stream(
+                SGn,
+                [&](long long idx) {
+                    Pair x;
+                    x.a = (double)rn[idx];
+                    x.b = (double)rj[idx];
+                    return x;
+                },
+                [&](long long, Pair x) { acc += x.a * x.b; });
             acc = bsum(acc, sh);
             if (threadIdx.x == 0) row[j] = acc;
         }
@@ -595,23 +662,37 @@ template <typename T> struct Inst {
         const long long SGn = SG();
         if (ok) {
             L.PulMix++;
-            const int nslot = in.scope + 1;
+            const int nslot = in.scope + 1, pI = L.pI;
+            const double th0 = L.theta[0], th1 = L.theta[kBS - 1];
+            const long long G = in.G;
             for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-                int s = (int)(idx / in.G);
-                double th = L.theta[s], acc = 0.;
-                for (int p = 0; p < L.pI; p++) {
-                    long long off = (long long)((head + p) % nslot) * SGn + idx;
+                const double th = idx >= G ? th1 : th0;  // S <= 2
+                double acc = 0.;
+                int slot = head;  // physical slot of logical entry p
+#pragma unroll 4
+                for (int p = 0; p < pI; p++) {
+                    const long long off = (long long)slot * SGn + idx;
                     acc += coef[p] * ((double)nin[off] + th * (double)res[off]);
+                    slot = slot + 1 == nslot ? 0 : slot + 1;
                 }
                 Den[idx] = (T)acc;
             }
         } else {
             L.LinMix++;
-            for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-                int s = (int)(idx / in.G);
-                double th = L.theta[s];
-                Den[idx] = (T)(th * (double)Den[idx] + (1. - th) * (double)OldDen[idx]);
-            }
+            const double th0 = L.theta[0], th1 = L.theta[kBS - 1];
+            const long long G = in.G;
+            stream(
+                SGn,
+                [&](long long idx) {
+                    Pair x;
+                    x.a = (double)Den[idx];
+                    x.b = (double)OldDen[idx];
+                    return x;
+                },
+                [&](long long idx, Pair x) {
+                    const double th = idx >= G ? th1 : th0;  // S <= 2
+                    Den[idx] = (T)(th * x.a + (1. - th) * x.b);
+                });
         }
         __syncthreads();
         if (L.TrappedQ)
@@ -648,10 +729,11 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
         I.W = Work;
         I.floor_rel = a.floor_rel;
         I.stall_max = a.stall_max;
+        I.relacc = fmax(in.RelAcc, a.relacc_floor);
         double *outDen = a.fields + in.out_off, *outV = outDen + SGn;
         // InitializeEnvironments (:6362-6390): Env, and V <- Env (or the stored start)
         for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-            int s = (int)(idx / in.G);
+            int s = idx >= in.G ? 1 : 0;  // S <= 2
             long long i = idx - (long long)s * in.G;
             double r[3];
             coords(in, i, r);
@@ -670,7 +752,7 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             L.muEval[s] = L.mu[s];
         }
         L.DMM = a.polish ? a.out[k].DMM : in.DMM0;
-        L.RelAcc = in.RelAcc;
+        L.RelAcc = I.relacc;
         L.incV = in.incV0;
         L.CC = L.OldCC = 0.;
         L.best = 1.0e+300;
@@ -771,7 +853,7 @@ struct mpdpft_batch {
     hipStream_t st = nullptr;
     std::string err;
     double seconds = 0., phase_seconds[2] = {0., 0.};
-    long long phase_iterations[2] = {0, 0};
+    long long phase_iterations[2] = {0, 0}, phase_evaluations[2] = {0, 0};
 };
 
 static int bfail(mpdpft_batch *b, int rc, const std::string &m) {
@@ -901,9 +983,12 @@ extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, in
 // Reduced-precision stop: FP32 fields hold V to a few ulp of max|V|, so the
 // FP32 loop stops at max|dV| < 1e-6 max|V| (density mode: 1e-12) or after 16
 // iterations without progress; the FP64 polish then meets the instance's own
-// criterion.
+// criterion.  The FP32 loop's mu search stops at |N - N0| / 2N0 <= 1e-8: the
+// rounding of the stored FP32 densities moves N by more than the usual 1e-11
+// every iteration, which would cost a full bisection each time.
 constexpr double kF32Floor = 1.0e-6;
 constexpr int kF32Stall = 16;
+constexpr double kF32RelAcc = 1.0e-8;
 
 template <typename T> static int run(mpdpft_batch *b, int polish) {
     const long long per = (5LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
@@ -920,6 +1005,7 @@ template <typename T> static int run(mpdpft_batch *b, int polish) {
     a.start = b->d_start;
     a.floor_rel = sizeof(T) == 4 ? kF32Floor : 0.;
     a.stall_max = sizeof(T) == 4 ? kF32Stall : 0;
+    a.relacc_floor = sizeof(T) == 4 ? kF32RelAcc : 0.;
     BHIP(hipMemsetAsync(b->d_next, 0, sizeof(int), b->st));
     hipLaunchKernelGGL(k_batch_solve<T>, dim3(b->nslots), dim3(kBT), 0, b->st, a);
     BHIP(hipGetLastError());
@@ -934,9 +1020,13 @@ extern "C" int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *res) {
     const bool fp32 = (b->flags & MPDPFT_BATCH_FP32) != 0;
     b->phase_seconds[0] = b->phase_seconds[1] = 0.;
     b->phase_iterations[0] = b->phase_iterations[1] = 0;
+    b->phase_evaluations[0] = b->phase_evaluations[1] = 0;
     auto count = [&](int ph) -> int {  // iterations of the phase just run (host copy of the results)
         BHIP(hipMemcpy(b->out.data(), b->d_out, sizeof(BOut) * b->n, hipMemcpyDeviceToHost));
-        for (int i = 0; i < b->n; i++) b->phase_iterations[ph] += b->out[i].SCcount;
+        for (int i = 0; i < b->n; i++) {
+            b->phase_iterations[ph] += b->out[i].SCcount;
+            b->phase_evaluations[ph] += b->out[i].bisection;
+        }
         return MPDPFT_OK;
     };
     if (fp32) {
@@ -1002,11 +1092,13 @@ extern "C" int mpdpft_batch_get_fields(mpdpft_batch *b, int i, double *den, doub
 
 extern "C" double mpdpft_batch_seconds(const mpdpft_batch *b) { return b ? b->seconds : 0.; }
 
-extern "C" int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, long long *iterations) {
+extern "C" int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, long long *iterations,
+                                        long long *bisection_steps) {
     if (!b) return MPDPFT_EINVAL;
     for (int ph = 0; ph < 2; ph++) {
         if (seconds) seconds[ph] = b->phase_seconds[ph];
         if (iterations) iterations[ph] = b->phase_iterations[ph];
+        if (bisection_steps) bisection_steps[ph] = b->phase_evaluations[ph];
     }
     return MPDPFT_OK;
 }
