@@ -230,6 +230,39 @@ typedef struct mpdpft_stage_stats {
 int mpdpft_enable_timing(mpdpft_ctx *ctx, int on);
 int mpdpft_get_stage_stats(mpdpft_ctx *ctx, mpdpft_stage_stats *out, int max, int *n);
 
+/* ---- derivatives of the converged solution -------------------------------- */
+/* No counterpart in the reference, which differentiates by finite differences
+ * over whole solves (its task loops, mpDPFTmain.cpp:77-214, mpDPFT.cpp:8365-8584).
+ *
+ * mpdpft_adjoint: for a loss L(n) of the density of the last converged solve,
+ * given dL/dn ([G], one species, plain sum over grid points), returns
+ * lambda[i] = dL/dEnv_i, the derivative with respect to an additive change of
+ * the external potential at grid point i, with the particle number held
+ * fixed.  It linearises the fixed point V = Env + K[n], n = F(V, mu),
+ * sum w n = N (implicit differentiation, no unrolled iterations) and solves
+ * (I - F_V^T P K'^T) lambda = F_V^T P dL/dn by restarted GMRES on the device.
+ * Supported: DIM 3, S = 1, DensityExpression 7 (TF) or 5 (nAiT, DelFieldMethod
+ * 4/5 with its taper), interactions 8 (HartreeMode 1), 10, 11, -1, -12.
+ * info (may be NULL): tol / max_iterations in (0: 1e-12 / 400), the rest out. */
+typedef struct mpdpft_adjoint_info {
+    double tol;               /* in: relative residual target of the linear solve */
+    int max_iterations;       /* in: GMRES iteration cap */
+    int iterations;           /* out */
+    double relative_residual; /* out */
+    double dN_dmu;            /* out: -sum w dn/dU, the particle-number response to mu */
+} mpdpft_adjoint_info;
+int mpdpft_adjoint(mpdpft_ctx *ctx, const double *dL_dn, double *lambda, mpdpft_adjoint_info *info);
+/* Environment 41: out[3c + d] = sum_i lambda[i] dEnv_i/dR_{c,d} (chain rule of
+ * mpdpft_adjoint's lambda to the centre coordinates). */
+int mpdpft_centre_gradient(mpdpft_ctx *ctx, const double *lambda, double *out);
+/* Environment 41: Hellmann-Feynman derivative of the total energy with respect
+ * to the centre coordinates, out[3c + d] = sum_i w_i dx^3 n_i dEnv_i/dR_{c,d}
+ * (the loop's energy is stationary in V and n at self-consistency). */
+int mpdpft_centre_forces(mpdpft_ctx *ctx, double *out);
+/* Replace the centres list (n x {x, y, z, Z, a}) and rebuild Env of the
+ * environment-41 species; the next solve starts from the new Env. */
+int mpdpft_set_centres(mpdpft_ctx *ctx, int n, const double *centres);
+
 /* ---- batched solves: many independent instances in one launch ------------ */
 /* The jobs the reference runs as separate runY directories (mpDPFT.sh,
  * mpDPFTmain.cpp:46-86 once per job), here in one persistent launch: one

@@ -44,7 +44,7 @@ class Context:
     def __init__(self, params, device=-1):
         self._lib = load_library()
         self._h = ctypes.c_void_p()
-        self.params = params
+        self.params = Params.from_buffer_copy(params)  # set_centres edits this copy, not the caller's
         rc = self._lib.mpdpft_create(ctypes.byref(params), device, ctypes.byref(self._h))
         if rc != _abi.OK:
             msg = self._lib.mpdpft_last_error(self._h).decode() if self._h else ""
@@ -130,6 +130,44 @@ class Context:
             return
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
         self._check(self._lib.mpdpft_set_initial_potential(self._h, _dp(v)))
+
+    def adjoint(self, dL_dn, tol=0.0, max_iterations=0):
+        """dL/dEnv of a loss of the last converged density (mpdpft_adjoint):
+        dL_dn is dL/dn at every grid point (one species).  Returns (lambda,
+        info dict)."""
+        g = np.ascontiguousarray(dL_dn, dtype=np.float64).reshape(-1)
+        if g.size != self.G:
+            raise ValueError("dL_dn needs %d values" % self.G)
+        lam = np.zeros(self.G)
+        info = _abi.AdjointInfo()
+        info.tol = tol
+        info.max_iterations = max_iterations
+        rc = self._check(self._lib.mpdpft_adjoint(self._h, _dp(g), _dp(lam), ctypes.byref(info)),
+                         (_abi.OK, _abi.ENOTCONV))
+        return lam, dict(iterations=info.iterations, relative_residual=info.relative_residual,
+                         dN_dmu=info.dN_dmu, converged=rc == _abi.OK)
+
+    def centre_gradient(self, lam):
+        """sum_i lam_i dEnv_i/dR_c for the environment-41 centres, shape (n, 3)."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+        out = np.zeros(3 * self.params.n_centres)
+        self._check(self._lib.mpdpft_centre_gradient(self._h, _dp(lam), _dp(out)))
+        return out.reshape(-1, 3)
+
+    def centre_forces(self):
+        """dE/dR_c of the last converged solve (Hellmann-Feynman), shape (n, 3)."""
+        out = np.zeros(3 * self.params.n_centres)
+        self._check(self._lib.mpdpft_centre_forces(self._h, _dp(out)))
+        return out.reshape(-1, 3)
+
+    def set_centres(self, centres):
+        """Replace the environment-41 centres (rows x, y, z, Z, a) and rebuild Env."""
+        c = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 5)
+        self._check(self._lib.mpdpft_set_centres(self._h, c.shape[0], _dp(c)))
+        self.params.n_centres = c.shape[0]
+        for i in range(c.shape[0]):
+            for k in range(5):
+                self.params.centres[i][k] = c[i, k]
 
     def set_break_file(self, path):
         self._check(self._lib.mpdpft_set_break_file(self._h, str(path).encode() if path else None))

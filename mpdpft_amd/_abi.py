@@ -49,6 +49,11 @@ class Params(ctypes.Structure):
     ]
 
 
+class AdjointInfo(ctypes.Structure):
+    _fields_ = [("tol", _d), ("max_iterations", _i), ("iterations", _i), ("relative_residual", _d),
+                ("dN_dmu", _d)]
+
+
 class Result(ctypes.Structure):
     _fields_ = [
         ("S", _i), ("SCcount", _i), ("converged", _i), ("LinMix", _i), ("PulMix", _i),
@@ -78,6 +83,7 @@ EXPORTS = [
     "mpdpft_import_v", "mpdpft_set_initial_potential", "mpdpft_set_break_file",
     "mpdpft_batch_create", "mpdpft_batch_solve", "mpdpft_batch_get_fields", "mpdpft_batch_seconds",
     "mpdpft_batch_phase_stats", "mpdpft_batch_last_error", "mpdpft_batch_destroy",
+    "mpdpft_adjoint", "mpdpft_centre_gradient", "mpdpft_centre_forces", "mpdpft_set_centres",
 ]
 BATCH_FP32 = 1
 
@@ -152,6 +158,14 @@ def load_library(path=LIB_PATH):
     lib.mpdpft_batch_last_error.restype = ctypes.c_char_p
     lib.mpdpft_batch_destroy.argtypes = [vp]
     lib.mpdpft_batch_destroy.restype = None
+    lib.mpdpft_adjoint.argtypes = [vp, dp, dp, ctypes.POINTER(AdjointInfo)]
+    lib.mpdpft_adjoint.restype = _i
+    lib.mpdpft_centre_gradient.argtypes = [vp, dp, dp]
+    lib.mpdpft_centre_gradient.restype = _i
+    lib.mpdpft_centre_forces.argtypes = [vp, dp]
+    lib.mpdpft_centre_forces.restype = _i
+    lib.mpdpft_set_centres.argtypes = [vp, _i, dp]
+    lib.mpdpft_set_centres.restype = _i
     if lib.mpdpft_params_sizeof() != ctypes.sizeof(Params):
         raise OSError("mpdpft_params layout mismatch: C %d vs ctypes %d"
                       % (lib.mpdpft_params_sizeof(), ctypes.sizeof(Params)))

@@ -4,6 +4,7 @@ C0: 2D two-species TF + contact gas (N = 100 each, 128^2 on [-16,16]^2).
 C1: 3D single-species electrons, softened Coulomb Z = 10, a = 0.5, N = 10,
     Hartree (isolated, zero-padded 256^3) + Dirac + PW92, 128^3 on a 20 bohr cube.
 C3: seeded batches of 2D two-species TF + contact gases in anisotropic traps.
+C4: 3D cluster of softened Coulomb centres (environment 41) for the adjoint.
 """
 import math
 
@@ -172,3 +173,31 @@ def c3(n=512, seed=20260101, grids=(64, 128, 256), tol=1e-8, maxsc=5000):
         _set(p.thetaVec, [0.1, 0.1])
         out.append(p)
     return out
+
+
+def centre_positions(n=32, seed=424242, box=8.0):
+    """n points by rejection sampling: x, y, z = -box + 2 box u (SplitMix64),
+    kept when |r| < box."""
+    rng = SplitMix64(seed)
+    out = []
+    while len(out) < n:
+        r = [-box + 2.0 * box * rng.uniform() for _ in range(3)]
+        if math.sqrt(r[0] ** 2 + r[1] ** 2 + r[2] ** 2) < box:
+            out.append(r)
+    return out
+
+
+def c4(steps=96, edge=30.0, N=32.0, n_centres=32, Z=1.0, a=0.6, seed=424242, density_expression=5,
+       kwidth=0.3, tol=1e-10, maxsc=3000, T=0.05, positions=None):
+    """3D cluster: sum of softened Coulomb centres -Z/sqrt(|r - R_c|^2 + a^2)
+    (environment 41) at seeded positions inside a sphere of radius 8 bohr;
+    the functional of c1 (nAiT or TF, isolated Hartree, Dirac, PW92)."""
+    p = c1(steps=steps, edge=edge, N=N, density_expression=density_expression, hartree_mode=1, kwidth=kwidth,
+           tol=tol, maxsc=maxsc, T=T)
+    p.Environments[0] = 41
+    pos = positions if positions is not None else centre_positions(n_centres, seed)
+    p.n_centres = len(pos)
+    for i, r in enumerate(pos):
+        _set(p.centres[i], [r[0], r[1], r[2], Z, a])
+    p.RelAcc = 2e-12
+    return p

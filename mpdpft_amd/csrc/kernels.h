@@ -57,10 +57,12 @@ hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, 
                              double *im, double scale, int post, double c1, const double2 *tw, hipStream_t st);
 
 // Isolated-boundary Hartree: zero-padded (2s)^D circular convolution with the
-// precomputed real kernel spectrum khat; out[i] += gammaH dx^D conv (i in [0,s]^D).
+// precomputed real kernel spectrum khat; out[i] += gammaH dx^D conv (i in [0,s]^D)
+// of the sources on [0,s)^D.  transpose = 1 applies the transposed operator
+// (sources on [0,s]^D, result on [0,s)^D, 0 at the periodic images).
 hipError_t hartree_isolated(const Grid &g, const double *n, double2 *pad, const double *khat,
                             const double2 *tw2, double gamma_dxD_over_M, double *out, int accumulate,
-                            hipStream_t st);
+                            hipStream_t st, int transpose = 0);
 // Build khat = DFT((2s)^D kernel 1/r, g0 at r=0) into pad then copy real part to khat.
 hipError_t build_isolated_kernel(const Grid &g, double2 *pad, const double2 *tw2, double g0, double *khat,
                                  hipStream_t st);
@@ -92,6 +94,9 @@ struct VintParams {
 };
 hipError_t update_v(const Grid &g, const VintParams &vp, const double *den_all, const double *env, double *V,
                     hipStream_t st);
+// out = inc * sum_a dV_a/dn of the pointwise interactions (10, 11, -1, -12) of one
+// species at density den (the diagonal of the interaction Jacobian without Hartree).
+hipError_t vint_dvdn(const Grid &g, const VintParams &vp, const double *den, double *out, hipStream_t st);
 
 // RelDev dots per species + field stats: partials layout per species s:
 // [<a,a>, <b,b>, <a,b>] sums and [denmin, denmax, vmin, vmax].

@@ -89,6 +89,26 @@ __device__ double pw92_eps(double n, double *vc) {
     return ec;
 }
 
+// d v_c / d n of PW92: v_c = eps - (rs/3) eps', drs/dn = -rs/(3n), so
+// dv_c/dn = -(rs/(3n)) ((2/3) eps' - (rs/3) eps'').
+__device__ double pw92_dvdn(double n) {
+    const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+    if (!(n > 0.)) return 0.;
+    double rs = cbrt(3. / (4. * PI_D * n));
+    double srs = sqrt(rs);
+    double Q0 = -2. * A * (1. + a1 * rs), Q0p = -2. * A * a1;
+    double Q1 = 2. * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+    double Q1p = A * (b1 / srs + 2. * b2 + 3. * b3 * srs + 4. * b4 * rs);
+    double Q1pp = A * (-0.5 * b1 / (rs * srs) + 1.5 * b3 / srs + 4. * b4);
+    double den = Q1 * Q1 + Q1;
+    double lg = log(1. + 1. / Q1);
+    double lgp = -Q1p / den;
+    double lgpp = -Q1pp / den + Q1p * Q1p * (2. * Q1 + 1.) / (den * den);
+    double dec = Q0p * lg + Q0 * lgp;
+    double d2ec = 2. * Q0p * lgp + Q0 * lgpp;
+    return -(rs / (3. * n)) * ((2. / 3.) * dec - (rs / 3.) * d2ec);
+}
+
 // LDA xc (id -1 exchange, -12 PW92) per particle energy and potential,
 // with the Angstrom/eV conversion of Units 2.
 __device__ void xc_lda(int id, int units, double n, double *e, double *v) {
@@ -107,6 +127,19 @@ __device__ void xc_lda(int id, int units, double n, double *e, double *v) {
     }
     *e = ce * ee;
     *v = ce * vv;
+}
+
+// d v / d n of the LDA potentials above (Units 2 conversion included)
+__device__ double xc_lda_dvdn(int id, int units, double n) {
+    const double Hartree = 27.211386245988, Bohr = 0.529177210903;
+    double cn = units == 2 ? Bohr * Bohr * Bohr : 1., ce = units == 2 ? Hartree : 1.;
+    double rho = n > 0. ? cn * n : 0.;
+    if (!(rho > 0.)) return 0.;
+    if (id == -1) {
+        double c = cbrt(3. / PI_D);
+        return ce * cn * (-c / 3.) * cbrt(rho) / rho;
+    }
+    return ce * cn * pw92_dvdn(rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -278,12 +311,12 @@ struct StGrid {
 };
 
 // zero-padded loads of the isolated Hartree transform
-struct LdGridPadZ {
+struct LdGridPadZ {  // z < lim of the grid field, zeros beyond
     const double *n;
-    int EL, s;
+    int EL, lim;
     __device__ double2 operator()(int x, int y, int z) const {
         double2 v = {0., 0.};
-        if (z < s) v.x = n[(x * EL + y) * EL + z];
+        if (z < lim) v.x = n[(x * EL + y) * EL + z];
         return v;
     }
 };
@@ -310,10 +343,12 @@ struct StGridIso {  // keep [0,s]^D of the circular convolution
     double *out;
     double scale;
     int EL, s, D, accumulate;
+    int images0;  // write 0 at the periodic images (a coordinate equal to s)
     __device__ void operator()(int x, int y, int z, double2 v) const {
         if (z > s) return;
         int idx = (x * EL + y) * EL + z;
         double a = scale * v.x;
+        if (images0 && (z == s || y == s || (D == 3 && x == s))) a = 0.;
         if (accumulate) out[idx] += a;
         else out[idx] = a;
     }
@@ -403,24 +438,27 @@ hipError_t build_isolated_kernel(const Grid &g, double2 *pad, const double2 *tw2
                                  hipStream_t st);
 
 hipError_t hartree_isolated(const Grid &g, const double *n, double2 *pad, const double *khat,
-                            const double2 *tw2, double scale, double *out, int accumulate, hipStream_t st) {
-    int s = g.s, M = 2 * s;
+                            const double2 *tw2, double scale, double *out, int accumulate, hipStream_t st,
+                            int transpose) {
+    // forward: sources [0,s)^D (the periodic images carry no charge), potential on [0,s]^D;
+    // transpose: sources [0,s]^D, result on [0,s)^D and 0 at the images
+    int s = g.s, M = 2 * s, lim = transpose ? s + 1 : s;
     StBuf sb{pad, M, M};
     LdBuf lb{pad, M, M};
     StBufMulK sk{pad, khat, M, M};
-    StGridIso so{out, scale, g.EL, s, g.D, accumulate};
+    StGridIso so{out, scale, g.EL, s, g.D, accumulate, transpose};
     if (g.D == 3) {
         // forward, pruned: only lines that can hold nonzeros
-        CHK(pass<2>(M, LdGridPadZ{n, g.EL, s}, sb, s, s, -1, tw2, st));
-        CHK(pass<1>(M, LdBufPad{pad, M, M, 1, s}, sb, s, M, -1, tw2, st));
-        CHK(pass<0>(M, LdBufPad{pad, M, M, 0, s}, sk, M, M, -1, tw2, st));
+        CHK(pass<2>(M, LdGridPadZ{n, g.EL, lim}, sb, lim, lim, -1, tw2, st));
+        CHK(pass<1>(M, LdBufPad{pad, M, M, 1, lim}, sb, lim, M, -1, tw2, st));
+        CHK(pass<0>(M, LdBufPad{pad, M, M, 0, lim}, sk, M, M, -1, tw2, st));
         // backward, pruned: only lines that reach [0,s]^3
         CHK(pass<0>(M, lb, sb, M, M, +1, tw2, st));
         CHK(pass<1>(M, lb, sb, s + 1, M, +1, tw2, st));
         CHK(pass<2>(M, lb, so, s + 1, s + 1, +1, tw2, st));
     } else {
-        CHK(pass<2>(M, LdGridPadZ{n, g.EL, s}, sb, 1, s, -1, tw2, st));
-        CHK(pass<1>(M, LdBufPad{pad, M, M, 1, s}, sk, 1, M, -1, tw2, st));
+        CHK(pass<2>(M, LdGridPadZ{n, g.EL, lim}, sb, 1, lim, -1, tw2, st));
+        CHK(pass<1>(M, LdBufPad{pad, M, M, 1, lim}, sk, 1, M, -1, tw2, st));
         CHK(pass<1>(M, lb, sb, 1, M, +1, tw2, st));
         CHK(pass<2>(M, lb, so, 1, s + 1, +1, tw2, st));
     }
@@ -580,6 +618,31 @@ __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const d
 hipError_t update_v(const Grid &g, const VintParams &vp, const double *den_all, const double *env, double *V,
                     hipStream_t st) {
     hipLaunchKernelGGL(k_update_v, dim3(2048), dim3(256), 0, st, g, vp, den_all, env, V);
+    return hipGetLastError();
+}
+
+// d V_s / d n_s of the pointwise interactions of k_update_v (one species)
+__global__ __launch_bounds__(256) void k_vint_dvdn(Grid g, VintParams vp, const double *__restrict__ den,
+                                                   double *out) {
+    const double DD = (double)g.D;
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < g.G; i += (size_t)gridDim.x * 256) {
+        double ns = den[i], acc = 0.;
+        for (int a = 0; a < vp.n_inter; a++) {
+            int t = vp.inter[a];
+            if (t == 10) {
+                acc += 2. * vp.beta;
+            } else if (t == 11) {  // v = -gammaH 2 (D n / pi)^{1/D} / (D - 1)
+                if (g.D > 1 && ns > 0.)
+                    acc += -vp.gammaH * 2. / (DD - 1.) * pow(ns * DD / PI_D, 1. / DD) / (DD * ns);
+            } else if (t == -1 || t == -12) {
+                acc += xc_lda_dvdn(t, vp.units, ns);
+            }
+        }
+        out[i] = vp.inc * acc;
+    }
+}
+hipError_t vint_dvdn(const Grid &g, const VintParams &vp, const double *den, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_vint_dvdn, dim3(2048), dim3(256), 0, st, g, vp, den, out);
     return hipGetLastError();
 }
 

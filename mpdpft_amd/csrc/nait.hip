@@ -467,6 +467,226 @@ __global__ __launch_bounds__(256) void k_nait_density(
 }
 
 // ---------------------------------------------------------------------------
+// Derivatives of the computed density for the adjoint: the same quadrature
+// tree (lower limit, panels and acceptance tests of the value integral), with
+// the integrand's partial derivatives summed over the accepted panels, i.e.
+// the exact derivative of the computed n(U, a_r, beta2) wherever the tree is
+// locally constant.
+// ---------------------------------------------------------------------------
+
+// quintic Hermite basis derivatives d/dt (see qherm)
+__device__ __forceinline__ double qherm_d(double t, double h, double f0, double g0, double c0, double f1, double g1,
+                                          double c1) {
+    double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+    double H0 = -30. * t2 + 60. * t3 - 30. * t4, H1 = 1. - 18. * t2 + 32. * t3 - 15. * t4;
+    double H2 = t - 4.5 * t2 + 6. * t3 - 2.5 * t4, H3 = 30. * t2 - 60. * t3 + 30. * t4;
+    double H4 = -12. * t2 + 28. * t3 - 15. * t4, H5 = 1.5 * t2 - 4. * t3 + 2.5 * t4;
+    return f0 * H0 + h * g0 * H1 + h * h * c0 * H2 + f1 * H3 + h * g1 * H4 + h * h * c1 * H5;
+}
+
+// d/dx of polylog_j(t, j, x) as computed, branch by branch
+__device__ double polylog_j_d(const TabView &t, int j, double x) {
+    const double sj = 1.5 - (double)j;
+    if (x < -699.) return 0.;
+    if (x > 699.) return -sj * xpow(j, x) / x * t.ginv[j];
+    if (x < t.px0) {
+        double e = exp(x);
+        return -e + 2. * e * e * t.pw2[j] - 3. * e * e * e * t.pw3[j];
+    }
+    if (x > t.pxmax) {
+        double x2 = 1. / (x * x), acc = 0., dacc = 0.;
+        for (int k = 11; k >= 0; k--) {
+            acc = x2 * (t.asym[12 * j + k] + acc);
+            dacc = x2 * ((double)(k + 1) * t.asym[12 * j + k] + dacc);
+        }
+        // S = sum_k a_k x^-2(k+1), S' = -2 sum_k (k+1) a_k x^-2(k+1) / x
+        return -t.ginv[j] * (sj * xpow(j, x) / x * (1. + acc) + xpow(j, x) * (-2. * dacc / x));
+    }
+    int k = (int)floor((x - t.px0) * t.pinv);
+    if (k > t.pn - 2) k = t.pn - 2;
+    double u = (x - (t.px0 + t.ph * (double)k)) * t.pinv;
+    const double *a = t.p + 6 * k, *b = a + 6;
+    return qherm_d(u, t.ph, a[j], a[j + 1], a[j + 2], b[j], b[j + 1], b[j + 2]) * t.pinv;
+}
+
+// polylog_pair and the x-derivatives of its two values, branch by branch
+__device__ void polylog_pair_d(const TabView &t, double x, double &l1, double &l3, double &d1, double &d3) {
+    if (x < -699.) {
+        l1 = l3 = d1 = d3 = 0.;
+        return;
+    }
+    if (x < t.px0 || x > t.pxmax) {
+        polylog_pair(t, x, l1, l3);
+        d1 = polylog_j_d(t, 1, x);
+        d3 = polylog_j_d(t, 3, x);
+        return;
+    }
+    polylog_pair(t, x, l1, l3);
+    int k = (int)floor((x - t.px0) * t.pinv);
+    if (k > t.pn - 2) k = t.pn - 2;
+    double u = (x - (t.px0 + t.ph * (double)k)) * t.pinv;
+    const double *a = t.p + 6 * k, *b = a + 6;
+    d1 = qherm_d(u, t.ph, a[1], a[2], a[3], b[1], b[2], b[3]) * t.pinv;
+    d3 = qherm_d(u, t.ph, a[3], a[4], a[5], b[3], b[4], b[5]) * t.pinv;
+}
+
+struct Deriv4 {
+    double v, u, a, b;  // value and d/dU, d/da_r, d/dbeta2
+};
+
+__device__ __forceinline__ Deriv4 integrand_d(const TabView &t, const PointArgs &q, double beta1, double x) {
+    double arg = -(q.U - x * q.ar) * q.invT;
+    double l1, l3, d1, d3;
+    polylog_pair_d(t, arg, l1, l3, d1, d3);
+    double base = q.pref * curly_a(t, x);
+    double ds = d1 + q.pref2 * d3;
+    Deriv4 r;
+    r.v = base * (l1 + q.pref2 * l3);
+    r.u = -base * ds * q.invT;
+    r.a = r.v / q.ar + base * ds * x * q.invT;  // pref is linear in a_r
+    r.b = base * l3 / beta1;                    // pref2 = beta2 / beta1
+    return r;
+}
+
+__device__ Deriv4 adaptive_boole_wave_d(const TabView &t, const PointArgs &q, double beta1, int BP, int maxIter,
+                                        double rtpna, double compare, double maxexp, double a0, double b0,
+                                        WaveStack &S, int lane) {
+    int sp = 0;
+    S.a[0] = a0;
+    S.b[0] = b0;
+    S.prev[0] = 0.;
+    S.it[0] = 0;
+    sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    const int n = 4 * BP + 1, tot = 2 * n;
+    Deriv4 total{0., 0., 0., 0.};
+    while (sp > 0) {
+        sp--;
+        double a = S.a[sp], b = S.b[sp], prev = S.prev[sp];
+        int iter = S.it[sp];
+        __builtin_amdgcn_wave_barrier();
+        double half = 0.5 * (b - a), Delta = 0.25 * half / ((double)BP);
+        Deriv4 sa{0., 0., 0., 0.}, sb{0., 0., 0., 0.};
+        for (int base = 0; base < tot; base += 64) {
+            int idx = base + lane;
+            if (idx < tot) {
+                bool right = idx >= n;
+                int yi = right ? idx - n : idx;
+                double x = a + (double)yi * Delta;
+                if (right) x = x + half;
+                double w = boole_w(yi, n);
+                Deriv4 f = integrand_d(t, q, beta1, x);
+                Deriv4 &acc = right ? sb : sa;
+                acc.v += w * f.v;
+                acc.u += w * f.u;
+                acc.a += w * f.a;
+                acc.b += w * f.b;
+            }
+        }
+        sa.v = wave_sum(sa.v);
+        sb.v = wave_sum(sb.v);
+        sa.u = wave_sum(sa.u);
+        sb.u = wave_sum(sb.u);
+        sa.a = wave_sum(sa.a);
+        sb.a = wave_sum(sb.a);
+        sa.b = wave_sum(sa.b);
+        sb.b = wave_sum(sb.b);
+        double ha = (a + half - a) / (((double)BP) * 4.), hb = (b - (a + half)) / (((double)BP) * 4.);
+        double ca = 2. * ha / 45., cb = 2. * hb / 45.;
+        double Inta = sa.v * ca, Intb = sb.v * cb;
+        double Int = Inta + Intb;
+        bool accept;
+        if (Int != Int) accept = true;
+        else if (iter > 1 && fabs(Int) > 0. && fabs((Int - prev) / Int) < rtpna) accept = true;
+        else if (iter > maxIter) accept = true;
+        else if (negligible(compare, pow(2., (double)iter) * Int, maxexp)) accept = true;
+        else if (Delta < kMP) accept = true;
+        else accept = false;
+        if (accept) {
+            total.v += Int;
+            total.u += sa.u * ca + sb.u * cb;
+            total.a += sa.a * ca + sb.a * cb;
+            total.b += sa.b * ca + sb.b * cb;
+        } else {
+            if (sp + 2 > 64) return Deriv4{__builtin_nan(""), 0., 0., 0.};
+            S.a[sp] = a + half;
+            S.b[sp] = b;
+            S.prev[sp] = Intb;
+            S.it[sp] = iter + 1;
+            S.a[sp + 1] = a;
+            S.b[sp + 1] = a + half;
+            S.prev[sp + 1] = Inta;
+            S.it[sp + 1] = iter + 1;
+            sp += 2;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    return total;
+}
+
+// One wavefront per grid point: n and its partial derivatives with respect to
+// U = V - mu, |grad V|^2 and lap V (through a_r and beta2).
+__global__ __launch_bounds__(256) void k_nait_partials(mk::Grid g, TabView t, DensityArgs da, double *n,
+                                                       double *dU, double *dg2, double *dl) {
+    __shared__ WaveStack stacks[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    WaveStack &S = stacks[wv];
+    const int G = (int)g.G;
+    const double tau = 1. / da.T, beta0 = -da.degeneracy / pow(2. * kPI, 1.5), beta1 = pow(da.alpha * tau, -1.5);
+    const double sqrt_at = sqrt(da.alpha * tau);
+    const double maxexp = -log10(da.rtpna);
+    for (int i = blockIdx.x * 4 + wv; i < G; i += gridDim.x * 4) {
+        double U = da.V[i] - da.mu;
+        double g2 = da.grad2[i];
+        double ar = 0.5 * pow(da.alpha * 1. * g2, 1. / 3.);
+        double beta2 = -da.lap[i] / (12. * sqrt_at);
+        Deriv4 r;
+        if (ar < fabs(da.thr)) {
+            double arg = -tau * U;
+            double P0 = polylog_j(t, 0, arg), P2 = polylog_j(t, 2, arg);
+            r.v = beta0 * (beta1 * P0 + beta2 * P2);
+            r.u = beta0 * (beta1 * polylog_j_d(t, 0, arg) + beta2 * polylog_j_d(t, 2, arg)) * (-tau);
+            r.a = 0.;
+            r.b = beta0 * P2;
+        } else {
+            PointArgs q;
+            q.invT = 1. / da.T;
+            q.U = U;
+            q.ar = ar;
+            q.pref = beta0 * ar * tau * beta1;
+            q.pref2 = beta2 / beta1;
+            bool small = true;
+            if (lane < 27) {
+                double x = -ldexp(1., lane + 1), arg = -tau * (U - x * ar);
+                double l1 = polylog_j(t, 1, arg), l3 = polylog_j(t, 3, arg);
+                double test = beta0 * curly_a(t, x) * ar * tau * (beta1 * l1 + beta2 * l3);
+                small = !(fabs(test) > kMP);
+            }
+            unsigned long long bal = __ballot(small ? 1 : 0);
+            int k = __ffsll((long long)bal) - 1;
+            if (k < 0 || k > 26) k = 26;
+            double x = -ldexp(1., k + 1);
+            Deriv4 r1 = adaptive_boole_wave_d(t, q, beta1, da.BP, da.maxIter, da.rtpna, da.compare, maxexp, 2. * x,
+                                              x, S, lane);
+            Deriv4 r2 = adaptive_boole_wave_d(t, q, beta1, da.BP, da.maxIter, da.rtpna, da.compare, maxexp, x, 100.,
+                                              S, lane);
+            r.v = r1.v + r2.v;
+            r.u = r1.u + r2.u;
+            r.a = r1.a + r2.a;
+            r.b = r1.b + r2.b;
+        }
+        if (r.v != r.v) r = Deriv4{0., 0., 0., 0.};
+        if (lane == 0) {
+            n[i] = r.v;
+            dU[i] = r.u;
+            // a_r = (alpha g2)^{1/3} / 2: da_r/dg2 = a_r / (3 g2); beta2 = -lap / (12 sqrt(alpha tau))
+            dg2[i] = (g2 > 0. && r.a != 0.) ? r.a * ar / (3. * g2) : 0.;
+            dl[i] = -r.b / (12. * sqrt_at);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Panel-reuse variant of the same quadrature.  A node [a, b] is sampled at
 // k = 0..8BP (left half a + k Delta, right half (a + k' Delta) + half with
 // k = 4BP + k'); the shared midpoint is evaluated once.  When a node is
@@ -1098,6 +1318,37 @@ void nait_reset_state(mpdpft_ctx *c) {
     if (!c->nait) return;
     for (int s = 0; s < c->S; s++)
         for (int k = 0; k < 3; k++) c->nait->Mpp[s][k] = c->p.Mpp[k];
+}
+
+int nait_partials(mpdpft_ctx *c, int s, double mu, double *n, double *dU, double *dg2, double *dl) {
+    NaitTables *t = c->nait;
+    if (!t) return ctx_fail(c, MPDPFT_EINVAL, "nAiT tables not loaded");
+    size_t G = c->G;
+    if (c->delfield_valid != s) {
+        int rc = ctx_delfield(c, c->d_V + s * G, s, c->p.DelFieldMethod);
+        if (rc) return rc;
+        c->delfield_valid = s;
+    }
+    DensityArgs da;
+    da.V = c->d_V + s * G;
+    da.grad2 = c->d_Grad2 + s * G;
+    da.lap = c->d_Lap + s * G;
+    da.mu = mu;
+    da.T = c->TVec[s];
+    da.degeneracy = c->p.degeneracy;
+    da.alpha = c->p.Units == 2 ? 1. / 0.1312342130759153 : 1.;
+    da.thr = t->Mpp[s][2];
+    da.rtpna = c->p.InternalAcc;
+    da.compare = c->p.Abundances[s] / std::pow(c->p.edge, (double)c->D);
+    da.BP = (int)t->Mpp[s][0];
+    da.maxIter = (int)(t->Mpp[s][1] + 0.5);
+    if (da.BP < 1 || 4 * da.BP + 1 > 4096 || da.maxIter > kStack - 4)
+        return ctx_fail(c, MPDPFT_EINVAL, "nAiT Mpp out of range");
+    unsigned blocks = (unsigned)std::min<size_t>(4096, (G + 3) / 4);
+    hipLaunchKernelGGL(k_nait_partials, dim3(blocks), dim3(256), 0, c->st, c->g, t->view, da, n, dU, dg2, dl);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "k_nait_partials");
+    return MPDPFT_OK;
 }
 
 int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {

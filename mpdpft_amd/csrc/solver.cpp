@@ -344,6 +344,7 @@ extern "C" void mpdpft_destroy(mpdpft_ctx *c) {
     if (!c) return;
     if (c->st) hipStreamSynchronize(c->st);
     nait_free(c);
+    adjoint_free(c);
     for (void *q : c->allocations) hipFree(q);
     for (double *q : c->nin) hipFree(q);
     for (double *q : c->res) hipFree(q);

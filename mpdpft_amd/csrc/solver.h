@@ -9,7 +9,8 @@
 #include "kernels.h"
 #include "mpdpft.h"
 
-struct NaitTables;  // nait.hip
+struct NaitTables;    // nait.hip
+struct AdjointState;  // adjoint.hip
 
 struct mpdpft_ctx {
     mpdpft_params p;
@@ -66,6 +67,7 @@ struct mpdpft_ctx {
     int delfield_valid = -1;  // species whose Grad2/Lap hold DelField(V_s)
 
     NaitTables *nait = nullptr;
+    AdjointState *adj = nullptr;  // buffers of the fixed-point adjoint (allocated on first use)
 
     // per-stage GPU timing (HIP events on st)
     int timing = 0;
@@ -85,10 +87,16 @@ void stage_begin(mpdpft_ctx *c);
 void stage_end(mpdpft_ctx *c, const char *name, double units, double bytes, double flops);
 void stage_add(mpdpft_ctx *c, const char *name, double units, double bytes, double flops);
 
+// adjoint.hip
+void adjoint_free(mpdpft_ctx *c);
+
 // nait.hip
 int nait_init(mpdpft_ctx *c);
 void nait_free(mpdpft_ctx *c);
 int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N);
+// n[V, mu] of species s and its partial derivatives d/dU (U = V - mu), d/d|grad V|^2
+// and d/d lap V at every grid point (same quadrature tree as the density)
+int nait_partials(mpdpft_ctx *c, int s, double mu, double *n, double *dU, double *dg2, double *dl);
 int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E);
 void nait_adapt(mpdpft_ctx *c);
 void nait_state(const mpdpft_ctx *c, int s, double *BP, double *maxIter);
