@@ -255,9 +255,12 @@ int mpdpft_adjoint(mpdpft_ctx *ctx, const double *dL_dn, double *lambda, mpdpft_
 /* Environment 41: out[3c + d] = sum_i lambda[i] dEnv_i/dR_{c,d} (chain rule of
  * mpdpft_adjoint's lambda to the centre coordinates). */
 int mpdpft_centre_gradient(mpdpft_ctx *ctx, const double *lambda, double *out);
-/* Environment 41: Hellmann-Feynman derivative of the total energy with respect
- * to the centre coordinates, out[3c + d] = sum_i w_i dx^3 n_i dEnv_i/dR_{c,d}
- * (the loop's energy is stationary in V and n at self-consistency). */
+/* Environment 41: derivative of the last solve's total energy (GetEnergy,
+ * mpDPFT.cpp:5414-5436) with respect to the centre coordinates through the
+ * fixed point: out[3c + d] = sum_i (w_i dx^3 n_i + lambda_E,i) dEnv_i/dR_{c,d},
+ * lambda_E from the adjoint system with the energy's own sources (dE/dn of the
+ * interaction and environment energies, dE_disp/dV and dE_disp/dmu of the
+ * kinetic energy's quadrature).  EkinTypes 0 only. */
 int mpdpft_centre_forces(mpdpft_ctx *ctx, double *out);
 /* Replace the centres list (n x {x, y, z, Z, a}) and rebuild Env of the
  * environment-41 species; the next solve starts from the new Env. */

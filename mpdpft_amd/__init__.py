@@ -155,7 +155,7 @@ class Context:
         return out.reshape(-1, 3)
 
     def centre_forces(self):
-        """dE/dR_c of the last converged solve (Hellmann-Feynman), shape (n, 3)."""
+        """dE/dR_c of the last converged solve's total energy through the fixed point, shape (n, 3)."""
         out = np.zeros(3 * self.params.n_centres)
         self._check(self._lib.mpdpft_centre_forces(self._h, _dp(out)))
         return out.reshape(-1, 3)

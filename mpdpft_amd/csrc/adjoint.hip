@@ -325,13 +325,51 @@ __global__ __launch_bounds__(256) void k_centre_grad(Grid g, const double *centr
     }
 }
 
-// lam = w dx^3 n (the Hellmann-Feynman integrand weights)
+// out = w dx^3 n (the explicit part of dE/dEnv)
 __global__ void k_weighted(Grid g, double dxD, const double *n, double *out) {
     for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < g.G; i += (size_t)gridDim.x * kT) {
         int c[3];
         dcoords(g, i, c);
         out[i] = trapw3(g, c) * dxD * n[i];
     }
+}
+
+// dE/dn of the loop's total energy (explicit n dependence):
+// w dx^3 (Env + v_xc + K n / 2) + K^T (w dx^3 n) / 2
+__global__ void k_energy_source(Grid g, double dxD, const double *envxc, const double *Kn, const double *KtWn,
+                                double *out) {
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < g.G; i += (size_t)gridDim.x * kT) {
+        int c[3];
+        dcoords(g, i, c);
+        double W = trapw3(g, c) * dxD;
+        double v = W * envxc[i];
+        if (Kn) v += 0.5 * (W * Kn[i] + KtWn[i]);
+        out[i] = v;
+    }
+}
+
+// TF dispersal energy pref sum w (mu - V)_+^{5/2}: d/dU = -(5/2) pref (mu - V)_+^{3/2}
+__global__ void k_tf_energy_dU(size_t G, const double *V, double mu, double pref, double *out) {
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < G; i += (size_t)gridDim.x * kT) {
+        double t = mu - V[i];
+        out[i] = t > 0. ? -2.5 * pref * t * sqrt(t) : 0.;
+    }
+}
+
+// out = s w dx^3 a (a may alias out); partial sums of out
+__global__ __launch_bounds__(256) void k_weight_scale(Grid g, double dxD, double s, const double *a, double *out,
+                                                      double *partials) {
+    __shared__ double sh[4];
+    double acc = 0.;
+    for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < g.G; i += (size_t)gridDim.x * kT) {
+        int c[3];
+        dcoords(g, i, c);
+        double v = s * trapw3(g, c) * dxD * a[i];
+        out[i] = v;
+        acc += v;
+    }
+    double r = block_sum(acc, sh);
+    if (partials && threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
 }  // namespace
@@ -416,8 +454,8 @@ int delfield_adjoint(mpdpft_ctx *c, const double *mug, const double *mul, double
     return MPDPFT_OK;
 }
 
-// out = F_V^T P (ell + K'^T lam); lam or ell may be null (term dropped)
-int apply_adjoint_map(mpdpft_ctx *c, const double *lam, const double *ell, double *out) {
+// out = F_V^T (P (ell + K'^T lam) - w alpha_add); lam or ell may be null (term dropped)
+int apply_adjoint_map(mpdpft_ctx *c, const double *lam, const double *ell, double *out, double alpha_add = 0.) {
     AdjointState &A = *c->adj;
     const Grid &g = c->g;
     const size_t G = c->G;
@@ -434,7 +472,7 @@ int apply_adjoint_map(mpdpft_ctx *c, const double *lam, const double *ell, doubl
                        lam, (const double *)A.dfdU, A.x, A.partials);
     double fx;
     RC(reduce_to_host(c, A.partials, 1, &fx));
-    const double alpha = fx / A.fmu_w;
+    const double alpha = fx / A.fmu_w + alpha_add;
     hipLaunchKernelGGL(k_adj_nu, dim3(nb), dim3(kT), 0, c->st, g, dxD(c), (const double *)A.x, alpha,
                        (const double *)A.dfdU, (const double *)A.dfdg2, (const double *)A.dfdl, A.direct,
                        A.nait ? A.mug : nullptr, A.nait ? A.mul : nullptr);
@@ -697,16 +735,92 @@ extern "C" int mpdpft_centre_gradient(mpdpft_ctx *c, const double *lambda, doubl
     return centre_contract(c, c->adj->t3, out);
 }
 
+// The total energy's derivative: E(V, mu, n = F(V, mu); theta) of GetEnergy
+// (:5414-5436) differentiated through the fixed point.  Sources: dE/dn =
+// w (Env + v_xc + K n / 2) + K^T (w n) / 2 (Hartree, Dirac, LDA energies are
+// exact antiderivatives of their potentials); k_V = dE_disp/dV through U and
+// the DelField terms (the kinetic energy's own quadrature, differentiated with
+// its refinements held fixed); k_mu = dE_disp/dmu.  Then
+// dE/dtheta = sum_i (w n + lambda_E)_i dEnv_i/dtheta.
 extern "C" int mpdpft_centre_forces(mpdpft_ctx *c, double *out) {
     if (!c || !out) return MPDPFT_EINVAL;
     HIPC(hipSetDevice(c->device));
     RC(centres_ok(c));
-    if (c->SCcount < 1) return ctx_fail(c, MPDPFT_EINVAL, "centre forces: no converged solve in this context");
-    if (!c->adj) RC(setup(c, 40));
-    hipLaunchKernelGGL(k_weighted, dim3(gblocks(c->G)), dim3(kT), 0, c->st, c->g, dxD(c), (const double *)c->d_Den,
-                       c->adj->t3);
+    RC(check_supported(c));
+    const mpdpft_params &p = c->p;
+    for (int e = 0; e < p.n_ekin; e++)
+        if (p.EkinTypes[e] != 0) return ctx_fail(c, MPDPFT_EINVAL, "centre forces: EkinTypes 0 only");
+    RC(setup(c, 40));
+    AdjointState &A = *c->adj;
+    const Grid &g = c->g;
+    const size_t G = c->G;
+    const unsigned nb = gblocks(G);
+    const double dV = dxD(c);
+    const char *vb = std::getenv("MPDPFT_VERBOSE");
+    A.verbose = vb && *vb && *vb != '0';
+    // Env + v_xc (pointwise interactions; V of UpdateV without Hartree)
+    mk::VintParams vp{};
+    for (int a = 0; a < p.n_interactions; a++)
+        if (p.Interactions[a] != 8) vp.inter[vp.n_inter++] = p.Interactions[a];
+    vp.S = 1;
+    vp.s = c->s;
+    vp.beta = p.beta;
+    vp.gammaH = p.gammaH;
+    vp.inc = 1.;
+    vp.units = p.Units;
+    HIPC(mk::update_v(g, vp, c->d_Den, c->d_Env, A.t1, c->st));
+    const double *Kn = nullptr, *KtWn = nullptr;
+    if (A.hartree) {
+        const int M = 2 * c->s;
+        double sc = p.gammaH * dV / std::pow((double)M, (double)c->D);
+        HIPC(mk::hartree_isolated(g, c->d_Den, c->d_pad, c->d_khat, c->d_tw2, sc, A.t2, 0, c->st, 0));
+        hipLaunchKernelGGL(k_weighted, dim3(nb), dim3(kT), 0, c->st, g, dV, (const double *)c->d_Den, A.fbar);
+        HIPC(mk::hartree_isolated(g, A.fbar, c->d_pad, c->d_khat, c->d_tw2, sc, A.t3, 0, c->st, 1));
+        Kn = A.t2;
+        KtWn = A.t3;
+    }
+    hipLaunchKernelGGL(k_energy_source, dim3(nb), dim3(kT), 0, c->st, g, dV, (const double *)A.t1, Kn, KtWn, A.ell);
     HIPC(hipGetLastError());
-    return centre_contract(c, c->adj->t3, out);
+    // kinetic (dispersal) energy sources: kV into A.w, k_mu
+    double kmu = 0.;
+    if (A.nait) {
+        double pref;
+        RC(nait_energy_partials(c, 0, A.direct, A.mug, A.mul, &pref));
+        hipLaunchKernelGGL(k_weight_scale, dim3(kRedBlocks), dim3(kT), 0, c->st, g, dV, pref, (const double *)A.direct,
+                           A.direct, A.partials);
+        double sU;
+        RC(reduce_to_host(c, A.partials, 1, &sU));
+        kmu = -sU;
+        hipLaunchKernelGGL(k_weight_scale, dim3(kRedBlocks), dim3(kT), 0, c->st, g, dV, pref, (const double *)A.mug,
+                           A.mug, (double *)nullptr);
+        hipLaunchKernelGGL(k_weight_scale, dim3(kRedBlocks), dim3(kT), 0, c->st, g, dV, pref, (const double *)A.mul,
+                           A.mul, (double *)nullptr);
+        RC(delfield_adjoint(c, A.mug, A.mul, A.w));
+        hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(kT), 0, c->st, G, 1., (const double *)A.direct, 1., A.w);
+    } else {
+        double U = p.Units == 2 ? 7.619964158443410 : 1.;
+        double pref = p.degeneracy * 4. * kPI * std::pow(2., 2.5) / (std::pow(2. * kPI * std::sqrt(U), 3.) * 10.);
+        hipLaunchKernelGGL(k_tf_energy_dU, dim3(nb), dim3(kT), 0, c->st, G, (const double *)c->d_V, c->muVec[0], pref,
+                           A.direct);
+        hipLaunchKernelGGL(k_weight_scale, dim3(kRedBlocks), dim3(kT), 0, c->st, g, dV, 1., (const double *)A.direct,
+                           A.w, A.partials);
+        double sU;
+        RC(reduce_to_host(c, A.partials, 1, &sU));
+        kmu = -sU;
+    }
+    HIPC(hipGetLastError());
+    // rhs = F_V^T (P l_E - w k_mu / F_mu.w) + k_V
+    RC(apply_adjoint_map(c, nullptr, A.ell, A.rhs, kmu / A.fmu_w));
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(kT), 0, c->st, G, 1., (const double *)A.w, 1., A.rhs);
+    int its = 0;
+    double rr = 0.;
+    RC(gmres(c, A.rhs, A.w, 1e-12, 400, &its, &rr));
+    if (rr > 1e-12) return ctx_fail(c, MPDPFT_ENOTCONV, "centre forces: adjoint GMRES did not reach 1e-12");
+    // lambda_E + w n
+    hipLaunchKernelGGL(k_weighted, dim3(nb), dim3(kT), 0, c->st, g, dV, (const double *)c->d_Den, A.t3);
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(kT), 0, c->st, G, 1., (const double *)A.w, 1., A.t3);
+    HIPC(hipGetLastError());
+    return centre_contract(c, A.t3, out);
 }
 
 extern "C" int mpdpft_set_centres(mpdpft_ctx *c, int n, const double *centres) {

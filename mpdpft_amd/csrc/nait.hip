@@ -1078,6 +1078,100 @@ __global__ __launch_bounds__(64) void k_ekin_ait(const double2 *__restrict__ ap,
     }
 }
 
+// d Ai / dy of airy_fast as computed (polynomial derivative; asymptotic form below -300)
+__device__ double airy_fast_d(const double2 *__restrict__ ap, double y) {
+    if (y <= -300.) {
+        double z = -y, q = pow(z, 0.25), phi = 2. * z * sqrt(z) / 3. - 0.25 * kPI;
+        double dAdz = (-sin(phi) * sqrt(z) / q - cos(phi) * 0.25 / (q * z)) / sqrt(kPI);
+        return -dAdz;
+    }
+    int k = (int)floor((y - kAX0) * kAINV);
+    k = k < 0 ? 0 : (k > kAN - 2 ? kAN - 2 : k);
+    double u = (y - (kAX0 + kAH * (double)k)) * kAINV;
+    const double2 *c = ap + 3 * k;
+    double2 c01 = c[0], c23 = c[1], c45 = c[2];
+    return ((((5. * c45.y * u + 4. * c45.x) * u + 3. * c23.y) * u + 2. * c23.x) * u + c01.y) * kAINV;
+}
+
+// Partial derivatives of the EkinAiT03D energy density (k_ekin_ait with the
+// final MaxIt, no resume) with respect to U = V - mu, |grad V|^2 and lap V:
+// the refinement that ends the point's loop is found as in k_ekin_ait, and
+// its trapezoid sum is differentiated with its node count held fixed (the
+// nodes x_j = j X / (gs - 1) move with X = max(1, 100 - U / a)).
+__global__ __launch_bounds__(64) void k_ekin_partials(const double2 *__restrict__ ap, EkinArgs ea, double *eU,
+                                                      double *eg2, double *el) {
+    const int lane = threadIdx.x;
+    const long long i = blockIdx.x;
+    if (i >= ea.G) return;
+    const double U = ea.V[i] - ea.mu, g2 = fabs(ea.grad2[i]), a = 0.5 * ea.c0 * pow(g2, 1. / 3.);
+    const double lap = ea.lap[i];
+    if (a < ea.thr) {
+        double dU = 0., dl = 0.;
+        if (-U > 0.) {
+            double t = 2. * (-U), st = sqrt(t);
+            dU = -ea.c1 * t * st + ea.c2 * lap / (6. * st);
+            dl = -ea.c2 * st / 6.;
+        }
+        if (lane == 0) {
+            eU[i] = dU;
+            eg2[i] = 0.;
+            el[i] = dl;
+        }
+        return;
+    }
+    const double Ua = U / a, X = fmax(1., 100. - Ua), xmax = fmax(1., 100. - U / a);
+    const bool Xvar = 100. - Ua > 1.;
+    const double dXdU = Xvar ? -1. / a : 0., dXda = Xvar ? U / (a * a) : 0.;
+    const double k1 = ea.c1 / 5., k2 = ea.c2 * lap / 6., twoa = 2. * a;
+    double prev = 1., val = 0., dU = 0., da = 0., dl = 0.;
+    for (int iteration = 0;; iteration++) {
+        const int gs = 64 << iteration;
+        const double dg = (double)(gs - 1), inv = 1. / dg;
+        double r = 0., rU = 0., ra = 0., rl = 0.;
+        for (int j = lane; j < gs; j += 64) {
+            double num = (double)j * X, q = num * inv;
+            double x = fma(fma(-q, dg, num), inv, q), y = x + Ua;
+            double AI = y > 100. ? 0. : airy_fast(ap, y);
+            double AId = y > 100. ? 0. : airy_fast_d(ap, y);
+            double tt = twoa * x, sq = sqrt(tt);
+            double h = k1 * tt * tt * sq - k2 * sq;
+            double wj = (j == 0 || j == gs - 1) ? 0.5 : 1.;
+            double dxU = (double)j * inv * dXdU, dxa = (double)j * inv * dXda;
+            double dyU = dxU + 1. / a, dya = dxa - U / (a * a);
+            // dh/dx and dh/da (both vanish with x; the x = 0 node contributes no x-derivative)
+            double hx = 0., ha = 0.;
+            if (x > 0.) {
+                hx = 5. * a * k1 * tt * sq - a * k2 / sq;
+                ha = 5. * x * k1 * tt * sq - x * k2 / sq;
+            }
+            r += wj * AI * h;
+            rU += wj * (AId * dyU * h + AI * hx * dxU);
+            ra += wj * (AId * dya * h + AI * (hx * dxa + ha));
+            rl += wj * AI * (-(ea.c2 / 6.) * sq);
+        }
+        r = wave_total(r);
+        rU = wave_total(rU);
+        ra = wave_total(ra);
+        rl = wave_total(rl);
+        val = r * (xmax - 0.) / ((double)gs - 1.);
+        dU = (dXdU * r + X * rU) * inv;
+        da = (dXda * r + X * ra) * inv;
+        dl = X * rl * inv;
+        double ac = (fabs(val) < kMP && fabs(prev) < kMP) ? 0. : fabs((val - prev) / (val + prev));
+        double as = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
+        if (iteration > 0 &&
+            (ac < kMP || (fabs(as) < ea.internal_acc && negligible(ea.cmp, ea.area_pref * val, ea.neg_exp))))
+            break;
+        if (iteration >= ea.MaxIt) break;
+        prev = val;
+    }
+    if (lane == 0) {
+        eU[i] = dU;
+        eg2[i] = g2 > 0. ? da * a / (3. * g2) : 0.;  // a = c0 |g2|^{1/3} / 2
+        el[i] = dl;
+    }
+}
+
 }  // namespace
 
 struct NaitTables {
@@ -1466,6 +1560,40 @@ void nait_state(const mpdpft_ctx *c, int s, double *BP, double *maxIter) {
 }
 
 // EkinAiT03D (mpDPFT.cpp:5610-5709); c->MaxIteration persists across calls.
+int nait_energy_partials(mpdpft_ctx *c, int s, double *eU, double *eg2, double *el, double *prefactor) {
+    NaitTables *t = c->nait;
+    if (!t) return ctx_fail(c, MPDPFT_EINVAL, "nAiT tables not loaded");
+    size_t G = c->G;
+    if (c->delfield_valid != s) {
+        int rc = ctx_delfield(c, c->d_V + s * G, s, c->p.DelFieldMethod);
+        if (rc) return rc;
+        c->delfield_valid = s;
+    }
+    double alpha = c->p.Units == 2 ? 1. / 0.1312342130759153 : 1.;
+    EkinArgs ea;
+    ea.V = c->d_V + s * G;
+    ea.grad2 = c->d_Grad2 + s * G;
+    ea.lap = c->d_Lap + s * G;
+    ea.mu = c->muVec[s];
+    ea.thr = t->Mpp[s][2];
+    ea.c0 = std::pow(alpha, 1. / 3.);
+    ea.c1 = std::pow(alpha, -1.5);
+    ea.c2 = std::pow(alpha, -0.5);
+    ea.internal_acc = c->p.InternalAcc;
+    ea.cmp = c->EnvE[s];
+    *prefactor = c->p.degeneracy / (4. * kPI * kPI);
+    ea.area_pref = std::pow(c->p.edge, (double)c->D) * (*prefactor);
+    ea.neg_exp = -std::log(c->p.InternalAcc);
+    ea.G = (long long)G;
+    ea.MaxIt = c->MaxIteration;
+    ea.resume = 0;
+    if (c->MaxIteration > 24) return ctx_fail(c, MPDPFT_EINVAL, "EkinAiT03D: MaxIteration above 24");
+    hipLaunchKernelGGL(k_ekin_partials, dim3((unsigned)G), dim3(64), 0, c->st, t->d_ap, ea, eU, eg2, el);
+    hipError_t he = hipGetLastError();
+    if (he != hipSuccess) return hip_fail(c, he, "k_ekin_partials");
+    return MPDPFT_OK;
+}
+
 int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E) {
     NaitTables *t = c->nait;
     if (!t) return ctx_fail(c, MPDPFT_EINVAL, "nAiT tables not loaded");

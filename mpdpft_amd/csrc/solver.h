@@ -98,6 +98,9 @@ int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N);
 // and d/d lap V at every grid point (same quadrature tree as the density)
 int nait_partials(mpdpft_ctx *c, int s, double mu, double *n, double *dU, double *dg2, double *dl);
 int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E);
+// d/dU, d/d|grad V|^2, d/d lap V of the EkinAiT03D energy density per point at the
+// final MaxIteration; the energy is prefactor * sum_i w_i dx^3 (density)_i
+int nait_energy_partials(mpdpft_ctx *c, int s, double *eU, double *eg2, double *el, double *prefactor);
 void nait_adapt(mpdpft_ctx *c);
 void nait_state(const mpdpft_ctx *c, int s, double *BP, double *maxIter);
 void nait_reset_state(mpdpft_ctx *c);
