@@ -22,6 +22,14 @@ loop solving the same instances one after another; the polish results are
 compared with the FP64 results (energy, density, mu) and, on a sample, with
 the oracle, which is also timed there as the CPU baseline of that workload.
 
+The "adjoint" object adds the differentiation workload: a 96^3 cluster of 32
+softened Coulomb centres (Z = 1, a = 0.6, seed 424242) on a 30 bohr cube with
+N = 32 and the functional above; the loss int (n - n_target)^2 (target: the
+centres of seed 434343) and the total energy are differentiated with respect to
+the 96 centre coordinates by the implicit adjoint of the converged fixed point,
+and one coordinate of each is checked against central differences (h = 1e-4)
+of warm-started solves.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128]
 """
 import argparse
@@ -140,6 +148,67 @@ def batch_leg(n, plain_n, cpu_n, cpu_on):
     return out
 
 
+def adjoint_leg(steps, h):
+    """C4: converged solve, adjoint gradient of the density-mismatch loss and
+    total-energy forces for all centre coordinates, FD checks."""
+    import numpy as np
+    from mpdpft_amd import Context, configs
+    os.environ["MPDPFT_VERBOSE"] = "0"
+    pos = configs.centre_positions(32, 424242)
+    p = configs.c4(steps=steps, tol=1e-10, positions=pos)
+    pt = configs.c4(steps=steps, tol=1e-10, positions=configs.centre_positions(32, 434343))
+    EL = steps + 1
+    w1 = np.ones(EL)
+    w1[0] = w1[-1] = 0.5
+    w = (w1[:, None, None] * w1[None, :, None] * w1[None, None, :]).reshape(-1) * (p.edge / steps) ** 3
+    ct = Context(pt)
+    ct.solve()
+    nt = ct.density()[0].copy()
+    ct.close()
+    ctx = Context(p)
+    t0 = time.perf_counter()
+    r = ctx.solve()
+    t_solve = time.perf_counter() - t0
+    n = ctx.density()[0].copy()
+    V = ctx.potential().copy()
+    t0 = time.perf_counter()
+    lam, info = ctx.adjoint(2.0 * w * (n - nt))
+    grad = ctx.centre_gradient(lam)
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    forces = ctx.centre_forces()
+    t_forces = time.perf_counter() - t0
+    C = np.array([[q[0], q[1], q[2], 1.0, 0.6] for q in pos])
+    ci, d = 0, 0
+    vals = []
+    for sgn in (1, -1):
+        Cp = C.copy()
+        Cp[ci, d] += sgn * h
+        ctx.set_centres(Cp)
+        ctx.set_initial_potential(V)
+        rr = ctx.solve()
+        vals.append((float(np.sum(w * (ctx.density()[0] - nt) ** 2)), rr.TotalEnergy, bool(rr.converged)))
+    ctx.close()
+    fdL = (vals[0][0] - vals[1][0]) / (2 * h)
+    fdE = (vals[0][1] - vals[1][1]) / (2 * h)
+    return {
+        "workload": "C4: 96^3 nAiT(T=0.05) + isolated Hartree + Dirac + PW92 on a 30 bohr cube, 32 softened "
+                    "Coulomb centres (Z=1, a=0.6, seed 424242), N=32, max|dV|<1e-10; loss int (n - n_target)^2 "
+                    "(target seed 434343) and total energy vs the 96 centre coordinates" if steps == 96 else
+                    "C4 geometry at %d^3" % steps,
+        "solve_seconds": round(t_solve, 3), "sc_iterations": r.SCcount - 1, "converged": bool(r.converged),
+        "total_energy": r.TotalEnergy,
+        "adjoint_seconds": round(t_adj, 4), "gmres_iterations": info["iterations"],
+        "gmres_relative_residual": info["relative_residual"], "forces_seconds": round(t_forces, 4),
+        "gradient_norm": float(np.linalg.norm(grad)), "force_norm": float(np.linalg.norm(forces)),
+        "fd_check": {"coordinate": "centre 0 x", "h": h, "dL_adjoint": float(grad[ci, d]), "dL_fd": fdL,
+                     "dL_rel_err": abs(grad[ci, d] - fdL) / abs(fdL), "dE_adjoint": float(forces[ci, d]),
+                     "dE_fd": fdE, "dE_rel_err": abs(forces[ci, d] - fdE) / abs(fdE),
+                     "fd_solves_converged": all(v[2] for v in vals)},
+        "fd_gradient_cost_estimate_s": round(2 * 96 * t_solve, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +222,7 @@ def main():
     ap.add_argument("--batch-plain", type=int, default=512,
                     help="instances the plain FP64 loop solves for the batch comparison")
     ap.add_argument("--batch-cpu", type=int, default=16, help="instances the oracle solves for the batch baseline")
+    ap.add_argument("--adjoint-grid", type=int, default=96, help="grid of the differentiation workload (0: skip)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "nait_reuse_128_traffic.md"),
                     help="rocprof summary holding 'traffic_bytes_per_launch = N' for the nAiT kernel at this grid")
     args = ap.parse_args()
@@ -220,6 +290,11 @@ def main():
         print("bench: batched workload, %d instances" % args.batch, file=sys.stderr, flush=True)
         batch = batch_leg(args.batch, args.batch_plain, args.batch_cpu, not args.no_cpu_baseline)
 
+    adjoint = None
+    if args.adjoint_grid > 0:
+        print("bench: differentiation workload, %d^3" % args.adjoint_grid, file=sys.stderr, flush=True)
+        adjoint = adjoint_leg(args.adjoint_grid, 1e-4)
+
     line = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -243,6 +318,7 @@ def main():
         "stages_hbm": hbm,
         "stage_ms": stages,
         "batch": batch,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 
