@@ -42,6 +42,7 @@ struct MulParams {
     const double *kq;     // k per standard index (kLattice values), length s
     const double *kq_h;   // k per standard index (k0 + p dk), Hartree
     const double *f0slot; // device scalar: dx^D sum n (Hartree)
+    const double2 *kek;   // optional (k, EXP(sigma k^2)) per standard index: replaces kq and ek lookups
     double dxD;           // dx^D
     double dkzero;        // 1e-6 dk^2
     int regularize;       // RegularizeCoulombSingularity
@@ -55,6 +56,14 @@ struct MulParams {
 // spectrum can feed several backward transforms.
 hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, const MulParams &m, double *re,
                              double *im, double scale, int post, double c1, const double2 *tw, hipStream_t st);
+// Forward transforms with DelField's pointwise preparation fused into the first
+// pass: the TaperOff shift (-*shift_dev holds the shell maximum) and taper of V
+// (T: taper_factor's table), or the sum of squares of the derivative fields
+// (sum_squares3; c may be null).
+hipError_t fft_forward_taper(const Grid &g, const double *V, const double *T, const double *shift_dev, double2 *spec,
+                             const double2 *tw, hipStream_t st);
+hipError_t fft_forward_sumsq(const Grid &g, const double *a, const double *b, const double *c, double2 *spec,
+                             const double2 *tw, hipStream_t st);
 
 // Isolated-boundary Hartree: zero-padded (2s)^D circular convolution with the
 // precomputed real kernel spectrum khat; out[i] += gammaH dx^D conv (i in [0,s]^D)
@@ -119,6 +128,12 @@ hipError_t linear_mix(size_t n_per_species, int S, const double *theta_dev, cons
 
 // Shift by the shell maximum and taper (DelField's TaperOff, mpDPFT.cpp:7251-7264).
 hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials, hipStream_t st);
+// The taper's geometry, once per grid: T[i] = EXP(-2500/L^2 (r - rt)^2) beyond rt, else 1
+// (k_taper's factor); flag[i] = 1 on the shell |r - rt| < 2 dx (k_shell_max's test).
+hipError_t taper_factor(const Grid &g, double rt, double *T, hipStream_t st);
+hipError_t shell_mark(const Grid &g, double rt, int *flag, hipStream_t st);
+// *out = min over j < n of -f[idx[j]] (k_shell_max over a precomputed shell list; partials: 64 doubles)
+hipError_t shell_max_list(const double *f, const int *idx, int n, double *partials, double *out, hipStream_t st);
 hipError_t taper(const Grid &g, const double *in, const double *shift_dev, double rt, double *out,
                  hipStream_t st);
 hipError_t sum_squares3(const Grid &g, const double *a, const double *b, const double *c, double *out,

@@ -156,6 +156,31 @@ struct LdGrid {  // real grid fields -> complex (im may be null)
         return v;
     }
 };
+// DelField's input with the TaperOff shift and taper fused: (V - max) T with
+// the per-point factor table T of k_taper_factor (1 inside the taper radius)
+struct LdGridTaper {
+    const double *re, *T;
+    const double *shift;  // device slot holding -max over the shell
+    int EL;
+    __device__ double2 operator()(int x, int y, int z) const {
+        int i = (x * EL + y) * EL + z;
+        double sft = -(*shift);
+        if (sft < -1e307 || sft > 1e307) sft = 0.;
+        return double2{(re[i] - sft) * T[i], 0.};
+    }
+};
+// |grad V|^2 = Dx^2 + Dy^2 (+ Dz^2) formed while loading (sum_squares3 fused)
+struct LdGridSumSq {
+    const double *a, *b, *c;
+    int EL;
+    __device__ double2 operator()(int x, int y, int z) const {
+        int i = (x * EL + y) * EL + z;
+        double v = a[i] * a[i];
+        if (b) v += b[i] * b[i];
+        if (c) v += c[i] * c[i];
+        return double2{v, 0.};
+    }
+};
 struct LdBuf {
     const double2 *p;
     int n1, n2;
@@ -177,10 +202,11 @@ struct StBufF0 {  // also captures dx^D * DC term
     }
 };
 
-// multiplier applied on the first backward load
-struct LdMul {
+// multiplier applied on the first backward load (D a compile-time constant, so
+// the per-axis lookups of a line's fixed coordinates are shared by its loads)
+template <int D, int KIND = -1> struct LdMulD {  // KIND >= 0: the multiplier kind fixed at compile time
     const double2 *p;
-    int n1, n2, D, s;
+    int n1, n2, Dr, s;
     MulParams m;
     __device__ __forceinline__ double smooth(const int *q) const {
         if (!m.ek) return 1.;
@@ -193,22 +219,31 @@ struct LdMul {
         int q3[3] = {x, y, z};
         int q[3];
         for (int d = 0; d < D; d++) q[d] = q3[3 - D + d];
-        double k[3] = {0., 0., 0.}, k2 = 0.;
-        for (int d = 0; d < D; d++) {
-            k[d] = m.kq[q[d]];
-            k2 += k[d] * k[d];
+        double k[3] = {0., 0., 0.}, k2 = 0., fk = 1.;
+        if (m.kek) {  // k and the smoothing factor from one table entry per axis
+            for (int d = 0; d < D; d++) {
+                double2 t = m.kek[q[d]];
+                k[d] = t.x;
+                k2 += k[d] * k[d];
+                fk = d == 0 ? t.y : fk * t.y;
+            }
+        } else {
+            for (int d = 0; d < D; d++) {
+                k[d] = m.kq[q[d]];
+                k2 += k[d] * k[d];
+            }
         }
         double2 o;
-        switch (m.kind) {
+        switch (KIND >= 0 ? KIND : m.kind) {
         case MUL_SMOOTH: {
-            double f = smooth(q);
+            double f = m.kek ? fk : smooth(q);
             o.x = v.x * f;
             o.y = v.y * f;
             return o;
         }
         case MUL_DERIV_PAIR:
         case MUL_DERIV_LAP: {
-            double f = smooth(q);
+            double f = m.kek ? fk : smooth(q);
             // A = i ka f v  (Nyquist plane of axis a dropped)
             double2 A = {0., 0.}, B = {0., 0.};
             if (m.axis_a >= 0 && q[m.axis_a] != s / 2) {
@@ -216,7 +251,7 @@ struct LdMul {
                 A.x = -c * v.y;
                 A.y = c * v.x;
             }
-            if (m.kind == MUL_DERIV_PAIR) {
+            if ((KIND >= 0 ? KIND : m.kind) == MUL_DERIV_PAIR) {
                 if (q[m.axis_b] != s / 2) {
                     double c = k[m.axis_b] * f;
                     B.x = -c * v.y;
@@ -302,11 +337,23 @@ struct StGrid {
         if (im) im[idx] = scale * v.y;
     }
     __device__ void operator()(int x, int y, int z, double2 v) const {
-        int xs[2] = {x, s}, ys[2] = {y, s}, zs[2] = {z, s};
-        int nx = (D == 3 && x == 0) ? 2 : 1, ny = (y == 0) ? 2 : 1, nz = (z == 0) ? 2 : 1;
-        for (int a = 0; a < nx; a++)
-            for (int b = 0; b < ny; b++)
-                for (int c = 0; c < nz; c++) put((xs[a] * EL + ys[b]) * EL + zs[c], v);
+        // the point and its periodic images (index 0 repeated at s along each axis)
+        const int i0 = (x * EL + y) * EL + z, sy = s * EL;
+        put(i0, v);
+        if (z == 0) put(i0 + s, v);
+        if (y == 0) {
+            put(i0 + sy, v);
+            if (z == 0) put(i0 + sy + s, v);
+        }
+        if (D == 3 && x == 0) {
+            const int ix = i0 + sy * EL;
+            put(ix, v);
+            if (z == 0) put(ix + s, v);
+            if (y == 0) {
+                put(ix + sy, v);
+                if (z == 0) put(ix + sy + s, v);
+            }
+        }
     }
 };
 
@@ -419,17 +466,64 @@ hipError_t fft_forward_real(const Grid &g, const double *re, const double *im, d
 hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, const MulParams &m, double *re,
                              double *im, double scale, int post, double c1, const double2 *tw, hipStream_t st) {
     int s = g.s;
-    LdMul lm{spec, s, s, g.D, s, m};
     StBuf sb{work, s, s};
     LdBuf lb{work, s, s};
     StGrid sg{re, im, scale, post, g.EL, s, g.D, g.lat, m.f0slot, c1};
     if (g.D == 3) {
-        CHK(pass<0>(s, lm, sb, s, s, +1, tw, st));
+        // DelField's multipliers (with the (k, smoothing) table) compiled per kind
+        if (m.kind == MUL_DERIV_PAIR && m.kek) {
+            LdMulD<3, MUL_DERIV_PAIR> lm{spec, s, s, 3, s, m};
+            CHK(pass<0>(s, lm, sb, s, s, +1, tw, st));
+        } else if (m.kind == MUL_DERIV_LAP && m.kek) {
+            LdMulD<3, MUL_DERIV_LAP> lm{spec, s, s, 3, s, m};
+            CHK(pass<0>(s, lm, sb, s, s, +1, tw, st));
+        } else if (m.kind == MUL_SMOOTH && m.kek) {
+            LdMulD<3, MUL_SMOOTH> lm{spec, s, s, 3, s, m};
+            CHK(pass<0>(s, lm, sb, s, s, +1, tw, st));
+        } else {
+            LdMulD<3> lm{spec, s, s, 3, s, m};
+            CHK(pass<0>(s, lm, sb, s, s, +1, tw, st));
+        }
         CHK(pass<1>(s, lb, sb, s, s, +1, tw, st));
         CHK(pass<2>(s, lb, sg, s, s, +1, tw, st));
     } else {
+        LdMulD<2> lm{spec, s, s, 2, s, m};
         CHK(pass<1>(s, lm, sb, 1, s, +1, tw, st));
         CHK(pass<2>(s, lb, sg, 1, s, +1, tw, st));
+    }
+    return hipSuccess;
+}
+
+hipError_t fft_forward_taper(const Grid &g, const double *V, const double *T, const double *shift_dev, double2 *spec,
+                             const double2 *tw, hipStream_t st) {
+    int s = g.s;
+    LdGridTaper lg{V, T, shift_dev, g.EL};
+    StBuf sb{spec, s, s};
+    LdBuf lb{spec, s, s};
+    if (g.D == 3) {
+        CHK(pass<2>(s, lg, sb, s, s, -1, tw, st));
+        CHK(pass<1>(s, lb, sb, s, s, -1, tw, st));
+        CHK(pass<0>(s, lb, sb, s, s, -1, tw, st));
+    } else {
+        CHK(pass<2>(s, lg, sb, 1, s, -1, tw, st));
+        CHK(pass<1>(s, lb, sb, 1, s, -1, tw, st));
+    }
+    return hipSuccess;
+}
+
+hipError_t fft_forward_sumsq(const Grid &g, const double *a, const double *b, const double *c, double2 *spec,
+                             const double2 *tw, hipStream_t st) {
+    int s = g.s;
+    LdGridSumSq lg{a, b, c, g.EL};
+    StBuf sb{spec, s, s};
+    LdBuf lb{spec, s, s};
+    if (g.D == 3) {
+        CHK(pass<2>(s, lg, sb, s, s, -1, tw, st));
+        CHK(pass<1>(s, lb, sb, s, s, -1, tw, st));
+        CHK(pass<0>(s, lb, sb, s, s, -1, tw, st));
+    } else {
+        CHK(pass<2>(s, lg, sb, 1, s, -1, tw, st));
+        CHK(pass<1>(s, lb, sb, 1, s, -1, tw, st));
     }
     return hipSuccess;
 }
@@ -916,6 +1010,51 @@ hipError_t linear_mix(size_t n_per_species, int S, const double *theta_dev, cons
                       hipStream_t st) {
     hipLaunchKernelGGL(k_linear_mix, dim3(gblocks(n_per_species * S)), dim3(256), 0, st, n_per_species, S,
                        theta_dev, old, den);
+    return hipGetLastError();
+}
+__global__ void k_taper_factor(Grid g, double rt, double *T) {
+    double a = -2500. / (g.edge * g.edge);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < g.G; i += (size_t)gridDim.x * blockDim.x) {
+        double r[3];
+        coords(g, (int)i, r);
+        double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        T[i] = rn > rt ? d_exp(a * (rn - rt) * (rn - rt)) : 1.;
+    }
+}
+__global__ void k_shell_mark(Grid g, double rt, int *flag) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < g.G; i += (size_t)gridDim.x * blockDim.x) {
+        double r[3];
+        coords(g, (int)i, r);
+        double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        flag[i] = fabs(rn - rt) < 2. * g.dx ? 1 : 0;
+    }
+}
+__global__ __launch_bounds__(256) void k_shell_max_list(const double *f, const int *idx, int n, double *partials) {
+    __shared__ double sh[4];
+    double m = 1e308;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) m = fmin(m, -f[idx[j]]);
+    double r = block_min(m, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+hipError_t taper_factor(const Grid &g, double rt, double *T, hipStream_t st) {
+    hipLaunchKernelGGL(k_taper_factor, dim3(gblocks(g.G)), dim3(256), 0, st, g, rt, T);
+    return hipGetLastError();
+}
+hipError_t shell_mark(const Grid &g, double rt, int *flag, hipStream_t st) {
+    hipLaunchKernelGGL(k_shell_mark, dim3(gblocks(g.G)), dim3(256), 0, st, g, rt, flag);
+    return hipGetLastError();
+}
+__global__ __launch_bounds__(256) void k_min_small(const double *partials, int n, double *out) {
+    __shared__ double sh[4];
+    double m = 1e308;
+    for (int j = threadIdx.x; j < n; j += 256) m = fmin(m, partials[j]);
+    double r = block_min(m, sh);
+    if (threadIdx.x == 0) *out = r;
+}
+hipError_t shell_max_list(const double *f, const int *idx, int n, double *partials, double *out, hipStream_t st) {
+    constexpr int kBlocks = 64;
+    hipLaunchKernelGGL(k_shell_max_list, dim3(kBlocks), dim3(256), 0, st, f, idx, n, partials);
+    hipLaunchKernelGGL(k_min_small, dim3(1), dim3(256), 0, st, (const double *)partials, kBlocks, out);
     return hipGetLastError();
 }
 hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials, hipStream_t st) {

@@ -293,6 +293,10 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
             ek[q] = x < -700. ? 0. : (x < 700. ? std::exp(x) : 1.0142320547350045e+304);
         }
         HIPC(hipMemcpy(c->d_ek, ek.data(), sizeof(double) * c->s, hipMemcpyHostToDevice));
+        std::vector<double2> kek(c->s);
+        for (int q = 0; q < c->s; q++) kek[q] = double2{kq[q], ek[q]};
+        RC(dmalloc(c, &c->d_kek, (size_t)c->s));
+        HIPC(hipMemcpy(c->d_kek, kek.data(), sizeof(double2) * c->s, hipMemcpyHostToDevice));
     }
     HIPC(hipMemcpy(c->d_kqh, kqh.data(), sizeof(double) * c->s, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d_tw, tw.data(), sizeof(double2) * c->s, hipMemcpyHostToDevice));
@@ -600,23 +604,27 @@ static int update_v(mpdpft_ctx *c, const double *den) {
 }
 
 // DelField methods 4/5 (:7243-7582) as 5 fused transforms (3D):
-// fwd(V); bwd(i kx e, i ky e) -> Dx,Dy; bwd(i kz e, -k^2 e^2) -> Dz,Lap;
-// fwd(sum D^2); bwd(e) -> POS -> Grad2.
+// fwd(taper(V)); bwd(i kx e, i ky e) -> Dx,Dy; bwd(i kz e, -k^2 e^2) -> Dz,Lap;
+// fwd(sum D^2); bwd(e) -> POS -> Grad2.  The taper and the sum of squares are
+// formed in the first pass of their forward transforms.
 static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method);
 int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method) {
     stage_begin(c);
     int rc = delfield_impl(c, f_dev, s, method);
     if (rc) return rc;
-    // HBM bytes the kernels move: a forward transform from a real grid reads
-    // 8F and writes 16F in its first pass, then 32F per pass; a backward one
-    // to k real grids moves 32F per pass and 16F + 8kG in its last; plus the
-    // taper (shell max 8G, taper 16G) and the square sum (8(D+1)G)
+    // HBM bytes the kernels move: a forward transform from k real grid fields
+    // reads 8kF and writes 16F in its first pass, then 32F per pass; a backward
+    // one to k real grids moves 32F per pass and 16F + 8kG in its last; the
+    // taper reads its factor table (8G) and the shell maximum 12 bytes per shell
+    // point; method 4 forms the square sum in its own kernel (8(D+1)G), method 5
+    // in the first pass of the forward transform of it
     const double F = (double)c->F, Gd = (double)c->G, D = (double)c->D;
-    auto fwd = [&]() { return 24. * F + 32. * F * (D - 1.); };
+    auto fwd = [&](double k) { return 8. * k * F + 16. * F + 32. * F * (D - 1.); };
     auto bwd = [&](double k) { return 32. * F * (D - 1.) + 16. * F + 8. * k * Gd; };
-    double bytes = fwd() + bwd(2.) + bwd(c->D == 3 ? 2. : 1.) + 8. * (D + 1.) * Gd;
-    if (method == 5) bytes += fwd() + bwd(1.);
-    if (c->p.TaperOff && (method == 4 || method == 5)) bytes += 24. * Gd;
+    double bytes = fwd(1.) + bwd(2.) + bwd(c->D == 3 ? 2. : 1.);
+    if (method == 5) bytes += fwd(D) + bwd(1.);
+    else bytes += 8. * (D + 1.) * Gd;
+    if (c->p.TaperOff && (method == 4 || method == 5)) bytes += 8. * Gd + 12. * std::max(0, c->n_shell);
     stage_end(c, "delfield", method == 5 ? 5. : 3., bytes, 0.);
     return MPDPFT_OK;
 }
@@ -624,23 +632,42 @@ static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) 
     const mpdpft_params &p = c->p;
     c->delfield_valid = -1;
     size_t G = c->G;
-    const double *src = f_dev;
-    if (p.TaperOff && (method == 4 || method == 5)) {
+    if (p.TaperOff && (method == 4 || method == 5)) {  // shift by the shell maximum, taper (fused)
         double rt = p.edge * p.RegularizationThreshold;
-        HIPC(mk::shell_max(c->g, f_dev, rt, c->d_partials, c->st));
-        HIPC(mk::finish_minmax(c->d_partials, 1, c->d_scal + 210, c->st));
-        HIPC(mk::taper(c->g, f_dev, c->d_scal + 210, p.RegularizationThreshold * p.edge, c->d_tmp3, c->st));
-        src = c->d_tmp3;
+        if (c->n_shell < 0) {  // the taper's geometry, once: factor table and shell point list
+            RC(dmalloc(c, &c->d_taperT, G));
+            HIPC(mk::taper_factor(c->g, rt, c->d_taperT, c->st));
+            int *flag = nullptr;
+            HIPC(hipMalloc((void **)&flag, sizeof(int) * G));
+            HIPC(mk::shell_mark(c->g, rt, flag, c->st));
+            std::vector<int> hf(G), idx;
+            hipError_t e = hipMemcpyAsync(hf.data(), flag, sizeof(int) * G, hipMemcpyDeviceToHost, c->st);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+            (void)hipFree(flag);
+            if (e != hipSuccess) return hip_fail(c, e, "shell_mark");
+            for (size_t i = 0; i < G; i++)
+                if (hf[i]) idx.push_back((int)i);
+            c->n_shell = (int)idx.size();
+            int *di = nullptr;
+            RC(dmalloc(c, &di, std::max<size_t>(1, idx.size())));
+            c->d_shell_idx = di;
+            if (!idx.empty())
+                HIPC(hipMemcpy(c->d_shell_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+        }
+        HIPC(mk::shell_max_list(f_dev, c->d_shell_idx, c->n_shell, c->d_partials, c->d_scal + 210, c->st));
+        HIPC(mk::fft_forward_taper(c->g, f_dev, c->d_taperT, c->d_scal + 210, c->d_spec, c->d_tw, c->st));
+    } else {
+        HIPC(mk::fft_forward_real(c->g, f_dev, nullptr, c->d_spec, c->d_tw, nullptr, c->st));
     }
     double sig = method == 5 ? c->sigma : 0.;
     double scale = 1. / (double)c->F;
     double *Dx = c->d_Dx + s * G, *Dy = c->d_Dy + s * G, *Dz = c->d_Dz + s * G;
     double *g2 = c->d_Grad2 + s * G, *lap = c->d_Lap + s * G;
-    HIPC(mk::fft_forward_real(c->g, src, nullptr, c->d_spec, c->d_tw, nullptr, c->st));
     mk::MulParams m{};
     m.kq = c->d_kq;
     m.sigma = sig;
     m.ek = method == 5 ? c->d_ek : nullptr;
+    m.kek = method == 5 ? c->d_kek : nullptr;
     m.kind = mk::MUL_DERIV_PAIR;
     m.axis_a = 0;
     m.axis_b = 1;
@@ -649,14 +676,15 @@ static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) 
     m.axis_a = c->D == 3 ? 2 : -1;
     HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_work, m, c->D == 3 ? Dz : nullptr, lap, scale, 0, 0., c->d_tw,
                                c->st));
-    HIPC(mk::sum_squares3(c->g, Dx, Dy, c->D == 3 ? Dz : nullptr, g2, c->st));
+    if (method != 5) HIPC(mk::sum_squares3(c->g, Dx, Dy, c->D == 3 ? Dz : nullptr, g2, c->st));
     if (method == 5) {
-        HIPC(mk::fft_forward_real(c->g, g2, nullptr, c->d_spec, c->d_tw, nullptr, c->st));
+        HIPC(mk::fft_forward_sumsq(c->g, Dx, Dy, c->D == 3 ? Dz : nullptr, c->d_spec, c->d_tw, c->st));
         mk::MulParams ms{};
         ms.kind = mk::MUL_SMOOTH;
         ms.kq = c->d_kq;
         ms.sigma = sig;
         ms.ek = c->d_ek;
+        ms.kek = c->d_kek;
         HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_spec, ms, g2, nullptr, scale, 1, 0., c->d_tw, c->st));
     }
     return MPDPFT_OK;

@@ -25,6 +25,10 @@ struct mpdpft_ctx {
     // device buffers
     double *d_lat = nullptr, *d_kq = nullptr, *d_kqh = nullptr;
     double *d_ek = nullptr;  // EXP(sigma kq^2) per standard index (DelField smoothing)
+    double2 *d_kek = nullptr;  // (kq, ek) per standard index
+    double *d_taperT = nullptr;   // DelField taper factor per grid point (built on first use)
+    int *d_shell_idx = nullptr;   // grid points of the taper's shell |r - rt| < 2 dx
+    int n_shell = -1;             // their number (-1: not built)
     double2 *d_tw = nullptr, *d_tw2 = nullptr;
     double2 *d_spec = nullptr, *d_work = nullptr, *d_pad = nullptr;
     double *d_khat = nullptr;
