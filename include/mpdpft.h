@@ -122,7 +122,9 @@ typedef struct mpdpft_params {
     int HartreeMode;       /* 0: periodic + Gaussian compensation (reference), 1: isolated, zero-padded 2s grid */
     double SmoothingKWidth;/* >0: DelField smoothing exp(-k^2/(2 w^2)); <=0: reference exp(-(6dx)^2 k^2/2) */
     int ConvergenceMode;   /* 0: reference overlap rule CC<SCcriterion; 1: max|V-V_old| < SCcriterion */
-    int Precision;         /* 0: FP64 throughout; 1: FP32 bulk + FP64 reductions/mu/Pulay + FP64 polish */
+    int Precision;         /* 0: FP64 throughout; 1: FP32 transforms, multipliers and pointwise interaction
+                              potentials (FP64 fields, reductions, mu, Pulay, density quadrature, energies)
+                              until the FP32 floor, then FP64 polish iterations; DIM 3, isolated Hartree */
     int WriteFiles;        /* 1: mpdpft_run_dir writes Den/V/Control files (default 1) */
     int n_centres;         /* environment 41 */
     double centres[MPDPFT_MAX_CENTRES][5]; /* x, y, z, Z, a */
@@ -147,6 +149,7 @@ typedef struct mpdpft_result {
     double AuxEkin[MPDPFT_MAX_SPECIES][4];
     double solve_seconds;  /* wall time of the SC loop (device-synchronised) */
     int bisection_steps;   /* total density evaluations inside mu bisection */
+    int fp32_iterations;   /* Precision 1: SC iterations run with the FP32 stages (the rest are FP64) */
 } mpdpft_result;
 
 typedef struct mpdpft_ctx mpdpft_ctx;

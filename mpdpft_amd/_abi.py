@@ -61,7 +61,7 @@ class Result(ctypes.Structure):
         ("FinalAbundances", _d * MAX_SPECIES), ("DeltamuModifier", _d), ("TotalEnergy", _d),
         ("DispersalEnergies", _d * MAX_SPECIES), ("EnvironmentalEnergies", _d * MAX_SPECIES),
         ("InteractionEnergies", _d * MAX_LIST), ("AuxEkin", (_d * 4) * MAX_SPECIES),
-        ("solve_seconds", _d), ("bisection_steps", _i),
+        ("solve_seconds", _d), ("bisection_steps", _i), ("fp32_iterations", _i),
     ]
 
 

@@ -24,7 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I
 # Packed FP32 instructions stay disabled in all device code.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["params.cpp", "kernels.hip", "solver.cpp", "nait.hip", "output.cpp", "batch.hip", "adjoint.hip"]
+LIB_SOURCES = ["params.cpp", "kernels.hip", "kernels_fp32.hip", "solver.cpp", "nait.hip", "output.cpp", "batch.hip",
+               "adjoint.hip"]
 
 
 def _compile(src):

@@ -100,6 +100,7 @@ struct VintParams {
     double beta, gammaH, inc;
     int units;
     int hartree_in_V;
+    int fp32;  // pointwise interaction potentials evaluated in FP32
 };
 hipError_t update_v(const Grid &g, const VintParams &vp, const double *den_all, const double *env, double *V,
                     hipStream_t st);
@@ -166,6 +167,22 @@ struct EnergyParams {
 };
 hipError_t energy_integrand(const Grid &g, const EnergyParams &ep, const double *a, const double *b,
                             const double *c, const double *d, double *partials, hipStream_t st);
+
+// ---- FP32 transforms of the precision variant (kernels_fp32.hip; 3D only).
+// Spectra, twiddles, multipliers and accumulation in FP32; grid fields FP64.
+// fft32_forward: V, or (V - max) T when T is given (shift_dev as fft_forward_taper).
+hipError_t fft32_forward(const Grid &g, const double *V, const double *T, const double *shift_dev, float2 *spec,
+                         const float2 *tw, hipStream_t st);
+hipError_t fft32_forward_sumsq(const Grid &g, const double *a, const double *b, const double *c, float2 *spec,
+                               const float2 *tw, hipStream_t st);
+// kind: MUL_DERIV_PAIR / MUL_DERIV_LAP / MUL_SMOOTH / MUL_NONE with kek[q] = (k, smoothing factor)
+hipError_t fft32_backward(const Grid &g, const float2 *spec, float2 *work, int kind, int axis_a, int axis_b,
+                          const float2 *kek, double *re, double *im, float scale, int post, const float2 *tw,
+                          hipStream_t st);
+hipError_t hartree_isolated32(const Grid &g, const double *n, float2 *pad, const float *khat, const float2 *tw2,
+                              float scale, double *out, hipStream_t st);
+hipError_t to_float(size_t n, const double *a, float *b, hipStream_t st);
+hipError_t to_float2(size_t n, const double2 *a, float2 *b, hipStream_t st);
 
 // Fill a device array with a value.
 hipError_t fill(size_t n, double v, double *a, hipStream_t st);

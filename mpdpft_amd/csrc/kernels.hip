@@ -129,6 +129,30 @@ __device__ void xc_lda(int id, int units, double n, double *e, double *v) {
     *v = ce * vv;
 }
 
+// FP32 forms of the pointwise potentials (the precision variant)
+__device__ float pw92_v_f(float n) {
+    const float A = 0.031091f, a1 = 0.21370f, b1 = 7.5957f, b2 = 3.5876f, b3 = 1.6382f, b4 = 0.49294f;
+    // below 1e-30 (r_s > 6e9) v_c ~ -a1 / (b4 r_s) < 1e-10, and r_s would overflow the FP32 forms
+    if (!(n > 1e-30f)) return 0.f;
+    float rs = cbrtf(3.f / (4.f * (float)PI_D * n));
+    float srs = sqrtf(rs);
+    float Q0 = -2.f * A * (1.f + a1 * rs);
+    float Q1 = 2.f * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+    float Q1p = A * (b1 / srs + 2.f * b2 + 3.f * b3 * srs + 4.f * b4 * rs);
+    float lg = logf(1.f + 1.f / Q1);
+    float ec = Q0 * lg;
+    float dec = -2.f * A * a1 * lg - Q0 * Q1p / (Q1 * Q1 + Q1);
+    return ec - rs / 3.f * dec;
+}
+__device__ float xc_lda_v_f(int id, int units, float n) {
+    const float Hartree = 27.211386245988f, Bohr = 0.529177210903f;
+    float cn = units == 2 ? Bohr * Bohr * Bohr : 1.f, ce = units == 2 ? Hartree : 1.f;
+    float rho = n > 0.f ? cn * n : 0.f;
+    if (!(rho > 0.f)) return 0.f;
+    if (id == -1) return -ce * cbrtf(3.f / (float)PI_D) * cbrtf(rho);
+    return ce * pw92_v_f(rho);
+}
+
 // d v / d n of the LDA potentials above (Units 2 conversion included)
 __device__ double xc_lda_dvdn(int id, int units, double n) {
     const double Hartree = 27.211386245988, Bohr = 0.529177210903;
@@ -698,10 +722,20 @@ __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const d
             } else if (t == 10) {
                 v = den[i] * (2. * vp.beta);
             } else if (t == 11) {
-                v = g.D > 1 ? -vp.gammaH * (2. * pow(d_pos(ns) * DD / PI_D, 1. / DD)) / (DD - 1.) : 0.;
+                if (vp.fp32) {
+                    float nf = (float)d_pos(ns), Df = (float)g.D;
+                    v = g.D > 1 ? (double)(-(float)vp.gammaH * (2.f * powf(nf * Df / (float)PI_D, 1.f / Df)) / (Df - 1.f))
+                                : 0.;
+                } else {
+                    v = g.D > 1 ? -vp.gammaH * (2. * pow(d_pos(ns) * DD / PI_D, 1. / DD)) / (DD - 1.) : 0.;
+                }
             } else if (t == -1 || t == -12) {
-                double e;
-                xc_lda(t, vp.units, ns, &e, &v);
+                if (vp.fp32) {
+                    v = (double)xc_lda_v_f(t, vp.units, (float)ns);
+                } else {
+                    double e;
+                    xc_lda(t, vp.units, ns, &e, &v);
+                }
             }
             acc += v;
         }

@@ -180,7 +180,15 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         if ((t == 3 || t == 40) && p->n_mpp < 2 * p->S) return ctx_fail(c, MPDPFT_EINVAL, "mpp needs 2S values");
         if (t == 1 && p->n_mpp < p->S) return ctx_fail(c, MPDPFT_EINVAL, "mpp needs S values");
     }
-    if (p->Precision != 0) return ctx_fail(c, MPDPFT_EINVAL, "Precision 1 is not built yet");
+    if (p->Precision != 0 && p->Precision != 1) return ctx_fail(c, MPDPFT_EINVAL, "Precision must be 0 or 1");
+    if (p->Precision == 1) {
+        if (p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "Precision 1: DIM 3");
+        for (int a = 0; a < p->n_interactions; a++)
+            if (p->Interactions[a] == 8 && p->HartreeMode != 1)
+                return ctx_fail(c, MPDPFT_EINVAL, "Precision 1: isolated Hartree (HartreeMode 1)");
+        if (p->DensityExpression == 5 && p->DelFieldMethod != 4 && p->DelFieldMethod != 5)
+            return ctx_fail(c, MPDPFT_EINVAL, "Precision 1: DelFieldMethod 4 or 5");
+    }
     if (p->InterpolVQ != 0 && p->InterpolVQ != 1)
         return ctx_fail(c, MPDPFT_EINVAL, "InterpolVQ must be 0 or 1 (2, 3 come from task loops)");
     for (int i = 0; i < p->n_schedule; i++) {
@@ -320,6 +328,28 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
                                : 4. * std::log(1. + std::sqrt(2.))) /
                     c->dx;
         HIPC(mk::build_isolated_kernel(c->g, c->d_pad, c->d_tw2, g0, c->d_khat, c->st));
+    }
+    if (p->Precision == 1) {  // FP32 spectra, twiddles, multiplier and kernel tables
+        RC(dmalloc(c, &c->d_specf, c->F));
+        RC(dmalloc(c, &c->d_workf, c->F));
+        RC(dmalloc(c, &c->d_twf, (size_t)c->s));
+        RC(dmalloc(c, &c->d_kekf, (size_t)c->s));
+        HIPC(mk::to_float2((size_t)c->s, c->d_tw, c->d_twf, c->st));
+        std::vector<double2> kek_h(c->s);
+        HIPC(hipMemcpy(kek_h.data(), c->d_kek, sizeof(double2) * c->s, hipMemcpyDeviceToHost));
+        std::vector<float2> kekf(c->s);
+        for (int q = 0; q < c->s; q++)
+            kekf[q] = float2{(float)kek_h[q].x, p->DelFieldMethod == 5 ? (float)kek_h[q].y : 1.f};
+        HIPC(hipMemcpy(c->d_kekf, kekf.data(), sizeof(float2) * c->s, hipMemcpyHostToDevice));
+        if (c->has_hartree) {
+            int M = 2 * c->s;
+            size_t tot = (size_t)M * M * M;
+            RC(dmalloc(c, &c->d_padf, tot));
+            RC(dmalloc(c, &c->d_khatf, tot));
+            RC(dmalloc(c, &c->d_tw2f, (size_t)M));
+            HIPC(mk::to_float2((size_t)M, c->d_tw2, c->d_tw2f, c->st));
+            HIPC(mk::to_float(tot, c->d_khat, c->d_khatf, c->st));
+        }
     }
 
     // environments and V <- Env (InitializeEnvironments, :6362-6390)
@@ -553,6 +583,14 @@ static double iso_bytes(const mpdpft_ctx *c) {
 static int hartree(mpdpft_ctx *c, const double *n, double *out) {
     const mpdpft_params &p = c->p;
     double dxD = std::pow(c->dx, (double)c->D);
+    if (p.HartreeMode == 1 && c->fp32_phase) {
+        double M = 2. * c->s, tot = std::pow(M, (double)c->D);
+        stage_begin(c);
+        HIPC(mk::hartree_isolated32(c->g, n, c->d_padf, c->d_khatf, c->d_tw2f, (float)(p.gammaH * dxD / tot), out,
+                                    c->st));
+        stage_end(c, "hartree_isolated_fp32", 1., 0.5 * iso_bytes(c) + 4. * std::pow(M, 3.), 0.);
+        return MPDPFT_OK;
+    }
     if (p.HartreeMode == 1) {
         double M = 2. * c->s, tot = std::pow(M, (double)c->D);
         stage_begin(c);
@@ -594,6 +632,7 @@ static int update_v(mpdpft_ctx *c, const double *den) {
     vp.inc = c->incV;
     vp.units = p.Units;
     vp.hartree_in_V = c->has_hartree;
+    vp.fp32 = c->fp32_phase;
     stage_begin(c);
     HIPC(mk::update_v(c->g, vp, den, c->d_Env, c->d_V, c->st));
     stage_end(c, "update_v", (double)c->SG, 8. * c->SG * (2. + c->S + c->has_hartree), 0.);
@@ -619,13 +658,14 @@ int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method) {
     // point; method 4 forms the square sum in its own kernel (8(D+1)G), method 5
     // in the first pass of the forward transform of it
     const double F = (double)c->F, Gd = (double)c->G, D = (double)c->D;
-    auto fwd = [&](double k) { return 8. * k * F + 16. * F + 32. * F * (D - 1.); };
-    auto bwd = [&](double k) { return 32. * F * (D - 1.) + 16. * F + 8. * k * Gd; };
+    const double cb = c->fp32_phase ? 8. : 16.;  // bytes of a complex spectrum value
+    auto fwd = [&](double k) { return 8. * k * F + cb * F + 2. * cb * F * (D - 1.); };
+    auto bwd = [&](double k) { return 2. * cb * F * (D - 1.) + cb * F + 8. * k * Gd; };
     double bytes = fwd(1.) + bwd(2.) + bwd(c->D == 3 ? 2. : 1.);
     if (method == 5) bytes += fwd(D) + bwd(1.);
     else bytes += 8. * (D + 1.) * Gd;
     if (c->p.TaperOff && (method == 4 || method == 5)) bytes += 8. * Gd + 12. * std::max(0, c->n_shell);
-    stage_end(c, "delfield", method == 5 ? 5. : 3., bytes, 0.);
+    stage_end(c, c->fp32_phase ? "delfield_fp32" : "delfield", method == 5 ? 5. : 3., bytes, 0.);
     return MPDPFT_OK;
 }
 static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) {
@@ -655,10 +695,31 @@ static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) 
                 HIPC(hipMemcpy(c->d_shell_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
         }
         HIPC(mk::shell_max_list(f_dev, c->d_shell_idx, c->n_shell, c->d_partials, c->d_scal + 210, c->st));
-        HIPC(mk::fft_forward_taper(c->g, f_dev, c->d_taperT, c->d_scal + 210, c->d_spec, c->d_tw, c->st));
-    } else {
-        HIPC(mk::fft_forward_real(c->g, f_dev, nullptr, c->d_spec, c->d_tw, nullptr, c->st));
     }
+    const bool taper = p.TaperOff && (method == 4 || method == 5);
+    if (c->fp32_phase) {  // the same transforms in FP32 (Precision 1)
+        const float sc = (float)(1. / (double)c->F);
+        double *Dx = c->d_Dx + s * G, *Dy = c->d_Dy + s * G, *Dz = c->d_Dz + s * G;
+        double *g2 = c->d_Grad2 + s * G, *lap = c->d_Lap + s * G;
+        HIPC(mk::fft32_forward(c->g, f_dev, taper ? c->d_taperT : nullptr, c->d_scal + 210, c->d_specf, c->d_twf,
+                               c->st));
+        HIPC(mk::fft32_backward(c->g, c->d_specf, c->d_workf, mk::MUL_DERIV_PAIR, 0, 1, c->d_kekf, Dx, Dy, sc, 0,
+                                c->d_twf, c->st));
+        HIPC(mk::fft32_backward(c->g, c->d_specf, c->d_workf, mk::MUL_DERIV_LAP, 2, -1, c->d_kekf, Dz, lap, sc, 0,
+                                c->d_twf, c->st));
+        if (method == 5) {
+            HIPC(mk::fft32_forward_sumsq(c->g, Dx, Dy, Dz, c->d_specf, c->d_twf, c->st));
+            HIPC(mk::fft32_backward(c->g, c->d_specf, c->d_specf, mk::MUL_SMOOTH, -1, -1, c->d_kekf, g2, nullptr, sc,
+                                    1, c->d_twf, c->st));
+        } else {
+            HIPC(mk::sum_squares3(c->g, Dx, Dy, Dz, g2, c->st));
+        }
+        return MPDPFT_OK;
+    }
+    if (taper)
+        HIPC(mk::fft_forward_taper(c->g, f_dev, c->d_taperT, c->d_scal + 210, c->d_spec, c->d_tw, c->st));
+    else
+        HIPC(mk::fft_forward_real(c->g, f_dev, nullptr, c->d_spec, c->d_tw, nullptr, c->st));
     double sig = method == 5 ? c->sigma : 0.;
     double scale = 1. / (double)c->F;
     double *Dx = c->d_Dx + s * G, *Dy = c->d_Dy + s * G, *Dz = c->d_Dz + s * G;
@@ -1202,6 +1263,34 @@ static int iteration(mpdpft_ctx *c) {
         if (c->TVec[s] < 0.) c->TVec[s] = std::max(std::fabs(c->muVec[s] - c->VMin[s]), std::fabs(c->muVec[s] - c->VMax[s])) / 50.;
     RC(get_densities(c));
     RC(check_convergence(c));
+    if (c->fp32_phase) {
+        // Precision 1: the FP32 stages end at max|dV| < max(criterion, 1e-6 max|V|) for three
+        // iterations (overlap mode: CC < 1e-12), or after 16 iterations without a new smallest CC;
+        // the loop then continues in FP64 (polish) with the instance's own criterion
+        double vmax = 0.;
+        for (int s = 0; s < c->S; s++) vmax = std::max(vmax, std::max(std::fabs(c->VMin[s]), std::fabs(c->VMax[s])));
+        double floorA = p.ConvergenceMode == 1 ? std::max(p.SCcriterion, 1e-6 * vmax) : std::max(p.SCcriterion, 1e-12);
+        bool stop = c->ExitSCloop != 0;
+        if (c->SCcount > 0) {
+            c->fp32_hits = c->CC < floorA ? c->fp32_hits + 1 : 0;
+            if (c->CC < c->fp32_best) {
+                c->fp32_best = c->CC;
+                c->fp32_stall = 0;
+            } else {
+                c->fp32_stall++;
+            }
+            if (c->fp32_hits >= 3 || c->fp32_stall >= 16) stop = true;
+        }
+        if (stop) {
+            c->fp32_phase = 0;
+            c->fp32_iterations = c->SCcount + 1;
+            c->ExitSCcount = 0;
+            c->ExitSCloop = 0;
+            c->delfield_valid = -1;
+            c->control << " ***** FP32 stages done after " << c->fp32_iterations << " iterations (CC = " << c->CC
+                       << "); FP64 polish *****\n";
+        }
+    }
     // Intercept (:4852-4879)
     if (c->PrintSC == 1)
         c->control << " ***** SCcount = " << c->SCcount << "(" << c->ExitSCcount << "/3);  CC = "
@@ -1257,6 +1346,11 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
         HIPC(hipMemcpyAsync(c->d_V, c->d_Env, sizeof(double) * c->SG, hipMemcpyDeviceToDevice, c->st));
     else  // LoadV (InterpolVQ = 1, :184)
         HIPC(hipMemcpyAsync(c->d_V, c->V0.data(), sizeof(double) * c->SG, hipMemcpyHostToDevice, c->st));
+    c->fp32_phase = p.Precision == 1 ? 1 : 0;
+    c->fp32_iterations = 0;
+    c->fp32_stall = 0;
+    c->fp32_hits = 0;
+    c->fp32_best = 1.0e300;
     RC(init_loop(c));
     c->PrintSC = p.Print > 0 ? 1 : 0;
     if (p.maxSCcount > 0) {
@@ -1286,6 +1380,7 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
     r->TotalEnergy = c->Etot;
     r->solve_seconds = c->solve_seconds;
     r->bisection_steps = c->bisection_steps;
+    r->fp32_iterations = c->fp32_iterations;
     for (int s = 0; s < c->S; s++) {
         r->muVec[s] = c->muVec[s];
         r->FinalAbundances[s] = c->TmpAb[s];

@@ -29,6 +29,12 @@ struct mpdpft_ctx {
     double *d_taperT = nullptr;   // DelField taper factor per grid point (built on first use)
     int *d_shell_idx = nullptr;   // grid points of the taper's shell |r - rt| < 2 dx
     int n_shell = -1;             // their number (-1: not built)
+    // precision variant (Precision 1): FP32 transforms until the FP32 floor, then FP64 polish
+    float2 *d_specf = nullptr, *d_workf = nullptr, *d_padf = nullptr, *d_twf = nullptr, *d_tw2f = nullptr,
+           *d_kekf = nullptr;
+    float *d_khatf = nullptr;
+    int fp32_phase = 0, fp32_iterations = 0, fp32_stall = 0, fp32_hits = 0;
+    double fp32_best = 0.;
     double2 *d_tw = nullptr, *d_tw2 = nullptr;
     double2 *d_spec = nullptr, *d_work = nullptr, *d_pad = nullptr;
     double *d_khat = nullptr;
