@@ -3,6 +3,7 @@
 C0: 2D two-species TF + contact gas (N = 100 each, 128^2 on [-16,16]^2).
 C1: 3D single-species electrons, softened Coulomb Z = 10, a = 0.5, N = 10,
     Hartree (isolated, zero-padded 256^3) + Dirac + PW92, 128^3 on a 20 bohr cube.
+C2: the precision variant of C1 at 256^3 (Z = N = 18, a = 0.4, 24 bohr).
 C3: seeded batches of 2D two-species TF + contact gases in anisotropic traps.
 C4: 3D cluster of softened Coulomb centres (environment 41) for the adjoint.
 """
@@ -126,6 +127,15 @@ def c1(steps=128, edge=20.0, Z=10.0, a=0.5, N=10.0, density_expression=7, hartre
     p.HartreeMode = hartree_mode
     p.SmoothingKWidth = kwidth
     p.Print = 0
+    return p
+
+
+def c2(steps=256, edge=24.0, Z=18.0, a=0.4, N=18.0, precision=1, tol=1e-8, maxsc=2000, T=0.05, kwidth=0.3):
+    """Precision variant of c1 at 256^3 on a 24 bohr cube, Z = 18, N = 18, a = 0.4;
+    precision 1: FP32 transforms and pointwise potentials, then FP64 polish."""
+    p = c1(steps=steps, edge=edge, Z=Z, a=a, N=N, density_expression=5, hartree_mode=1, kwidth=kwidth, tol=tol,
+           maxsc=maxsc, T=T)
+    p.Precision = precision
     return p
 
 

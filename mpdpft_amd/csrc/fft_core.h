@@ -267,7 +267,8 @@ template <int N> constexpr int lines_per_block() {
 
 template <int N, int AXIS, typename T2, class Ld, class St>
 inline hipError_t launch_pass(Ld ld, St st, int L0, int L1, int sign, const T2 *tw, hipStream_t stream) {
-    constexpr int B = lines_per_block<N>();
+    // FP32 lines are half the bytes: twice as many per block keeps the LDS tile and the contiguous runs
+    constexpr int B = lines_per_block<N>() * (sizeof(T2) == 8 ? 2 : 1);
     PassGeom g{L0, L1, sign};
     unsigned blocks = (unsigned)(L0 * ((L1 + B - 1) / B));
     if (blocks == 0) return hipSuccess;
