@@ -204,14 +204,12 @@ static std::string final_block(mpdpft_ctx *c, const mpdpft_result &r) {
     return t.str();
 }
 
-extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
-    std::string d = dir ? dir : "";
-    mpdpft_params p;
-    mpdpft_params_default(&p);
-    int rc = mpdpft_parse_input(join(d, "mpDPFT.input").c_str(), &p);
-    if (rc) return rc;
+// One GetData call (mpDPFT.cpp:135-244) in directory d with parameters p:
+// LoadV when InterpolVQ is 1, the SC loop, and ExportData when write_files.
+static int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
+                   const std::string &notes, double *mpp_out) {
     mpdpft_ctx *c = nullptr;
-    rc = mpdpft_create(&p, device, &c);
+    int rc = mpdpft_create(&p, device, &c);
     if (rc) {
         if (c) {
             FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");
@@ -223,6 +221,7 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
         mpdpft_destroy(c);
         return rc;
     }
+    c->notes << notes;
     mpdpft_set_break_file(c, join(d, "mpDPFTmanualSCloopBreakQ.dat").c_str());
     if (p.InterpolVQ == 1) {  // LoadV (:184)
         std::vector<double> v0(c->SG);
@@ -240,11 +239,13 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     mpdpft_result r;
     int src = mpdpft_solve(c, &r);
     if (res) *res = r;
+    if (mpp_out && p.DensityExpression == 5)  // the adapted (BoolePairs, maxIter) per species
+        for (int s = 0; s < c->S; s++) nait_state(c, s, &mpp_out[2 * s], &mpp_out[2 * s + 1]);
     if (src != MPDPFT_OK && src != MPDPFT_ENOTCONV) {
         mpdpft_destroy(c);
         return src;
     }
-    if (c->p.WriteFiles) {
+    if (write_files && c->p.WriteFiles) {
         std::vector<double> den(c->SG), V(c->SG);
         rc = mpdpft_get_density(c, den.data());
         if (!rc) rc = mpdpft_get_potential(c, V.data());
@@ -267,4 +268,75 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     }
     mpdpft_destroy(c);
     return rc ? rc : src;
+}
+
+// The outer task loop of main (mpDPFTmain.cpp:77-214) for task types 0, 1 and 8,
+// with RetrieveTask's per-task parameter changes (mpDPFT.cpp:6070-6121):
+// 1: rectangular grid of starting chemical potentials mu_s (1 + delta)^p_s,
+//    delta = 1.1^(1/AxesPoints) - 1 (two species); files of the last task only;
+// 8: successive refinement: task k runs on steps 2^k with InterpolVQ = 1 (LoadV of
+//    the previous task's mpDPFT_V.dat), the previous mu and adapted nAiT Mpp, and
+//    incrementalV = 1; every task writes its files.
+// mpDPFT_ControlTask.dat gets one line per task.
+extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
+    std::string d = dir ? dir : "";
+    mpdpft_params p;
+    mpdpft_params_default(&p);
+    int rc = mpdpft_parse_input(join(d, "mpDPFT.input").c_str(), &p);
+    if (rc) return rc;
+    const int type = p.TaskType;
+    if (type != 0 && type != 1 && type != 8) {
+        FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");
+        if (f) {
+            std::fprintf(f, "TaskType %d is not built (0, 1, 8 are)\n", type);
+            std::fclose(f);
+        }
+        return MPDPFT_EINVAL;
+    }
+    int count = (type == 0 || p.TaskParameterCount < 1) ? 1 : p.TaskParameterCount;
+    if (type == 1 && p.TaskParameterCount <= 1) count = 1;
+    std::ostringstream task;
+    task << std::setprecision(16);
+    task << " ***** mpdpft_hip task loop: TaskType " << type << ", " << count << " task(s) *****\n";
+    mpdpft_result r{};
+    double mpp[2 * MPDPFT_MAX_SPECIES] = {};
+    int last_rc = MPDPFT_OK;
+    for (int k = 0; k < count; k++) {
+        mpdpft_params q = p;
+        std::ostringstream notes;
+        bool files = type != 1 || k == count - 1;
+        if (type == 1 && count > 1) {
+            if (q.S < 2) return MPDPFT_EINVAL;
+            int axes = (int)(std::pow((double)count, 1. / (double)q.S) + 0.5);
+            int p0 = (k - k % axes) / axes, p1 = k % axes;
+            double delta = std::pow(1.1, 1. / (double)axes) - 1.;
+            q.muVec[0] *= std::pow(1. + delta, (double)p0);
+            q.muVec[1] *= std::pow(1. + delta, (double)p1);
+            notes << "-------> TASK = " << k + 1 << "/" << count << "\n";
+        } else if (type == 8 && k > 0) {
+            q.steps = p.steps * (int)(std::pow(2., (double)k) + 0.5);
+            q.InterpolVQ = 1;
+            if (q.incrementalV < 1.) q.incrementalV = 1.;
+            for (int s = 0; s < q.S; s++) q.muVec[s] = r.muVec[s];
+            if (q.DensityExpression == 5) {  // the adapted Mpp (one list for all species here)
+                q.Mpp[0] = mpp[0];
+                q.Mpp[1] = mpp[1];
+                if (k == count - 1) q.DeltamuModifier = r.DeltamuModifier;
+            }
+            notes << "successive InterpolV; task.count = " << k << "\nsteps -> " << q.steps << "\n";
+        }
+        last_rc = run_one(d, q, device, &r, files, notes.str(), mpp);
+        if (last_rc != MPDPFT_OK && last_rc != MPDPFT_ENOTCONV) return last_rc;
+        task << "task " << k << ": steps " << q.steps << "  muVec " << vec_to_str(r.muVec, q.S) << "  NVec "
+             << vec_to_str(r.FinalAbundances, q.S) << "  TotalEnergy " << r.TotalEnergy << "  SCiterations "
+             << r.SCcount - 1 << "  converged " << r.converged << "\n";
+    }
+    if (res) *res = r;
+    FILE *f = std::fopen(join(d, "mpDPFT_ControlTask.dat").c_str(), "w");
+    if (f) {
+        std::string sc = task.str();
+        std::fwrite(sc.data(), 1, sc.size(), f);
+        std::fclose(f);
+    }
+    return last_rc;
 }

@@ -157,7 +157,8 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         return ctx_fail(c, MPDPFT_EINVAL, "DensityExpression must be 7 (TF) or 5 (nAiT)");
     if (p->DensityExpression == 5 && p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "nAiT is built for DIM 3");
     if (p->method != 1) return ctx_fail(c, MPDPFT_EINVAL, "integration method must be 1 (Riemann)");
-    if (p->TaskType != 0) return ctx_fail(c, MPDPFT_EINVAL, "TaskType must be 0");
+    if (p->TaskType != 0 && p->TaskType != 1 && p->TaskType != 8)  // the task loop itself is mpdpft_run_dir's
+        return ctx_fail(c, MPDPFT_EINVAL, "TaskType must be 0, 1 or 8");
     if (p->Symmetry != 0) return ctx_fail(c, MPDPFT_EINVAL, "Symmetry must be 0");
     if (p->Noise != 0.) return ctx_fail(c, MPDPFT_EINVAL, "Noise must be 0");
     if ((int)(p->Mixer[0] + 0.5) > 1) return ctx_fail(c, MPDPFT_EINVAL, "Mixer[0] must be 0 or 1");
