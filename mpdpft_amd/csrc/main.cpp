@@ -1,6 +1,7 @@
-// main.cpp -- mpdpft_hip: drop-in for ./mpDPFT (type-0 task).  Runs in the
-// given directory (default: the current one), reading mpDPFT.input and
-// writing mpDPFT_Den.dat, mpDPFT_V.dat and mpDPFT_Control.dat.
+// main.cpp -- mpdpft_hip: drop-in for ./mpDPFT (task types 0, 1 and 8).  Runs
+// in the given directory (default: the current one), reading mpDPFT.input and
+// writing mpDPFT_Den.dat, mpDPFT_V.dat, mpDPFT_Control.dat and
+// mpDPFT_ControlTask.dat.
 #include <cstdio>
 
 #include "mpdpft.h"
