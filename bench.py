@@ -209,6 +209,34 @@ def adjoint_leg(steps, h):
     }
 
 
+def precision_leg(steps):
+    """C2: the same solve in FP64 and as Precision 1 (FP32 transforms and pointwise
+    potentials until the FP32 floor, then FP64 polish); errors and times."""
+    import numpy as np
+    from mpdpft_amd import Context, configs
+    out = {"workload": "C2: %d^3 nAiT(T=0.05) + isolated Hartree + Dirac + PW92 on a 24 bohr cube, Z=18, a=0.4, "
+                       "N=18, max|dV|<1e-8" % steps}
+    res = {}
+    for prec in (0, 1):
+        ctx = Context(configs.c2(steps=steps, precision=prec))
+        ctx.enable_timing(True)
+        t0 = time.perf_counter()
+        r = ctx.solve()
+        t = time.perf_counter() - t0
+        res[prec] = (r, ctx.density()[0].copy(), t)
+        stats = ctx.stage_stats()
+        ctx.close()
+        out["fp64" if prec == 0 else "precision1"] = {
+            "seconds": round(t, 2), "sc_iterations": r.SCcount - 1, "fp32_iterations": r.fp32_iterations,
+            "converged": bool(r.converged), "total_energy": r.TotalEnergy, "mu": r.muVec[0],
+            "stage_ms": {k: round(v["total_ms"], 1) for k, v in stats.items() if v["launches"]}}
+    (r0, n0, _), (r1, n1, _) = res[0], res[1]
+    out["rel_dE"] = abs(r1.TotalEnergy - r0.TotalEnergy) / abs(r0.TotalEnergy)
+    out["rel_dmu"] = abs(r1.muVec[0] - r0.muVec[0]) / abs(r0.muVec[0])
+    out["max_dn_over_max_n"] = float(np.abs(n1 - n0).max() / np.abs(n0).max())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +251,8 @@ def main():
                     help="instances the plain FP64 loop solves for the batch comparison")
     ap.add_argument("--batch-cpu", type=int, default=16, help="instances the oracle solves for the batch baseline")
     ap.add_argument("--adjoint-grid", type=int, default=96, help="grid of the differentiation workload (0: skip)")
+    ap.add_argument("--precision-grid", type=int, default=0,
+                    help="grid of the precision-variant comparison (256 = C2; 0: skip, about 15 min at 256)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "nait_reuse_128_traffic.md"),
                     help="rocprof summary holding 'traffic_bytes_per_launch = N' for the nAiT kernel at this grid")
     args = ap.parse_args()
@@ -290,6 +320,11 @@ def main():
         print("bench: batched workload, %d instances" % args.batch, file=sys.stderr, flush=True)
         batch = batch_leg(args.batch, args.batch_plain, args.batch_cpu, not args.no_cpu_baseline)
 
+    precision = None
+    if args.precision_grid > 0:
+        print("bench: precision variant, %d^3" % args.precision_grid, file=sys.stderr, flush=True)
+        precision = precision_leg(args.precision_grid)
+
     adjoint = None
     if args.adjoint_grid > 0:
         print("bench: differentiation workload, %d^3" % args.adjoint_grid, file=sys.stderr, flush=True)
@@ -319,6 +354,7 @@ def main():
         "stage_ms": stages,
         "batch": batch,
         "adjoint": adjoint,
+        "precision": precision,
     }
     print(json.dumps(line), flush=True)
 
