@@ -127,16 +127,15 @@ hipError_t pulay_mix(const Grid &g, int S, int I, const double *const *nin, cons
 hipError_t linear_mix(size_t n_per_species, int S, const double *theta_dev, const double *old, double *den,
                       hipStream_t st);
 
-// Shift by the shell maximum and taper (DelField's TaperOff, mpDPFT.cpp:7251-7264).
+// DelField's TaperOff shift by the shell maximum (mpDPFT.cpp:7251-7264); the taper itself is
+// applied in fft_forward_taper's first pass.
 hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials, hipStream_t st);
 // The taper's geometry, once per grid: T[i] = EXP(-2500/L^2 (r - rt)^2) beyond rt, else 1
-// (k_taper's factor); flag[i] = 1 on the shell |r - rt| < 2 dx (k_shell_max's test).
+// (DelField's taper, mpDPFT.cpp:7251-7264); flag[i] = 1 on the shell |r - rt| < 2 dx (k_shell_max's test).
 hipError_t taper_factor(const Grid &g, double rt, double *T, hipStream_t st);
 hipError_t shell_mark(const Grid &g, double rt, int *flag, hipStream_t st);
 // *out = min over j < n of -f[idx[j]] (k_shell_max over a precomputed shell list; partials: 64 doubles)
 hipError_t shell_max_list(const double *f, const int *idx, int n, double *partials, double *out, hipStream_t st);
-hipError_t taper(const Grid &g, const double *in, const double *shift_dev, double rt, double *out,
-                 hipStream_t st);
 hipError_t sum_squares3(const Grid &g, const double *a, const double *b, const double *c, double *out,
                         hipStream_t st);
 

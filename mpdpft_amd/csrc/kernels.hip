@@ -905,19 +905,6 @@ __global__ __launch_bounds__(256) void k_shell_max(Grid g, const double *f, doub
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
-__global__ void k_taper(Grid g, const double *in, const double *shift, double rt, double *out) {
-    double sft = -(*shift);  // shift slot holds -max
-    if (sft < -1e307 || sft > 1e307) sft = 0.;
-    double a = -2500. / (g.edge * g.edge);
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < g.G; i += (size_t)gridDim.x * blockDim.x) {
-        double r[3];
-        coords(g, (int)i, r);
-        double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        double v = in[i] - sft;
-        if (rn > rt) v *= d_exp(a * (rn - rt) * (rn - rt));
-        out[i] = v;
-    }
-}
 
 __global__ void k_sum_squares3(size_t G, const double *a, const double *b, const double *c, double *out) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < G; i += (size_t)gridDim.x * blockDim.x) {
@@ -1093,11 +1080,6 @@ hipError_t shell_max_list(const double *f, const int *idx, int n, double *partia
 }
 hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials, hipStream_t st) {
     hipLaunchKernelGGL(k_shell_max, dim3(kRedBlocks), dim3(256), 0, st, g, f, rt, partials);
-    return hipGetLastError();
-}
-hipError_t taper(const Grid &g, const double *in, const double *shift_dev, double rt, double *out,
-                 hipStream_t st) {
-    hipLaunchKernelGGL(k_taper, dim3(gblocks(g.G)), dim3(256), 0, st, g, in, shift_dev, rt, out);
     return hipGetLastError();
 }
 hipError_t sum_squares3(const Grid &g, const double *a, const double *b, const double *c, double *out,

@@ -16,7 +16,7 @@ namespace {
 
 using fftk::launch_pass;
 
-// DelField input: V, or (V - max) T with the taper table T (k_taper's factor)
+// DelField input: V, or (V - max) T with the taper table T (taper_factor)
 struct F32LdTaper {
     const double *re, *T, *shift;
     int EL;
