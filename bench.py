@@ -46,6 +46,20 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+def host_cpu():
+    """Model name of the host CPU (first entry of /proc/cpuinfo) and os.cpu_count()."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return "%s (%s logical CPUs visible)" % (name, os.cpu_count())
+
+
 def cpu_baseline(p, mu, V, density_evals, budget_s):
     """Oracle (plain C, OpenMP over all host threads) timing of one nAiT
     density evaluation at the solve's own grid and converged potential,
@@ -66,8 +80,8 @@ def cpu_baseline(p, mu, V, density_evals, budget_s):
         "kind": "port",
         "sample": ("one nAiT density evaluation (DelField + per-point adaptive Airy quadrature) of the oracle at "
                    "%d^3 on the converged potential took %.2f s; a solve makes %d such evaluations "
-                   "(mu bracketing/bisection included) -> %.0f s per solve" % (p.steps, t_eval, density_evals,
-                                                                                  solve_s)),
+                   "(mu bracketing/bisection included) -> %.0f s per solve; host: %s" % (
+                       p.steps, t_eval, density_evals, solve_s, host_cpu())),
     }
 
 
@@ -144,7 +158,8 @@ def batch_leg(n, plain_n, cpu_n, cpu_on):
         out["cpu_baseline"] = {"value": round(cpu_n / t_cpu, 3), "unit": "solves/s", "cores": cores,
                                "kind": "port",
                                "sample": "the oracle's SC loop on the first %d instances, one after another "
-                                         "(OpenMP over %d threads): %.2f s" % (cpu_n, cores, t_cpu)}
+                                         "(OpenMP over %d threads): %.2f s; host: %s" % (cpu_n, cores, t_cpu,
+                                                                                         host_cpu())}
     return out
 
 
