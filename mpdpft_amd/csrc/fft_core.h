@@ -265,15 +265,40 @@ template <int N> constexpr int lines_per_block() {
     return N <= 64 ? 32 : (N <= 192 ? 16 : 8);
 }
 
-template <int N, int AXIS, typename T2, class Ld, class St>
-inline hipError_t launch_pass(Ld ld, St st, int L0, int L1, int sign, const T2 *tw, hipStream_t stream) {
-    // FP32 lines are half the bytes: twice as many per block keeps the LDS tile and the contiguous runs
-    constexpr int B = lines_per_block<N>() * (sizeof(T2) == 8 ? 2 : 1);
+// Block shape choice per transform length and precision, set by the
+// autotuner (mk::fft_autotune): 0 = half the default lines per block,
+// 1 = the default, 2 = twice the default.  Host-side table, default 1.
+inline int &bsel_slot(int N, bool fp32) {
+    static int table[2][10] = {{1, 1, 1, 1, 1, 1, 1, 1, 1, 1}, {1, 1, 1, 1, 1, 1, 1, 1, 1, 1}};
+    int k = N == 16 ? 0 : N == 32 ? 1 : N == 64 ? 2 : N == 96 ? 3 : N == 128 ? 4 : N == 192 ? 5 : N == 256 ? 6
+          : N == 512 ? 7 : 8;
+    return table[fp32 ? 1 : 0][k];
+}
+
+template <int N, int B, int AXIS, typename T2, class Ld, class St>
+inline hipError_t launch_pass_b(Ld ld, St st, int L0, int L1, int sign, const T2 *tw, hipStream_t stream) {
     PassGeom g{L0, L1, sign};
     unsigned blocks = (unsigned)(L0 * ((L1 + B - 1) / B));
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL((fft_pass<N, B, AXIS, T2, Ld, St>), dim3(blocks), dim3(kThreads), 0, stream, ld, st, g, tw);
     return hipGetLastError();
+}
+
+// LDS of a pass (lines + twiddles) within the 160 KB of a CU
+template <int N, int B, typename T2> constexpr bool lds_fits() { return (B * (N + 1) + N) * sizeof(T2) <= 160 * 1024; }
+
+template <int N, int AXIS, typename T2, class Ld, class St>
+inline hipError_t launch_pass(Ld ld, St st, int L0, int L1, int sign, const T2 *tw, hipStream_t stream) {
+    // FP32 lines are half the bytes: twice as many per block keeps the LDS tile and the contiguous runs
+    constexpr bool f32 = sizeof(T2) == 8;
+    constexpr int B1 = lines_per_block<N>() * (f32 ? 2 : 1);
+    constexpr int B0 = B1 / 2 > 0 ? B1 / 2 : 1, B2 = B1 * 2;
+    const int sel = bsel_slot(N, f32);
+    if (sel == 0) return launch_pass_b<N, B0, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    if constexpr (lds_fits<N, B2, T2>()) {
+        if (sel == 2) return launch_pass_b<N, B2, AXIS>(ld, st, L0, L1, sign, tw, stream);
+    }
+    return launch_pass_b<N, B1, AXIS>(ld, st, L0, L1, sign, tw, stream);
 }
 
 }  // namespace fftk

@@ -72,6 +72,14 @@ hipError_t fft_forward_sumsq(const Grid &g, const double *a, const double *b, co
 hipError_t hartree_isolated(const Grid &g, const double *n, double2 *pad, const double *khat,
                             const double2 *tw2, double gamma_dxD_over_M, double *out, int accumulate,
                             hipStream_t st, int transpose = 0);
+// Block-shape autotuner: times each lines-per-block choice of the s-point transform
+// (a forward transform of 'field') and, with pad given, of the 2s-point isolated
+// Hartree convolution, and keeps the fastest for the process (results do not
+// depend on the choice: every line is transformed by the same arithmetic).
+hipError_t fft_autotune(const Grid &g, const double *field, double2 *spec, const double2 *tw, double2 *pad,
+                        const double *khat, const double2 *tw2, double *out, hipStream_t st);
+hipError_t fft32_autotune(const Grid &g, const double *field, float2 *spec, const float2 *tw, float2 *pad,
+                          const float *khat, const float2 *tw2, double *out, hipStream_t st);
 // Build khat = DFT((2s)^D kernel 1/r, g0 at r=0) into pad then copy real part to khat.
 hipError_t build_isolated_kernel(const Grid &g, double2 *pad, const double2 *tw2, double g0, double *khat,
                                  hipStream_t st);

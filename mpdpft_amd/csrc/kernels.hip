@@ -518,6 +518,48 @@ hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, 
     return hipSuccess;
 }
 
+// Time each block shape of a transform and keep the fastest (once per length
+// and precision per process).  run() enqueues one transform on st.
+template <class Run> hipError_t tune_one(int N, bool fp32, Run run, hipStream_t st) {
+    static bool done[2][10] = {};
+    int k = N == 16 ? 0 : N == 32 ? 1 : N == 64 ? 2 : N == 96 ? 3 : N == 128 ? 4 : N == 192 ? 5 : N == 256 ? 6
+          : N == 512 ? 7 : 8;
+    if (done[fp32 ? 1 : 0][k]) return hipSuccess;
+    hipEvent_t e0, e1;
+    CHK(hipEventCreate(&e0));
+    CHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bestSel = 1;
+    for (int sel = 0; sel < 3; sel++) {
+        fftk::bsel_slot(N, fp32) = sel;
+        CHK(run());  // warm-up (first launch of this shape)
+        CHK(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; r++) CHK(run());
+        CHK(hipEventRecord(e1, st));
+        CHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bestSel = sel;
+        }
+    }
+    fftk::bsel_slot(N, fp32) = bestSel;
+    done[fp32 ? 1 : 0][k] = true;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return hipSuccess;
+}
+
+hipError_t fft_autotune(const Grid &g, const double *field, double2 *spec, const double2 *tw, double2 *pad,
+                        const double *khat, const double2 *tw2, double *out, hipStream_t st) {
+    CHK(tune_one(g.s, false, [&]() { return fft_forward_real(g, field, nullptr, spec, tw, nullptr, st); }, st));
+    if (pad && g.D == 3)
+        CHK(tune_one(2 * g.s, false, [&]() { return hartree_isolated(g, field, pad, khat, tw2, 1., out, 0, st); },
+                     st));
+    return hipSuccess;
+}
+
 hipError_t fft_forward_taper(const Grid &g, const double *V, const double *T, const double *shift_dev, double2 *spec,
                              const double2 *tw, hipStream_t st) {
     int s = g.s;

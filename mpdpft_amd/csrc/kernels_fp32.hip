@@ -256,6 +256,44 @@ hipError_t hartree_isolated32(const Grid &g, const double *n, float2 *pad, const
     return hipSuccess;
 }
 
+hipError_t fft32_autotune(const Grid &g, const double *field, float2 *spec, const float2 *tw, float2 *pad,
+                          const float *khat, const float2 *tw2, double *out, hipStream_t st) {
+    if (g.D != 3) return hipErrorInvalidValue;
+    static bool done[10] = {};
+    auto tune = [&](int N, auto run) -> hipError_t {
+        int k = N == 16 ? 0 : N == 32 ? 1 : N == 64 ? 2 : N == 96 ? 3 : N == 128 ? 4 : N == 192 ? 5 : N == 256 ? 6
+              : N == 512 ? 7 : 8;
+        if (done[k]) return hipSuccess;
+        hipEvent_t e0, e1;
+        CHK(hipEventCreate(&e0));
+        CHK(hipEventCreate(&e1));
+        float best = 1e30f;
+        int bestSel = 1;
+        for (int sel = 0; sel < 3; sel++) {
+            fftk::bsel_slot(N, true) = sel;
+            CHK(run());
+            CHK(hipEventRecord(e0, st));
+            for (int r = 0; r < 3; r++) CHK(run());
+            CHK(hipEventRecord(e1, st));
+            CHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bestSel = sel;
+            }
+        }
+        fftk::bsel_slot(N, true) = bestSel;
+        done[k] = true;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return hipSuccess;
+    };
+    CHK(tune(g.s, [&]() { return fft32_forward(g, field, nullptr, nullptr, spec, tw, st); }));
+    if (pad) CHK(tune(2 * g.s, [&]() { return hartree_isolated32(g, field, pad, khat, tw2, 1.f, out, st); }));
+    return hipSuccess;
+}
+
 hipError_t to_float(size_t n, const double *a, float *b, hipStream_t st) {
     hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, st, n, a, b);
     return hipGetLastError();

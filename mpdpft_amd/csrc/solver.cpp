@@ -360,6 +360,17 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
         HIPC(mk::environment(c->g, e, c->d_Env + s * G, c->st));
     }
     HIPC(hipMemcpyAsync(c->d_V, c->d_Env, sizeof(double) * SG, hipMemcpyDeviceToDevice, c->st));
+    {  // FFT block shapes: time the candidates once per length and precision (MPDPFT_FFT_TUNE=0: defaults)
+        const char *tune = std::getenv("MPDPFT_FFT_TUNE");
+        if (!(tune && tune[0] == '0')) {
+            const bool iso = c->has_hartree && p->HartreeMode == 1;
+            HIPC(mk::fft_autotune(c->g, c->d_Env, c->d_spec, c->d_tw, iso ? c->d_pad : nullptr, c->d_khat, c->d_tw2,
+                                  c->d_tmp1, c->st));
+            if (p->Precision == 1)
+                HIPC(mk::fft32_autotune(c->g, c->d_Env, c->d_specf, c->d_twf, iso ? c->d_padf : nullptr, c->d_khatf,
+                                        c->d_tw2f, c->d_tmp1, c->st));
+        }
+    }
     HIPC(hipStreamSynchronize(c->st));
     for (int s = 0; s < c->S; s++) {  // scalar state used by the stage entry points
         c->muVec[s] = p->muVec[s];
