@@ -28,7 +28,7 @@ N = 32 and the functional above; the loss int (n - n_target)^2 (target: the
 centres of seed 434343) and the total energy are differentiated with respect to
 the 96 centre coordinates by the implicit adjoint of the converged fixed point,
 and one coordinate of each is checked against central differences (h = 1e-4)
-of warm-started solves.
+of solves started from the converged potential and stopped at max|dV| < 2e-11.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128]
 """
@@ -193,17 +193,21 @@ def adjoint_leg(steps, h):
     t0 = time.perf_counter()
     forces = ctx.centre_forces()
     t_forces = time.perf_counter() - t0
+    ctx.close()
+    # central differences from solves stopped at max|dV| < 2e-11 (their stopping noise over 2h is the
+    # difference's error), each started from the converged V
     C = np.array([[q[0], q[1], q[2], 1.0, 0.6] for q in pos])
     ci, d = 0, 0
     vals = []
+    fd = Context(configs.c4(steps=steps, tol=2e-11, maxsc=500, positions=pos))
     for sgn in (1, -1):
         Cp = C.copy()
         Cp[ci, d] += sgn * h
-        ctx.set_centres(Cp)
-        ctx.set_initial_potential(V)
-        rr = ctx.solve()
-        vals.append((float(np.sum(w * (ctx.density()[0] - nt) ** 2)), rr.TotalEnergy, bool(rr.converged)))
-    ctx.close()
+        fd.set_centres(Cp)
+        fd.set_initial_potential(V)
+        rr = fd.solve()
+        vals.append((float(np.sum(w * (fd.density()[0] - nt) ** 2)), rr.TotalEnergy, bool(rr.converged)))
+    fd.close()
     fdL = (vals[0][0] - vals[1][0]) / (2 * h)
     fdE = (vals[0][1] - vals[1][1]) / (2 * h)
     return {

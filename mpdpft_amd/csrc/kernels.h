@@ -142,8 +142,10 @@ hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials
 // (DelField's taper, mpDPFT.cpp:7251-7264); flag[i] = 1 on the shell |r - rt| < 2 dx (k_shell_max's test).
 hipError_t taper_factor(const Grid &g, double rt, double *T, hipStream_t st);
 hipError_t shell_mark(const Grid &g, double rt, int *flag, hipStream_t st);
-// *out = min over j < n of -f[idx[j]] (k_shell_max over a precomputed shell list; partials: 64 doubles)
-hipError_t shell_max_list(const double *f, const int *idx, int n, double *partials, double *out, hipStream_t st);
+// *out = min over j < n of -f[idx[j]] (k_shell_max over a precomputed shell list) in one launch;
+// partials: 64 doubles; ticket: a device counter that starts at 0 (left at 0)
+hipError_t shell_max_list(const double *f, const int *idx, int n, double *partials, unsigned *ticket, double *out,
+                          hipStream_t st);
 hipError_t sum_squares3(const Grid &g, const double *a, const double *b, const double *c, double *out,
                         hipStream_t st);
 

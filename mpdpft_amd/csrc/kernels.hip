@@ -1092,12 +1092,31 @@ __global__ void k_shell_mark(Grid g, double rt, int *flag) {
         flag[i] = fabs(rn - rt) < 2. * g.dx ? 1 : 0;
     }
 }
-__global__ __launch_bounds__(256) void k_shell_max_list(const double *f, const int *idx, int n, double *partials) {
+// Block minima, then the last block to finish (ticket counter) reduces them and
+// resets the counter: one launch.
+__global__ __launch_bounds__(256) void k_shell_max_list(const double *f, const int *idx, int n, double *partials,
+                                                        unsigned *ticket, double *out) {
     __shared__ double sh[4];
+    __shared__ bool last;
     double m = 1e308;
     for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) m = fmin(m, -f[idx[j]]);
     double r = block_min(m, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = r;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double v = 1e308;
+        for (int j = threadIdx.x; j < (int)gridDim.x; j += 256) v = fmin(v, partials[j]);
+        double t = block_min(v, sh);
+        if (threadIdx.x == 0) {
+            *out = t;
+            *ticket = 0u;
+        }
+    }
 }
 hipError_t taper_factor(const Grid &g, double rt, double *T, hipStream_t st) {
     hipLaunchKernelGGL(k_taper_factor, dim3(gblocks(g.G)), dim3(256), 0, st, g, rt, T);
@@ -1107,17 +1126,10 @@ hipError_t shell_mark(const Grid &g, double rt, int *flag, hipStream_t st) {
     hipLaunchKernelGGL(k_shell_mark, dim3(gblocks(g.G)), dim3(256), 0, st, g, rt, flag);
     return hipGetLastError();
 }
-__global__ __launch_bounds__(256) void k_min_small(const double *partials, int n, double *out) {
-    __shared__ double sh[4];
-    double m = 1e308;
-    for (int j = threadIdx.x; j < n; j += 256) m = fmin(m, partials[j]);
-    double r = block_min(m, sh);
-    if (threadIdx.x == 0) *out = r;
-}
-hipError_t shell_max_list(const double *f, const int *idx, int n, double *partials, double *out, hipStream_t st) {
+hipError_t shell_max_list(const double *f, const int *idx, int n, double *partials, unsigned *ticket, double *out,
+                          hipStream_t st) {
     constexpr int kBlocks = 64;
-    hipLaunchKernelGGL(k_shell_max_list, dim3(kBlocks), dim3(256), 0, st, f, idx, n, partials);
-    hipLaunchKernelGGL(k_min_small, dim3(1), dim3(256), 0, st, (const double *)partials, kBlocks, out);
+    hipLaunchKernelGGL(k_shell_max_list, dim3(kBlocks), dim3(256), 0, st, f, idx, n, partials, ticket, out);
     return hipGetLastError();
 }
 hipError_t shell_max(const Grid &g, const double *f, double rt, double *partials, hipStream_t st) {

@@ -214,6 +214,8 @@ static void twiddles(int n, std::vector<double2> &w) {
     }
 }
 
+static int taper_geometry(mpdpft_ctx *c);
+
 extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **out) {
     if (!p || !out) return MPDPFT_EINVAL;
     *out = nullptr;
@@ -360,6 +362,9 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
         HIPC(mk::environment(c->g, e, c->d_Env + s * G, c->st));
     }
     HIPC(hipMemcpyAsync(c->d_V, c->d_Env, sizeof(double) * SG, hipMemcpyDeviceToDevice, c->st));
+    if (p->TaperOff && (p->DelFieldMethod == 4 || p->DelFieldMethod == 5) &&
+        (p->DensityExpression == 5 || p->DIM == 3))  // DelField will run (nAiT, or the 3D energies)
+        RC(taper_geometry(c));
     {  // FFT block shapes: time the candidates once per length and precision (MPDPFT_FFT_TUNE=0: defaults)
         const char *tune = std::getenv("MPDPFT_FFT_TUNE");
         if (!(tune && tune[0] == '0')) {
@@ -658,6 +663,34 @@ static int update_v(mpdpft_ctx *c, const double *den) {
 // fwd(taper(V)); bwd(i kx e, i ky e) -> Dx,Dy; bwd(i kz e, -k^2 e^2) -> Dz,Lap;
 // fwd(sum D^2); bwd(e) -> POS -> Grad2.  The taper and the sum of squares are
 // formed in the first pass of their forward transforms.
+// The taper's geometry (DelField TaperOff, :7251-7264), built once per context:
+// the per-point factor table and the list of the shell's points.
+static int taper_geometry(mpdpft_ctx *c) {
+    const mpdpft_params &p = c->p;
+    const size_t G = c->G;
+    const double rt = p.edge * p.RegularizationThreshold;
+    RC(dmalloc(c, &c->d_taperT, G));
+    HIPC(mk::taper_factor(c->g, rt, c->d_taperT, c->st));
+    int *flag = nullptr;
+    HIPC(hipMalloc((void **)&flag, sizeof(int) * G));
+    HIPC(mk::shell_mark(c->g, rt, flag, c->st));
+    std::vector<int> hf(G), idx;
+    hipError_t e = hipMemcpyAsync(hf.data(), flag, sizeof(int) * G, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(flag);
+    if (e != hipSuccess) return hip_fail(c, e, "shell_mark");
+    for (size_t i = 0; i < G; i++)
+        if (hf[i]) idx.push_back((int)i);
+    c->n_shell = (int)idx.size();
+    int *di = nullptr;
+    RC(dmalloc(c, &di, std::max<size_t>(1, idx.size())));
+    c->d_shell_idx = di;
+    if (!idx.empty()) HIPC(hipMemcpy(c->d_shell_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+    RC(dmalloc(c, &c->d_ticket, 1));
+    HIPC(hipMemset(c->d_ticket, 0, sizeof(unsigned)));
+    return MPDPFT_OK;
+}
+
 static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method);
 int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method) {
     stage_begin(c);
@@ -685,28 +718,9 @@ static int delfield_impl(mpdpft_ctx *c, const double *f_dev, int s, int method) 
     c->delfield_valid = -1;
     size_t G = c->G;
     if (p.TaperOff && (method == 4 || method == 5)) {  // shift by the shell maximum, taper (fused)
-        double rt = p.edge * p.RegularizationThreshold;
-        if (c->n_shell < 0) {  // the taper's geometry, once: factor table and shell point list
-            RC(dmalloc(c, &c->d_taperT, G));
-            HIPC(mk::taper_factor(c->g, rt, c->d_taperT, c->st));
-            int *flag = nullptr;
-            HIPC(hipMalloc((void **)&flag, sizeof(int) * G));
-            HIPC(mk::shell_mark(c->g, rt, flag, c->st));
-            std::vector<int> hf(G), idx;
-            hipError_t e = hipMemcpyAsync(hf.data(), flag, sizeof(int) * G, hipMemcpyDeviceToHost, c->st);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-            (void)hipFree(flag);
-            if (e != hipSuccess) return hip_fail(c, e, "shell_mark");
-            for (size_t i = 0; i < G; i++)
-                if (hf[i]) idx.push_back((int)i);
-            c->n_shell = (int)idx.size();
-            int *di = nullptr;
-            RC(dmalloc(c, &di, std::max<size_t>(1, idx.size())));
-            c->d_shell_idx = di;
-            if (!idx.empty())
-                HIPC(hipMemcpy(c->d_shell_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
-        }
-        HIPC(mk::shell_max_list(f_dev, c->d_shell_idx, c->n_shell, c->d_partials, c->d_scal + 210, c->st));
+        if (c->n_shell < 0) RC(taper_geometry(c));
+        HIPC(mk::shell_max_list(f_dev, c->d_shell_idx, c->n_shell, c->d_partials, c->d_ticket, c->d_scal + 210,
+                                c->st));
     }
     const bool taper = p.TaperOff && (method == 4 || method == 5);
     if (c->fp32_phase) {  // the same transforms in FP32 (Precision 1)

@@ -29,6 +29,7 @@ struct mpdpft_ctx {
     double *d_taperT = nullptr;   // DelField taper factor per grid point (built on first use)
     int *d_shell_idx = nullptr;   // grid points of the taper's shell |r - rt| < 2 dx
     int n_shell = -1;             // their number (-1: not built)
+    unsigned *d_ticket = nullptr; // completion counter of the one-launch shell maximum
     // precision variant (Precision 1): FP32 transforms until the FP32 floor, then FP64 polish
     float2 *d_specf = nullptr, *d_workf = nullptr, *d_padf = nullptr, *d_twf = nullptr, *d_tw2f = nullptr,
            *d_kekf = nullptr;
