@@ -161,22 +161,16 @@ __global__ void k_spec_combine(int s, const double *kq, const double *ek, const 
     }
 }
 
-// the smallest index on the taper shell where f equals the shell maximum
-// (-*shift); -1 when the shell is empty (the shift is then 0)
-__global__ __launch_bounds__(256) void k_shell_argmax(Grid g, const double *f, const double *shift, double rt,
+// the smallest grid index of the taper shell's point list (ascending) where f equals
+// the shell maximum (-*shift); -1 when the shell is empty (the shift is then 0)
+__global__ __launch_bounds__(256) void k_shell_argmax(const double *f, const int *idx, int n, const double *shift,
                                                       int *out) {
     __shared__ int sh[256];
     const double sft = -(*shift);
     int best = 0x7fffffff;
-    if (!(sft < -1e307 || sft > 1e307)) {
-        for (size_t i = threadIdx.x; i < g.G; i += 256) {
-            int c[3];
-            dcoords(g, i, c);
-            double r0 = g.lat[c[0]], r1 = g.lat[c[1]], r2 = g.lat[c[2]];
-            double rn = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-            if (fabs(rn - rt) < 2. * g.dx && f[i] == sft && (int)i < best) best = (int)i;
-        }
-    }
+    if (!(sft < -1e307 || sft > 1e307))
+        for (int j = threadIdx.x; j < n; j += 256)
+            if (f[idx[j]] == sft && idx[j] < best) best = idx[j];
     sh[threadIdx.x] = best;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -638,18 +632,17 @@ int setup(mpdpft_ctx *c, int restart) {
     A.taper = A.nait && p.TaperOff && (A.method == 4 || A.method == 5);
     A.hartree = c->has_hartree;
     if (A.nait) {
+        c->delfield_valid = -1;  // DelField(V) now: its fields and shell maximum are the linearisation's
         RC(nait_partials(c, 0, mu, A.nbuf, A.dfdU, A.dfdg2, A.dfdl));
         // the DelField fields of V (nait_partials leaves DelField(V) in the context)
         HIPC(hipMemcpyAsync(A.Dx, c->d_Dx, sizeof(double) * G, hipMemcpyDeviceToDevice, c->st));
         HIPC(hipMemcpyAsync(A.Dy, c->d_Dy, sizeof(double) * G, hipMemcpyDeviceToDevice, c->st));
         HIPC(hipMemcpyAsync(A.Dz, c->d_Dz, sizeof(double) * G, hipMemcpyDeviceToDevice, c->st));
         HIPC(hipMemcpyAsync(A.g2, c->d_Grad2, sizeof(double) * G, hipMemcpyDeviceToDevice, c->st));
-        if (A.taper) {
-            double rt = p.edge * p.RegularizationThreshold;
-            HIPC(mk::shell_max(c->g, c->d_V, rt, c->d_partials, c->st));
-            HIPC(mk::finish_minmax(c->d_partials, 1, c->d_scal + 210, c->st));
-            hipLaunchKernelGGL(k_shell_argmax, dim3(1), dim3(256), 0, c->st, c->g, (const double *)c->d_V,
-                               (const double *)(c->d_scal + 210), rt, A.istar);
+        if (A.taper) {  // the forward's shell points and maximum (left by nait_partials' DelField of V)
+            if (c->n_shell < 0) return ctx_fail(c, MPDPFT_EINVAL, "adjoint: taper geometry missing");
+            hipLaunchKernelGGL(k_shell_argmax, dim3(1), dim3(256), 0, c->st, (const double *)c->d_V,
+                               (const int *)c->d_shell_idx, c->n_shell, (const double *)(c->d_scal + 210), A.istar);
         }
     } else {
         double g = p.degeneracy;
