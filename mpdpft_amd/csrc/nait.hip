@@ -980,6 +980,7 @@ __global__ __launch_bounds__(256) void k_nait_sums(mk::Grid g, const double *__r
 struct EkinArgs {
     const double *V, *grad2, *lap;
     double mu, thr, c0, c1, c2, internal_acc, cmp, area_pref, neg_exp;
+    double tail;  // nodes with y > max(U/a, 0) + tail are left out (infinity: every node)
     int MaxIt;
     int resume;  // continue each point from the refinement the previous launch stopped at
     long long G;
@@ -994,6 +995,16 @@ __device__ __noinline__ double airy_asym(double y) {
     return cos(2. * (-y) * sqrt(-y) / 3. - 0.25 * kPI) / (sqrt(kPI) * pow(-y, 0.25));
 }
 
+// the table part of airy_fast (y > -300; the index is clamped, so any y reads in bounds)
+__device__ __forceinline__ double airy_poly(const double2 *__restrict__ ap, double y) {
+    int k = (int)floor((y - kAX0) * kAINV);
+    k = k < 0 ? 0 : (k > kAN - 2 ? kAN - 2 : k);
+    double u = (y - (kAX0 + kAH * (double)k)) * kAINV;
+    const double2 *c = ap + 3 * k;
+    double2 c01 = c[0], c23 = c[1], c45 = c[2];
+    return ((((c45.y * u + c45.x) * u + c23.y) * u + c23.x) * u + c01.y) * u + c01.x;
+}
+
 __device__ __forceinline__ double airy_fast(const double2 *__restrict__ ap, double y) {
     if (y <= -300.) return airy_asym(y);
     int k = (int)floor((y - kAX0) * kAINV);
@@ -1002,6 +1013,39 @@ __device__ __forceinline__ double airy_fast(const double2 *__restrict__ ap, doub
     const double2 *c = ap + 3 * k;
     double2 c01 = c[0], c23 = c[1], c45 = c[2];
     return ((((c45.y * u + c45.x) * u + c23.y) * u + c23.x) * u + c01.y) * u + c01.x;
+}
+
+// Trapezoid node j of EkinAiT03D's refinement with gs = dg + 1 nodes on [0, X]:
+// x = j X / dg, correctly rounded (product by the reciprocal plus one residual
+// correction; the residual is exact by FMA).
+__device__ __forceinline__ double ekin_node(int j, double X, double dg, double inv) {
+    double num = (double)j * X, q = num * inv;
+    return fma(fma(-q, dg, num), inv, q);
+}
+
+// c1 (2ax)^{5/2}/5 - c2 lap (2ax)^{1/2}/6 times Ai(y); tt = 2ax
+__device__ __forceinline__ double ekin_term(double AI, double tt, double k1, double k2) {
+    double sq = sqrt(tt);
+    return AI * (k1 * tt * tt * sq - k2 * sq);
+}
+
+// Node ranges of one refinement.  [0, asym): nodes that may have y = x + U/a <=
+// -300, where Ai takes its asymptotic form (two nodes of margin; y is monotone in
+// j).  (last, gs): nodes beyond y = max(U/a, 0) + tail.  There |Ai(y)| <
+// exp(-(2/3) tail^{3/2}), 8e-38 for tail = 25, of Ai's size where the integrand
+// lives, so with the (2ax)^{5/2} growth over x <= 400 a left-out term is below
+// 1e-30 of the terms before it and below half an ulp of the lane's partial sum:
+// leaving the tail out does not change the sum.  A NaN U/a keeps every node.
+struct EkinNodes {
+    int asym, last;
+};
+__device__ __forceinline__ EkinNodes ekin_nodes(int gs, double dg, double X, double Ua, double tail) {
+    EkinNodes e;
+    const double xa = -300. - Ua;  // y <= -300 for x <= xa (xa < X)
+    e.asym = xa >= 0. ? min(gs, (int)(xa / X * dg) + 2) : 0;
+    const double xc = fmax(Ua, 0.) + tail - Ua;
+    e.last = xc < X ? min(gs - 1, (int)(xc / X * dg) + 1) : gs - 1;
+    return e;
 }
 
 // Airy kinetic-energy density (EkinAiT03D inner loop, mpDPFT.cpp:5624-5709),
@@ -1046,16 +1090,28 @@ __global__ __launch_bounds__(64) void k_ekin_ait(const double2 *__restrict__ ap,
     for (;;) {
         const int gs = 64 << iteration;
         const double dg = (double)(gs - 1), inv = 1. / dg;
+        const EkinNodes nd = ekin_nodes(gs, dg, xr, Ua, ea.tail);
         double r = 0.;
-        for (int j = lane; j < gs; j += 64) {
-            // x = j xr / dg, correctly rounded: product by the reciprocal plus one
-            // residual correction (exact residual by FMA)
-            double num = (double)j * xr, q = num * inv;
-            double x = fma(fma(-q, dg, num), inv, q), y = x + Ua;
+        // each lane sums the nodes j = lane (mod 64) in increasing order
+        int j = lane;
+        for (; j < nd.asym; j += 64) {
+            double x = ekin_node(j, xr, dg, inv), y = x + Ua;
             double AI = y > 100. ? 0. : airy_fast(ap, y);
-            double tt = twoa * x, sq = sqrt(tt);
-            double f = AI * (k1 * tt * tt * sq - k2 * sq);  // c1 (2ax)^{5/2}/5 - c2 lap (2ax)^{1/2}/6
-            r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * f;
+            r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * ekin_term(AI, twoa * x, k1, k2);
+        }
+        for (; j + 64 <= nd.last; j += 128) {  // two nodes per pass: both table reads in flight
+            double x0 = ekin_node(j, xr, dg, inv), y0 = x0 + Ua;
+            double x1 = ekin_node(j + 64, xr, dg, inv), y1 = x1 + Ua;
+            double A0 = airy_poly(ap, y0), A1 = airy_poly(ap, y1);
+            double f0 = ekin_term(y0 > 100. ? 0. : A0, twoa * x0, k1, k2);
+            double f1 = ekin_term(y1 > 100. ? 0. : A1, twoa * x1, k1, k2);
+            r += (j == 0 ? 0.5 : 1.) * f0;
+            r += (j + 64 == gs - 1 ? 0.5 : 1.) * f1;
+        }
+        if (j <= nd.last) {
+            double x = ekin_node(j, xr, dg, inv), y = x + Ua;
+            double AI = airy_poly(ap, y);
+            r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * ekin_term(y > 100. ? 0. : AI, twoa * x, k1, k2);
         }
         r = wave_total(r);
         r *= (xmax - 0.) / ((double)gs - 1.);
@@ -1127,10 +1183,10 @@ __global__ __launch_bounds__(64) void k_ekin_partials(const double2 *__restrict_
     for (int iteration = 0;; iteration++) {
         const int gs = 64 << iteration;
         const double dg = (double)(gs - 1), inv = 1. / dg;
+        const int last = ekin_nodes(gs, dg, X, Ua, ea.tail).last;  // the energy's node range
         double r = 0., rU = 0., ra = 0., rl = 0.;
-        for (int j = lane; j < gs; j += 64) {
-            double num = (double)j * X, q = num * inv;
-            double x = fma(fma(-q, dg, num), inv, q), y = x + Ua;
+        for (int j = lane; j <= last; j += 64) {
+            double x = ekin_node(j, X, dg, inv), y = x + Ua;
             double AI = y > 100. ? 0. : airy_fast(ap, y);
             double AId = y > 100. ? 0. : airy_fast_d(ap, y);
             double tt = twoa * x, sq = sqrt(tt);
@@ -1559,6 +1615,13 @@ void nait_state(const mpdpft_ctx *c, int s, double *BP, double *maxIter) {
     *maxIter = c->nait ? c->nait->Mpp[s][1] : 0.;
 }
 
+// Where EkinAiT03D's node sums stop (see ekin_nodes); MPDPFT_EKIN_FULL=1 sums
+// every node up to y = 100 as the reference's loop does (same results).
+static double ekin_tail() {
+    const char *e = getenv("MPDPFT_EKIN_FULL");
+    return (e && *e && *e != '0') ? HUGE_VAL : 25.;
+}
+
 // EkinAiT03D (mpDPFT.cpp:5610-5709); c->MaxIteration persists across calls.
 int nait_energy_partials(mpdpft_ctx *c, int s, double *eU, double *eg2, double *el, double *prefactor) {
     NaitTables *t = c->nait;
@@ -1584,6 +1647,7 @@ int nait_energy_partials(mpdpft_ctx *c, int s, double *eU, double *eg2, double *
     *prefactor = c->p.degeneracy / (4. * kPI * kPI);
     ea.area_pref = std::pow(c->p.edge, (double)c->D) * (*prefactor);
     ea.neg_exp = -std::log(c->p.InternalAcc);
+    ea.tail = ekin_tail();
     ea.G = (long long)G;
     ea.MaxIt = c->MaxIteration;
     ea.resume = 0;
@@ -1616,6 +1680,7 @@ int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E) {
     double prefactor = c->p.degeneracy / (4. * kPI * kPI);
     ea.area_pref = std::pow(c->p.edge, (double)c->D) * prefactor;
     ea.neg_exp = -std::log(c->p.InternalAcc);
+    ea.tail = ekin_tail();
     ea.G = (long long)G;
     double e = 0., prev = 0.;
     int Iteration = 0, MAXIteration = c->MaxIteration + 6;

@@ -306,3 +306,21 @@ def test_nait_trajectory(oracle, iters):
     assert ro.SCcount == iters + 1 and rg.SCcount == iters + 1
     check_loop(ro, dO, VO, rg, dG, VG, 1)
     assert rg.bisection_steps == ro.bisection_steps
+
+
+@pytest.mark.gpu
+def test_airy_energy_tail_is_exact(monkeypatch):
+    """The Airy kinetic energy leaves out the trapezoid nodes where Ai is below
+    1e-37 of its scale; summing every node (MPDPFT_EKIN_FULL=1, the reference's
+    loop) gives the same dispersal energy to the bit."""
+    print("case airy energy tail", flush=True)
+    p = configs.c1(steps=32, density_expression=5, hartree_mode=1, maxsc=3)
+    out = []
+    for full in ("0", "1"):
+        monkeypatch.setenv("MPDPFT_EKIN_FULL", full)
+        ctx = Context(p)
+        r = ctx.solve()
+        ctx.close()
+        out.append((r.DispersalEnergies[0], r.TotalEnergy))
+        print("full=%s  E_disp %.17g  E %.17g" % (full, out[-1][0], out[-1][1]), flush=True)
+    assert out[0] == out[1]
