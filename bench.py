@@ -272,7 +272,7 @@ def main():
     ap.add_argument("--adjoint-grid", type=int, default=96, help="grid of the differentiation workload (0: skip)")
     ap.add_argument("--precision-grid", type=int, default=0,
                     help="grid of the precision-variant comparison (256 = C2; 0: skip, about 15 min at 256)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "nait_reuse_128_traffic.md"),
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "nait_lanes_128_traffic.md"),
                     help="rocprof summary holding 'traffic_bytes_per_launch = N' for the nAiT kernel at this grid")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -310,7 +310,8 @@ def main():
                     traffic = float(ln.split("=")[1])
         roof = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
-                "kernel": "k_nait_reuse", "avg_launch_ms": round(avg_s * 1e3, 4),
+                "kernel": "k_nait_lanes" if p.Mpp[0] <= 15 else "k_nait_reuse",
+                "avg_launch_ms": round(avg_s * 1e3, 4),
                 "flops_per_launch": flops_per_launch,
                 "note": "FP64 vector (VALU) compute bound, no MFMA: integrand evaluations actually performed "
                         "x 50 nominal FP64 flops each, against the 78.6 TFLOP/s FP64 vector peak"}

@@ -955,6 +955,182 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     }
 }
 
+// The same walk with a fixed lane layout, for BP <= 15 (2BP + 1 <= 32): lanes
+// 0..31 hold the left half of a node, lanes 32..63 the right half.  Lane m of a
+// half holds that half's even sample 2m (m <= 2BP) and odd sample 2m + 1 (m <
+// 2BP); the midpoint is the last even sample of the left half and the first of
+// the right one.  A lane's Boole weights are then constants of the lane, and
+// one row-sum reduction of w_e e + 32 o yields both half sums (rows 0-1: left,
+// rows 2-3: right).  Abscissae, panels and acceptance tests are those of
+// k_nait_reuse; only the order of the panel-sum additions differs.
+constexpr int kLanesMaxBP = 15;
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nait_lanes(
+    ReuseTab rt, const TabView *__restrict__ tv, PointScalars ps, double *__restrict__ n, int *__restrict__ counts) {
+    extern __shared__ double lds[];
+    const TabView &t = *tv;
+    const int lane = threadIdx.x;
+    const long long i = xcd_point(ps.G);
+    if (i >= ps.G) return;
+    const int BP = ps.BP, h4 = 4 * BP, full = 8 * BP, H = 2 * BP;
+    double *cur = lds;                                // [8BP + 1] samples of the current node
+    double *stk = lds + full + 1;                     // [depth][4BP + 1] pending right halves
+    double *stn = stk + (size_t)ps.depth * (h4 + 1);  // [depth][4]: a, b, prev, iter
+    double *cst = stn + 4 * (size_t)ps.depth;         // [2]: log10(compare), -log10(rtpna)
+    if (lane == 0) {
+        cst[0] = ps.log10cmp;
+        cst[1] = ps.maxexp;
+    }
+    // this lane's slots: half hs, index m within the half
+    const int hs = lane >> 5, m = lane & 31;
+    const bool has_even = m <= H, has_odd = m < H;
+    const int ke = hs * h4 + 2 * m, ko = ke + 1;  // sample indices in the node (0..8BP)
+    const int je = hs * H + m;                    // where the next node's even sample 2 je comes from
+    const double we = !has_even ? 0. : ((m == 0 || m == H) ? 7. : (((2 * m - 1) & 3) == 1 ? 12. : 14.));
+    const double wo = has_odd ? 32. : 0.;
+
+    const double U = ps.V[i] - ps.mu;
+    const double ar = 0.5 * pow(ps.alpha * 1. * ps.grad2[i], 1. / 3.);
+    const double beta2 = -ps.lap[i] / (12. * ps.sqrt_at);
+    double res = 0.;
+    int evals = 0, nim = 0, nra = 0;
+    if (ar < fabs(ps.thr)) {
+        double arg = -ps.tau * U;
+        res = ps.beta0 * (ps.beta1 * polylog_j(t, 0, arg) + beta2 * polylog_j(t, 2, arg));
+    } else {
+        PointArgs q;
+        q.invT = ps.invT;
+        q.U = U;
+        q.ar = ar;
+        q.pref = ps.beta0 * ar * ps.tau * ps.beta1;
+        q.pref2 = beta2 / ps.beta1;
+        // lower limit: the first x = -2^k (k = 1, 2, ...) where the integrand
+        // is below 1e-12, stopping once x < -1e8 (k = 27)
+        bool small = true;
+        if (lane < 27) {
+            double x = -ldexp(1., lane + 1), arg = -ps.tau * (U - x * ar);
+            double l1, l3;
+            polylog_pair(t, arg, l1, l3);
+            double test = ps.beta0 * curly_a(t, x) * ar * ps.tau * (ps.beta1 * l1 + beta2 * l3);
+            small = !(fabs(test) > kMP);
+        }
+        unsigned long long bal = __ballot(small ? 1 : 0);
+        int kx = __ffsll((long long)bal) - 1;
+        if (kx < 0 || kx > 26) kx = 26;
+        const double xlow = -ldexp(1., kx + 1);
+        evals = 27;
+#pragma unroll 1
+        for (int root = 0; root < 2; root++) {
+            double a = root ? xlow : 2. * xlow, b = root ? 100. : xlow, prev = 0.;
+            const double half0 = 0.5 * (b - a), Delta0 = 0.25 * half0 / ((double)BP);
+            const double c0 = 2. * (((a + half0) - a) / (((double)BP) * 4.)) / 45.;
+            int iter = 0, sp = 0;
+            bool fresh = true;  // the root evaluates its even samples too
+            double e = 0.;      // this lane's even sample (placed by the parent step)
+            for (;;) {
+                a = uniform_d(a);
+                b = uniform_d(b);
+                prev = uniform_d(prev);
+                iter = __builtin_amdgcn_readfirstlane(iter);
+                sp = __builtin_amdgcn_readfirstlane(sp);
+                const double half = 0.5 * (b - a);
+                double Delta, ca, cb;
+                if (half == ldexp(half0, -iter) && (a + half) - a == half && b - (a + half) == half) {
+                    Delta = ldexp(Delta0, -iter);
+                    ca = cb = ldexp(c0, -iter);
+                } else {
+                    Delta = 0.25 * half / ((double)BP);
+                    ca = 2. * (((a + half) - a) / (((double)BP) * 4.)) / 45.;
+                    cb = 2. * ((b - (a + half)) / (((double)BP) * 4.)) / 45.;
+                }
+                if (fresh) {
+                    if (has_even) {
+                        double x = ke <= h4 ? a + (double)ke * Delta : (a + (double)(ke - h4) * Delta) + half;
+                        e = integrand_fast(rt, q, x);
+                        cur[ke] = e;
+                    }
+                    evals += full + 1;
+                } else {
+                    evals += h4;
+                }
+                fresh = false;
+                double o = 0.;
+                if (has_odd) {
+                    double x = ko <= h4 ? a + (double)ko * Delta : (a + (double)(ko - h4) * Delta) + half;
+                    o = integrand_fast(rt, q, x);
+                    cur[ko] = o;
+                }
+                double v = we * e + wo * o;
+                v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+                v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+                v += dpp_mov<0x141>(v);  // row_half_mirror
+                v += dpp_mov<0x140>(v);  // row_mirror: every lane holds its row's sum
+                const double Sa = readlane_d(v, 0) + readlane_d(v, 16), Sb = readlane_d(v, 32) + readlane_d(v, 48);
+                const double Inta = Sa * ca, Intb = Sb * cb, Int = Inta + Intb;
+                bool accept = true;
+                if (Int != Int) {
+                } else if (iter > 1 && fabs(Int) > 0. && relacc_ok(Int, prev, ps.rtpna)) {
+                    nra++;
+                } else if (iter > ps.maxIter) {
+                    nim++;
+                } else {
+                    double sm = ldexp(Int, iter);  // 2^iter Int (NegligibleMagnitudeQ)
+                    bool negl;
+                    if (sm == 0.) negl = true;
+                    else if (ps.compare == 0. || ps.compare < sm) negl = false;
+                    else if (fabs(sm) < ps.negl_lo) negl = true;
+                    else if (fabs(sm) > ps.negl_hi) negl = false;
+                    else negl = negl_exact(cst, sm);
+                    accept = negl || Delta < kMP;
+                }
+                accept = __builtin_amdgcn_readfirstlane((int)accept) != 0;
+                __syncthreads();  // cur complete
+                const double *src;
+                if (accept) {
+                    res += Int;
+                    if (sp == 0) break;
+                    sp--;
+                    a = uniform_d(stn[4 * sp]);
+                    b = uniform_d(stn[4 * sp + 1]);
+                    prev = uniform_d(stn[4 * sp + 2]);
+                    iter = (int)uniform_d(stn[4 * sp + 3]);
+                    src = stk + (size_t)sp * (h4 + 1);
+                } else {
+                    if (sp >= ps.depth) {  // deeper than maxIter allows; cannot happen
+                        res = __builtin_nan("");
+                        break;
+                    }
+                    double *dst = stk + (size_t)sp * (h4 + 1);
+                    if (lane <= h4) dst[lane] = cur[h4 + lane];  // 4BP + 1 <= 61 samples
+                    if (lane == 0) {
+                        stn[4 * sp] = a + half;
+                        stn[4 * sp + 1] = b;
+                        stn[4 * sp + 2] = Intb;
+                        stn[4 * sp + 3] = (double)(iter + 1);
+                    }
+                    sp++;
+                    b = a + half;
+                    prev = Inta;
+                    iter++;
+                    src = cur;  // the left child: the left half of cur, in place
+                }
+                // the next node's even sample 2 je is src[je]
+                e = has_even ? src[je] : 0.;
+                __syncthreads();
+                if (has_even) cur[ke] = e;
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        const long long G = ps.G;
+        n[i] = res != res ? 0. : res;
+        counts[i] = evals;
+        counts[G + i] = nim;
+        counts[2 * G + i] = nra;
+    }
+}
+
 // Trapezoid-weighted density sum and the three per-point counters, as
 // per-block partials (fixed order).
 __global__ __launch_bounds__(256) void k_nait_sums(mk::Grid g, const double *__restrict__ n,
@@ -1563,7 +1739,12 @@ int nait_density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) {
     rt.ginv1 = t->view.ginv[1];
     rt.ginv3 = t->view.ginv[3];
     stage_begin(c);
-    if (reuse) {
+    if (reuse && da.BP <= kLanesMaxBP && !c->nait_generic) {
+        hipLaunchKernelGGL(k_nait_lanes, dim3(xcd_blocks((long long)G)), dim3(64), lds, c->st, rt, t->d_view, ps, den_out,
+                           t->d_counts);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(c, e, "k_nait_lanes");
+    } else if (reuse) {
         hipLaunchKernelGGL(k_nait_reuse, dim3(xcd_blocks((long long)G)), dim3(64), lds, c->st, rt, t->d_view, ps, den_out,
                            t->d_counts);
         e = hipGetLastError();

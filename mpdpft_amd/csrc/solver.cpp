@@ -387,6 +387,7 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
     c->verbose = vb && *vb && *vb != '0';
     const char *nk = std::getenv("MPDPFT_NAIT_KERNEL");
     c->nait_plain = nk && std::strcmp(nk, "plain") == 0;
+    c->nait_generic = nk && std::strcmp(nk, "reuse") == 0;
     if (p->DensityExpression == 5) RC(nait_init(c));
     return MPDPFT_OK;
 }

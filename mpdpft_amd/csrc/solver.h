@@ -69,6 +69,7 @@ struct mpdpft_ctx {
     int has_hartree = 0;
     int verbose = 0;  // MPDPFT_VERBOSE: one line per SC iteration on stderr
     int nait_plain = 0;  // MPDPFT_NAIT_KERNEL=plain: per-node nAiT kernel without panel reuse
+    int nait_generic = 0;  // MPDPFT_NAIT_KERNEL=reuse: panel reuse without the fixed lane layout (any BP)
     std::vector<double> V0;   // initial potential of a solve (empty: V = Env)
     std::ostringstream notes; // lines for the next solve's control stream (LoadV)
     int importWarnings = 0;   // warnings of that import, added to the solve's count

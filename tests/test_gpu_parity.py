@@ -278,11 +278,14 @@ def test_not_converged_is_reported(oracle):
 
 
 # ------------------------------------------------------------------ nAiT (C1 functional)
-@pytest.mark.parametrize("kernel,steps,bp", [("reuse", 16, 10), ("plain", 16, 10), ("reuse", 16, 17),
-                                              ("reuse", 32, 10)])
+@pytest.mark.parametrize("kernel,steps,bp", [("lanes", 16, 10), ("lanes", 16, 15), ("lanes", 16, 3),
+                                              ("reuse", 16, 10), ("plain", 16, 10), ("reuse", 16, 17),
+                                              ("lanes", 32, 10)])
 def test_nait_density_stage(oracle, monkeypatch, kernel, steps, bp):
-    """nAiT density at fixed V and mu: the panel-reuse kernel (one or two
-    samples per lane and half) and the per-node kernel against the oracle."""
+    """nAiT density at fixed V and mu against the oracle: the fixed-lane-layout
+    kernel (default for BP <= 15), the panel-reuse kernel for any BP (one or two
+    samples per lane and half; BP = 17 also takes it by default) and the
+    per-node kernel."""
     print("case nait density stage", kernel, steps, bp, flush=True)
     monkeypatch.setenv("MPDPFT_NAIT_KERNEL", kernel)
     p = configs.c1(steps=steps, density_expression=5, hartree_mode=1)
