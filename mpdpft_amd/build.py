@@ -26,12 +26,20 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 
 LIB_SOURCES = ["params.cpp", "kernels.hip", "kernels_fp32.hip", "solver.cpp", "nait.hip", "output.cpp", "batch.hip",
                "adjoint.hip"]
+# The FFT passes are instantiated once per block shape, and the tuner picks a
+# shape per length by timing.  With multiply-add contraction the compiler may
+# fuse different products in different instantiations, so the results would
+# change with the (run-dependent) choice; these units therefore round every
+# product and sum as written (explicit fma() calls only).
+NO_CONTRACT = {"kernels.hip", "kernels_fp32.hip"}
 
 
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src + ".o")
     cmd = [HIPCC] + COMMON + DEVICE
+    if src in NO_CONTRACT:
+        cmd += ["-ffp-contract=off"]
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
     cmd += ["-c", path, "-o", obj]

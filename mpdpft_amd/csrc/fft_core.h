@@ -275,6 +275,11 @@ inline int &bsel_slot(int N, bool fp32) {
     return table[fp32 ? 1 : 0][k];
 }
 
+// Every length and precision to one block shape (the tuner then leaves them)
+inline void bsel_force(int sel) {
+    for (int N : {16, 32, 64, 96, 128, 192, 256, 512}) bsel_slot(N, false) = bsel_slot(N, true) = sel;
+}
+
 template <int N, int B, int AXIS, typename T2, class Ld, class St>
 inline hipError_t launch_pass_b(Ld ld, St st, int L0, int L1, int sign, const T2 *tw, hipStream_t stream) {
     PassGeom g{L0, L1, sign};

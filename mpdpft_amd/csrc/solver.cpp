@@ -15,6 +15,7 @@
 #include <cstring>
 #include <iomanip>
 
+#include "fft_core.h"
 #include "solver.h"
 
 static const double kPI = 3.141592653589793;
@@ -365,9 +366,13 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
     if (p->TaperOff && (p->DelFieldMethod == 4 || p->DelFieldMethod == 5) &&
         (p->DensityExpression == 5 || p->DIM == 3))  // DelField will run (nAiT, or the 3D energies)
         RC(taper_geometry(c));
-    {  // FFT block shapes: time the candidates once per length and precision (MPDPFT_FFT_TUNE=0: defaults)
+    {  // FFT block shapes: time the candidates once per length and precision (MPDPFT_FFT_TUNE=0: defaults;
+       // MPDPFT_FFT_B=0|1|2: half, default or twice the default lines per block for every length)
         const char *tune = std::getenv("MPDPFT_FFT_TUNE");
-        if (!(tune && tune[0] == '0')) {
+        const char *force = std::getenv("MPDPFT_FFT_B");
+        if (force && force[0] >= '0' && force[0] <= '2' && force[1] == 0) {
+            fftk::bsel_force(force[0] - '0');
+        } else if (!(tune && tune[0] == '0')) {
             const bool iso = c->has_hartree && p->HartreeMode == 1;
             HIPC(mk::fft_autotune(c->g, c->d_Env, c->d_spec, c->d_tw, iso ? c->d_pad : nullptr, c->d_khat, c->d_tw2,
                                   c->d_tmp1, c->st));

@@ -65,6 +65,30 @@ def test_delfield(oracle, D, steps, method):
     ctx.close()
 
 
+@pytest.mark.parametrize("steps", [64, 128])
+def test_fft_block_shape_does_not_change_results(monkeypatch, steps):
+    """The FFT autotuner picks the lines per block of each transform length by
+    timing, which varies from run to run; DelField and the isolated Hartree
+    potential must not depend on that choice, bit for bit (C1 sizes: 128^3
+    and its 256^3 padding)."""
+    print("case fft block shapes", steps, flush=True)
+    p = configs.c1(steps=steps, density_expression=5, hartree_mode=1)
+    x = np.linspace(-1.0, 1.0, p.steps + 1)
+    r2 = (x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2).reshape(-1)
+    den = 3.0 * np.exp(-8.0 * r2)
+    out = []
+    for sel in ("0", "1", "2"):
+        monkeypatch.setenv("MPDPFT_FFT_B", sel)
+        ctx = Context(p)
+        f = ctx.environment()[0] + 0.3 * np.sin(np.arange(ctx.G) * 0.001)
+        d = ctx.delfield(f)
+        out.append([d[k] for k in ("dx", "dy", "dz", "gradsq", "laplacian")] + [ctx.interaction_potential(8, 0, den)])
+        ctx.close()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("case", ["contact2d", "contact3", "dirac", "ldax", "pw92", "hartree_per", "hartree_iso",
                                   "hartree_iso2d", "single_contact"])
 def test_interaction_potentials(oracle, case):
