@@ -194,9 +194,31 @@ __device__ __forceinline__ double curly_a_fast(const ReuseTab &t, double y) {
     return c.x + u * (c.y + u * (c.z + u * c.w));
 }
 
-// exp for the far branch (arg < -40), out of line: inlined, its polynomial
-// constants are hoisted into registers the walk needs.
-__device__ __noinline__ double exp_far(double x) { return exp(x); }
+// e^x for the far branch, -699 <= x < -40, inline: Cody-Waite reduction x = n
+// ln2 + r with the two-part ln2 of fdlibm (n ln2_hi exact), |r| <= ln2/2, the
+// degree-13 Taylor polynomial of e^r (truncation < 5e-18 relative), then the
+// exponent n.  Results stay normal in this range (e^-699 ~ 2.6e-304); within
+// 2 ulp of exp.
+__device__ __forceinline__ double exp_far(double x) {
+    const double n = rint(x * 1.4426950408889634074);
+    double r = fma(-n, 6.93147180369123816490e-01, x);
+    r = fma(-n, 1.90821492927058770002e-10, r);
+    double p = 1. / 6227020800.;  // 1/13!
+    p = fma(p, r, 1. / 479001600.);
+    p = fma(p, r, 1. / 39916800.);
+    p = fma(p, r, 1. / 3628800.);
+    p = fma(p, r, 1. / 362880.);
+    p = fma(p, r, 1. / 40320.);
+    p = fma(p, r, 1. / 5040.);
+    p = fma(p, r, 1. / 720.);
+    p = fma(p, r, 1. / 120.);
+    p = fma(p, r, 1. / 24.);
+    p = fma(p, r, 1. / 6.);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.);
+    p = fma(p, r, 1.);
+    return ldexp(p, (int)n);
+}
 
 // The integrand's Li pair for the panel-reuse kernel.  Below px0 and above
 // 699 the branches of polylog_j; on [px0, pxmax] the quintic Hermite
