@@ -27,8 +27,10 @@ softened Coulomb centres (Z = 1, a = 0.6, seed 424242) on a 30 bohr cube with
 N = 32 and the functional above; the loss int (n - n_target)^2 (target: the
 centres of seed 434343) and the total energy are differentiated with respect to
 the 96 centre coordinates by the implicit adjoint of the converged fixed point,
-and one coordinate of each is checked against central differences (h = 1e-4)
-of solves started from the converged potential and stopped at max|dV| < 2e-11.
+and one coordinate of each is checked against central differences (h = 2e-4,
+which keeps the O(h^2) truncation near 2e-8 relative while halving the effect
+of the solves' stopping noise over 2h) of solves started from the converged
+potential and stopped at max|dV| < 2e-11.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128]
 """
@@ -348,7 +350,7 @@ def main():
     adjoint = None
     if args.adjoint_grid > 0:
         print("bench: differentiation workload, %d^3" % args.adjoint_grid, file=sys.stderr, flush=True)
-        adjoint = adjoint_leg(args.adjoint_grid, 1e-4)
+        adjoint = adjoint_leg(args.adjoint_grid, 2e-4)
 
     line = {
         "metric": METRIC,
