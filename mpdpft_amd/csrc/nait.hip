@@ -1050,9 +1050,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             bool fresh = true;  // the root evaluates its even samples too
             double e = 0.;      // this lane's even sample (placed by the parent step)
             for (;;) {
-                a = uniform_d(a);
-                b = uniform_d(b);
-                prev = uniform_d(prev);
+                // a, b, prev are wave-uniform by construction and only feed VALU arithmetic (FP64 has no
+                // scalar ALU), so they stay in vector registers; iter and sp drive scalar branches
                 iter = __builtin_amdgcn_readfirstlane(iter);
                 sp = __builtin_amdgcn_readfirstlane(sp);
                 const double half = 0.5 * (b - a);
@@ -1112,10 +1111,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
                     res += Int;
                     if (sp == 0) break;
                     sp--;
-                    a = uniform_d(stn[4 * sp]);
-                    b = uniform_d(stn[4 * sp + 1]);
-                    prev = uniform_d(stn[4 * sp + 2]);
-                    iter = (int)uniform_d(stn[4 * sp + 3]);
+                    a = stn[4 * sp];
+                    b = stn[4 * sp + 1];
+                    prev = stn[4 * sp + 2];
+                    iter = (int)stn[4 * sp + 3];
                     src = stk + (size_t)sp * (h4 + 1);
                 } else {
                     if (sp >= ps.depth) {  // deeper than maxIter allows; cannot happen
