@@ -1,7 +1,8 @@
 // output.cpp -- the drop-in directory contract: read mpDPFT.input, solve,
-// write mpDPFT_Den.dat, mpDPFT_V.dat and mpDPFT_Control.dat in the reference
-// formats (ExportData, mpDPFT.cpp:12399-12520; final block :12716-12800;
-// WriteControlFile :11989-11995).
+// write mpDPFT_Den.dat, mpDPFT_V.dat, mpDPFT_ContourData.dat, mpDPFT_CutData.dat
+// and mpDPFT_Control.dat in the reference formats (ExportData,
+// mpDPFT.cpp:12399-12660; final block :12716-12800; WriteControlFile
+// :11989-11995).
 #include <cmath>
 #include <cstdio>
 #include <iomanip>
@@ -60,6 +61,153 @@ static int write_v(mpdpft_ctx *c, const std::string &path, const std::vector<dou
         for (int s = 0; s < c->S; s++) {
             put(f, V[s * c->G + i], prec);
             std::fputc(' ', f);
+        }
+        std::fputc('\n', f);
+    }
+    std::fclose(f);
+    return MPDPFT_OK;
+}
+
+// Euclidean norm as the reference's Norm (mpDPFT.h:259): squares summed in order
+static double norm(const double *v, int n) {
+    double s = 0.;
+    for (int i = 0; i < n; i++) s += v[i] * v[i];
+    return std::sqrt(s);
+}
+
+// Grid index of (i, j) in the x-y plane: 2D row-major, 3D at z index steps/2
+// (GetFieldInterpolations, mpDPFT.cpp:4545-4546; ContourData :12537-12538).
+static size_t plane_index(const mpdpft_ctx *c, int i, int j) {
+    const size_t EL = (size_t)c->EL;
+    return c->D == 2 ? i * EL + j : (i * EL + j) * EL + (size_t)(c->s / 2);
+}
+
+// mpDPFT_ContourData.dat (ExportData, mpDPFT.cpp:12527-12550), DIM 2 and 3: per
+// point of the x-y plane its position, |position| and, per species, density,
+// environment and potential; a blank line before each new x row and one at
+// the end (gnuplot pm3d blocks).
+static int write_contour(mpdpft_ctx *c, const std::string &path, const std::vector<double> &den,
+                         const std::vector<double> &env, const std::vector<double> &V) {
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    const int prec = c->p.OutPrecision, EL = c->EL;
+    double posxinit = c->lat_h[0];
+    for (int i = 0; i < EL; i++)
+        for (int j = 0; j < EL; j++) {
+            const size_t idx = plane_index(c, i, j);
+            double pos[3] = {c->lat_h[i], c->lat_h[j], c->D == 3 ? c->lat_h[c->s / 2] : 0.};
+            if (pos[0] > posxinit) {
+                std::fputc('\n', f);
+                posxinit = pos[0];
+            }
+            put(f, pos[0], prec);
+            std::fputc(' ', f);
+            put(f, pos[1], prec);
+            std::fputc(' ', f);
+            put(f, norm(pos, c->D), prec);
+            for (int s = 0; s < c->S; s++) {
+                const size_t k = (size_t)s * c->G + idx;
+                std::fputc(' ', f);
+                put(f, den[k], prec);
+                std::fputc(' ', f);
+                put(f, env[k], prec);
+                std::fputc(' ', f);
+                put(f, V[k], prec);
+            }
+            std::fputc('\n', f);
+        }
+    std::fputc('\n', f);
+    std::fclose(f);
+    return MPDPFT_OK;
+}
+
+// Bilinear interpolation on the square grid X x X of F[i n + j] = field(X[i], X[j])
+// at (px, py): the cell of each coordinate by bisection (x[l] < t <= x[l+1], the
+// first cell at the left end), then the four-corner weights.  The reference
+// builds ALGLIB's bilinear spline over the same plane (spline2dbuildbilinearv,
+// mpDPFT.cpp:4559) and evaluates it with the coordinates swapped to match its
+// row order (:12241); the value is this one.
+static void find_cell(const std::vector<double> &X, double t, int &l, double &w) {
+    int lo = 0, hi = (int)X.size() - 1;
+    while (lo != hi - 1) {
+        int h = (lo + hi) / 2;
+        if (X[h] >= t) hi = h;
+        else lo = h;
+    }
+    l = lo;
+    w = (t - X[l]) / (X[l + 1] - X[l]);
+}
+static double bilinear(const std::vector<double> &X, const std::vector<double> &F, double px, double py) {
+    const int n = (int)X.size();
+    int ix, iy;
+    double tx, ty;
+    find_cell(X, px, ix, tx);
+    find_cell(X, py, iy, ty);
+    const double f00 = F[(size_t)ix * n + iy], f10 = F[(size_t)(ix + 1) * n + iy];
+    const double f11 = F[(size_t)(ix + 1) * n + iy + 1], f01 = F[(size_t)ix * n + iy + 1];
+    return (1. - ty) * (1. - tx) * f00 + ty * (1. - tx) * f01 + ty * tx * f11 + (1. - ty) * tx * f10;
+}
+
+// mpDPFT_CutData.dat along the default cut OutCut = (-edge/2, 0) -> (edge/2, 0)
+// (mpDPFT.cpp:1367): GetCutPositions (:12151-12173), GetFieldInterpolationsAlongCut
+// (:12217-12253) and GetCutDataStream (:12175-12213).  Row i: the cut coordinate,
+// per species the density, environment and potential at the cut point (DIM 1:
+// the grid values; DIM 2/3: interpolated in the x-y plane), and in 3D the lattice
+// coordinate and per species the column density sum_k n(i, steps/2, k) dx.
+static int write_cut(mpdpft_ctx *c, const std::string &path, const std::vector<double> &den,
+                     const std::vector<double> &env, const std::vector<double> &V) {
+    const int EL = c->EL, steps = c->s, prec = c->p.OutPrecision;
+    std::vector<double> cx(EL), cy(EL, 0.);
+    if (c->D == 1) {
+        for (int i = 0; i < EL; i++) cx[i] = c->lat_h[i];
+    } else {
+        const double x0 = -c->p.edge / 2., x1 = c->p.edge / 2.;
+        const double DeltaX = (x1 - x0) / (double)steps, DeltaY = (0. - 0.) / (double)steps;
+        for (int i = 0; i < EL; i++) {
+            cx[i] = x0 + (double)i * DeltaX;
+            cy[i] = 0. + (double)i * DeltaY;
+        }
+    }
+    double dv[2] = {cx[0] - cx[EL - 1], cy[0] - cy[EL - 1]};
+    const double CutLine = norm(dv, 2);
+    // fi[s][p][i], p = density, environment, potential
+    std::vector<double> fi((size_t)c->S * 3 * EL);
+    const std::vector<double> *src[3] = {&den, &env, &V};
+    if (c->D == 1) {
+        for (int s = 0; s < c->S; s++)
+            for (int pp = 0; pp < 3; pp++)
+                for (int i = 0; i < EL; i++) fi[((size_t)s * 3 + pp) * EL + i] = (*src[pp])[(size_t)s * c->G + i];
+    } else {
+        std::vector<double> X(EL), F((size_t)EL * EL);
+        for (int i = 0; i < EL; i++) X[i] = -c->p.edge / 2. + (double)i * c->p.edge / ((double)steps);
+        for (int s = 0; s < c->S; s++)
+            for (int pp = 0; pp < 3; pp++) {
+                for (int i = 0; i < EL; i++)
+                    for (int j = 0; j < EL; j++)
+                        F[(size_t)i * EL + j] = (*src[pp])[(size_t)s * c->G + plane_index(c, i, j)];
+                for (int i = 0; i < EL; i++) fi[((size_t)s * 3 + pp) * EL + i] = bilinear(X, F, cx[i], cy[i]);
+            }
+    }
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    for (int i = 0; i < EL; i++) {
+        put(f, -CutLine / 2. + (double)i * CutLine / ((double)steps), prec);
+        for (int s = 0; s < c->S; s++)
+            for (int pp = 0; pp < 3; pp++) {
+                std::fputc(' ', f);
+                put(f, fi[((size_t)s * 3 + pp) * EL + i], prec);
+            }
+        if (c->D == 3) {
+            std::fputc(' ', f);
+            put(f, c->lat_h[i], prec);
+            const int j = steps / 2;
+            for (int s = 0; s < c->S; s++) {
+                double col = 0.;
+                for (int k = 0; k < EL; k++) col += den[(size_t)s * c->G + ((size_t)i * EL + j) * EL + k];
+                col *= c->dx;
+                std::fputc(' ', f);
+                put(f, col, prec);
+            }
         }
         std::fputc('\n', f);
     }
@@ -256,6 +404,17 @@ static int run_one(const std::string &d, const mpdpft_params &p, int device, mpd
             if (!rc) rc = write_v(c, join(d, "mpDPFT_V.dat"), V);
         } else
             c->control << " ***** skip V ****\n";
+        std::vector<double> env;
+        if (!rc) {
+            env.resize(c->SG);
+            rc = mpdpft_get_environment(c, env.data());
+        }
+        if (c->D > 1) {
+            c->control << " ***** export ContourData ****\n";
+            if (!rc) rc = write_contour(c, join(d, "mpDPFT_ContourData.dat"), den, env, V);
+        }
+        c->control << " ***** export CutData ****\n";
+        if (!rc) rc = write_cut(c, join(d, "mpDPFT_CutData.dat"), den, env, V);
         c->control << final_block(c, r);
         c->control << " " << r.errorCount << " Error(s);  " << r.warningCount << " Warning(s)\n EndOfControlFile\n";
         FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");

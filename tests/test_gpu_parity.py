@@ -235,6 +235,66 @@ def test_drop_in_directory_run(oracle, tmp_path):
     assert rel(V, VO) < RTOL
     it = [l for l in ctl.split("\\\\") if l.startswith("SCiterations")][0]
     assert it.split("=")[1].strip() == "%d/%d" % (ro.SCcount - 1, p.maxSCcount)
+    check_plot_files(tmp_path, p, den, V)
+
+
+def check_plot_files(d, p, den, V):
+    """mpDPFT_ContourData.dat and mpDPFT_CutData.dat (ExportData, mpDPFT.cpp:12527-12660)
+    against the run's own Den/V files and the environment, recomputed here: the
+    x-y plane (3D: z index steps/2) with a blank line before each new x row, and
+    the default cut along the x axis, where the interpolated fields are the grid
+    values of row y = 0 (3D: plus the lattice coordinate and the column density
+    sum_k n dx)."""
+    EL, S, D = p.steps + 1, p.S, p.DIM
+    ctx = Context(p)
+    env = ctx.environment().reshape(S, -1)
+    ctx.close()
+    x = -p.edge / 2 + np.arange(EL) * (p.edge / p.steps)
+    plane = lambda f, i, j: f[..., i * EL + j] if D == 2 else f[..., (i * EL + j) * EL + p.steps // 2]
+    lines = (d / "mpDPFT_ContourData.dat").read_text().split("\n")
+    assert lines[-1] == "" and lines[-2] == ""  # the closing blank line
+    blocks = "\n".join(lines).strip("\n").split("\n\n")
+    assert len(blocks) == EL
+    for i, blk in enumerate(blocks):
+        rows = np.array([[float(t) for t in l.split(" ")] for l in blk.split("\n")])
+        assert rows.shape == (EL, 3 + 3 * S)
+        zc = x[p.steps // 2] if D == 3 else 0.0
+        assert np.allclose(rows[:, 0], x[i], rtol=0, atol=1e-13) and np.allclose(rows[:, 1], x, rtol=0, atol=1e-13)
+        assert np.allclose(rows[:, 2], np.sqrt(x[i] ** 2 + x ** 2 + zc ** 2), rtol=1e-14, atol=1e-13)
+        for s in range(S):
+            j = np.arange(EL)
+            assert np.array_equal(rows[:, 3 + 3 * s], plane(den[s], i, j))
+            assert np.allclose(rows[:, 4 + 3 * s], plane(env[s], i, j), rtol=1e-15, atol=0)
+            assert np.array_equal(rows[:, 5 + 3 * s], plane(V[s], i, j))
+    cut = np.array([[float(t) for t in l.split(" ")] for l in (d / "mpDPFT_CutData.dat").read_text().splitlines()])
+    assert cut.shape == (EL, 1 + 3 * S + (1 + S if D == 3 else 0))
+    assert np.allclose(cut[:, 0], x, rtol=0, atol=1e-13)
+    i = np.arange(EL)
+    for s in range(S):
+        for k, f in enumerate((den[s], env[s], V[s])):
+            ref = plane(f, i, p.steps // 2)
+            assert np.allclose(cut[:, 1 + 3 * s + k], ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    if D == 3:
+        assert np.allclose(cut[:, 1 + 3 * S], x, rtol=0, atol=1e-13)
+        cube = den.reshape(S, EL, EL, EL)
+        for s in range(S):
+            col = cube[s, :, p.steps // 2, :].sum(axis=1) * (p.edge / p.steps)
+            assert np.allclose(cut[:, 2 + 3 * S + s], col, rtol=1e-13, atol=0)
+
+
+def test_drop_in_plot_files_3d(tmp_path):
+    """The 3D form of the plot files: contour plane at z index steps/2, cut with
+    the lattice and column-density columns (TF + Hartree ion, 16^3, 3 iterations)."""
+    print("case drop-in plot files 3d", flush=True)
+    p = configs.c1(steps=16, density_expression=7, hartree_mode=1, maxsc=3)
+    p.n_interactions = 1
+    configs.write_input(p, tmp_path / "mpDPFT.input")
+    cli = os.path.join(os.path.dirname(mpdpft_amd.__file__), "bin", "mpdpft_hip")
+    out = subprocess.run([cli, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode in (0, 3), out.stderr  # 3: not converged after maxSCcount (files written)
+    den = np.array([[float(t) for t in l.split()] for l in (tmp_path / "mpDPFT_Den.dat").read_text().splitlines()]).T
+    V = np.array([[float(t) for t in l.split()[3:]] for l in (tmp_path / "mpDPFT_V.dat").read_text().splitlines()]).T
+    check_plot_files(tmp_path, p, den, V)
 
 
 def test_warm_start_from_v_file(oracle, tmp_path):
