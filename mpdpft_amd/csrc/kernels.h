@@ -31,7 +31,8 @@ enum MulKind {
     MUL_DERIV_PAIR = 2,  // re <- i k_a e^{sigma k^2}, im <- i k_b e^{sigma k^2} (Nyquist plane of the axis zeroed)
     MUL_DERIV_LAP = 3,   // re <- i k_z e^{sigma k^2}, im <- -|k|^2 e^{2 sigma k^2}
     MUL_HARTREE_PER = 4, // reference periodic Hartree with Gaussian compensation
-    MUL_SECOND = 5       // re <- -k_a^2 (method 4 Laplacian pieces)
+    MUL_SECOND = 5,      // re <- -k_a^2 (method 4 Laplacian pieces)
+    MUL_HARTREE_2D = 6   // 2D periodic Hartree: (f - f0 e^{-k^2}) / |k|, 0 for |k| <= dkzero
 };
 struct MulParams {
     int kind;

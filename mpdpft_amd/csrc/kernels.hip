@@ -318,21 +318,72 @@ template <int D, int KIND = -1> struct LdMulD {  // KIND >= 0: the multiplier ki
             }
             return o;
         }
+        case MUL_HARTREE_2D: {  // HartreeRepulsion 2D (mpDPFT.cpp:2627-2645); dkzero is the |k| threshold
+            double kk2 = 0.;
+            int par = 0;
+            for (int d = 0; d < D; d++) {
+                double kh = m.kq_h[q[d]];
+                kk2 += kh * kh;
+                par += q[d];
+            }
+            const double k = sqrt(kk2), f0 = *m.f0slot;
+            if (k > m.dkzero) {
+                double sg = (par & 1) ? -1. : 1.;
+                o.x = (m.dxD * v.x - sg * f0 * d_exp(-k * k)) / k;
+                o.y = m.dxD * v.y / k;
+            } else {
+                o.x = 0.;
+                o.y = 0.;
+            }
+            return o;
+        }
         default:
             return v;
         }
     }
 };
 
+// e^-x I0(x), x >= 0 (the 2D Hartree compensation; the reference calls
+// gsl_sf_bessel_I0): the power series sum_k ((x/2)^k / k!)^2 of positive terms
+// for x <= 30, the asymptotic series (2 pi x)^-1/2 sum_k a_k x^-k with a_k =
+// a_{k-1} (2k-1)^2 / (8k) beyond, each summed until the next term is below 1e-17
+// of the sum (within 2e-15 of the exact value).
+__device__ double exp_i0(double x) {
+    if (x <= 30.) {
+        double q = 0.25 * x * x, t = 1., sum = 1.;
+        for (int k = 1; k < 400; k++) {
+            t *= q / ((double)k * (double)k);
+            sum += t;
+            if (t < 1e-17 * sum) break;
+        }
+        return exp(-x) * sum;
+    }
+    double a = 1., sum = 1., xi = 1. / x;
+    for (int k = 1; k < 200; k++) {
+        double nx = a * (2. * k - 1.) * (2. * k - 1.) / (8. * k) * xi;
+        if (nx < 1e-17 * sum || nx > a) break;
+        a = nx;
+        sum += a;
+    }
+    return sum / sqrt(2. * PI_D * x);
+}
+
 // store the real/imag parts onto the (s+1)^D grid with the periodic images
 struct StGrid {
     double *re, *im;
     double scale;
-    int post;  // 0 plain, 1 POS(re), 2 Hartree compensation on re
+    int post;  // 0 plain, 1 POS(re), 2 Hartree compensation on re (3D erf form), 3 (2D Bessel form)
     int EL, s, D;
     const double *lat;
     const double *f0slot;
-    double c1;  // 4 pi gammaH for post 2
+    double c1;  // 4 pi gammaH for post 2, 2 pi gammaH for post 3
+    // post 3: 2 pi gammaH (val + f0 e^{-z^2/8} I0(z^2/8) / (4 sqrt(pi))) (mpDPFT.cpp:2647-2655)
+    __device__ double hart2d(int idx, double val) const {
+        int a = idx / EL, b = idx - a * EL;
+        double z = sqrt(lat[a] * lat[a] + lat[b] * lat[b]), e = z * z / 8.;
+        double ei0 = e < 1.0e+300 ? exp_i0(e) : 1. / sqrt(2. * PI_D * e);
+        return c1 * (val + (*f0slot) * ei0 / (4. * sqrt(PI_D)));
+    }
     __device__ double hart(int idx, double val) const {
         double r[3];
         int i = idx;
@@ -356,6 +407,7 @@ struct StGrid {
             double a = scale * v.x;
             if (post == 1) a = d_pos(a);
             else if (post == 2) a = hart(idx, a);
+            else if (post == 3) a = hart2d(idx, a);
             re[idx] = a;
         }
         if (im) im[idx] = scale * v.y;

@@ -170,8 +170,6 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         int t = p->Interactions[a];
         if (!(t == 0 || t == 1 || t == 8 || t == 10 || t == 11 || t == -1 || t == -12))
             return ctx_fail(c, MPDPFT_EINVAL, "unsupported interaction type " + std::to_string(t));
-        if (t == 8 && p->DIM == 2 && p->HartreeMode == 0)
-            return ctx_fail(c, MPDPFT_EINVAL, "2D Hartree needs HartreeMode 1 (isolated)");
     }
     for (int r = 0; r < p->n_regularize; r++)
         if (p->regularize[r] != 0) return ctx_fail(c, MPDPFT_EINVAL, "regularize must be 0");
@@ -622,6 +620,20 @@ static int hartree(mpdpft_ctx *c, const double *n, double *out) {
         return MPDPFT_OK;
     }
     stage_begin(c);
+    if (c->D == 2) {  // periodic + Gaussian compensation, 2D (HartreeRepulsion :2627-2659)
+        HIPC(mk::fft_forward_real(c->g, n, nullptr, c->d_spec, c->d_tw, c->d_scal + 200, c->st));
+        mk::MulParams m{};
+        m.kind = mk::MUL_HARTREE_2D;
+        m.kq = c->d_kq;
+        m.kq_h = c->d_kqh;
+        m.f0slot = c->d_scal + 200;
+        m.dxD = dxD;
+        m.dkzero = 0.00001 * c->dk;
+        HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_spec, m, out, nullptr, std::pow(1. / p.edge, 2.), 3,
+                                   2. * kPI * p.gammaH, c->d_tw, c->st));
+        stage_end(c, "hartree_periodic", 2., 2. * 32. * c->D * (double)c->F, 0.);
+        return MPDPFT_OK;
+    }
     // periodic + Gaussian compensation (HartreeRepulsion 3D, :2624-2712)
     HIPC(mk::fft_forward_real(c->g, n, nullptr, c->d_spec, c->d_tw, c->d_scal + 200, c->st));
     mk::MulParams m{};

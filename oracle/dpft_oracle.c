@@ -788,6 +788,70 @@ static void hartree_isolated(orc_state *o, const double *n, double *out) {
     free(buf);
 }
 
+/* e^-x I0(x), x >= 0, for the 2D compensation term (the reference calls GSL's
+ * gsl_sf_bessel_I0, absent here): the power series sum_k ((x/2)^k / k!)^2
+ * (positive terms) up to x = 30, the asymptotic series
+ * (2 pi x)^-1/2 sum_k a_k x^-k, a_k = a_{k-1} (2k-1)^2 / (8k), beyond; both
+ * summed until the next term is below 1e-17 of the sum.  Pinned against
+ * scipy.special.i0e in the tests. */
+double oracle_exp_i0(double x) {
+    if (x <= 30.) {
+        double q = 0.25 * x * x, t = 1., sum = 1.;
+        for (int k = 1; k < 400; k++) {
+            t *= q / ((double)k * (double)k);
+            sum += t;
+            if (t < 1e-17 * sum) break;
+        }
+        return exp(-x) * sum;
+    }
+    double a = 1., sum = 1., xi = 1. / x;
+    for (int k = 1; k < 200; k++) {
+        double nx = a * (2. * k - 1.) * (2. * k - 1.) / (8. * k) * xi;
+        if (nx < 1e-17 * sum || nx > a) break;
+        a = nx;
+        sum += a;
+    }
+    return sum / sqrt(2. * PI_ * x);
+}
+
+/* HartreeRepulsion in 2D, HartreeType 0 (mpDPFT.cpp:2627-2659): the centred
+ * transform, (f - f0 e^{-k^2}) / |k| for |k| > 1e-5 dk and 0 at k = 0, back,
+ * plus the potential of the subtracted Gaussian f0 e^{-z^2/8} I0(z^2/8) /
+ * (4 sqrt(pi)), times 2 pi gammaH. */
+static int hartree_2d(orc_state *o, int EnergyQ, const double *n, double *out) {
+    for (long long i = 0; i < o->G; i++) {
+        o->CF[i].re = n[i];
+        o->CF[i].im = 0.;
+    }
+    fft_centered(o, o->CF, -1);
+    double f0 = o->CF[o->CentreIndex].re;
+    int EL = o->EL;
+    for (int i = 0; i < EL; i++) {
+        double qi = o->k0 + (double)i * o->dk, qi2 = qi * qi;
+        for (int j = 0; j < EL; j++) {
+            double qj = o->k0 + (double)j * o->dk, qj2 = qj * qj;
+            long long idx = (long long)i * EL + j;
+            double k = sqrt(qi2 + qj2);
+            if (k > 0.00001 * o->dk) {
+                o->CF[idx].re = (o->CF[idx].re - f0 * EXPs(-k * k)) / k;
+                o->CF[idx].im /= k;
+            } else {
+                o->CF[idx].re = 0.;
+                o->CF[idx].im = 0.;
+            }
+        }
+    }
+    fft_centered(o, o->CF, +1);
+    for (long long i = 0; i < o->G; i++) {
+        double z = o->rnorm[i], e = z * z / 8.;
+        double ei0 = e < 1.0e+300 ? oracle_exp_i0(e) : 1. / sqrt(2. * PI_ * e);
+        double v = o->CF[i].re + f0 * ei0 / (4. * sqrt(PI_));
+        out[i] = 2. * PI_ * o->p.gammaH * v;
+        if (EnergyQ) out[i] *= 0.5 * n[i];
+    }
+    return 0;
+}
+
 /* HartreeRepulsion, HartreeType 0 (mpDPFT.cpp:2583-2717): periodic transform
  * with Gaussian compensation of the k = 0 singularity. */
 static int hartree(orc_state *o, int EnergyQ, int ss, double *out) {
@@ -798,7 +862,7 @@ static int hartree(orc_state *o, int EnergyQ, int ss, double *out) {
             for (long long i = 0; i < o->G; i++) out[i] *= 0.5 * n[i];
         return 0;
     }
-    if (o->D != 3) return MPDPFT_EINVAL; /* 2D Bessel-I0 variant: not on this path */
+    if (o->D == 2) return hartree_2d(o, EnergyQ, n, out);
     for (long long i = 0; i < o->G; i++) {
         o->CF[i].re = n[i];
         o->CF[i].im = 0.;

@@ -48,6 +48,8 @@ double oracle_cube_inverse_r(void);
 double oracle_curlyA(double y);
 double oracle_polylog(int j, double x);
 double oracle_airy(double y);
+/* e^-x I0(x) for x >= 0 (2D periodic Hartree compensation) */
+double oracle_exp_i0(double x);
 
 /* Timed partial workload for the CPU baseline: n_iter SC iterations (no
  * convergence test) starting from the initial density; returns seconds. */

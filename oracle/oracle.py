@@ -55,7 +55,7 @@ def lib():
         L.oracle_cube_inverse_r.restype = ctypes.c_double
         L.oracle_time_iterations.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.oracle_time_iterations.restype = ctypes.c_double
-        for n in ("oracle_curlyA", "oracle_airy"):
+        for n in ("oracle_curlyA", "oracle_airy", "oracle_exp_i0"):
             getattr(L, n).argtypes = [ctypes.c_double]
             getattr(L, n).restype = ctypes.c_double
         L.oracle_polylog.argtypes = [ctypes.c_int, ctypes.c_double]
@@ -184,3 +184,8 @@ def polylog(j, x):
 
 def airy(y):
     return lib().oracle_airy(float(y))
+
+
+def exp_i0(x):
+    """e^-x I0(x) (2D periodic Hartree compensation)."""
+    return lib().oracle_exp_i0(float(x))

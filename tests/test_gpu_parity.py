@@ -90,10 +90,10 @@ def test_fft_block_shape_does_not_change_results(monkeypatch, steps):
 
 
 @pytest.mark.parametrize("case", ["contact2d", "contact3", "dirac", "ldax", "pw92", "hartree_per", "hartree_iso",
-                                  "hartree_iso2d", "single_contact"])
+                                  "hartree_iso2d", "hartree_per2d", "single_contact"])
 def test_interaction_potentials(oracle, case):
     print("case vint", case, flush=True)
-    if case in ("contact2d", "hartree_iso2d"):
+    if case in ("contact2d", "hartree_iso2d", "hartree_per2d"):
         p = configs.c0(steps=64, edge=12.0)
     elif case == "contact3":
         p = configs.c0(steps=32, edge=12.0)
@@ -105,7 +105,7 @@ def test_interaction_potentials(oracle, case):
     else:
         p = configs.c1(steps=32, edge=12.0, hartree_mode=1 if case.startswith("hartree_iso") else 0)
     typ = {"contact2d": 1, "contact3": 1, "dirac": 11, "ldax": -1, "pw92": -12, "hartree_per": 8,
-           "hartree_iso": 8, "hartree_iso2d": 8, "single_contact": 10}[case]
+           "hartree_iso": 8, "hartree_iso2d": 8, "hartree_per2d": 8, "single_contact": 10}[case]
     p.n_interactions = 1
     p.Interactions[0] = typ
     if case == "hartree_iso2d":
@@ -199,6 +199,26 @@ def test_3d_tf_hartree_xc_trajectory(oracle, hmode, iters):
     assert ro.SCcount == iters + 1 and rg.SCcount == iters + 1
     check_loop(ro, dO, VO, rg, dG, VG, 1)
     assert rg.bisection_steps == ro.bisection_steps
+
+
+@pytest.mark.parametrize("maxsc", [4, 400])
+def test_2d_periodic_hartree_loop(oracle, maxsc):
+    """2D two-species TF gas with contact and the periodic Hartree term (Bessel-I0
+    compensation, mpDPFT.cpp:2627-2659): a 4-iteration trajectory and a converged
+    solve on both paths."""
+    print("case 2d periodic hartree", maxsc, flush=True)
+    p = configs.c0(steps=64, edge=16.0, maxsc=maxsc)
+    p.n_interactions = 2
+    p.Interactions[1] = 8
+    p.HartreeMode = 0
+    p.gammaH = 0.05
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    if maxsc == 4:
+        assert ro.SCcount == 5 and rg.SCcount == 5
+        check_loop(ro, dO, VO, rg, dG, VG, p.S)
+    else:
+        assert ro.converged and rg.converged
+        check_loop(ro, dO, VO, rg, dG, VG, p.S, exact_count=False)
 
 
 def test_3d_tf_ion_converges(oracle):

@@ -129,6 +129,43 @@ def test_hartree_periodic_gaussian_closed_form(oracle):
     assert np.abs(V - exact).max() < 1e-6 * np.abs(exact).max()
 
 
+def test_exp_i0_matches_scipy(oracle):
+    """e^-x I0(x) of the 2D Hartree compensation (power series / asymptotic
+    series) against scipy.special.i0e, on both sides of the switch at x = 30."""
+    from scipy import special
+    xs = np.concatenate([np.linspace(0.0, 60.0, 6001), np.geomspace(1e-10, 1e5, 300)])
+    got = np.array([oracle.exp_i0(x) for x in xs])
+    assert np.abs(got / special.i0e(xs) - 1.0).max() < 5e-15
+
+
+@pytest.mark.filterwarnings("ignore::scipy.integrate.IntegrationWarning")
+def test_hartree_2d_periodic_gaussian_closed_form(oracle):
+    """2D: n = A exp(-r^2/4) has n~ = 4 pi A exp(-k^2) = f0 exp(-k^2), so the
+    compensated periodic solve (mpDPFT.cpp:2627-2659) leaves only the Gaussian's
+    in-plane Coulomb potential gH 2 pi^{3/2} A e^{-r^2/8} I0(r^2/8); its value at
+    r = 0 is checked against the quadrature int n(r)/r d^2r = 2 pi A int e^{-r^2/4} dr,
+    and the radial form against a direct polar quadrature at r = 3."""
+    from scipy import special
+    p = configs.c0(steps=64, edge=24.0)
+    p.n_interactions = 1
+    p.Interactions[0] = 8
+    p.HartreeMode = 0
+    p.gammaH = 0.7
+    x, y = grid(p)
+    r = np.sqrt(x * x + y * y)
+    A = 1.5
+    n = A * np.exp(-r * r / 4)
+    V = oracle.vint(p, 8, 0, np.concatenate([n, n]))
+    exact = p.gammaH * 2 * math.pi ** 1.5 * A * special.i0e(r * r / 8)
+    assert np.abs(V - exact).max() < 1e-12 * np.abs(exact).max()
+    v0, _ = sint.quad(lambda t: math.exp(-t * t / 4), 0, np.inf, epsabs=1e-14)
+    assert abs(p.gammaH * 2 * math.pi * A * v0 - exact[r.argmin()]) < 1e-12 * exact.max()
+    # V(3, 0) = int n(r') / |r - r'| d^2 r' in polar coordinates around the origin
+    f = lambda phi, t: A * math.exp(-t * t / 4) * t / math.sqrt(9.0 + t * t - 6.0 * t * math.cos(phi))
+    v3, _ = sint.dblquad(f, 0, 40, 0, 2 * math.pi, epsabs=1e-11, epsrel=1e-11)
+    assert abs(p.gammaH * v3 - p.gammaH * 2 * math.pi ** 1.5 * A * special.i0e(9.0 / 8)) < 1e-8
+
+
 @pytest.mark.parametrize("D,steps", [(3, 12), (2, 24)])
 def test_hartree_isolated_fft_equals_direct_sum(oracle, D, steps):
     p = configs.c1(steps=steps, edge=8.0, hartree_mode=1) if D == 3 else configs.c0(steps=steps, edge=8.0)
