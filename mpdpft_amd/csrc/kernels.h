@@ -168,7 +168,9 @@ enum EnergyOp {
     E_POW_POS = 4,       // POS(a)^p
     E_GRADCORR = 5,      // b/|a| if |a| > thr else 0   (a = n, b = grad2)
     E_VIRIAL = 6,        // a * (x b + y c + z d)
-    E_HALF_PROD = 7      // 0.5 a b
+    E_HALF_PROD = 7,     // 0.5 a b
+    E_GOMBAS = 8,        // Gombas correlation energy density of a, scale p0 (= gammaH)
+    E_VWN = 9            // VWN correlation energy density of a (eV, density in 1/Angstrom^3)
 };
 struct EnergyParams {
     int op;

@@ -793,6 +793,36 @@ hipError_t tf_density_multi(const Grid &g, const double *V, const double *mus_de
 }
 
 // ---------------------------------------------------------------------------
+// Gombas and VWN correlation (GombasCorrelation / VWNCorrelation,
+// mpDPFT.cpp:2774-2828), densities in 1/Angstrom^3, energies in eV.
+// Gombas: e(n) = -gammaH n (a1 t / (a2 + c t) + (b1 / c) ln(1 + b2 c t)),
+// t = POS(n)^(1/3); v = de/dn.  VWN: e(n) = n A F(x) for n > 1e-15, x =
+// (3 / (4 pi c^3 n))^(1/6) = sqrt(r_s), F = Y + (2b/Q) T - (b x0 / X0)(Z + 2(b +
+// 2 x0) T / Q) with X = x^2 + b x + c', Y = ln(x^2/X), Z = ln((x - x0)^2 / X),
+// T = atan(Q / (2x + b)); v = de/dn = A (F - (x/6) dF/dx) for n > 1e-12.  The
+// potentials are used in 3D with Units 2 only (0 otherwise), as in the reference.
+// ---------------------------------------------------------------------------
+constexpr double kGa1 = 0.0357, kGa2 = 0.0562, kGb1 = 0.0311, kGb2 = 2.39, kBohrA = 0.52917721067;
+__device__ __forceinline__ void gombas(double n, double gammaH, double *e, double *v) {
+    double t = pow(d_pos(n), 1. / 3.), dn = kGa2 + kBohrA * t, la = 1. + kGb2 * kBohrA * t;
+    double f = kGa1 * t / dn + (kGb1 / kBohrA) * log(la);
+    *e = -gammaH * n * f;
+    *v = -gammaH * (f + (t / 3.) * (kGa1 * kGa2 / (dn * dn) + kGb1 * kGb2 / la));
+}
+constexpr double kVA = 0.846021, kVb = 13.072, kVc = 42.7198, kVQ = 0.0448998, kVx0 = -0.409286, kVX0 = 37.53713;
+__device__ __forceinline__ void vwn_F(double x, double *F, double *dF) {
+    double X = x * x + kVb * x + kVc, dX = 2. * x + kVb;
+    double Y = log(x * x / X), Z = log((x - kVx0) * (x - kVx0) / X), T = atan(kVQ / (2. * x + kVb));
+    double dT = -2. * kVQ / ((2. * x + kVb) * (2. * x + kVb) + kVQ * kVQ);
+    const double g = kVb * kVx0 / kVX0, h = 2. * (kVb + 2. * kVx0) / kVQ;
+    *F = Y + 2. * kVb / kVQ * T - g * (Z + h * T);
+    *dF = (2. / x - dX / X) + 2. * kVb / kVQ * dT - g * ((2. / (x - kVx0) - dX / X) + h * dT);
+}
+__device__ __forceinline__ double vwn_x(double n) {
+    return pow(3. / (4. * PI_D * kBohrA * kBohrA * kBohrA * n), 1. / 6.);
+}
+
+// ---------------------------------------------------------------------------
 // potentials
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const double *__restrict__ den,
@@ -829,6 +859,16 @@ __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const d
                 } else {
                     double e;
                     xc_lda(t, vp.units, ns, &e, &v);
+                }
+            } else if (t == 12 && g.D == 3 && vp.units == 2) {
+                double e;
+                gombas(ns, vp.gammaH, &e, &v);
+            } else if (t == 13 && g.D == 3 && vp.units == 2) {
+                const double d = d_pos(ns);
+                if (d > 1.0e-12) {
+                    double x = vwn_x(d), F, dF;
+                    vwn_F(x, &F, &dF);
+                    v = kVA * (F - x / 6. * dF);
                 }
             }
             acc += v;
@@ -1053,6 +1093,21 @@ __global__ __launch_bounds__(256) void k_energy(Grid g, EnergyParams ep, const d
             break;
         }
         case E_POW_POS: v = pow(d_pos(a[i]), ep.p0); break;
+        case E_GOMBAS: {
+            double vv;
+            gombas(a[i], ep.p0, &v, &vv);
+            break;
+        }
+        case E_VWN: {
+            const double d = d_pos(a[i]);
+            v = 0.;
+            if (d > 1.0e-15) {
+                double F, dF;
+                vwn_F(vwn_x(d), &F, &dF);
+                v = d * kVA * F;
+            }
+            break;
+        }
         case E_GRADCORR: {
             double rho = fabs(a[i]);
             v = rho > ep.p0 ? b[i] / rho : 0.;

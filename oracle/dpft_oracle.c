@@ -950,6 +950,59 @@ static void dirac(orc_state *o, int EnergyQ, int ss, double *out) {
     }
 }
 
+/* GombasCorrelation (mpDPFT.cpp:2774-2796): energy density
+ * e(n) = -gammaH n (a1 t/(a2 + c t) + (b1/c) ln(1 + b2 c t)), t = POS(n)^(1/3);
+ * the potential (3D, Units 2 only; 0 otherwise) is de/dn. */
+static void gombas(orc_state *o, int EnergyQ, int ss, double *out) {
+    const double *n = o->Den + (long long)ss * o->G;
+    const double a1 = 0.0357, a2 = 0.0562, b1 = 0.0311, b2 = 2.39, c = 0.52917721067;
+    for (long long i = 0; i < o->G; i++) {
+        double t = pow(POSs(n[i]), 1. / 3.), dn = a2 + c * t, la = 1. + b2 * c * t;
+        if (EnergyQ)
+            out[i] = -o->p.gammaH * n[i] * (a1 * t / dn + (b1 / c) * log(la));
+        else if (o->D == 3 && o->p.Units == 2)
+            out[i] = -o->p.gammaH * (a1 * t / dn + (b1 / c) * log(la) + (t / 3.) * (a1 * a2 / (dn * dn) + b1 * b2 / la));
+        else
+            out[i] = 0.;
+    }
+}
+
+/* VWNCorrelation (mpDPFT.cpp:2798-2828): energy density e(n) = n A F(x) for
+ * n > 1e-15 (0 otherwise), x = (3 / (4 pi c^3 n))^(1/6) (c = 0.52917721067),
+ * F = Y + (2b/Q) T - (b x0 / X0)(Z + 2(b + 2x0) T / Q), X = x^2 + b x + c',
+ * Y = ln(x^2/X), Z = ln((x - x0)^2 / X), T = atan(Q / (2x + b)).  The
+ * potential (3D, Units 2, n > 1e-12; 0 otherwise) is de/dn = A (F - (x/6) F'),
+ * derived here from F (the reference writes out the same derivative in closed
+ * form; the tests compare the two and a finite difference of e). */
+static void vwn_parts(double x, double *F, double *dF) {
+    const double b = 13.072, cc = 42.7198, Q = 0.0448998, x0 = -0.409286, X0 = 37.53713;
+    double X = x * x + b * x + cc, dX = 2. * x + b;
+    double Y = log(x * x / X), Z = log((x - x0) * (x - x0) / X), T = atan(Q / (2. * x + b));
+    double dY = 2. / x - dX / X, dZ = 2. / (x - x0) - dX / X;
+    double dT = -2. * Q / ((2. * x + b) * (2. * x + b) + Q * Q);
+    *F = Y + 2. * b / Q * T - b * x0 / X0 * (Z + 2. * (b + 2. * x0) * T / Q);
+    *dF = dY + 2. * b / Q * dT - b * x0 / X0 * (dZ + 2. * (b + 2. * x0) * dT / Q);
+}
+static void vwn(orc_state *o, int EnergyQ, int ss, double *out) {
+    const double *n = o->Den + (long long)ss * o->G;
+    const double A = 0.846021, c3 = 0.52917721067 * 0.52917721067 * 0.52917721067;
+    for (long long i = 0; i < o->G; i++) {
+        double d = POSs(n[i]), F, dF;
+        out[i] = 0.;
+        if (EnergyQ) {
+            if (d > 1.0e-15) {
+                double x = pow(3. / (4. * PI_ * c3 * d), 1. / 6.);
+                vwn_parts(x, &F, &dF);
+                out[i] = d * A * F;
+            }
+        } else if (o->D == 3 && o->p.Units == 2 && d > MP_) {
+            double x = pow(3. / (4. * PI_ * c3 * d), 1. / 6.);
+            vwn_parts(x, &F, &dF);
+            out[i] = A * (F - x / 6. * dF);
+        }
+    }
+}
+
 static int getvint(orc_state *o, int type, int ss, double *out) {
     long long G = o->G;
     if (type == 0) {
@@ -968,6 +1021,10 @@ static int getvint(orc_state *o, int type, int ss, double *out) {
         for (long long i = 0; i < G; i++) out[i] = o->Den[i] * (2. * o->p.beta);
     } else if (type == 11) {
         dirac(o, 0, ss, out);
+    } else if (type == 12) {
+        gombas(o, 0, ss, out);
+    } else if (type == 13) {
+        vwn(o, 0, ss, out);
     } else
         return MPDPFT_EINVAL;
     return 0;
@@ -1619,6 +1676,11 @@ static double interaction_energy(orc_state *o, int type) { /* :5805-5918 */
     }
     if (o->D == 3 && o->S == 1 && type == 11) {
         dirac(o, 1, 0, o->Tmp2);
+        return integrate(o, o->Tmp2);
+    }
+    if (o->D == 3 && o->S == 1 && (type == 12 || type == 13)) {
+        if (type == 12) gombas(o, 1, 0, o->Tmp2);
+        else vwn(o, 1, 0, o->Tmp2);
         return integrate(o, o->Tmp2);
     }
     return 0.;

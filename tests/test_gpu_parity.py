@@ -90,7 +90,7 @@ def test_fft_block_shape_does_not_change_results(monkeypatch, steps):
 
 
 @pytest.mark.parametrize("case", ["contact2d", "contact3", "dirac", "ldax", "pw92", "hartree_per", "hartree_iso",
-                                  "hartree_iso2d", "hartree_per2d", "single_contact"])
+                                  "hartree_iso2d", "hartree_per2d", "single_contact", "gombas", "vwn"])
 def test_interaction_potentials(oracle, case):
     print("case vint", case, flush=True)
     if case in ("contact2d", "hartree_iso2d", "hartree_per2d"):
@@ -105,7 +105,10 @@ def test_interaction_potentials(oracle, case):
     else:
         p = configs.c1(steps=32, edge=12.0, hartree_mode=1 if case.startswith("hartree_iso") else 0)
     typ = {"contact2d": 1, "contact3": 1, "dirac": 11, "ldax": -1, "pw92": -12, "hartree_per": 8,
-           "hartree_iso": 8, "hartree_iso2d": 8, "hartree_per2d": 8, "single_contact": 10}[case]
+           "hartree_iso": 8, "hartree_iso2d": 8, "hartree_per2d": 8, "single_contact": 10, "gombas": 12,
+           "vwn": 13}[case]
+    if case in ("gombas", "vwn"):
+        p.Units = 2  # the reference evaluates these potentials in 3D with Units 2 only
     p.n_interactions = 1
     p.Interactions[0] = typ
     if case == "hartree_iso2d":
@@ -219,6 +222,22 @@ def test_2d_periodic_hartree_loop(oracle, maxsc):
     else:
         assert ro.converged and rg.converged
         check_loop(ro, dO, VO, rg, dG, VG, p.S, exact_count=False)
+
+
+def test_3d_gombas_vwn_trajectory(oracle):
+    """3D TF in Units 2 with isolated Hartree, Dirac exchange and the Gombas and VWN
+    correlation terms (potentials and energies): 3 iterations on both paths."""
+    print("case gombas vwn trajectory", flush=True)
+    p = configs.c1(steps=16, density_expression=7, hartree_mode=1, maxsc=3)
+    p.Units = 2
+    p.n_interactions = 4
+    for k, t in enumerate((8, 11, 12, 13)):
+        p.Interactions[k] = t
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    assert ro.SCcount == 4 and rg.SCcount == 4
+    check_loop(ro, dO, VO, rg, dG, VG, 1)
+    for k in range(4):
+        assert abs(rg.InteractionEnergies[k] - ro.InteractionEnergies[k]) <= 1e-10 * abs(ro.InteractionEnergies[k]) + 1e-14
 
 
 def test_3d_tf_ion_converges(oracle):

@@ -129,6 +129,54 @@ def test_hartree_periodic_gaussian_closed_form(oracle):
     assert np.abs(V - exact).max() < 1e-6 * np.abs(exact).max()
 
 
+# Gombas and VWN correlation (mpDPFT.cpp:2774-2828) in Units 2 (1/Angstrom^3, eV):
+# the published energy densities; the potentials must be their derivatives.
+def _gombas_e(n, gH):
+    t = np.cbrt(np.maximum(n, 0.0))
+    c = 0.52917721067
+    return -gH * n * (0.0357 * t / (0.0562 + c * t) + (0.0311 / c) * np.log(1 + 2.39 * c * t))
+
+
+def _vwn_e(n):
+    A, b, c, Q, x0, X0 = 0.846021, 13.072, 42.7198, 0.0448998, -0.409286, 37.53713
+    x = (3 / (4 * math.pi * 0.52917721067 ** 3 * n)) ** (1 / 6)
+    X = x * x + b * x + c
+    T = np.arctan(Q / (2 * x + b))
+    F = np.log(x * x / X) + 2 * b / Q * T - b * x0 / X0 * (np.log((x - x0) ** 2 / X) + 2 * (b + 2 * x0) * T / Q)
+    return n * A * F
+
+
+@pytest.mark.parametrize("typ", [12, 13])
+def test_gombas_vwn_potential_is_energy_derivative(oracle, typ):
+    p = configs.c1(steps=8, edge=4.0)
+    p.Units = 2
+    p.gammaH = 0.8
+    n = np.geomspace(1e-6, 50.0, (p.steps + 1) ** 3)
+    v = oracle.vint(p, typ, 0, n)
+    e = (lambda m: _gombas_e(m, p.gammaH)) if typ == 12 else _vwn_e
+    h = 1e-5 * n
+    fd = (e(n + h) - e(n - h)) / (2 * h)
+    assert np.abs(v - fd).max() < 1e-7 * np.abs(fd).max()
+    assert np.all(np.abs(v - fd) <= 1e-6 * np.abs(fd) + 1e-12)
+    q = configs.c1(steps=8, edge=4.0)  # Units 1: the reference sets these potentials to 0
+    assert np.all(oracle.vint(q, typ, 0, n) == 0.0)
+
+
+def test_vwn_energy_rpa_form(oracle):
+    """The reference's VWN constants (b = 13.072, c = 42.7198, x0 = -0.409286) are
+    VWN's fit to the RPA correlation energy: its high-density limit is A ln r_s
+    with A = 0.0310907 Ha (0.846021 eV; x = sqrt(r_s) for n in 1/Angstrom^3), and
+    the RPA lies below the Ceperley-Alder correlation (PW92 here) by 25-70 % for
+    r_s = 0.5 ... 10."""
+    eps = lambda rs: float(_vwn_e(np.array([3 / (4 * math.pi * rs ** 3) / 0.52917721067 ** 3]))[0]) / (
+        3 / (4 * math.pi * rs ** 3) / 0.52917721067 ** 3) / 27.211386245988
+    slope = (eps(1e-7) - eps(1e-8)) / math.log(10.0)
+    assert abs(slope / 0.0310907 - 1) < 1e-3
+    for rs in (0.5, 1.0, 2.0, 5.0, 10.0):
+        ratio = eps(rs) / oracle.pw92(3 / (4 * math.pi * rs ** 3))[0]
+        assert 1.2 < ratio < 1.75, (rs, ratio)
+
+
 def test_exp_i0_matches_scipy(oracle):
     """e^-x I0(x) of the 2D Hartree compensation (power series / asymptotic
     series) against scipy.special.i0e, on both sides of the switch at x = 30."""
