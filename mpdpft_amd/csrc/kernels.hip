@@ -821,6 +821,35 @@ __device__ __forceinline__ void vwn_F(double x, double *F, double *dF) {
 __device__ __forceinline__ double vwn_x(double n) {
     return pow(3. / (4. * PI_D * kBohrA * kBohrA * kBohrA * n), 1. / 6.);
 }
+// the pointwise potentials for k_update_v, out of line: inlined, their
+// transcendentals set the register count (and occupancy) of every update
+__device__ __noinline__ double xc_v(int t, int units, double n) {
+    double e, v;
+    xc_lda(t, units, n, &e, &v);
+    return v;
+}
+__device__ __noinline__ double xc_v_f(int t, int units, double n) { return (double)xc_lda_v_f(t, units, (float)n); }
+// DiracExchange (mpDPFT.cpp:2766-2769): -gammaH 2 (D n / pi)^{1/D} / (D - 1), D > 1
+__device__ __noinline__ double dirac_v(double n, int D, double gammaH) {
+    const double DD = (double)D;
+    return D > 1 ? -gammaH * (2. * pow(d_pos(n) * DD / PI_D, 1. / DD)) / (DD - 1.) : 0.;
+}
+__device__ __noinline__ double dirac_v_f(double n, int D, double gammaH) {
+    float nf = (float)d_pos(n), Df = (float)D;
+    return D > 1 ? (double)(-(float)gammaH * (2.f * powf(nf * Df / (float)PI_D, 1.f / Df)) / (Df - 1.f)) : 0.;
+}
+__device__ __noinline__ double gombas_v(double n, double gammaH) {
+    double e, v;
+    gombas(n, gammaH, &e, &v);
+    return v;
+}
+__device__ __noinline__ double vwn_v(double n) {
+    const double d = d_pos(n);
+    if (!(d > 1.0e-12)) return 0.;
+    double x = vwn_x(d), F, dF;
+    vwn_F(x, &F, &dF);
+    return kVA * (F - x / 6. * dF);
+}
 
 // ---------------------------------------------------------------------------
 // potentials
@@ -828,7 +857,6 @@ __device__ __forceinline__ double vwn_x(double n) {
 __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const double *__restrict__ den,
                                                   const double *__restrict__ env, double *V) {
     size_t G = g.G, tot = G * (size_t)vp.S;
-    double DD = (double)g.D;
     for (size_t idx = blockIdx.x * 256 + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * 256) {
         int s = (int)(idx / G);
         size_t i = idx - (size_t)s * G;
@@ -846,30 +874,13 @@ __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const d
             } else if (t == 10) {
                 v = den[i] * (2. * vp.beta);
             } else if (t == 11) {
-                if (vp.fp32) {
-                    float nf = (float)d_pos(ns), Df = (float)g.D;
-                    v = g.D > 1 ? (double)(-(float)vp.gammaH * (2.f * powf(nf * Df / (float)PI_D, 1.f / Df)) / (Df - 1.f))
-                                : 0.;
-                } else {
-                    v = g.D > 1 ? -vp.gammaH * (2. * pow(d_pos(ns) * DD / PI_D, 1. / DD)) / (DD - 1.) : 0.;
-                }
+                v = vp.fp32 ? dirac_v_f(ns, g.D, vp.gammaH) : dirac_v(ns, g.D, vp.gammaH);
             } else if (t == -1 || t == -12) {
-                if (vp.fp32) {
-                    v = (double)xc_lda_v_f(t, vp.units, (float)ns);
-                } else {
-                    double e;
-                    xc_lda(t, vp.units, ns, &e, &v);
-                }
+                v = vp.fp32 ? xc_v_f(t, vp.units, ns) : xc_v(t, vp.units, ns);
             } else if (t == 12 && g.D == 3 && vp.units == 2) {
-                double e;
-                gombas(ns, vp.gammaH, &e, &v);
+                v = gombas_v(ns, vp.gammaH);
             } else if (t == 13 && g.D == 3 && vp.units == 2) {
-                const double d = d_pos(ns);
-                if (d > 1.0e-12) {
-                    double x = vwn_x(d), F, dF;
-                    vwn_F(x, &F, &dF);
-                    v = kVA * (F - x / 6. * dF);
-                }
+                v = vwn_v(ns);
             }
             acc += v;
         }
