@@ -1217,7 +1217,21 @@ __global__ __launch_bounds__(256) void k_shell_max_list(const double *f, const i
     __shared__ double sh[4];
     __shared__ bool last;
     double m = 1e308;
-    for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) m = fmin(m, -f[idx[j]]);
+    // eight list entries per thread and round: their index loads, then their
+    // value loads, are in flight together (the gathers are latency bound)
+    constexpr int U = 8;
+    const int stride = gridDim.x * 256;
+    for (int j0 = blockIdx.x * 256 + threadIdx.x; j0 < n; j0 += U * stride) {
+        int id[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = j0 + u * stride;
+            id[u] = j < n ? idx[j] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (id[u] >= 0) m = fmin(m, -f[id[u]]);
+    }
     double r = block_min(m, sh);
     if (threadIdx.x == 0) {
         partials[blockIdx.x] = r;
