@@ -228,7 +228,10 @@ struct StBufF0 {  // also captures dx^D * DC term
 
 // multiplier applied on the first backward load (D a compile-time constant, so
 // the per-axis lookups of a line's fixed coordinates are shared by its loads)
-template <int D, int KIND = -1> struct LdMulD {  // KIND >= 0: the multiplier kind fixed at compile time
+// KIND >= 0: the multiplier kind fixed at compile time; those instantiations are
+// launched only with the (k, smoothing) table kek, so they read nothing else
+// (with the table choice left to run time the compiler loads both forms).
+template <int D, int KIND = -1> struct LdMulD {
     const double2 *p;
     int n1, n2, Dr, s;
     MulParams m;
@@ -244,7 +247,7 @@ template <int D, int KIND = -1> struct LdMulD {  // KIND >= 0: the multiplier ki
         int q[3];
         for (int d = 0; d < D; d++) q[d] = q3[3 - D + d];
         double k[3] = {0., 0., 0.}, k2 = 0., fk = 1.;
-        if (m.kek) {  // k and the smoothing factor from one table entry per axis
+        if (KIND >= 0 || m.kek) {  // k and the smoothing factor from one table entry per axis
             for (int d = 0; d < D; d++) {
                 double2 t = m.kek[q[d]];
                 k[d] = t.x;
@@ -260,14 +263,14 @@ template <int D, int KIND = -1> struct LdMulD {  // KIND >= 0: the multiplier ki
         double2 o;
         switch (KIND >= 0 ? KIND : m.kind) {
         case MUL_SMOOTH: {
-            double f = m.kek ? fk : smooth(q);
+            double f = (KIND >= 0 || m.kek) ? fk : smooth(q);
             o.x = v.x * f;
             o.y = v.y * f;
             return o;
         }
         case MUL_DERIV_PAIR:
         case MUL_DERIV_LAP: {
-            double f = m.kek ? fk : smooth(q);
+            double f = (KIND >= 0 || m.kek) ? fk : smooth(q);
             // A = i ka f v  (Nyquist plane of axis a dropped)
             double2 A = {0., 0.}, B = {0., 0.};
             if (m.axis_a >= 0 && q[m.axis_a] != s / 2) {
