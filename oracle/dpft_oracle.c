@@ -14,13 +14,19 @@
  *    (mpDPFT.cpp:8850-8979), round trip, Parseval, wrap-fill.
  *  - DelField: analytic derivatives of a centred Gaussian; smoothing closed form.
  *  - Hartree periodic mode: the Gaussian-density closed form of the
- *    compensation scheme; isolated mode: direct O(G^2) summation.
+ *    compensation scheme (3D erf form; 2D Bessel-I0 form, with e^-x I0(x)
+ *    against scipy.special.i0e and the 2D Coulomb potential of a Gaussian by
+ *    polar quadrature); isolated mode: direct O(G^2) summation.
+ *  - Gombas and VWN correlation: potentials against central differences of
+ *    the energy densities; VWN's high-density slope A ln r_s and its RPA-fit
+ *    position relative to PW92.
  *  - TF + contact: the 2D continuum closed form mu = sqrt(N(2pi+g)/pi).
  *  - PW92: published values; Dirac: closed form.
  *  - Pulay small solve: Eigen 3.4 FullPivHouseholderQR (vendored in the
  *    reference) through oracle/_ref.
  *  - mu bisection, convergence, mixing: invariants (monotone N(mu), exit rule,
- *    linear/Pulay fixed points).
+ *    linear/Pulay fixed points) -- parity unpinned beyond these invariants and
+ *    the full-loop closed forms; the same holds for AdaptnAiT and EkinAiT03D.
  */
 #include "dpft_oracle.h"
 
