@@ -1028,12 +1028,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         q.pref2 = beta2 / ps.beta1;
         // lower limit: the first x = -2^k (k = 1, 2, ...) where the integrand
         // is below 1e-12, stopping once x < -1e8 (k = 27)
+        // (the table forms of the walk decide; within 1e-12 of the threshold, where
+        // they and the interpolants of the lower-limit test could disagree, the latter)
         bool small = true;
         if (lane < 27) {
             double x = -ldexp(1., lane + 1), arg = -ps.tau * (U - x * ar);
             double l1, l3;
-            polylog_pair(t, arg, l1, l3);
-            double test = ps.beta0 * curly_a(t, x) * ar * ps.tau * (ps.beta1 * l1 + beta2 * l3);
+            polylog_pair_fast(rt, arg, l1, l3);
+            double test = ps.beta0 * curly_a_fast(rt, x) * ar * ps.tau * (ps.beta1 * l1 + beta2 * l3);
+            const double at = fabs(test);
+            if (at > kMP * (1. - 1e-12) && at < kMP * (1. + 1e-12)) {
+                polylog_pair(t, arg, l1, l3);
+                test = ps.beta0 * curly_a(t, x) * ar * ps.tau * (ps.beta1 * l1 + beta2 * l3);
+            }
             small = !(fabs(test) > kMP);
         }
         unsigned long long bal = __ballot(small ? 1 : 0);
