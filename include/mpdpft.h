@@ -150,6 +150,9 @@ typedef struct mpdpft_result {
     double solve_seconds;  /* wall time of the SC loop (device-synchronised) */
     int bisection_steps;   /* total density evaluations inside mu bisection */
     int fp32_iterations;   /* Precision 1: SC iterations run with the FP32 stages (the rest are FP64) */
+    /* GetAuxEint (mpDPFT.cpp:5518-5549): for S = 1, DIM 3, Units 2 the Dirac exchange energy, its
+       potential-functional form, the Gombas and the VWN correlation energies; 0 otherwise */
+    double AuxEint[MPDPFT_MAX_SPECIES][4];
 } mpdpft_result;
 
 typedef struct mpdpft_ctx mpdpft_ctx;

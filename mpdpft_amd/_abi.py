@@ -62,6 +62,7 @@ class Result(ctypes.Structure):
         ("DispersalEnergies", _d * MAX_SPECIES), ("EnvironmentalEnergies", _d * MAX_SPECIES),
         ("InteractionEnergies", _d * MAX_LIST), ("AuxEkin", (_d * 4) * MAX_SPECIES),
         ("solve_seconds", _d), ("bisection_steps", _i), ("fp32_iterations", _i),
+        ("AuxEint", (_d * 4) * MAX_SPECIES),
     ]
 
 

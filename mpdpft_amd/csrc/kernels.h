@@ -170,7 +170,8 @@ enum EnergyOp {
     E_VIRIAL = 6,        // a * (x b + y c + z d)
     E_HALF_PROD = 7,     // 0.5 a b
     E_GOMBAS = 8,        // Gombas correlation energy density of a, scale p0 (= gammaH)
-    E_VWN = 9            // VWN correlation energy density of a (eV, density in 1/Angstrom^3)
+    E_VWN = 9,           // VWN correlation energy density of a (eV, density in 1/Angstrom^3)
+    E_QT2 = 10           // (2 * 0.131163 * (p0 - a))^2 where p0 > a, else 0 (a = V, p0 = mu)
 };
 struct EnergyParams {
     int op;

@@ -1112,6 +1112,12 @@ __global__ __launch_bounds__(256) void k_energy(Grid g, EnergyParams ep, const d
             gombas(a[i], ep.p0, &v, &vv);
             break;
         }
+        case E_QT2: {
+            const double t = ep.p0 - a[i];
+            const double q = 2. * 0.131163 * t;
+            v = t > 0. ? q * q : 0.;
+            break;
+        }
         case E_VWN: {
             const double d = d_pos(a[i]);
             v = 0.;

@@ -336,15 +336,20 @@ static std::string final_block(mpdpft_ctx *c, const mpdpft_result &r) {
     t << "Linear/Pulay-Mixings         = " << r.LinMix << "/" << r.PulMix << "\\\\";
     for (int s = 0; s < c->S; s++)
         t << "AuxilliaryEkin[s=" << s << "]          = " << vec_to_str(r.AuxEkin[s], 4) << "\\\\";
+    for (int s = 0; s < c->S; s++)
+        t << "AuxilliaryEint[s=" << s << "]          = " << vec_to_str(r.AuxEint[s], 4) << "\\\\";
     double ed[MPDPFT_MAX_SPECIES], ee[MPDPFT_MAX_SPECIES];
     for (int s = 0; s < c->S; s++) {
         ed[s] = r.DispersalEnergies[s];
         ee[s] = r.EnvironmentalEnergies[s];
     }
     t << "DispersalEnergies            = " << vec_to_str(ed, c->S) << "\\\\";
+    const double zeros[MPDPFT_MAX_SPECIES] = {};  // the reference leaves these at 0 (:5416, :5644)
+    t << "AlternativeDispersalEnergies = " << vec_to_str(zeros, c->S) << "\\\\";
     t << "EnvironmentalEnergies        = " << vec_to_str(ee, c->S) << "\\\\";
     t << "InteractionEnergies          = " << vec_to_str(r.InteractionEnergies, p.n_interactions) << "\\\\";
     t << "TotalEnergy                  = " << r.TotalEnergy << "\\\\";
+    t << "FigureOfMerit                = " << 0. << "\\\\";  // GetFigureOfMerit (:13743-13793) is 0 for task types 0, 1, 8
     t << "SCiterations                 = " << r.SCcount - 1 << "/" << p.maxSCcount << "\\\\";
     t << "ConvergenceCheck             = " << std::setprecision(6) << r.CC << "\\\\";
     t << "ComputationalTime            = " << std::setprecision(6) << r.solve_seconds << "\\\\";

@@ -1103,6 +1103,35 @@ static int interaction_energy(mpdpft_ctx *c, int type, double *E) {  // :5805-59
     return MPDPFT_OK;
 }
 
+// GetAuxEint (:5518-5549): for S = 1, DIM 3, Units 2 the Dirac exchange energy,
+// its potential-functional form -gammaH g^2/(16 pi^5) int (2*0.131163*(mu - V)_+)^2,
+// and the Gombas and VWN correlation energies; zeros otherwise.
+static int aux_eint(mpdpft_ctx *c) {
+    const mpdpft_params &p = c->p;
+    for (int s = 0; s < c->S; s++)
+        for (int k = 0; k < 4; k++) c->AuxEint[s][k] = 0.;
+    if (!(c->S == 1 && c->D == 3 && p.Units == 2)) return MPDPFT_OK;
+    const double *n = c->d_Den;
+    mk::EnergyParams ep{};
+    ep.units = p.Units;
+    double DD = 3.;
+    double cXD = 2. * std::sqrt(4. * kPI) / ((DD - 1.) * kPI) * std::pow(0.5 * std::pow(0.75 * std::sqrt(kPI), DD - 2.), 1. / DD);
+    ep.op = mk::E_DIRAC;
+    ep.p0 = -p.gammaH * cXD * DD / (1. + DD);
+    RC(energy_integral(c, ep, n, nullptr, nullptr, nullptr, &c->AuxEint[0][0]));
+    ep.op = mk::E_QT2;
+    ep.p0 = c->muVec[0];
+    double v;
+    RC(energy_integral(c, ep, c->d_V, nullptr, nullptr, nullptr, &v));
+    c->AuxEint[0][1] = -p.gammaH * std::pow(p.degeneracy, 2.) / (16. * std::pow(kPI, 5.)) * v;
+    ep.op = mk::E_GOMBAS;
+    ep.p0 = p.gammaH;
+    RC(energy_integral(c, ep, n, nullptr, nullptr, nullptr, &c->AuxEint[0][2]));
+    ep.op = mk::E_VWN;
+    RC(energy_integral(c, ep, n, nullptr, nullptr, nullptr, &c->AuxEint[0][3]));
+    return MPDPFT_OK;
+}
+
 static int aux_ekin(mpdpft_ctx *c) {  // GetAuxEkin (:5461-5516)
     const mpdpft_params &p = c->p;
     size_t G = c->G;
@@ -1183,6 +1212,7 @@ static int get_energy(mpdpft_ctx *c) {
     }
     c->Etot += c->EInt;
     RC(aux_ekin(c));
+    RC(aux_eint(c));
     for (int s = 0; s < c->S; s++) {
         mk::EnergyParams ep{};
         ep.op = mk::E_PROD;
@@ -1436,6 +1466,7 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
         r->DispersalEnergies[s] = c->DispE[s];
         r->EnvironmentalEnergies[s] = c->EnvE[s];
         for (int k = 0; k < 4; k++) r->AuxEkin[s][k] = c->AuxEkin[s][k];
+        for (int k = 0; k < 4; k++) r->AuxEint[s][k] = c->AuxEint[s][k];
     }
     for (int a = 0; a < p.n_interactions; a++) r->InteractionEnergies[a] = c->IntE[a];
     return (r->converged || p.maxSCcount == 0) ? MPDPFT_OK : MPDPFT_ENOTCONV;

@@ -65,6 +65,7 @@ struct mpdpft_ctx {
     double EInt = 0, Etot = 0;
     double IntE[MPDPFT_MAX_LIST], DispE[MPDPFT_MAX_SPECIES], EnvE[MPDPFT_MAX_SPECIES];
     double AuxEkin[MPDPFT_MAX_SPECIES][4];
+    double AuxEint[MPDPFT_MAX_SPECIES][4];
     double solve_seconds = 0;
     int has_hartree = 0;
     int verbose = 0;  // MPDPFT_VERBOSE: one line per SC iteration on stderr

@@ -241,6 +241,7 @@ struct orc_state {
     double EInt, Etot;
     double IntE[MPDPFT_MAX_LIST], DispE[MPDPFT_MAX_SPECIES], EnvE[MPDPFT_MAX_SPECIES];
     double AuxEkin[MPDPFT_MAX_SPECIES][4];
+    double AuxEint[MPDPFT_MAX_SPECIES][4];
 };
 
 static double *dalloc(long long n) {
@@ -1692,6 +1693,25 @@ static double interaction_energy(orc_state *o, int type) { /* :5805-5918 */
     return 0.;
 }
 
+/* GetAuxEint (:5518-5549): S = 1, DIM 3, Units 2 only (zeros otherwise). */
+static void aux_eint(orc_state *o) {
+    long long G = o->G;
+    memset(o->AuxEint, 0, sizeof(o->AuxEint));
+    if (!(o->S == 1 && o->D == 3 && o->p.Units == 2)) return;
+    dirac(o, 1, 0, o->Tmp2);
+    o->AuxEint[0][0] = integrate(o, o->Tmp2);
+    for (long long i = 0; i < G; i++) {
+        double t = o->muVec[0] - o->V[i];
+        double q = 2. * 0.131163 * t;
+        o->Tmp1[i] = t > 0. ? q * q : 0.;
+    }
+    o->AuxEint[0][1] = -o->p.gammaH * pow(o->p.degeneracy, 2.) / (16. * pow(PI_, 5.)) * integrate(o, o->Tmp1);
+    gombas(o, 1, 0, o->Tmp2);
+    o->AuxEint[0][2] = integrate(o, o->Tmp2);
+    vwn(o, 1, 0, o->Tmp2);
+    o->AuxEint[0][3] = integrate(o, o->Tmp2);
+}
+
 static void aux_ekin(orc_state *o) { /* GetAuxEkin, :5461-5516 */
     long long G = o->G;
     int D = o->D;
@@ -1771,6 +1791,7 @@ static void get_energy(orc_state *o) {
     }
     o->Etot += o->EInt;
     aux_ekin(o);
+    aux_eint(o);
     for (int s = 0; s < o->S; s++) {
         long long G = o->G;
         for (long long i = 0; i < G; i++) o->Tmp1[i] = o->Env[s * G + i] * o->Den[s * G + i];
@@ -1906,6 +1927,7 @@ int oracle_solve_v0(const mpdpft_params *p, const double *v0, mpdpft_result *r, 
         r->DispersalEnergies[s] = o->DispE[s];
         r->EnvironmentalEnergies[s] = o->EnvE[s];
         for (int k = 0; k < 4; k++) r->AuxEkin[s][k] = o->AuxEkin[s][k];
+        for (int k = 0; k < 4; k++) r->AuxEint[s][k] = o->AuxEint[s][k];
     }
     for (int a = 0; a < p->n_interactions; a++) r->InteractionEnergies[a] = o->IntE[a];
     size_t n = sizeof(double) * (size_t)(o->S * o->G);

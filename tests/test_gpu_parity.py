@@ -238,6 +238,9 @@ def test_3d_gombas_vwn_trajectory(oracle):
     check_loop(ro, dO, VO, rg, dG, VG, 1)
     for k in range(4):
         assert abs(rg.InteractionEnergies[k] - ro.InteractionEnergies[k]) <= 1e-10 * abs(ro.InteractionEnergies[k]) + 1e-14
+        # GetAuxEint (S = 1, 3D, Units 2): Dirac, its potential form, Gombas, VWN
+        assert ro.AuxEint[0][k] != 0.0
+        assert abs(rg.AuxEint[0][k] - ro.AuxEint[0][k]) <= 1e-10 * abs(ro.AuxEint[0][k])
 
 
 def test_3d_tf_ion_converges(oracle):
@@ -265,7 +268,8 @@ def test_drop_in_directory_run(oracle, tmp_path):
     assert len(v_lines) == G and all(l.endswith(" ") and len(l.split()) == 4 for l in v_lines)
     assert v_lines[0].split()[:2] == ["-16", "-16"]
     ctl = (tmp_path / "mpDPFT_Control.dat").read_text()
-    for key in ("FinalAbundances", "muVec", "TotalEnergy", "SCiterations", "ConvergenceCheck"):
+    for key in ("FinalAbundances", "muVec", "TotalEnergy", "SCiterations", "ConvergenceCheck", "AuxilliaryEint[s=1]",
+                "AlternativeDispersalEnergies", "FigureOfMerit"):
         assert key in ctl
     rc, ro, dO, VO = oracle.solve(p)
     den = np.array([[float(x) for x in l.split()] for l in den_lines]).T
