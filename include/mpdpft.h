@@ -21,7 +21,8 @@
 extern "C" {
 #endif
 
-#define MPDPFT_ABI_VERSION 1
+/* 2: mpdpft_result gained AuxEint */
+#define MPDPFT_ABI_VERSION 2
 
 #define MPDPFT_MAX_SPECIES 8
 #define MPDPFT_MAX_LIST 16
