@@ -95,3 +95,19 @@ def test_create_fails_loudly_on_invalid_params():
     with pytest.raises(mpdpft_amd.MpdpftError) as e:
         mpdpft_amd.Context(p)
     assert e.value.code in (_abi.EINVAL, _abi.ENODEV)
+
+
+def test_cli_fails_loudly_without_a_gpu(tmp_path):
+    """Without a HIP device the drop-in CLI writes no result files: it exits 1 and
+    leaves the reason in mpDPFT_Control.dat (there is no CPU fallback)."""
+    import subprocess
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    configs.write_input(configs.c0(steps=32), tmp_path / "mpDPFT.input")
+    cli = os.path.join(os.path.dirname(mpdpft_amd.__file__), "bin", "mpdpft_hip")
+    out = subprocess.run([cli, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1
+    assert "no HIP device" in (tmp_path / "mpDPFT_Control.dat").read_text()
+    assert not (tmp_path / "mpDPFT_Den.dat").exists() and not (tmp_path / "mpDPFT_V.dat").exists()
+
