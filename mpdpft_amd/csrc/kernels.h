@@ -171,7 +171,8 @@ enum EnergyOp {
     E_HALF_PROD = 7,     // 0.5 a b
     E_GOMBAS = 8,        // Gombas correlation energy density of a, scale p0 (= gammaH)
     E_VWN = 9,           // VWN correlation energy density of a (eV, density in 1/Angstrom^3)
-    E_QT2 = 10           // (2 * 0.131163 * (p0 - a))^2 where p0 > a, else 0 (a = V, p0 = mu)
+    E_QT2 = 10,          // (2 * 0.131163 * (p0 - a))^2 where p0 > a, else 0 (a = V, p0 = mu)
+    E_RENORM = 11        // renormalised 2D contact energy density of (a, b) = (n_0, n_1), p0 = scattering length
 };
 struct EnergyParams {
     int op;

@@ -824,6 +824,32 @@ __device__ __forceinline__ void vwn_F(double x, double *F, double *dF) {
 __device__ __forceinline__ double vwn_x(double n) {
     return pow(3. / (4. * PI_D * kBohrA * kBohrA * kBohrA * n), 1. / 6.);
 }
+// RenormalizedContact (mpDPFT.cpp:2891-2904), 2D, two species: with n' = POS(n) +
+// 1e-16 and eta = -2 / ln(4 pi a^2 n'), the energy density pi n'_0 n'_1 (b(eta_0) +
+// b(eta_1)) and the potential of species s, pi n'_o (b(eta_s) + b(eta_o) + b'(eta_s)
+// eta_s^2 / 2) (o the other species).  b and b' are the reference's separately
+// fitted functions (PiotrBeta, :4458-4472): fit coefficients kept as the reference
+// has them, since the results are defined by the fit; both branches meet at eta =
+// 1.55 to 1e-12 in value and in b'.
+__device__ __forceinline__ double qmc_beta(int der, double x) {
+    const double x2 = x * x, x4 = x2 * x2;
+    if (x < 0.) return 100000.;
+    if (der == 0) {
+        if (x > 1.55) return -0.4633747986519757 / x + 1.443565409987811 - 0.20465388562309392 / x2;
+        return -0.0004147527851895318 + 1.0062109233358525 * x - 0.0584541517808854 * x2 -
+               0.3073866986724988 * x4 + 0.25331504986348463 * x4 * x - 0.06139783376450124 * x4 * x2;
+    }
+    if (x > 1.55)
+        return -0.0011588876670877157 + 0.0663320388441727 / x + 1.745208782122636 / (x2 * x) -
+               1.25878015302857 / x4;
+    return 1 - 0.028100195459653712 * x - 0.3963015574364809 * x2 - 0.3445575747110073 * x2 * x -
+           0.0987982521417292 * x4 + 1.2591046941895094 * x4 * x - 1.2654959902790368 * x4 * x2 +
+           0.5086340329349458 * x4 * x2 * x - 0.07525628170004087 * x4 * x4;
+}
+__device__ __forceinline__ double renorm_eta(double n, double a) {
+    return -2. / log(4. * PI_D * a * a * (d_pos(n) + 1.0e-16));
+}
+
 // the pointwise potentials for k_update_v, out of line: inlined, their
 // transcendentals set the register count (and occupancy) of every update
 __device__ __noinline__ double xc_v(int t, int units, double n) {
@@ -840,6 +866,10 @@ __device__ __noinline__ double dirac_v(double n, int D, double gammaH) {
 __device__ __noinline__ double dirac_v_f(double n, int D, double gammaH) {
     float nf = (float)d_pos(n), Df = (float)D;
     return D > 1 ? (double)(-(float)gammaH * (2.f * powf(nf * Df / (float)PI_D, 1.f / Df)) / (Df - 1.f)) : 0.;
+}
+__device__ __noinline__ double renorm_v(double n_s, double n_o, double a) {
+    const double es = renorm_eta(n_s, a), eo = renorm_eta(n_o, a);
+    return PI_D * (d_pos(n_o) + 1.0e-16) * (qmc_beta(0, es) + qmc_beta(0, eo) + 0.5 * qmc_beta(1, es) * es * es);
 }
 __device__ __noinline__ double gombas_v(double n, double gammaH) {
     double e, v;
@@ -880,6 +910,8 @@ __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const d
                 v = vp.fp32 ? dirac_v_f(ns, g.D, vp.gammaH) : dirac_v(ns, g.D, vp.gammaH);
             } else if (t == -1 || t == -12) {
                 v = vp.fp32 ? xc_v_f(t, vp.units, ns) : xc_v(t, vp.units, ns);
+            } else if (t == 9) {  // two species (checked at create)
+                v = renorm_v(ns, den[(size_t)(1 - s) * G + i], vp.beta);
             } else if (t == 12 && g.D == 3 && vp.units == 2) {
                 v = gombas_v(ns, vp.gammaH);
             } else if (t == 13 && g.D == 3 && vp.units == 2) {
@@ -1110,6 +1142,11 @@ __global__ __launch_bounds__(256) void k_energy(Grid g, EnergyParams ep, const d
         case E_GOMBAS: {
             double vv;
             gombas(a[i], ep.p0, &v, &vv);
+            break;
+        }
+        case E_RENORM: {
+            const double e0 = renorm_eta(a[i], ep.p0), e1 = renorm_eta(b[i], ep.p0);
+            v = PI_D * (d_pos(a[i]) + 1.0e-16) * (d_pos(b[i]) + 1.0e-16) * (qmc_beta(0, e0) + qmc_beta(0, e1));
             break;
         }
         case E_QT2: {

@@ -168,8 +168,10 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
     if ((int)p->Mixer[1] > 60) return ctx_fail(c, MPDPFT_EINVAL, "Pulay scope must be <= 60");
     for (int a = 0; a < p->n_interactions; a++) {
         int t = p->Interactions[a];
-        if (!(t == 0 || t == 1 || t == 8 || t == 10 || t == 11 || t == 12 || t == 13 || t == -1 || t == -12))
+        if (!(t == 0 || t == 1 || t == 8 || t == 9 || t == 10 || t == 11 || t == 12 || t == 13 || t == -1 || t == -12))
             return ctx_fail(c, MPDPFT_EINVAL, "unsupported interaction type " + std::to_string(t));
+        if (t == 9 && !(p->DIM == 2 && p->S == 2))
+            return ctx_fail(c, MPDPFT_EINVAL, "interaction 9 (renormalised contact) needs DIM 2 and two species");
     }
     for (int r = 0; r < p->n_regularize; r++)
         if (p->regularize[r] != 0) return ctx_fail(c, MPDPFT_EINVAL, "regularize must be 0");
@@ -1087,6 +1089,11 @@ static int interaction_energy(mpdpft_ctx *c, int type, double *E) {  // :5805-59
         RC(energy_integral(c, ep, c->d_Den, c->d_Den, nullptr, nullptr, &v));
         *E = p.beta * v;
         return MPDPFT_OK;
+    }
+    if (c->D == 2 && type == 9) {  // RenormalizedContact energy (:5859-5862)
+        ep.op = mk::E_RENORM;
+        ep.p0 = p.beta;
+        return energy_integral(c, ep, c->d_Den, c->d_Den + G, nullptr, nullptr, E);
     }
     if (c->D == 3 && c->S == 1 && (type == 12 || type == 13)) {  // Gombas / VWN energy (:5871-5878)
         ep.op = type == 12 ? mk::E_GOMBAS : mk::E_VWN;

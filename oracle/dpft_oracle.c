@@ -957,6 +957,42 @@ static void dirac(orc_state *o, int EnergyQ, int ss, double *out) {
     }
 }
 
+/* PiotrBeta (mpDPFT.cpp:4458-4472): the fitted b(eta) of the 2D contact energy
+ * and its separately fitted derivative (fit coefficients, as in the reference). */
+double oracle_qmc_beta(int der, double x) {
+    double x2 = x * x, x4 = x2 * x2;
+    if (x < 0.) return 100000.;
+    if (der == 0) {
+        if (x > 1.55) return -0.4633747986519757 / x + 1.443565409987811 - 0.20465388562309392 / x2;
+        return -0.0004147527851895318 + 1.0062109233358525 * x - 0.0584541517808854 * x2 -
+               0.3073866986724988 * x4 + 0.25331504986348463 * x4 * x - 0.06139783376450124 * x4 * x2;
+    }
+    if (x > 1.55)
+        return -0.0011588876670877157 + 0.0663320388441727 / x + 1.745208782122636 / (x2 * x) -
+               1.25878015302857 / x4;
+    return 1 - 0.028100195459653712 * x - 0.3963015574364809 * x2 - 0.3445575747110073 * x2 * x -
+           0.0987982521417292 * x4 + 1.2591046941895094 * x4 * x - 1.2654959902790368 * x4 * x2 +
+           0.5086340329349458 * x4 * x2 * x - 0.07525628170004087 * x4 * x4;
+}
+
+/* RenormalizedContact (mpDPFT.cpp:2891-2904), 2D, two species, a = beta (the 2D
+ * scattering length): n' = POS(n) + 1e-16, eta = -2 / ln(4 pi a^2 n'); energy
+ * density pi n'_0 n'_1 (b(eta_0) + b(eta_1)) (species 0 focal); potential of
+ * species ss: pi n'_o (b(eta_ss) + b(eta_o) + b'(eta_ss) eta_ss^2 / 2). */
+static void renorm_contact(orc_state *o, int EnergyQ, int ss, double *out) {
+    long long G = o->G;
+    int so = ss == 0 ? 1 : 0;
+    double pref = 4. * PI_ * o->p.beta * o->p.beta;
+    for (long long i = 0; i < G; i++) {
+        double ds = POSs(o->Den[ss * G + i]) + 1.0e-16, dp = POSs(o->Den[so * G + i]) + 1.0e-16;
+        double es = -2. / log(pref * ds), ep = -2. / log(pref * dp);
+        if (EnergyQ)
+            out[i] = PI_ * ds * dp * (oracle_qmc_beta(0, es) + oracle_qmc_beta(0, ep));
+        else
+            out[i] = PI_ * dp * (oracle_qmc_beta(0, es) + oracle_qmc_beta(0, ep) + 0.5 * oracle_qmc_beta(1, es) * es * es);
+    }
+}
+
 /* GombasCorrelation (mpDPFT.cpp:2774-2796): energy density
  * e(n) = -gammaH n (a1 t/(a2 + c t) + (b1/c) ln(1 + b2 c t)), t = POS(n)^(1/3);
  * the potential (3D, Units 2 only; 0 otherwise) is de/dn. */
@@ -1028,6 +1064,8 @@ static int getvint(orc_state *o, int type, int ss, double *out) {
         for (long long i = 0; i < G; i++) out[i] = o->Den[i] * (2. * o->p.beta);
     } else if (type == 11) {
         dirac(o, 0, ss, out);
+    } else if (type == 9 && o->D == 2 && o->S == 2) {
+        renorm_contact(o, 0, ss, out);
     } else if (type == 12) {
         gombas(o, 0, ss, out);
     } else if (type == 13) {
@@ -1683,6 +1721,10 @@ static double interaction_energy(orc_state *o, int type) { /* :5805-5918 */
     }
     if (o->D == 3 && o->S == 1 && type == 11) {
         dirac(o, 1, 0, o->Tmp2);
+        return integrate(o, o->Tmp2);
+    }
+    if (o->D == 2 && type == 9 && o->S == 2) {
+        renorm_contact(o, 1, 0, o->Tmp2);
         return integrate(o, o->Tmp2);
     }
     if (o->D == 3 && o->S == 1 && (type == 12 || type == 13)) {

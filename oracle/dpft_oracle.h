@@ -48,6 +48,8 @@ double oracle_cube_inverse_r(void);
 double oracle_curlyA(double y);
 double oracle_polylog(int j, double x);
 double oracle_airy(double y);
+/* the fitted b(eta) (der 0) and b'(eta) (der 1) of the renormalised 2D contact */
+double oracle_qmc_beta(int der, double x);
 /* e^-x I0(x) for x >= 0 (2D periodic Hartree compensation) */
 double oracle_exp_i0(double x);
 

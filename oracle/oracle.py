@@ -58,6 +58,8 @@ def lib():
         for n in ("oracle_curlyA", "oracle_airy", "oracle_exp_i0"):
             getattr(L, n).argtypes = [ctypes.c_double]
             getattr(L, n).restype = ctypes.c_double
+        L.oracle_qmc_beta.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.oracle_qmc_beta.restype = ctypes.c_double
         L.oracle_polylog.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_polylog.restype = ctypes.c_double
         _lib = L
@@ -184,6 +186,11 @@ def polylog(j, x):
 
 def airy(y):
     return lib().oracle_airy(float(y))
+
+
+def qmc_beta(der, x):
+    """The fitted b(eta) (der 0) / b'(eta) (der 1) of the renormalised 2D contact."""
+    return lib().oracle_qmc_beta(int(der), float(x))
 
 
 def exp_i0(x):

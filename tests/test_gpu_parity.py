@@ -90,10 +90,10 @@ def test_fft_block_shape_does_not_change_results(monkeypatch, steps):
 
 
 @pytest.mark.parametrize("case", ["contact2d", "contact3", "dirac", "ldax", "pw92", "hartree_per", "hartree_iso",
-                                  "hartree_iso2d", "hartree_per2d", "single_contact", "gombas", "vwn"])
+                                  "hartree_iso2d", "hartree_per2d", "single_contact", "gombas", "vwn", "renorm"])
 def test_interaction_potentials(oracle, case):
     print("case vint", case, flush=True)
-    if case in ("contact2d", "hartree_iso2d", "hartree_per2d"):
+    if case in ("contact2d", "hartree_iso2d", "hartree_per2d", "renorm"):
         p = configs.c0(steps=64, edge=12.0)
     elif case == "contact3":
         p = configs.c0(steps=32, edge=12.0)
@@ -106,7 +106,9 @@ def test_interaction_potentials(oracle, case):
         p = configs.c1(steps=32, edge=12.0, hartree_mode=1 if case.startswith("hartree_iso") else 0)
     typ = {"contact2d": 1, "contact3": 1, "dirac": 11, "ldax": -1, "pw92": -12, "hartree_per": 8,
            "hartree_iso": 8, "hartree_iso2d": 8, "hartree_per2d": 8, "single_contact": 10, "gombas": 12,
-           "vwn": 13}[case]
+           "vwn": 13, "renorm": 9}[case]
+    if case == "renorm":
+        p.beta = 0.05  # 2D scattering length: 4 pi a^2 n stays below 1 (repulsive branch)
     if case in ("gombas", "vwn"):
         p.Units = 2  # the reference evaluates these potentials in 3D with Units 2 only
     p.n_interactions = 1
@@ -222,6 +224,20 @@ def test_2d_periodic_hartree_loop(oracle, maxsc):
     else:
         assert ro.converged and rg.converged
         check_loop(ro, dO, VO, rg, dG, VG, p.S, exact_count=False)
+
+
+def test_2d_renormalised_contact_trajectory(oracle):
+    """2D two-species TF gas with the renormalised contact interaction (mpDPFT.cpp:2891-2904)
+    instead of the plain contact: 4 iterations on both paths, with its energy."""
+    print("case renormalised contact trajectory", flush=True)
+    p = configs.c0(steps=64, edge=16.0, maxsc=4)
+    p.Interactions[0] = 9
+    p.beta = 0.05
+    ro, dO, VO, rg, dG, VG = solve_both(oracle, p)
+    assert ro.SCcount == 5 and rg.SCcount == 5
+    check_loop(ro, dO, VO, rg, dG, VG, p.S)
+    assert ro.InteractionEnergies[0] != 0.0
+    assert abs(rg.InteractionEnergies[0] - ro.InteractionEnergies[0]) <= 1e-10 * abs(ro.InteractionEnergies[0])
 
 
 def test_3d_gombas_vwn_trajectory(oracle):

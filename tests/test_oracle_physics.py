@@ -177,6 +177,42 @@ def test_vwn_energy_rpa_form(oracle):
         assert 1.2 < ratio < 1.75, (rs, ratio)
 
 
+def test_qmc_beta_fit(oracle):
+    """The renormalised-contact fit (PiotrBeta, mpDPFT.cpp:4458-4472): both branches of
+    b and of b' meet at eta = 1.55 (to 1e-11; a changed coefficient breaks this), b ~ eta
+    at weak coupling, and the separately fitted b' follows db/deta to 3e-2."""
+    for der in (0, 1):
+        assert abs(oracle.qmc_beta(der, 1.55) - oracle.qmc_beta(der, 1.55 + 1e-12)) < 1e-11
+    for x in (0.01, 0.02, 0.05):
+        assert abs(oracle.qmc_beta(0, x) - x) < 1e-3
+    h = 1e-6
+    for x in np.linspace(0.01, 5.0, 200):
+        fd = (oracle.qmc_beta(0, x + h) - oracle.qmc_beta(0, x - h)) / (2 * h)
+        assert abs(oracle.qmc_beta(1, x) - fd) < 3e-2
+
+
+def test_renormalised_contact_potential_structure(oracle):
+    """V_s = d/dn_s of e = pi n_s n_o (b(eta_s) + b(eta_o)), eta = -2/ln(4 pi a^2 n), with the
+    fitted b' in place of db/deta (mpDPFT.cpp:2891-2904): central differences of e agree to the
+    fits' mutual accuracy; swapping the two species swaps the potentials."""
+    p = configs.c0(steps=16, edge=8.0)
+    p.n_interactions = 1
+    p.Interactions[0] = 9
+    p.beta = 0.05
+    G = (p.steps + 1) ** 2
+    rng = np.random.default_rng(3)
+    n0, n1 = rng.uniform(0.05, 20.0, G), rng.uniform(0.05, 20.0, G)
+    v0 = oracle.vint(p, 9, 0, np.concatenate([n0, n1]))
+    v1 = oracle.vint(p, 9, 1, np.concatenate([n0, n1]))
+    assert np.array_equal(v1, oracle.vint(p, 9, 0, np.concatenate([n1, n0])))
+    eta = lambda n: -2 / np.log(4 * math.pi * p.beta ** 2 * (n + 1e-16))
+    b = np.vectorize(lambda x: oracle.qmc_beta(0, x))
+    e = lambda a, c: math.pi * (a + 1e-16) * (c + 1e-16) * (b(eta(a)) + b(eta(c)))
+    hh = 1e-6 * n0
+    fd = (e(n0 + hh, n1) - e(n0 - hh, n1)) / (2 * hh)
+    assert np.abs(v0 - fd).max() < 3e-2 * np.abs(fd).max()
+
+
 def test_exp_i0_matches_scipy(oracle):
     """e^-x I0(x) of the 2D Hartree compensation (power series / asymptotic
     series) against scipy.special.i0e, on both sides of the switch at x = 30."""
