@@ -24,9 +24,14 @@
  *  - PW92: published values; Dirac: closed form.
  *  - Pulay small solve: Eigen 3.4 FullPivHouseholderQR (vendored in the
  *    reference) through oracle/_ref.
+ *  - EkinAiT03D per point (ekin_point): the refined trapezoid sum against
+ *    scipy quadrature of the Airy average, the TF branch against the T = 0
+ *    Fermi gas, the deep-well limit (Airy -> TF, Laplacian term included) and
+ *    its first moment correction (int t^3 Ai = 2).
  *  - mu bisection, convergence, mixing: invariants (monotone N(mu), exit rule,
  *    linear/Pulay fixed points) -- parity unpinned beyond these invariants and
- *    the full-loop closed forms; the same holds for AdaptnAiT and EkinAiT03D.
+ *    the full-loop closed forms; the same holds for AdaptnAiT and for
+ *    EkinAiT03D's outer MaxIteration loop.
  */
 #include "dpft_oracle.h"
 
@@ -2354,6 +2359,42 @@ static void nait_adapt(orc_state *o) {
     }
 }
 
+/* One grid point of EkinAiT03D (:5632-5690): below the gradient threshold
+ * the T = 0 Thomas-Fermi kinetic energy density with its Laplacian term,
+ * above it the Airy average of that expression, a trapezoid sum on [0, xmax]
+ * (xmax = max(1, 100 - U/a)) with 64 * 2^k nodes, refined until successive
+ * sums agree (asymmetry below acc, or the scaled value negligible against
+ * cmp) or MaxIt refinements are spent. */
+static double ekin_point(double U, double a, double lap, double c1, double c2, double thr, double cmp, double scale,
+                         double acc, int MaxIt) {
+    if (a < thr) return (-U > 0.) ? c1 * pow(2. * (-U), 2.5) / 5. - c2 * lap * sqrt(2. * (-U)) / 6. : 0.;
+    double xmax = fmax(1., 100. - U / a);
+    double prev = 1., val = 0.;
+    int iteration = 0, gs = 64;
+    for (;;) {
+        double r = 0.;
+        for (int j = 0; j < gs; j++) {
+            double x = (double)j * xmax / ((double)(gs - 1)), y = x + U / a;
+            double AI = y > 100. ? 0. : airyAi(y);
+            double f = AI * (c1 * pow(2. * a * x, 2.5) / 5. - c2 * lap * sqrt(2. * a * x) / 6.);
+            r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * f;
+        }
+        r *= (xmax - 0.) / ((double)gs - 1.);
+        val = r;
+        double asymcrit = (fabs(val) < MP_ && fabs(prev) < MP_) ? 0. : fabs((val - prev) / (val + prev));
+        double asym = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
+        if (iteration > 0 &&
+            (asymcrit < MP_ || (fabs(asym) < acc && negligible_magnitude(cmp, scale * val, -log(acc)))))
+            break;
+        else if (iteration == MaxIt)
+            break;
+        iteration++;
+        gs *= 2;
+        prev = val;
+    }
+    return val;
+}
+
 /* EkinAiT03D (:5610-5709): Airy kinetic energy, with the reference's
  * MaxIteration state that persists across calls. */
 static double ekin_ait(orc_state *o, int s) {
@@ -2371,38 +2412,8 @@ static double ekin_ait(orc_state *o, int s) {
 #pragma omp parallel for schedule(dynamic, 64)
         for (long long i = 0; i < G; i++) {
             double U = o->V[s * G + i] - o->muVec[s], a = 0.5 * c0 * pow(fabs(o->Grad2[s * G + i]), 1. / 3.);
-            double lap = o->Lap[s * G + i];
-            double val;
-            if (a < g_ns.Mpp[s][2]) {
-                val = (-U > 0.) ? c1 * pow(2. * (-U), 2.5) / 5. - c2 * lap * sqrt(2. * (-U)) / 6. : 0.;
-            } else {
-                double xmax = fmax(1., 100. - U / a);
-                double prev = 1.;
-                int iteration = 0, gs = 64;
-                for (;;) {
-                    double r = 0.;
-                    for (int j = 0; j < gs; j++) {
-                        double x = (double)j * fmax(1., 100. - U / a) / ((double)(gs - 1)), y = x + U / a;
-                        double AI = y > 100. ? 0. : airyAi(y);
-                        double f = AI * (c1 * pow(2. * a * x, 2.5) / 5. - c2 * lap * sqrt(2. * a * x) / 6.);
-                        r += ((j == 0 || j == gs - 1) ? 0.5 : 1.) * f;
-                    }
-                    r *= (xmax - 0.) / ((double)gs - 1.);
-                    val = r;
-                    double asymcrit = (fabs(val) < MP_ && fabs(prev) < MP_) ? 0. : fabs((val - prev) / (val + prev));
-                    double asym = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
-                    if (iteration > 0 &&
-                        (asymcrit < MP_ || (fabs(asym) < o->p.InternalAcc &&
-                                            negligible_magnitude(cmp, area * prefactor * val, -log(o->p.InternalAcc)))))
-                        break;
-                    else if (iteration == MaxIt)
-                        break;
-                    iteration++;
-                    gs *= 2;
-                    prev = val;
-                }
-            }
-            o->Tmp1[i] = val;
+            o->Tmp1[i] = ekin_point(U, a, o->Lap[s * G + i], c1, c2, g_ns.Mpp[s][2], cmp, area * prefactor,
+                                    o->p.InternalAcc, MaxIt);
         }
         E = prefactor * integrate(o, o->Tmp1);
         double asymE = (E == 0. && prevE == 0.) ? 0. : (E - prevE) / (E + prevE);
@@ -2441,3 +2452,6 @@ int oracle_nait_before_energy(orc_state *o) {
 double oracle_curlyA(double y) { return load_tables() ? NAN : curlyA(y); }
 double oracle_polylog(int j, double x) { return load_tables() ? NAN : polylog_j(j, x); }
 double oracle_airy(double y) { return load_tables() ? NAN : airyAi(y); }
+double oracle_ekin_point(double U, double a, double lap, double thr, double acc, int MaxIt) {
+    return load_tables() ? NAN : ekin_point(U, a, lap, 1., 1., thr, 0., 1., acc, MaxIt);
+}

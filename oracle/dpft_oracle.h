@@ -48,6 +48,10 @@ double oracle_cube_inverse_r(void);
 double oracle_curlyA(double y);
 double oracle_polylog(int j, double x);
 double oracle_airy(double y);
+/* EkinAiT03D's kinetic energy density at one point (Units 1): U = V - mu,
+ * a = |grad V|^(2/3)/2, lap = Laplacian of V; TF form below the gradient
+ * threshold thr, else the refined Airy average (no negligibility exit). */
+double oracle_ekin_point(double U, double a, double lap, double thr, double acc, int MaxIt);
 /* the fitted b(eta) (der 0) and b'(eta) (der 1) of the renormalised 2D contact */
 double oracle_qmc_beta(int der, double x);
 /* e^-x I0(x) for x >= 0 (2D periodic Hartree compensation) */

@@ -61,6 +61,8 @@ def lib():
         L.oracle_qmc_beta.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_qmc_beta.restype = ctypes.c_double
         L.oracle_polylog.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.oracle_ekin_point.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
+        L.oracle_ekin_point.restype = ctypes.c_double
         L.oracle_polylog.restype = ctypes.c_double
         _lib = L
     return _lib
@@ -186,6 +188,11 @@ def polylog(j, x):
 
 def airy(y):
     return lib().oracle_airy(float(y))
+
+
+def ekin_point(U, a, lap, thr, acc, max_it):
+    """EkinAiT03D's kinetic energy density at one point (Units 1, no prefactor)."""
+    return lib().oracle_ekin_point(float(U), float(a), float(lap), float(thr), float(acc), int(max_it))
 
 
 def qmc_beta(der, x):

@@ -111,3 +111,64 @@ def test_airy_integrates_to_curlya(oracle):
         b = a + 0.7
         val = 0.5 * (b - a) * sum(w * oracle.airy(0.5 * (b - a) * x + 0.5 * (a + b)) for x, w in zip(gx, gw))
         assert abs(val - (oracle.curlyA(a) - oracle.curlyA(b))) < 1e-9
+
+
+def _ekin_quad(U, a, lap):
+    """Airy average int_0^xmax Ai(x + U/a) ((2ax)^{5/2}/5 - lap sqrt(2ax)/6) dx by adaptive
+    quadrature with scipy's Ai (xmax = max(1, 100 - U/a), Ai taken as 0 beyond argument 100)."""
+    from scipy import integrate as sint
+    y0 = U / a
+    xmax = max(1.0, 100.0 - y0)
+
+    def f(x):
+        if x + y0 > 100.0:
+            return 0.0
+        return sp.airy(x + y0)[0] * ((2 * a * x) ** 2.5 / 5 - lap * math.sqrt(2 * a * x) / 6)
+
+    pts = [-y0] if 0 < -y0 < xmax else None
+    return sint.quad(f, 0, xmax, points=pts, limit=2000, epsabs=0, epsrel=1e-13)[0]
+
+
+@pytest.mark.parametrize("U,a,lap,acc,max_it,tol", [
+    (-1.0, 0.3, 0.0, 1e-10, 16, 1e-12),
+    (-1.0, 0.05, 0.0, 1e-10, 16, 1e-12),
+    (0.4, 0.3, 0.0, 1e-10, 16, 1e-12),    # classically forbidden point: only the Airy tail
+    (-2.5, 0.6, 0.0, 1e-10, 16, 1e-12),
+    (-1.0, 0.3, 0.7, 1e-8, 18, 1e-8),     # sqrt(x) term: trapezoid error ~ h^1.5
+    (0.2, 0.5, -0.4, 1e-8, 18, 1e-8),
+])
+def test_ekin_point_is_the_airy_average(oracle, U, a, lap, acc, max_it, tol):
+    """EkinAiT03D's refined trapezoid sum equals the Airy average by independent quadrature."""
+    v = oracle.ekin_point(U, a, lap, 0.0, acc, max_it)
+    ref = _ekin_quad(U, a, lap)
+    assert abs(v - ref) <= tol * abs(ref)
+
+
+def test_ekin_point_tf_branch_is_the_fermi_gas(oracle):
+    """Below the gradient threshold (uniform gas, no Laplacian) the kinetic energy density
+    times 1/(4 pi^2) is the T = 0 Fermi gas's (3/5) n mu with n = (2 mu)^{3/2}/(6 pi^2);
+    zero outside the classically allowed region."""
+    for mu in (0.05, 0.7, 3.0):
+        tau = 0.6 * (2 * mu) ** 1.5 / (6 * math.pi ** 2) * mu
+        v = oracle.ekin_point(-mu, 0.0, 0.0, 1e-3, 1e-10, 10)
+        assert abs(v / (4 * math.pi ** 2) - tau) <= 1e-14 * tau
+    assert oracle.ekin_point(0.3, 0.0, 0.0, 1e-3, 1e-10, 10) == 0.0
+
+
+def test_ekin_point_deep_well_limit(oracle):
+    """Deep inside the well (X = -U/a >> 1) the Airy average tends to the TF branch, Laplacian
+    term included, and for the x^{5/2} part the gap is the first moment correction
+    f'''(X) M3/3! with M3 = int t^3 Ai(t) dt = 2 (the remainder, from the x = 0 endpoint,
+    falls off like X^{-3/2} relative to it).  A wrong sign of the Airy shift or a wrong power
+    breaks these by orders of magnitude."""
+    U, a = -1.0, 0.002                                  # X = 500
+    for lap in (0.0, 0.9):
+        air = oracle.ekin_point(U, a, lap, 0.0, 1e-10, 18)
+        tf = oracle.ekin_point(U, a, lap, 1e9, 1e-10, 18)
+        assert abs(air - tf) <= 1e-5 * abs(tf)
+    for a, bound in ((0.05, 0.05), (0.02, 0.015)):      # X = 20, 50
+        X = -U / a
+        c = (2 * a) ** 2.5 / 5
+        pred = c * (15 / 8) * X ** -0.5 * (2 / 6) + c * (-225 / 64) * X ** -3.5 * (40 / 720)
+        gap = oracle.ekin_point(U, a, 0.0, 0.0, 1e-12, 16) - oracle.ekin_point(U, a, 0.0, 1e9, 1e-12, 16)
+        assert abs(gap - pred) <= bound * pred
