@@ -24,6 +24,10 @@
  *  - PW92: published values; Dirac: closed form.
  *  - Pulay small solve: Eigen 3.4 FullPivHouseholderQR (vendored in the
  *    reference) through oracle/_ref.
+ *  - nAiT density per point (nait_point_at): nested quadrature of the Airy
+ *    average of the finite-T TF density (Fermi-Dirac integrals by
+ *    quadrature), the TF branch against F_{1/2}, and the exact
+ *    Boltzmann-regime series (int Ai(x) e^{px} dx = e^{p^3/3}).
  *  - EkinAiT03D per point (ekin_point): the refined trapezoid sum against
  *    scipy quadrature of the Airy average, the TF branch against the T = 0
  *    Fermi gas, the deep-well limit (Airy -> TF, Laplacian term included) and
@@ -2282,17 +2286,17 @@ static void nait_state_init(orc_state *o) {
     g_ns.init = 1;
 }
 
-/* getDenEnAt(EnergyQ = 0) in 3D for one grid point. */
-static double nait_point(orc_state *o, int s, long long i, long long *itermax, long long *relacc) {
-    double T = o->TVec[s];
-    double alpha = o->p.Units == 2 ? 1. / 0.1312342130759153 : 1.;
-    double tau = 1. / T, beta0 = -o->p.degeneracy / pow(2. * PI_, 1.5), beta1 = pow(alpha * tau, -1.5);
-    long long G = o->G;
-    double U = o->V[s * G + i] - o->muVec[s];
-    double ar = 0.5 * pow(alpha * 1. * o->Grad2[s * G + i], 1. / 3.);
-    double beta2 = -o->Lap[s * G + i] / (12. * sqrt(alpha * tau));
+/* getDenEnAt(EnergyQ = 0) in 3D at one point, from U = V - mu, a_r =
+ * (alpha |grad V|^2)^(1/3)/2 and the Laplacian of V: below the gradient
+ * threshold thr the finite-T Thomas-Fermi density with its Laplacian term,
+ * else the Airy average of that density written as int CurlyA(x) d/dx n_TF
+ * (U - a_r x) dx, lower limit found by doubling, adaptive Boole panels. */
+static double nait_point_at(double U, double ar, double lap, double T, double alpha, double degeneracy, double thr,
+                            int BP, int maxIter, double RTPNA, double Compare, long long *itermax, long long *relacc) {
+    double tau = 1. / T, beta0 = -degeneracy / pow(2. * PI_, 1.5), beta1 = pow(alpha * tau, -1.5);
+    double beta2 = -lap / (12. * sqrt(alpha * tau));
     double res;
-    if (ar < fabs(g_ns.Mpp[s][2])) {
+    if (ar < fabs(thr)) {
         double arg = -tau * U;
         res = beta0 * (beta1 * polylog_j(0, arg) + beta2 * polylog_j(2, arg));
     } else {
@@ -2302,10 +2306,10 @@ static double nait_point(orc_state *o, int s, long long i, long long *itermax, l
         q.ar = ar;
         q.prefactor = beta0 * ar * tau * beta1;
         q.prefactor2 = beta2 / beta1;
-        q.BP = (int)g_ns.Mpp[s][0];
-        q.maxIter = (int)(g_ns.Mpp[s][1] + 0.5);
-        q.RTPNA = o->p.InternalAcc;
-        q.Compare = o->p.Abundances[s] / pow(o->p.edge, (double)o->D);
+        q.BP = BP;
+        q.maxIter = maxIter;
+        q.RTPNA = RTPNA;
+        q.Compare = Compare;
         q.cnt_itermax = q.cnt_relacc = 0;
         double test = 1., x = -1., arg;
         while (fabs(test) > MP_) {
@@ -2322,6 +2326,16 @@ static double nait_point(orc_state *o, int s, long long i, long long *itermax, l
     }
     if (res != res) res = 0.;
     return res;
+}
+
+static double nait_point(orc_state *o, int s, long long i, long long *itermax, long long *relacc) {
+    double alpha = o->p.Units == 2 ? 1. / 0.1312342130759153 : 1.;
+    long long G = o->G;
+    double U = o->V[s * G + i] - o->muVec[s];
+    double ar = 0.5 * pow(alpha * 1. * o->Grad2[s * G + i], 1. / 3.);
+    return nait_point_at(U, ar, o->Lap[s * G + i], o->TVec[s], alpha, o->p.degeneracy, g_ns.Mpp[s][2],
+                         (int)g_ns.Mpp[s][0], (int)(g_ns.Mpp[s][1] + 0.5), o->p.InternalAcc,
+                         o->p.Abundances[s] / pow(o->p.edge, (double)o->D), itermax, relacc);
 }
 
 static long long g_last_itermax[MPDPFT_MAX_SPECIES], g_last_relacc[MPDPFT_MAX_SPECIES];
@@ -2452,6 +2466,10 @@ int oracle_nait_before_energy(orc_state *o) {
 double oracle_curlyA(double y) { return load_tables() ? NAN : curlyA(y); }
 double oracle_polylog(int j, double x) { return load_tables() ? NAN : polylog_j(j, x); }
 double oracle_airy(double y) { return load_tables() ? NAN : airyAi(y); }
+double oracle_nait_point(double U, double ar, double lap, double T, double thr, int BP, int maxIter, double acc) {
+    long long im = 0, ra = 0;
+    return load_tables() ? NAN : nait_point_at(U, ar, lap, T, 1., 1., thr, BP, maxIter, acc, 0., &im, &ra);
+}
 double oracle_ekin_point(double U, double a, double lap, double thr, double acc, int MaxIt) {
     return load_tables() ? NAN : ekin_point(U, a, lap, 1., 1., thr, 0., 1., acc, MaxIt);
 }

@@ -48,6 +48,12 @@ double oracle_cube_inverse_r(void);
 double oracle_curlyA(double y);
 double oracle_polylog(int j, double x);
 double oracle_airy(double y);
+/* The nAiT density at one point (Units 1, degeneracy 1): U = V - mu, ar =
+ * |grad V|^(2/3)/2, lap = Laplacian of V, temperature T; finite-T TF form
+ * below the gradient threshold thr, else the adaptive Boole Airy average
+ * with BP panels, maxIter levels and relative accuracy acc (no
+ * negligibility exit). */
+double oracle_nait_point(double U, double ar, double lap, double T, double thr, int BP, int maxIter, double acc);
 /* EkinAiT03D's kinetic energy density at one point (Units 1): U = V - mu,
  * a = |grad V|^(2/3)/2, lap = Laplacian of V; TF form below the gradient
  * threshold thr, else the refined Airy average (no negligibility exit). */

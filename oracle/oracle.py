@@ -61,6 +61,8 @@ def lib():
         L.oracle_qmc_beta.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_qmc_beta.restype = ctypes.c_double
         L.oracle_polylog.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.oracle_nait_point.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.oracle_nait_point.restype = ctypes.c_double
         L.oracle_ekin_point.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
         L.oracle_ekin_point.restype = ctypes.c_double
         L.oracle_polylog.restype = ctypes.c_double
@@ -188,6 +190,12 @@ def polylog(j, x):
 
 def airy(y):
     return lib().oracle_airy(float(y))
+
+
+def nait_point(U, ar, lap, T, thr, bp, max_iter, acc):
+    """The nAiT density at one point (Units 1, degeneracy 1)."""
+    return lib().oracle_nait_point(float(U), float(ar), float(lap), float(T), float(thr), int(bp), int(max_iter),
+                                   float(acc))
 
 
 def ekin_point(U, a, lap, thr, acc, max_it):

@@ -172,3 +172,80 @@ def test_ekin_point_deep_well_limit(oracle):
         pred = c * (15 / 8) * X ** -0.5 * (2 / 6) + c * (-225 / 64) * X ** -3.5 * (40 / 720)
         gap = oracle.ekin_point(U, a, 0.0, 0.0, 1e-12, 16) - oracle.ekin_point(U, a, 0.0, 1e9, 1e-12, 16)
         assert abs(gap - pred) <= bound * pred
+
+
+def _fd(eta, j):
+    """Fermi-Dirac integrals by quadrature in u = sqrt(t): F_{1/2} (j = 1) and F_{-3/2} (j = -3)."""
+    from scipy import integrate as sint
+    up = math.sqrt(max(eta, 0.0) + 40.0)
+    if j == 1:
+        f, pts = (lambda u: u * u * sp.expit(eta - u * u)), None
+        c = 4 / math.sqrt(math.pi)
+    else:
+        f, pts = (lambda u: sp.expit(u * u - eta) * sp.expit(eta - u * u)), ([math.sqrt(eta)] if eta > 0 else None)
+        c = 2 / math.sqrt(math.pi)
+    return c * sint.quad(f, 0, up, points=pts, epsabs=0, epsrel=1e-13, limit=200)[0]
+
+
+def _n_tf(U, lap, T):
+    """Finite-T Thomas-Fermi density (T/2pi)^{3/2} F_{1/2}(-U/T) with the Laplacian term in the
+    reference's form, -lap sqrt(T)/12 F_{-3/2}(-U/T)/(2pi)^{3/2} (mpDPFT.cpp:3375)."""
+    eta = -U / T
+    return (T ** 1.5 * _fd(eta, 1) - lap * math.sqrt(T) / 12 * _fd(eta, -3)) / (2 * math.pi) ** 1.5
+
+
+@pytest.mark.parametrize("U,ar,lap,T", [
+    (-1.0, 0.3, 0.0, 0.05), (-1.0, 0.3, 0.8, 0.05), (0.3, 0.4, 0.0, 0.1), (-0.5, 0.1, -0.5, 0.02),
+    (-2.0, 0.7, 1.5, 0.2),
+])
+def test_nait_point_is_the_airy_average(oracle, U, ar, lap, T):
+    """The nAiT density (CurlyA form, doubling lower limit, adaptive Boole panels) equals
+    int Ai(x) n_TF(U - a_r x) dx by nested quadrature: scipy's Ai outside, Fermi-Dirac
+    integrals inside."""
+    from scipy import integrate as sint
+    n = oracle.nait_point(U, ar, lap, T, 1e-9, 10, 20, 1e-12)
+    xlo = (U - 45 * T) / ar
+    pts = [U / ar] if xlo < U / ar < 30 else None
+    ref = sint.quad(lambda x: sp.airy(x)[0] * _n_tf(U - ar * x, lap, T), xlo, 30, points=pts, epsabs=0,
+                    epsrel=1e-12, limit=4000)[0]
+    assert abs(n - ref) <= 1e-10 * abs(ref)
+
+
+def test_nait_tf_branch_is_the_finite_t_fermi_gas(oracle):
+    """Below the gradient threshold: (T/2pi)^{3/2} F_{1/2}(mu/T), with F by quadrature, plus the
+    Laplacian term; degenerate, intermediate and Boltzmann regimes."""
+    for U, T in ((-1.0, 0.05), (-0.2, 0.3), (0.5, 0.1), (2.0, 0.4)):
+        for lap in (0.0, 0.9):
+            n = oracle.nait_point(U, 0.0, lap, T, 1e-3, 10, 20, 1e-12)
+            ref = _n_tf(U, lap, T)
+            assert abs(n - ref) <= 1e-13 * abs(ref)
+
+
+def test_nait_point_boltzmann_closed_form(oracle):
+    """Far above mu the Fermi factors expand as Li_s(-z) = sum_k (-1)^k z^k / k^s, and the Airy
+    average of each term is exact: int Ai(x) e^{p x} dx = e^{p^3/3}.  So n = beta0 sum_k (-1)^k
+    z^k e^{(k a_r/T)^3/3} (T^{3/2} k^{-3/2} - lap sqrt(T)/12 k^{1/2}), z = e^{-U/T}; the k = 1
+    factor e^{(a_r/T)^3/3} is the Wigner-Kirkwood beta^3 |grad V|^2/24 (|grad V|^2 = 8 a_r^3).
+    T = 4 keeps the integrand far above the absolute cut of the lower-limit search."""
+    T = 4.0
+    for Ut in (10, 14):
+        for bar in (0.3, 0.5, 0.7):
+            for lap in (0.0, 3.0):
+                U, ar = Ut * T, bar * T
+                z = math.exp(-Ut)
+                ref = -(2 * math.pi) ** -1.5 * sum(
+                    (-1) ** k * z ** k * math.exp((k * bar) ** 3 / 3) * (T ** 1.5 * k ** -1.5 - lap * math.sqrt(T) / 12
+                                                                          * k ** 0.5) for k in range(1, 5))
+                n = oracle.nait_point(U, ar, lap, T, 1e-12, 10, 20, 1e-12)
+                assert abs(n - ref) <= 1e-10 * abs(ref)
+
+
+def test_nait_laplacian_term_is_the_reference_form(oracle):
+    """The Laplacian term keeps the reference's coefficient -lap/(12 sqrt(alpha tau))
+    (mpDPFT.cpp:3375): in the Boltzmann regime it shifts n by -lap/(12 T), which is the
+    Wigner-Kirkwood -beta^2 lap/12 only at T = 1 (the hbar^2 expansion has 1/T^2)."""
+    for T in (1.0, 0.05, 0.3):
+        U = 30 * T
+        n0 = oracle.nait_point(U, 0.0, 0.0, T, 1e9, 10, 20, 1e-12)
+        n1 = oracle.nait_point(U, 0.0, 0.7, T, 1e9, 10, 20, 1e-12)
+        assert abs((n1 - n0) / n0 - (-0.7 / (12 * T))) <= 1e-10
