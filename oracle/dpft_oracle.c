@@ -32,7 +32,9 @@
  *    scipy quadrature of the Airy average, the TF branch against the T = 0
  *    Fermi gas, the deep-well limit (Airy -> TF, Laplacian term included) and
  *    its first moment correction (int t^3 Ai = 2).
- *  - mu bisection, convergence, mixing: invariants (monotone N(mu), exit rule,
+ *  - mu at convergence: the root of N(mu) = N on the exported potential
+ *    (scipy brentq), to RelAcc.
+ *  - mu bisection path, convergence, mixing: invariants (monotone N(mu), exit rule,
  *    linear/Pulay fixed points) -- parity unpinned beyond these invariants and
  *    the full-loop closed forms; the same holds for AdaptnAiT and for
  *    EkinAiT03D's outer MaxIteration loop.

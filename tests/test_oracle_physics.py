@@ -392,3 +392,25 @@ def test_linear_fixed_point_is_pulay_fixed_point(oracle):
     q.maxSCcount = 4000
     rc2, r2, den2, V2 = oracle.solve(q)
     assert np.abs(den - den2).max() < 1e-6 * den.max()
+
+
+@pytest.mark.parametrize("case", ["tf_contact_2d", "tf_ion_3d"])
+def test_mu_is_the_root_of_n_of_mu(oracle, case):
+    """At convergence the loop's mu (bracketing + bisection) is the root of N(mu) = N on the
+    exported potential, found independently with scipy's brentq on the density integral: both
+    N(mu) and mu agree to RelAcc, for each species."""
+    from scipy.optimize import brentq
+    if case == "tf_contact_2d":
+        p = configs.c0(steps=32, edge=32.0)
+    else:
+        p = configs.c1(steps=24, density_expression=7, hartree_mode=1, N=6.0, maxsc=600)
+        p.Interactions[1] = 0
+        p.Interactions[2] = 0
+    rc, r, den, V = oracle.solve(p)
+    assert rc == 0
+    for s in range(p.S):
+        mu, N = r.muVec[s], p.Abundances[s]
+        assert abs(oracle.density(p, s, mu, V[s])[1] - N) <= p.RelAcc * N
+        root = brentq(lambda m: oracle.density(p, s, m, V[s])[1] - N, mu - 0.5 * abs(mu) - 0.1,
+                      mu + 0.5 * abs(mu) + 0.1, xtol=1e-15, rtol=1e-15)
+        assert abs(mu - root) <= p.RelAcc * abs(root)
