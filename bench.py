@@ -1,44 +1,47 @@
 #!/usr/bin/env python3
 """Converged self-consistent DPFT solves per second on one MI355X.
 
-Workload (the FP64 GPU drop-in configuration): spin-unpolarised electrons,
-N = 10, in the softened Coulomb well -Z/sqrt(r^2 + a^2) (Z = 10, a = 0.5 bohr)
-on a 128^3 grid over a 20 bohr cube; density from the Airy-averaged
-quantum-corrected functional (nAiT, T = 0.05 Ha) with |grad V|^2 and lap V
-from FFT derivatives smoothed by a k-space Gaussian of width 0.3 bohr^-1;
-Hartree by an isolated-boundary convolution on the zero-padded 256^3 grid;
-Dirac exchange + PW92 correlation; Pulay history 10, mixing 0.05, converged
-when max|V - V_old| < 1e-8 for three consecutive iterations.
+Headline workload (a step): one batched solve of 512 independent 2D
+two-species Thomas-Fermi + contact gases (seed 20260101: N per species in
+[20, 400], anisotropic trap (x^2 + lambda^2 y^2)/2 with lambda in [1, 2],
+contact g in [0.2, 2.0], grids 64^2/128^2/256^2 with probabilities
+0.5/0.3/0.2 on a box 2.5 sqrt(2N), Pulay history 8, mixing 0.1, converged at
+max|dV| < 1e-8) in one persistent launch, FP64 throughout.  `value` is
+converged solves per second (512 x steps / wall time of the timed solves,
+inputs resident on the device).  The same line carries:
 
-A step is one converged solve from V = V_ext.  The JSON line carries the
-roofline of the dominant kernel (the nAiT density kernel, FP64) timed with
-HIP events on the solver's stream, the HBM rates of the FFT stages, and a
-CPU baseline from the oracle on a bounded sample.
+  roofline      the persistent kernel (k_batch_solve, FP64): its algorithmic
+                bytes per launch (each field access of every instance's loop,
+                counted by the kernel itself) over its launch time from HIP
+                events on the batch's stream, against 8 TB/s; the counter
+                traffic comes from a rocprof summary under profiles/
+  batch         the FP32 + FP64-polish batch (rate, |dE| against FP64 and
+                against the oracle) and the plain FP64 GPU loop on the same
+                512 instances (one context per instance, one after another)
+  c1            one converged solve of the 3D drop-in workload (128^3, nAiT
+                T = 0.05, isolated Hartree on the zero-padded 256^3 grid,
+                Dirac + PW92): its nAiT kernel roofline (FP64 VALU) and the
+                HBM rates of its transform and pointwise stages
+  adjoint       the 96^3 differentiation workload: solve, implicit adjoint
+                gradient of int (n - n_target)^2 and the total-energy forces
+                with respect to the 96 centre coordinates, checked against
+                central differences (h = 1e-4) on a few coordinates
+  cpu_baseline  the oracle (plain C restatement, FP64) on a bounded sample of
+                the same 512 instances: one instance per core, and in the
+                notes sequentially on all cores and on one thread
 
-The "batch" object adds the batched workload: 512 seeded 2D two-species TF +
-contact gases (grids 64^2/128^2/256^2, seed 20260101) solved in one persistent
-launch, in FP64 and as FP32 with an FP64 polish, against the plain FP64 GPU
-loop solving the same instances one after another; the polish results are
-compared with the FP64 results (energy, density, mu) and, on a sample, with
-the oracle, which is also timed there as the CPU baseline of that workload.
+Every leg runs under an internal deadline; a leg that does not fit is named
+as skipped and the line is always printed.
 
-The "adjoint" object adds the differentiation workload: a 96^3 cluster of 32
-softened Coulomb centres (Z = 1, a = 0.6, seed 424242) on a 30 bohr cube with
-N = 32 and the functional above; the loss int (n - n_target)^2 (target: the
-centres of seed 434343) and the total energy are differentiated with respect to
-the 96 centre coordinates by the implicit adjoint of the converged fixed point,
-and one coordinate of each is checked against central differences (h = 2e-4,
-which keeps the O(h^2) truncation near 2e-8 relative while halving the effect
-of the solves' stopping noise over 2h) of solves started from the converged
-potential and stopped at max|dV| < 2e-11.
-
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--deadline S]
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
+import traceback
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -46,10 +49,10 @@ sys.path.insert(0, ROOT)
 METRIC = "converged SC solves/s vs FP64 GPU at |ΔE|<1e-6; FFT/pointwise HBM GB/s % peak"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+N_BATCH = 512
 
 
 def host_cpu():
-    """Model name of the host CPU (first entry of /proc/cpuinfo) and os.cpu_count()."""
     name = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -59,118 +62,260 @@ def host_cpu():
                     break
     except OSError:
         pass
-    return "%s (%s logical CPUs visible)" % (name, os.cpu_count())
+    return name
 
 
-def cpu_baseline(p, mu, V, density_evals, budget_s):
-    """Oracle (plain C, OpenMP over all host threads) timing of one nAiT
-    density evaluation at the solve's own grid and converged potential,
-    scaled by the number of density evaluations one converged solve makes."""
-    import numpy as np
-    from oracle import oracle as O
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    O.lib()
-    v = np.ascontiguousarray(V[0])
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+
+def read_summary_value(path, key):
+    """'key = value' line of a committed rocprof summary, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            if ln.strip().startswith(key):
+                try:
+                    return float(ln.split("=", 1)[1].split()[0])
+                except (ValueError, IndexError):
+                    return None
+    return None
+
+
+# ---------------------------------------------------------------- CPU legs --
+_CPU_WORKER = r"""
+import ctypes, json, sys, time
+sys.path.insert(0, %r)
+from mpdpft_amd._abi import Params   # the struct layout only; the GPU library is not loaded here
+from oracle import oracle as O
+raw = open(sys.argv[1], "rb").read()
+sz = ctypes.sizeof(Params)
+idx = json.loads(sys.argv[2])
+O.lib()
+out = []
+for i in idx:
+    p = Params.from_buffer_copy(raw[i * sz:(i + 1) * sz])
     t0 = time.perf_counter()
-    _, N = O.density(p, 0, mu, v)
-    t_eval = time.perf_counter() - t0
-    solve_s = t_eval * density_evals
+    rc, r, den, V = O.solve(p)
+    out.append([i, time.perf_counter() - t0, r.TotalEnergy, int(r.SCcount), int(r.converged)])
+print(json.dumps(out))
+""" % ROOT
+_PARAMS_FILE = None
+
+
+def _params_file(n):
+    """The first n seeded C3 instances as raw mpdpft_params records (for the
+    oracle's worker processes, which load no GPU library)."""
+    global _PARAMS_FILE
+    if _PARAMS_FILE is None:
+        import tempfile
+        from mpdpft_amd import configs
+        fd, path = tempfile.mkstemp(prefix="c3_params_", suffix=".bin")
+        with os.fdopen(fd, "wb") as f:
+            for p in configs.c3(n=n):
+                f.write(bytes(p))
+        _PARAMS_FILE = path
+    return _PARAMS_FILE
+
+
+def _oracle_run(indices, threads):
+    """Oracle solves of the given C3 instances in a child process with
+    OMP_NUM_THREADS = threads (the oracle is plain C + OpenMP)."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, "-c", _CPU_WORKER, _params_file(N_BATCH), json.dumps(list(indices))], env=env,
+                       capture_output=True, text=True, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline_leg(budget_s):
+    """C3 on the host: sequentially with all threads (the reference's runY jobs
+    one after another), one instance per core (independent jobs side by side),
+    and sequentially on one thread, each on a bounded prefix of the seeded
+    instance list."""
+    cores = host_threads()
+    t_start = time.perf_counter()
+    # one instance per core: 2 x cores instances, one worker process each (OMP 1)
+    n_par = 2 * cores
+    procs = []
+    t0 = time.perf_counter()
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    for w in range(cores):
+        idx = list(range(w, n_par, cores))
+        procs.append(subprocess.Popen([sys.executable, "-c", _CPU_WORKER, _params_file(N_BATCH), json.dumps(idx)], env=env,
+                                      stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+    par = []
+    for p in procs:
+        out, err = p.communicate(timeout=max(30.0, budget_s))
+        if p.returncode != 0:
+            raise RuntimeError("oracle worker failed: " + err[-400:])
+        par += json.loads(out.strip().splitlines()[-1])
+    t_par = time.perf_counter() - t0
+    # sequential, all threads: the first instances until ~25 % of the budget is used
+    seq = []
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < 0.25 * budget_s and k < 64:
+        seq += _oracle_run(range(k, k + 4), cores)
+        k += 4
+    t_seq = sum(x[1] for x in seq)
+    # sequential, one thread: the first 4 instances
+    one = _oracle_run(range(4), 1)
+    t_one = sum(x[1] for x in one)
     return {
-        "value": 1.0 / solve_s,
+        "value": round(len(par) / t_par, 3),
         "unit": "solves/s",
         "cores": cores,
         "kind": "port",
-        "sample": ("one nAiT density evaluation (DelField + per-point adaptive Airy quadrature) of the oracle at "
-                   "%d^3 on the converged potential took %.2f s; a solve makes %d such evaluations "
-                   "(mu bracketing/bisection included) -> %.0f s per solve; host: %s" % (
-                       p.steps, t_eval, density_evals, solve_s, host_cpu())),
+        "sample": ("the oracle (plain C FP64 restatement of the SC loop) on the first %d of the 512 seeded "
+                   "instances, one instance per core (%d worker processes, 1 thread each): %.2f s wall; "
+                   "host %s" % (len(par), cores, t_par, host_cpu())),
+        "sequential_all_threads": {"instances": len(seq), "seconds": round(t_seq, 3),
+                                   "solves_per_s": round(len(seq) / t_seq, 3), "threads": cores},
+        "sequential_one_thread": {"instances": len(one), "seconds": round(t_one, 3),
+                                  "solves_per_s": round(len(one) / t_one, 3), "threads": 1},
+        "oracle_energies": {str(x[0]): x[2] for x in par},
+        "leg_seconds": round(time.perf_counter() - t_start, 1),
     }
 
 
-def batch_leg(n, plain_n, cpu_n, cpu_on):
-    """C3: the batched solver (FP64; FP32 + FP64 polish) against the plain FP64
-    loop on the same instances; errors of the polished results."""
+# ---------------------------------------------------------------- GPU legs --
+def batch_leg(steps, warmup, plain_n, oracle_energies):
+    """The headline: FP64 batch solves/s; FP32 + polish; the plain FP64 loop."""
     import numpy as np
     from mpdpft_amd import Batch, Context, configs
-    insts = configs.c3(n=n)
-    Batch(insts[:4]).solve()  # loads the kernels
+    insts = configs.c3(n=N_BATCH)
     b64 = Batch(insts)
-    r64 = b64.solve()
-    t64 = b64.seconds()
-    f64 = [b64.fields(i) for i in range(n)]
+    for _ in range(warmup):
+        b64.solve()
+    ms = by = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r64 = b64.solve()
+        (_, k_ms), (_, k_by), us = b64.kernel_stats()
+        ms += k_ms
+        by += k_by
+    elapsed = time.perf_counter() - t0
+    f64 = [b64.fields(i) for i in range(N_BATCH)]
     b64.close()
+    grid = np.array([p.steps for p in insts])
+    per_class = {str(g): {"instances": int((grid == g).sum()), "max_instance_ms": round(float(us[grid == g].max()) / 1e3, 2),
+                          "mean_instance_ms": round(float(us[grid == g].mean()) / 1e3, 2)}
+                 for g in (64, 128, 256)}
+    avg_ms = ms / steps
+    bytes_per_launch = by / steps
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    traffic = read_summary_value(os.path.join(ROOT, "profiles", "batch_c3_512_traffic.md"), "traffic_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_batch_solve<double>",
+            "avg_launch_ms": round(avg_ms, 3), "algorithmic_bytes_per_launch": bytes_per_launch,
+            "note": "one launch solves all 512 instances; algorithmic bytes = every field access of every "
+                    "instance's loop (mu evaluations, density, UpdateV, convergence, Pulay push + Gram row, "
+                    "mixing, energies, retire) counted once by the kernel itself"}
+    # FP32 fields + FP64 polish
     b32 = Batch(insts, fp32=True)
-    r32 = b32.solve()
-    t32 = b32.seconds()
+    b32.solve()
+    n32 = max(1, min(steps, 10))
+    t0 = time.perf_counter()
+    for _ in range(n32):
+        r32 = b32.solve()
+    t32 = (time.perf_counter() - t0) / n32
+    (ms32, by32, _) = b32.kernel_stats()
     (ph_s, ph_it, ph_bis) = b32.phase_stats()
-    f32 = [b32.fields(i) for i in range(n)]
+    f32 = [b32.fields(i) for i in range(N_BATCH)]
     b32.close()
     dE = max(abs(a.TotalEnergy - b.TotalEnergy) / abs(b.TotalEnergy) for a, b in zip(r32, r64))
     dmu = max(abs(a.muVec[s] - b.muVec[s]) / abs(b.muVec[s]) for a, b in zip(r32, r64) for s in range(a.S))
     dn = max(float(np.max(np.abs(a[0] - b[0])) / np.max(np.abs(b[0]))) for a, b in zip(f32, f64))
-    plain_n = min(plain_n, n)
-    t_plain = 0.0
-    plain_match = 0
-    for i in range(plain_n):
-        ctx = Context(insts[i])
-        t0 = time.perf_counter()
-        r = ctx.solve()
-        t_plain += time.perf_counter() - t0
-        ctx.close()
-        plain_match += int(abs(r.TotalEnergy - r64[i].TotalEnergy) <= 1e-10 * abs(r.TotalEnergy)
-                           and r.SCcount == r64[i].SCcount)
-    plain_rate = plain_n / t_plain
     out = {
-        "workload": "C3: %d 2D two-species TF + contact gases, seed 20260101, grids 64/128/256 (p 0.5/0.3/0.2), "
-                    "box 2.5 sqrt(2N), Pulay 8, theta 0.1, max|dV|<1e-8" % n,
-        "instances": n,
-        "grid_points": int(sum(p.S * (p.steps + 1) ** 2 for p in insts)),
-        "fp64_batch": {"seconds": round(t64, 4), "solves_per_s": round(n / t64, 2),
-                       "converged": sum(r.converged for r in r64),
-                       "sc_iterations": int(sum(r.SCcount for r in r64))},
-        "fp32_polish_batch": {"seconds": round(t32, 4), "solves_per_s": round(n / t32, 2),
+        "workload": "C3: 512 2D two-species TF + contact gases, seed 20260101, grids 64/128/256 (p 0.5/0.3/0.2), "
+                    "box 2.5 sqrt(2N), Pulay 8, theta 0.1, max|dV|<1e-8",
+        "fp64_batch": {"solves_per_s": round(N_BATCH * steps / elapsed, 2), "seconds_per_batch": round(elapsed / steps, 4),
+                       "converged": sum(r.converged for r in r64), "sc_iterations": int(sum(r.SCcount for r in r64)),
+                       "per_grid": per_class},
+        "fp32_polish_batch": {"solves_per_s": round(N_BATCH / t32, 2), "seconds_per_batch": round(t32, 4),
                               "converged": sum(r.converged for r in r32),
                               "fp32_seconds": round(ph_s[0], 4), "polish_seconds": round(ph_s[1], 4),
+                              "fp32_kernel_ms": round(ms32[0], 3), "polish_kernel_ms": round(ms32[1], 3),
+                              "fp32_algorithmic_GBps": round(by32[0] / (ms32[0] / 1e3) / 1e9, 1) if ms32[0] else None,
                               "fp32_iterations": int(ph_it[0]), "polish_iterations": int(ph_it[1]),
                               "fp32_bisection_steps": int(ph_bis[0]), "polish_bisection_steps": int(ph_bis[1]),
                               "max_rel_dE_vs_fp64": dE, "max_rel_dmu_vs_fp64": dmu,
                               "max_dn_over_max_n_vs_fp64": dn},
-        "plain_fp64_loop": {"instances": plain_n, "seconds": round(t_plain, 4),
-                            "solves_per_s": round(plain_rate, 2),
-                            "identical_to_fp64_batch": plain_match},
-        "speedup_fp32_polish_vs_plain_fp64": round((n / t32) / plain_rate, 2),
-        "speedup_fp64_batch_vs_plain_fp64": round((n / t64) / plain_rate, 2),
     }
-    if cpu_on:
-        from oracle import oracle as O
-        O.lib()
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        t_cpu = 0.0
-        eE = en = emu = 0.0
-        cpu_n = min(cpu_n, n)
-        for i in range(cpu_n):
+    if oracle_energies:
+        eo = max(abs(r32[int(i)].TotalEnergy - e) / abs(e) for i, e in oracle_energies.items())
+        e64 = max(abs(r64[int(i)].TotalEnergy - e) / abs(e) for i, e in oracle_energies.items())
+        out["fp32_polish_batch"]["max_rel_dE_vs_oracle"] = eo
+        out["fp64_batch"]["max_rel_dE_vs_oracle"] = e64
+        out["oracle_instances"] = len(oracle_energies)
+    # the plain FP64 GPU loop: one context per instance, one after another
+    plain_n = min(plain_n, N_BATCH)
+    if plain_n > 0:
+        t_plain = 0.0
+        match = 0
+        for i in range(plain_n):
+            ctx = Context(insts[i])
             t0 = time.perf_counter()
-            _, r, den, _ = O.solve(insts[i])
-            t_cpu += time.perf_counter() - t0
-            eE = max(eE, abs(r32[i].TotalEnergy - r.TotalEnergy) / abs(r.TotalEnergy))
-            emu = max(emu, max(abs(r32[i].muVec[s] - r.muVec[s]) / abs(r.muVec[s]) for s in range(r.S)))
-            en = max(en, float(np.max(np.abs(f32[i][0] - den)) / np.max(np.abs(den))))
-        out["fp32_polish_batch"].update({"max_rel_dE_vs_oracle": eE, "max_rel_dmu_vs_oracle": emu,
-                                         "max_dn_over_max_n_vs_oracle": en, "oracle_instances": cpu_n})
-        out["cpu_baseline"] = {"value": round(cpu_n / t_cpu, 3), "unit": "solves/s", "cores": cores,
-                               "kind": "port",
-                               "sample": "the oracle's SC loop on the first %d instances, one after another "
-                                         "(OpenMP over %d threads): %.2f s; host: %s" % (cpu_n, cores, t_cpu,
-                                                                                         host_cpu())}
-    return out
+            r = ctx.solve()
+            t_plain += time.perf_counter() - t0
+            ctx.close()
+            match += int(abs(r.TotalEnergy - r64[i].TotalEnergy) <= 1e-10 * abs(r.TotalEnergy))
+            if i % 128 == 127:
+                print("bench: plain loop %d/%d" % (i + 1, plain_n), file=sys.stderr, flush=True)
+        plain_rate = plain_n / t_plain
+        out["plain_fp64_loop"] = {"instances": plain_n, "seconds": round(t_plain, 3), "solves_per_s": round(plain_rate, 2),
+                                  "energy_within_1e-10_of_fp64_batch": match}
+        out["speedup_fp64_batch_vs_plain_fp64"] = round(N_BATCH * steps / elapsed / plain_rate, 2)
+        out["speedup_fp32_polish_vs_plain_fp64"] = round(N_BATCH / t32 / plain_rate, 2)
+    return N_BATCH * steps / elapsed, elapsed / steps, roof, out
 
 
-def adjoint_leg(steps, h):
-    """C4: converged solve, adjoint gradient of the density-mismatch loss and
-    total-energy forces for all centre coordinates, FD checks."""
+def c1_leg(grid, maxsc):
+    """One converged drop-in solve: nAiT kernel roofline and stage HBM rates."""
+    from mpdpft_amd import Context, configs
+    p = configs.c1(steps=grid, density_expression=5, hartree_mode=1, maxsc=maxsc)
+    ctx = Context(p)
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    res = ctx.solve()
+    t = time.perf_counter() - t0
+    stats = ctx.stage_stats()
+    ctx.close()
+    nait = stats.get("nait_density", {})
+    roof = None
+    if nait.get("launches"):
+        avg_s = nait["total_ms"] / nait["launches"] / 1e3
+        fl = nait["flops"] / nait["launches"]
+        ach = fl / avg_s / 1e12
+        traffic = read_summary_value(os.path.join(ROOT, "profiles", "nait_lanes_128_traffic.md"),
+                                     "traffic_bytes_per_launch") if grid == 128 else None
+        roof = {"bound": "valu", "achieved": round(ach, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / FP64_PEAK_TFLOPS, 5), "traffic": traffic, "kernel": "k_nait_lanes",
+                "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": fl,
+                "note": "FP64 vector (VALU) bound, no MFMA: integrand evaluations performed x 50 nominal FP64 flops"}
+    hbm = {}
+    for name in ("delfield", "hartree_isolated", "update_v", "tf_density", "pulay_gram", "pulay_mix"):
+        st = stats.get(name)
+        if st and st["launches"] and st["total_ms"] > 0:
+            gbs = st["bytes"] / (st["total_ms"] / 1e3) / 1e9
+            hbm[name] = {"GB/s": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
+                         "avg_ms": round(st["total_ms"] / st["launches"], 4), "launches": st["launches"]}
+    return {
+        "workload": "C1 drop-in: 3D nAiT(T=0.05) + isolated Hartree (zero-padded %d^3) + Dirac + PW92, %d^3 on a "
+                    "20 bohr cube, Z=10 a=0.5 N=10, Pulay 10, theta 0.05, max|dV|<1e-8" % (2 * grid, grid),
+        "seconds": round(t, 2), "solves_per_s": round(1.0 / t, 5), "converged": bool(res.converged),
+        "sc_iterations": res.SCcount - 1, "density_evaluations": res.bisection_steps + p.S * (1 + res.SCcount),
+        "total_energy": res.TotalEnergy, "mu": res.muVec[0], "nait_roofline": roof, "stages_hbm": hbm,
+        "stage_ms": {k: round(v["total_ms"], 2) for k, v in stats.items() if v["launches"]},
+    }
+
+
+def adjoint_leg(steps, h, n_coords, deadline):
+    """C4: converged solve, adjoint gradient of the density-mismatch loss and the
+    total-energy forces for all 96 centre coordinates, central-difference checks."""
     import numpy as np
     from mpdpft_amd import Context, configs
-    os.environ["MPDPFT_VERBOSE"] = "0"
     pos = configs.centre_positions(32, 424242)
     p = configs.c4(steps=steps, tol=1e-10, positions=pos)
     pt = configs.c4(steps=steps, tol=1e-10, positions=configs.centre_positions(32, 434343))
@@ -182,6 +327,7 @@ def adjoint_leg(steps, h):
     ct.solve()
     nt = ct.density()[0].copy()
     ct.close()
+    print("bench: adjoint target solved", file=sys.stderr, flush=True)
     ctx = Context(p)
     t0 = time.perf_counter()
     r = ctx.solve()
@@ -196,189 +342,128 @@ def adjoint_leg(steps, h):
     forces = ctx.centre_forces()
     t_forces = time.perf_counter() - t0
     ctx.close()
-    # central differences from solves stopped at max|dV| < 2e-11 (their stopping noise over 2h is the
-    # difference's error), each started from the converged V
     C = np.array([[q[0], q[1], q[2], 1.0, 0.6] for q in pos])
-    ci, d = 0, 0
-    vals = []
-    fd = Context(configs.c4(steps=steps, tol=2e-11, maxsc=500, positions=pos))
-    for sgn in (1, -1):
-        Cp = C.copy()
-        Cp[ci, d] += sgn * h
-        fd.set_centres(Cp)
-        fd.set_initial_potential(V)
-        rr = fd.solve()
-        vals.append((float(np.sum(w * (fd.density()[0] - nt) ** 2)), rr.TotalEnergy, bool(rr.converged)))
+    coords = [(0, 0), (5, 1), (11, 2), (17, 0), (23, 1), (31, 2)][:n_coords]
+    checks = []
+    fd = Context(configs.c4(steps=steps, tol=1e-12, maxsc=500, positions=pos))
+    for ci, d in coords:
+        if time.perf_counter() > deadline:
+            break
+        vals = []
+        for sgn in (1, -1):
+            Cp = C.copy()
+            Cp[ci, d] += sgn * h
+            fd.set_centres(Cp)
+            fd.set_initial_potential(V)
+            rr = fd.solve()
+            vals.append((float(np.sum(w * (fd.density()[0] - nt) ** 2)), rr.TotalEnergy, bool(rr.converged)))
+        fdL = (vals[0][0] - vals[1][0]) / (2 * h)
+        fdE = (vals[0][1] - vals[1][1]) / (2 * h)
+        print("bench: adjoint fd check centre %d %s done" % (ci, "xyz"[d]), file=sys.stderr, flush=True)
+        checks.append({"centre": ci, "axis": "xyz"[d], "dL_adjoint": float(grad[ci, d]), "dL_fd": fdL,
+                       "dL_rel_err": abs(grad[ci, d] - fdL) / abs(fdL), "dE_adjoint": float(forces[ci, d]),
+                       "dE_fd": fdE, "dE_rel_err": abs(forces[ci, d] - fdE) / abs(fdE),
+                       "fd_solves_converged": all(v[2] for v in vals)})
     fd.close()
-    fdL = (vals[0][0] - vals[1][0]) / (2 * h)
-    fdE = (vals[0][1] - vals[1][1]) / (2 * h)
     return {
-        "workload": "C4: 96^3 nAiT(T=0.05) + isolated Hartree + Dirac + PW92 on a 30 bohr cube, 32 softened "
-                    "Coulomb centres (Z=1, a=0.6, seed 424242), N=32, max|dV|<1e-10; loss int (n - n_target)^2 "
-                    "(target seed 434343) and total energy vs the 96 centre coordinates" if steps == 96 else
-                    "C4 geometry at %d^3" % steps,
+        "workload": ("C4: %d^3 nAiT(T=0.05) + isolated Hartree + Dirac + PW92 on a 30 bohr cube, 32 softened Coulomb "
+                     "centres (Z=1, a=0.6, seed 424242), N=32, max|dV|<1e-10; loss int (n - n_target)^2 (target "
+                     "seed 434343) and total energy vs the 96 centre coordinates" % steps),
         "solve_seconds": round(t_solve, 3), "sc_iterations": r.SCcount - 1, "converged": bool(r.converged),
         "total_energy": r.TotalEnergy,
         "adjoint_seconds": round(t_adj, 4), "gmres_iterations": info["iterations"],
         "gmres_relative_residual": info["relative_residual"], "forces_seconds": round(t_forces, 4),
         "gradient_norm": float(np.linalg.norm(grad)), "force_norm": float(np.linalg.norm(forces)),
-        "fd_check": {"coordinate": "centre 0 x", "h": h, "dL_adjoint": float(grad[ci, d]), "dL_fd": fdL,
-                     "dL_rel_err": abs(grad[ci, d] - fdL) / abs(fdL), "dE_adjoint": float(forces[ci, d]),
-                     "dE_fd": fdE, "dE_rel_err": abs(forces[ci, d] - fdE) / abs(fdE),
-                     "fd_solves_converged": all(v[2] for v in vals)},
+        "fd_h": h, "fd_checks": checks,
+        "max_dL_rel_err": max((c["dL_rel_err"] for c in checks), default=None),
+        "max_dE_rel_err": max((c["dE_rel_err"] for c in checks), default=None),
         "fd_gradient_cost_estimate_s": round(2 * 96 * t_solve, 1),
     }
 
 
-def precision_leg(steps):
-    """C2: the same solve in FP64 and as Precision 1 (FP32 transforms and pointwise
-    potentials until the FP32 floor, then FP64 polish); errors and times."""
-    import numpy as np
-    from mpdpft_amd import Context, configs
-    out = {"workload": "C2: %d^3 nAiT(T=0.05) + isolated Hartree + Dirac + PW92 on a 24 bohr cube, Z=18, a=0.4, "
-                       "N=18, max|dV|<1e-8" % steps}
-    res = {}
-    for prec in (0, 1):
-        ctx = Context(configs.c2(steps=steps, precision=prec))
-        ctx.enable_timing(True)
-        t0 = time.perf_counter()
-        r = ctx.solve()
-        t = time.perf_counter() - t0
-        res[prec] = (r, ctx.density()[0].copy(), t)
-        stats = ctx.stage_stats()
-        ctx.close()
-        out["fp64" if prec == 0 else "precision1"] = {
-            "seconds": round(t, 2), "sc_iterations": r.SCcount - 1, "fp32_iterations": r.fp32_iterations,
-            "converged": bool(r.converged), "total_energy": r.TotalEnergy, "mu": r.muVec[0],
-            "stage_ms": {k: round(v["total_ms"], 1) for k, v in stats.items() if v["launches"]}}
-    (r0, n0, _), (r1, n1, _) = res[0], res[1]
-    out["rel_dE"] = abs(r1.TotalEnergy - r0.TotalEnergy) / abs(r0.TotalEnergy)
-    out["rel_dmu"] = abs(r1.muVec[0] - r0.muVec[0]) / abs(r0.muVec[0])
-    out["max_dn_over_max_n"] = float(np.abs(n1 - n0).max() / np.abs(n0).max())
-    return out
-
-
 def main():
+    t_begin = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=20, help="timed batched solves")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--deadline", type=float, default=450.0, help="seconds after which no further leg starts")
+    ap.add_argument("--legs", default="cpu,batch,c1,adjoint",
+                    help="comma-separated subset of cpu, batch, plain, c1, adjoint")
+    ap.add_argument("--plain", type=int, default=512, help="instances the plain FP64 loop solves")
+    ap.add_argument("--c1-grid", type=int, default=128)
     ap.add_argument("--maxsc", type=int, default=300)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
-    ap.add_argument("--batch", type=int, default=512, help="instances of the batched workload (0: skip)")
-    ap.add_argument("--batch-plain", type=int, default=512,
-                    help="instances the plain FP64 loop solves for the batch comparison")
-    ap.add_argument("--batch-cpu", type=int, default=16, help="instances the oracle solves for the batch baseline")
-    ap.add_argument("--adjoint-grid", type=int, default=96, help="grid of the differentiation workload (0: skip)")
-    ap.add_argument("--precision-grid", type=int, default=0,
-                    help="grid of the precision-variant comparison (256 = C2; 0: skip, about 15 min at 256)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "nait_lanes_128_traffic.md"),
-                    help="rocprof summary holding 'traffic_bytes_per_launch = N' for the nAiT kernel at this grid")
+    ap.add_argument("--adjoint-grid", type=int, default=96)
+    ap.add_argument("--fd-coords", type=int, default=6)
+    ap.add_argument("--cpu-budget", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: runs on one GPU; --gpus %d is not supported (the batched and sharded "
-              "layouts are not built)" % args.gpus, file=sys.stderr)
+        print("bench.py runs on one GPU: the batched instances would shard across GPUs by queue share, which is "
+              "not built; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    deadline = t_begin + args.deadline
+    legs = set(args.legs.split(","))
+    os.environ.setdefault("MPDPFT_VERBOSE", "0")
 
-    os.environ.setdefault("MPDPFT_VERBOSE", "1")  # one line per SC iteration on stderr
-    from mpdpft_amd import Context, configs
+    skipped, errors = {}, {}
+    value = ms_per_step = None
+    roof = cpu = batch = c1 = adjoint = None
 
-    p = configs.c1(steps=args.grid, density_expression=5, hartree_mode=1, maxsc=args.maxsc)
-    ctx = Context(p)
-    for _ in range(args.warmup):
-        ctx.solve()
-    ctx.enable_timing(True)  # resets the stage statistics
-    t0 = time.perf_counter()
-    res = None
-    for _ in range(args.steps):
-        res = ctx.solve()  # returns after the stream has drained
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    stats = ctx.stage_stats()
-    V = ctx.potential()
-
-    nait = stats.get("nait_density", {})
-    roof = None
-    if nait.get("launches"):
-        avg_s = nait["total_ms"] / nait["launches"] / 1e3
-        flops_per_launch = nait["flops"] / nait["launches"]
-        achieved = flops_per_launch / avg_s / 1e12
-        traffic = None
-        if args.grid == 128 and args.traffic and os.path.exists(args.traffic):
-            for ln in open(args.traffic):
-                if ln.startswith("traffic_bytes_per_launch"):
-                    traffic = float(ln.split("=")[1])
-        roof = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
-                "kernel": "k_nait_lanes" if p.Mpp[0] <= 15 else "k_nait_reuse",
-                "avg_launch_ms": round(avg_s * 1e3, 4),
-                "flops_per_launch": flops_per_launch,
-                "note": "FP64 vector (VALU) compute bound, no MFMA: integrand evaluations actually performed "
-                        "x 50 nominal FP64 flops each, against the 78.6 TFLOP/s FP64 vector peak"}
-    hbm = {}
-    stages = {k: {"launches": v["launches"], "total_ms": round(v["total_ms"], 2)} for k, v in stats.items()}
-    for name in ("delfield", "hartree_isolated", "update_v", "tf_density"):
-        st = stats.get(name)
-        if st and st["launches"] and st["total_ms"] > 0:
-            gbs = st["bytes"] / (st["total_ms"] / 1e3) / 1e9
-            hbm[name] = {"GB/s": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
-                         "avg_ms": round(st["total_ms"] / st["launches"], 4), "launches": st["launches"]}
-    if "update_v" in hbm:  # Dirac exchange and PW92 per point: transcendental (pow, log) bound, not HBM
-        hbm["update_v"]["note"] = "FP64 transcendental bound (Dirac pow, PW92 log/pow per point)"
-
-    density_evals = res.bisection_steps + p.S * (1 + res.SCcount)
-    cpu = None
-    if not args.no_cpu_baseline:
+    def run(name, est_s, fn):
+        if name not in legs:
+            skipped[name] = "not requested"
+            return None
+        left = deadline - time.perf_counter()
+        if left < est_s:
+            skipped[name] = "deadline: %.0f s left, leg needs about %.0f s" % (left, est_s)
+            return None
+        print("bench: leg %s (%.0f s elapsed)" % (name, time.perf_counter() - t_begin), file=sys.stderr, flush=True)
         try:
-            cpu = cpu_baseline(p, res.muVec[0], V, density_evals, args.cpu_budget)
+            return fn()
         except Exception as e:  # noqa: BLE001
-            cpu = {"value": None, "error": str(e)}
+            errors[name] = "%s: %s" % (type(e).__name__, e)
+            traceback.print_exc()
+            return None
 
-    batch = None
-    if args.batch > 0:
-        os.environ["MPDPFT_VERBOSE"] = "0"  # 512 plain solves: no per-iteration lines
-        print("bench: batched workload, %d instances" % args.batch, file=sys.stderr, flush=True)
-        batch = batch_leg(args.batch, args.batch_plain, args.batch_cpu, not args.no_cpu_baseline)
-
-    precision = None
-    if args.precision_grid > 0:
-        print("bench: precision variant, %d^3" % args.precision_grid, file=sys.stderr, flush=True)
-        precision = precision_leg(args.precision_grid)
-
-    adjoint = None
-    if args.adjoint_grid > 0:
-        print("bench: differentiation workload, %d^3" % args.adjoint_grid, file=sys.stderr, flush=True)
-        adjoint = adjoint_leg(args.adjoint_grid, 2e-4)
-
-    line = {
-        "metric": METRIC,
-        "value": args.steps / elapsed,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "C1 drop-in: 3D nAiT(T=0.05) + isolated Hartree (zero-padded %d^3) + Dirac + PW92, "
-                               "%d^3 on a 20 bohr cube, Z=10 a=0.5 N=10, Pulay 10, theta 0.05, max|dV|<1e-8"
-                               % (2 * args.grid, args.grid),
-                   "grid": args.grid, "converged": bool(res.converged), "sc_iterations": res.SCcount - 1,
-                   "density_evaluations": density_evals, "total_energy": res.TotalEnergy, "mu": res.muVec[0]},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "stages_hbm": hbm,
-        "stage_ms": stages,
-        "batch": batch,
-        "adjoint": adjoint,
-        "precision": precision,
-    }
-    print(json.dumps(line), flush=True)
+    try:
+        # the host legs run before the GPU is touched (child processes only)
+        cpu = run("cpu", 20, lambda: cpu_baseline_leg(args.cpu_budget))
+        oracle_E = cpu.pop("oracle_energies", None) if cpu else None
+        got = run("batch", 30, lambda: batch_leg(args.steps, args.warmup,
+                                                 args.plain if "plain" in legs or "batch" in legs else 0, oracle_E))
+        if got:
+            value, sec, roof, batch = got
+            ms_per_step = 1e3 * sec
+        c1 = run("c1", 70, lambda: c1_leg(args.c1_grid, args.maxsc))
+        adjoint = run("adjoint", 60, lambda: adjoint_leg(args.adjoint_grid, 1e-4, args.fd_coords, deadline))
+    finally:
+        if _PARAMS_FILE and os.path.exists(_PARAMS_FILE):
+            os.unlink(_PARAMS_FILE)
+        line = {
+            "metric": METRIC,
+            "value": round(value, 2) if value else None,
+            "unit": "solves/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": round(ms_per_step, 3) if ms_per_step else None,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "C3 batched: 512 seeded 2D two-species TF + contact gases (64^2/128^2/256^2), "
+                                   "FP64, one persistent launch per solve", "instances": N_BATCH, "seed": 20260101},
+            "roofline": roof,
+            "cpu_baseline": cpu,
+            "batch": batch,
+            "c1": c1,
+            "adjoint": adjoint,
+            "skipped_legs": skipped,
+            "leg_errors": errors,
+            "bench_seconds": round(time.perf_counter() - t_begin, 1),
+        }
+        print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

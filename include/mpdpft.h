@@ -21,8 +21,8 @@
 extern "C" {
 #endif
 
-/* 2: mpdpft_result gained AuxEint */
-#define MPDPFT_ABI_VERSION 2
+/* 2: mpdpft_result gained AuxEint; 3: mpdpft_result gained bracket_failures and pulay_rejections */
+#define MPDPFT_ABI_VERSION 3
 
 #define MPDPFT_MAX_SPECIES 8
 #define MPDPFT_MAX_LIST 16
@@ -154,6 +154,12 @@ typedef struct mpdpft_result {
     /* GetAuxEint (mpDPFT.cpp:5518-5549): for S = 1, DIM 3, Units 2 the Dirac exchange energy, its
        potential-functional form, the Gombas and the VWN correlation energies; 0 otherwise */
     double AuxEint[MPDPFT_MAX_SPECIES][4];
+    /* recoverable events of the solve (neither is an error; the reference only prints them):
+       EnforceConstraints calls whose mu bracket was not found, mu reset to its previous value
+       (mpDPFT.cpp:4657-4661); PulayMixer solves rejected by the invertibility or 1e-8 validation
+       test, the iteration then mixing linearly (:5089-5107) */
+    int bracket_failures;
+    int pulay_rejections;
 } mpdpft_result;
 
 typedef struct mpdpft_ctx mpdpft_ctx;
@@ -220,6 +226,51 @@ int mpdpft_interaction_potential(mpdpft_ctx *ctx, int type, int s, const double 
 /* Density of species s at chemical potential mu for potential v [G]
  * (GetDensity, mpDPFT.cpp:2142-2238); returns the trapezoid integral in *N. */
 int mpdpft_density(mpdpft_ctx *ctx, int s, double mu, const double *v, double *den, double *N);
+
+/* ---- one iteration at a time, and the loop's own stages ------------------- */
+/* mpdpft_loop_begin starts a loop as mpdpft_solve does (V = Env or the
+ * initial potential, the initial densities :194-201) without iterating;
+ * each mpdpft_step then runs one SC iteration of GetData (:213-225:
+ * StoreCurrentIteration, UpdateV, GetDensities, CheckConvergence, Intercept,
+ * AdmixDensities, InjectSchedule, PrepareNextIteration) and reports the loop
+ * state in r (SCcount, CC, converged, muVec, FinalAbundances, mixing and
+ * event counters; energies stay 0) and *exit_loop (ExitSCloop, may be NULL).
+ * A step after the loop has exited changes nothing.  Stepping until
+ * exit_loop and calling mpdpft_energy gives mpdpft_solve's result. */
+int mpdpft_loop_begin(mpdpft_ctx *ctx);
+int mpdpft_step(mpdpft_ctx *ctx, mpdpft_result *r, int *exit_loop);
+/* GetEnergy (mpDPFT.cpp:5414-5436) of the context's current density,
+ * potential and mu; r receives the whole result (energies and loop state). */
+int mpdpft_energy(mpdpft_ctx *ctx, mpdpft_result *r);
+/* GetDensities + EnforceConstraints (mpDPFT.cpp:3306-3316, :4569-4743) on
+ * the context's current potential (mpdpft_set_potential or the loop's):
+ * N_s at mu[s], then the mu bracketing and bisection to |N - N_0| / 2N_0 <=
+ * RelAcc with the context's DeltamuModifier and RelAcc state (carried from
+ * call to call as across SC iterations).  mu (in/out, S values), N (out, may
+ * be NULL); the density of the last evaluated mu becomes the context's
+ * density (mpdpft_get_density).  Returns MPDPFT_ENOBRACKET when a species'
+ * bracket was not found (its mu reset to the value it entered with). */
+int mpdpft_enforce_constraints(mpdpft_ctx *ctx, double *mu, double *N);
+/* AdmixDensities with PulayMixer (mpDPFT.cpp:4921-4992, :5005-5167):
+ * n_out (host [S][G]) is the newest density, the context's current density
+ * the n_in of this step; the mixed density replaces the context's density.
+ * sc_count is the loop's SCcount (0: linear mixing, as the first iteration
+ * is); the Pulay history is the context's (mpdpft_loop_begin resets it).
+ * Returns MPDPFT_EPULAY when the Pulay solve was rejected and the step mixed
+ * linearly (the mixed density is valid). */
+int mpdpft_mix(mpdpft_ctx *ctx, int sc_count, const double *n_out);
+/* Forcing hooks for the loop's recovery paths (no counterpart in the
+ * reference, whose tests have none): at SC iteration `at` of the next solves,
+ * MPDPFT_FAULT_DUPLICATE_PULAY pushes the newest Pulay entry twice (a
+ * singular Gram matrix: the solve is rejected and the iteration mixes
+ * linearly, :5089-5107); MPDPFT_FAULT_NAN_POTENTIAL writes NaN into species
+ * 0's potential at the grid centre after UpdateV (the NaN reaches N(mu), the
+ * bracket is not found and the loop runs on without converging, as the
+ * reference's loop would for these density expressions).  kind 0 or at < 0
+ * clears the hook. */
+#define MPDPFT_FAULT_DUPLICATE_PULAY 1
+#define MPDPFT_FAULT_NAN_POTENTIAL 2
+int mpdpft_inject_fault(mpdpft_ctx *ctx, int kind, int at);
 
 /* ---- per-stage GPU timing (the reference's StartTimer/EndTimer, :11997) --- */
 /* Kernel time per stage, measured with HIP events on the context's stream.
@@ -299,6 +350,13 @@ double mpdpft_batch_seconds(const mpdpft_batch *b);
  * iterations only. */
 int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, long long *iterations,
                              long long *bisection_steps);
+/* The last solve's k_batch_solve launches (the persistent kernel), each argument may be NULL:
+ * kernel_ms[phase] is the launch's duration from HIP events recorded on the batch's stream around
+ * it, bytes[phase] its algorithmic traffic (every field a stage of an instance's loop reads or
+ * writes, counted once per access at the field's storage width, summed over the instances), phase
+ * [0] the FP32 loop and [1] the FP64 loop or polish; instance_us[i] (n values) the time instance i
+ * spent in the last phase between leaving the queue and retiring (device clock, 100 MHz). */
+int mpdpft_batch_kernel_stats(const mpdpft_batch *b, double *kernel_ms, double *bytes, double *instance_us);
 const char *mpdpft_batch_last_error(const mpdpft_batch *b);
 void mpdpft_batch_destroy(mpdpft_batch *b);
 

@@ -76,6 +76,41 @@ class Context:
         self._check(rc, (_abi.OK, _abi.ENOTCONV))
         return r
 
+    def loop_begin(self):
+        """Start a loop without iterating (mpdpft_loop_begin)."""
+        self._check(self._lib.mpdpft_loop_begin(self._h))
+
+    def step(self):
+        """One SC iteration (mpdpft_step): (Result with the loop state, exit_loop)."""
+        r = Result()
+        ex = ctypes.c_int()
+        self._check(self._lib.mpdpft_step(self._h, ctypes.byref(r), ctypes.byref(ex)))
+        return r, bool(ex.value)
+
+    def energy(self):
+        """GetEnergy of the current fields (mpdpft_energy): the whole Result."""
+        r = Result()
+        self._check(self._lib.mpdpft_energy(self._h, ctypes.byref(r)))
+        return r
+
+    def enforce_constraints(self, mu):
+        """GetDensities + EnforceConstraints on the current potential from mu
+        (S values): (mu, N, status) with status OK or ENOBRACKET."""
+        m = np.ascontiguousarray(mu, dtype=np.float64).copy()
+        N = np.zeros(self.S)
+        rc = self._check(self._lib.mpdpft_enforce_constraints(self._h, _dp(m), _dp(N)), (_abi.OK, _abi.ENOBRACKET))
+        return m, N, rc
+
+    def mix(self, sc_count, n_out):
+        """AdmixDensities of n_out ([S][G]) into the current density
+        (mpdpft_mix): OK, or EPULAY when the step fell back to linear mixing."""
+        n = np.ascontiguousarray(n_out, dtype=np.float64).reshape(-1)
+        return self._check(self._lib.mpdpft_mix(self._h, int(sc_count), _dp(n)), (_abi.OK, _abi.EPULAY))
+
+    def inject_fault(self, kind, at):
+        """Forcing hook at SC iteration `at` (mpdpft_inject_fault); at < 0 clears."""
+        self._check(self._lib.mpdpft_inject_fault(self._h, int(kind), int(at)))
+
     def _field(self, fn):
         a = np.empty(self.S * self.G)
         self._check(fn(self._h, _dp(a)))
@@ -226,6 +261,16 @@ class Batch:
         bis = (ctypes.c_longlong * 2)()
         self._check(self._lib.mpdpft_batch_phase_stats(self._h, sec, its, bis))
         return (sec[0], sec[1]), (its[0], its[1]), (bis[0], bis[1])
+
+    def kernel_stats(self):
+        """(kernel_ms[2], algorithmic bytes[2], per-instance microseconds[n]) of the
+        last solve's persistent launches: [0] the FP32 loop, [1] the FP64 loop or
+        polish (HIP events on the batch's stream)."""
+        ms = (ctypes.c_double * 2)()
+        by = (ctypes.c_double * 2)()
+        us = np.zeros(self.n)
+        self._check(self._lib.mpdpft_batch_kernel_stats(self._h, ms, by, _dp(us)))
+        return (ms[0], ms[1]), (by[0], by[1]), us
 
     def _check(self, rc):
         if rc != _abi.OK:

@@ -62,7 +62,7 @@ class Result(ctypes.Structure):
         ("DispersalEnergies", _d * MAX_SPECIES), ("EnvironmentalEnergies", _d * MAX_SPECIES),
         ("InteractionEnergies", _d * MAX_LIST), ("AuxEkin", (_d * 4) * MAX_SPECIES),
         ("solve_seconds", _d), ("bisection_steps", _i), ("fp32_iterations", _i),
-        ("AuxEint", (_d * 4) * MAX_SPECIES),
+        ("AuxEint", (_d * 4) * MAX_SPECIES), ("bracket_failures", _i), ("pulay_rejections", _i),
     ]
 
 
@@ -83,19 +83,25 @@ EXPORTS = [
     "mpdpft_density", "mpdpft_run_dir", "mpdpft_enable_timing", "mpdpft_get_stage_stats",
     "mpdpft_import_v", "mpdpft_set_initial_potential", "mpdpft_set_break_file",
     "mpdpft_batch_create", "mpdpft_batch_solve", "mpdpft_batch_get_fields", "mpdpft_batch_seconds",
-    "mpdpft_batch_phase_stats", "mpdpft_batch_last_error", "mpdpft_batch_destroy",
+    "mpdpft_batch_phase_stats", "mpdpft_batch_kernel_stats", "mpdpft_batch_last_error", "mpdpft_batch_destroy",
     "mpdpft_adjoint", "mpdpft_centre_gradient", "mpdpft_centre_forces", "mpdpft_set_centres",
+    "mpdpft_loop_begin", "mpdpft_step", "mpdpft_energy", "mpdpft_enforce_constraints", "mpdpft_mix",
+    "mpdpft_inject_fault",
 ]
 BATCH_FP32 = 1
+FAULT_DUPLICATE_PULAY = 1
+FAULT_NAN_POTENTIAL = 2
 
 _lib = None
 
 
-def load_library(path=LIB_PATH):
-    """Load libmpdpft.so; raises OSError when it has not been built."""
+def load_library(path=None):
+    """Load libmpdpft.so (MPDPFT_LIBRARY names another build of it); raises
+    OSError when it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("MPDPFT_LIBRARY") or LIB_PATH
     if not os.path.exists(path):
         raise OSError("libmpdpft.so not built (%s); run mpdpft_amd/build.py" % path)
     lib = ctypes.CDLL(path)
@@ -155,6 +161,8 @@ def load_library(path=LIB_PATH):
     lib.mpdpft_batch_seconds.restype = ctypes.c_double
     lib.mpdpft_batch_phase_stats.argtypes = [vp, vp, vp, vp]
     lib.mpdpft_batch_phase_stats.restype = ctypes.c_int
+    lib.mpdpft_batch_kernel_stats.argtypes = [vp, vp, vp, vp]
+    lib.mpdpft_batch_kernel_stats.restype = ctypes.c_int
     lib.mpdpft_batch_last_error.argtypes = [vp]
     lib.mpdpft_batch_last_error.restype = ctypes.c_char_p
     lib.mpdpft_batch_destroy.argtypes = [vp]
@@ -167,6 +175,18 @@ def load_library(path=LIB_PATH):
     lib.mpdpft_centre_forces.restype = _i
     lib.mpdpft_set_centres.argtypes = [vp, _i, dp]
     lib.mpdpft_set_centres.restype = _i
+    lib.mpdpft_loop_begin.argtypes = [vp]
+    lib.mpdpft_loop_begin.restype = _i
+    lib.mpdpft_step.argtypes = [vp, R, ctypes.POINTER(_i)]
+    lib.mpdpft_step.restype = _i
+    lib.mpdpft_energy.argtypes = [vp, R]
+    lib.mpdpft_energy.restype = _i
+    lib.mpdpft_enforce_constraints.argtypes = [vp, dp, dp]
+    lib.mpdpft_enforce_constraints.restype = _i
+    lib.mpdpft_mix.argtypes = [vp, _i, dp]
+    lib.mpdpft_mix.restype = _i
+    lib.mpdpft_inject_fault.argtypes = [vp, _i, _i]
+    lib.mpdpft_inject_fault.restype = _i
     if lib.mpdpft_params_sizeof() != ctypes.sizeof(Params):
         raise OSError("mpdpft_params layout mismatch: C %d vs ctypes %d"
                       % (lib.mpdpft_params_sizeof(), ctypes.sizeof(Params)))

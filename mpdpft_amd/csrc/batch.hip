@@ -13,8 +13,10 @@
 // longer than the number of resident workgroups.
 //
 // Fields of the running instance live in the workgroup's slot; per-slot
-// storage is Env, V, OldV, Den, OldDen and the Pulay history (n_in and the
-// residual per history entry), [S][G] each.  Field storage type T is double
+// storage is Env, V, OldV and the Pulay history (n_in and the residual per
+// history entry, plus the slot that holds the current mixed density), [S][G]
+// each.  An iteration's unmixed density is never stored: it is the TF density
+// of V at the evaluated mu, recomputed where it is used.  Field storage type T is double
 // (FP64 path) or float (FP32 bulk: integrals, mu, Pulay Gram matrix, QR and
 // energies stay FP64); an FP32 solve ends with an FP64 polish from its result.
 #include <hip/hip_runtime.h>
@@ -22,11 +24,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "mpdpft.h"
+
+const char *params_structure_error(const mpdpft_params *p);  // params.cpp
 
 namespace {
 
@@ -35,10 +40,15 @@ constexpr int kMaxScope = 16;   // Pulay history (entries) in the batch
 constexpr int kBS = 2;          // species per instance (the species of a field index is idx >= G)
 constexpr double kMPb = 1.0e-12;
 constexpr int kWork = 5 * (kMaxScope + 1) * (kMaxScope + 1);  // >= row + A + qr + inv + vectors + coef + flag
+constexpr int kGC = 8;   // stored Pulay entries dotted per sweep of the new residual
+constexpr int kMixC = 8; // Pulay entries combined per load batch of the mixing
+constexpr int kUA = 2;   // grid points per lane visit in sweep A
+constexpr int kUB = 1;   // grid points per lane visit in sweep B (more costs registers and time)
 
 struct BInst {
     int D, S, EL, env[kBS];
     long long G;
+    double invEL;              // 1 / EL (index arithmetic)
     double lat0, del, trapD;   // x_i = lat0 + del i; integral factor del^D
     double c0[kBS], c1[kBS];   // environment parameters
     double pref;               // TF prefactor
@@ -49,13 +59,18 @@ struct BInst {
 };
 
 struct BOut {
-    int SCcount, converged, LinMix, PulMix, errorCount, warningCount, bisection, pad;
+    int SCcount, converged, LinMix, PulMix, errorCount, warningCount, bisection, bracket_failures;
+    int pulay_rejections, pad;
     double CC, DMM, Etot, Eint;
     double mu[kBS], N[kBS], Edisp[kBS], Eenv[kBS], Aux0[kBS];
+    double bytes;                 // algorithmic bytes the instance's loop moved (each field access once)
+    unsigned long long t0, t1;    // s_memrealtime (100 MHz) when the instance was taken and retired
+    unsigned long long tph[4];    // time in sweep A, the mu search, sweep B, the Pulay solve (100 MHz ticks)
 };
 
 template <typename T> struct BArgs {
     const BInst *inst;
+    const int *order;     // queue order of the instances
     int n;
     int *next;            // queue head (atomic)
     T *slots;             // per slot: (5 + 2 (scope + 1)) S Gmax values
@@ -70,34 +85,81 @@ template <typename T> struct BArgs {
 };
 
 // ---- block primitives (deterministic, result broadcast to all threads) ----
+// Wave level: quad and row butterflies (DPP), then the four row sums in a fixed
+// order through readlane (wave-uniform result); block level: the four wave
+// values through LDS, combined in wave order.  Two barriers per reduction.
+template <int CTRL>
+__device__ __forceinline__ double bdpp(double v) {
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double blane(double v, int l) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+// A wave-uniform double moved to scalar registers (the value is the same in
+// every lane; the compiler then keeps what is derived from it scalar).
+__device__ __forceinline__ double uni(double v) {
+    int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wsum(double v) {
+    v += bdpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += bdpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += bdpp<0x141>(v);  // row_half_mirror
+    v += bdpp<0x140>(v);  // row_mirror
+    return (blane(v, 0) + blane(v, 16)) + (blane(v, 32) + blane(v, 48));
+}
+__device__ __forceinline__ double wmax(double v) {
+    v = fmax(v, bdpp<0xB1>(v));
+    v = fmax(v, bdpp<0x4E>(v));
+    v = fmax(v, bdpp<0x141>(v));
+    v = fmax(v, bdpp<0x140>(v));
+    return fmax(fmax(blane(v, 0), blane(v, 16)), fmax(blane(v, 32), blane(v, 48)));
+}
+constexpr int kWaves = kBT / 64;
+static_assert(kWaves == 4, "the block combine below is written for four wavefronts");
 __device__ __forceinline__ double bsum(double v, double *sh) {
-    const int t = threadIdx.x;
+    v = wsum(v);
+    __syncthreads();  // sh is free (the previous reduction's readers are done)
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    sh[t] = v;
-    __syncthreads();
-    for (int w = kBT / 2; w > 0; w >>= 1) {
-        if (t < w) sh[t] += sh[t + w];
-        __syncthreads();
-    }
-    return sh[0];
+    return uni((sh[0] + sh[1]) + (sh[2] + sh[3]));
 }
 __device__ __forceinline__ double bmax(double v, double *sh) {
-    const int t = threadIdx.x;
+    v = wmax(v);
     __syncthreads();
-    sh[t] = v;
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int w = kBT / 2; w > 0; w >>= 1) {
-        if (t < w) sh[t] = fmax(sh[t], sh[t + w]);
-        __syncthreads();
+    return uni(fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3])));
+}
+// n <= NV sums at once (the same order per value as bsum)
+template <int NV>
+__device__ __forceinline__ void bsum_n(double (&v)[NV], int n, double *sh) {
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        if (k < n) v[k] = wsum(v[k]);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            if (k < n) sh[w * NV + k] = v[k];
     }
-    return sh[0];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        if (k < n) v[k] = uni((sh[k] + sh[NV + k]) + (sh[2 * NV + k] + sh[3 * NV + k]));
 }
 
 // Visit the points i = threadIdx.x + k kBT < n in increasing k, kU at a time:
 // the kU loads of a step are issued before their values are used, so a lane
 // keeps kU loads in flight (one workgroup per instance is latency bound
 // otherwise).  Per-lane accumulation order is that of the plain strided loop.
-constexpr int kU = 8;
+constexpr int kU = 16;
 template <class Ld, class Use>
 __device__ __forceinline__ void stream(long long n, Ld ld, Use use) {
     long long i = threadIdx.x;
@@ -122,13 +184,24 @@ __device__ __forceinline__ double dEXP(double x) {  // EXP, mpDPFT.h:266
 }
 __device__ __forceinline__ double dSign(double x) { return x < 0. ? -1. : 1.; }
 
-// Grid indices fit 32 bits (G <= 257^3 < 2^32): 32-bit division only.
-__device__ __forceinline__ double trapw(const BInst &in, long long i) {
-    double w = 1.;
-    unsigned rem = (unsigned)i;
+// Exact i / d for 32-bit i through the FP64 reciprocal inv = 1/d, corrected
+// by one (the product can fall one below or above the quotient).
+__device__ __forceinline__ unsigned udiv(unsigned i, unsigned d, double inv) {
+    unsigned q = (unsigned)((double)i * inv);
+    const unsigned r = i - q * d;
+    if ((int)r < 0) q--;
+    else if (r >= d) q++;
+    return q;
+}
+
+// Trapezoid weight of grid point i (Riemann :7981-8026): 1/2 per axis on
+// which the point sits on the first or last lattice plane.
+__device__ __forceinline__ double trapw(const BInst &in, unsigned i) {
     const unsigned EL = (unsigned)in.EL;
+    double w = 1.;
+    unsigned rem = i;
     for (int d = 0; d < in.D; d++) {
-        unsigned q = rem / EL, c = rem - q * EL;
+        const unsigned q = udiv(rem, EL, in.invEL), c = rem - q * EL;
         rem = q;
         if (c == 0 || c == EL - 1) w *= 0.5;
     }
@@ -154,14 +227,15 @@ __device__ __forceinline__ void coords(const BInst &in, long long i, double *r) 
 struct Loop {
     double mu[kBS], PrevMu[kBS], OldMu[kBS], TmpAb[kBS], OldAb[kBS], theta[kBS], muEval[kBS];
     double DMM, RelAcc, incV, CC, OldCC, best;
-    int SCcount, ExitSCcount, ExitSCloop, LinMix, PulMix, TrappedQ, warn, err, bis, pI, stall;
+    double thMix[kBS], muMix[kBS];  // theta and mu_eval of the iteration whose mixing is pending
+    double bytes;  // algorithmic bytes: every field a stage reads or writes, counted once per access
+    int SCcount, ExitSCcount, ExitSCloop, LinMix, PulMix, TrappedQ, warn, err, bis, pI, stall, nobr, prej;
 };
 
 template <typename T> struct Inst {
     const BInst &in;
-    T *Env, *V, *OldV, *Den, *OldDen, *nin, *res;  // nin/res: [scope + 1][S][G]
+    T *Env, *V, *OldV, *nin, *res;                 // nin/res: [scope + 1][S][G]
     double *sh;                                    // LDS scratch (kBT doubles)
-    double *M;                                     // LDS Gram cache [(kMaxScope+1)^2]
     double *W;                                     // LDS work area [kWork] (Gram row, matrix, QR, coefficients)
     double floor_rel;                              // reduced-precision stop floor (0: the criterion as given)
     int stall_max;                                 // reduced-precision stall exit (0: none)
@@ -183,20 +257,9 @@ template <typename T> struct Inst {
             });
         return bsum(acc, sh) * in.trapD;
     }
-    __device__ void writeDen(int s, double mu) {
-        T *n = Den + (long long)s * in.G;
-        const T *v = V + (long long)s * in.G;
-        const double pref = in.pref;
-        const bool d2 = in.D == 2;
-        stream(
-            in.G, [&](long long i) { return (double)v[i]; },
-            [&](long long i, double vi) {
-                double t = mu - vi;
-                n[i] = (T)(t > 0. ? (d2 ? pref * t : pref * t * sqrt(t)) : 0.);
-            });
-    }
     __device__ double N_at(Loop &L, int s, double mu) {
         L.muEval[s] = mu;
+        L.bytes += (double)in.G * sizeof(T);
         return evalN(s, mu);
     }
     __device__ double N_eval(Loop &L, int s, double mu) {  // a density evaluation of the bisection
@@ -274,10 +337,10 @@ template <typename T> struct Inst {
                     }
                 }
                 if (L.TmpAb[s] >= target) hiF[s] = true;
-                if (!hiF[s] || !loF[s]) {
+                if (!hiF[s] || !loF[s]) {  // :4657-4661: printed there, not a warning
                     L.mu[s] = L.PrevMu[s];
                     abortQ = true;
-                    L.warn++;
+                    L.nobr++;
                 }
             }
             if (!abortQ) {
@@ -341,42 +404,306 @@ template <typename T> struct Inst {
             if (!isfinite(L.mu[s])) L.mu[s] = L.PrevMu[s];
     }
 
-    // GetDensities (:3306-3316): N at the current mu, EnforceConstraints, and
-    // the density field of the last evaluated mu
-    __device__ void densities(Loop &L) {
-        for (int s = 0; s < in.S; s++) L.TmpAb[s] = N_at(L, s, L.mu[s]);
-        enforce(L);
-        for (int s = 0; s < in.S; s++) writeDen(s, L.muEval[s]);
-        __syncthreads();
+    // TF density at one point (GetTFdensity :4415-4456): the value that enters
+    // the particle-number integral, and the value a stored density field holds
+    __device__ __forceinline__ double tfval(double mu, double v) const {
+        const double t = mu - v;
+        return t > 0. ? (in.D == 2 ? in.pref * t : in.pref * t * sqrt(t)) : 0.;
+    }
+    __device__ __forceinline__ double tfT(double mu, double v) const { return (double)(T)tfval(mu, v); }
+
+    // Pulay combination of NP history entries (element offsets off[q] of
+    // their slots) into the visit's accumulators: acc += c_q (n_in,q + theta
+    // R_q).  Every load of the chunk is issued before the first use.
+    template <int NS, int NP>
+    __device__ __forceinline__ void mix_chunk(double (&acc)[kUA][kBS], const unsigned (&ii)[kUA], const unsigned *off,
+                                              const double *c, double th0, double th1) const {
+        const unsigned G = (unsigned)in.G;
+        T xn[NP][kUA][NS], xr[NP][kUA][NS];
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+#pragma unroll
+            for (int u = 0; u < kUA; u++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    xn[q][u][s] = nin[off[q] + s * G + ii[u]];
+                    xr[q][u][s] = res[off[q] + s * G + ii[u]];
+                }
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+#pragma unroll
+            for (int u = 0; u < kUA; u++)
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+                    acc[u][s] += c[q] * ((double)xn[q][u][s] + (s == 0 ? th0 : th1) * (double)xr[q][u][s]);
+    }
+    template <int NS>
+    __device__ __forceinline__ void mix_any(int np, double (&acc)[kUA][kBS], const unsigned (&ii)[kUA],
+                                            const unsigned *off, const double *c, double th0, double th1) const {
+        switch (np) {
+        case 1: mix_chunk<NS, 1>(acc, ii, off, c, th0, th1); break;
+        case 2: mix_chunk<NS, 2>(acc, ii, off, c, th0, th1); break;
+        case 3: mix_chunk<NS, 3>(acc, ii, off, c, th0, th1); break;
+        case 4: mix_chunk<NS, 4>(acc, ii, off, c, th0, th1); break;
+        case 5: mix_chunk<NS, 5>(acc, ii, off, c, th0, th1); break;
+        case 6: mix_chunk<NS, 6>(acc, ii, off, c, th0, th1); break;
+        case 7: mix_chunk<NS, 7>(acc, ii, off, c, th0, th1); break;
+        case 8: mix_chunk<NS, 8>(acc, ii, off, c, th0, th1); break;
+        default: break;
+        }
     }
 
-    // UpdateV (:4815-4838): V_s = incV (beta sum_{s' != s} n_s') + Env_s from
-    // the mixed density (OldDen after the swap)
-    __device__ void updateV(Loop &L) {
-        const long long SGn = SG(), G = in.G;
-        const bool pair = in.contact && in.S == 2;  // S <= 2: the other species' density
-        const double beta = in.beta, incV = L.incV;
-        stream(
-            SGn,
-            [&](long long idx) {
-                Pair x;
-                x.a = (double)Env[idx];
-                x.b = pair ? (double)OldDen[idx >= G ? idx - G : idx + G] : 0.;
-                return x;
-            },
-            [&](long long idx, Pair x) {
-                double acc = 0.;
-                if (in.contact) acc += x.b * beta;
-                if (incV < 1.) acc *= incV;
-                V[idx] = (T)(acc + x.a);
-            });
-        __syncthreads();
-        if (L.SCcount > 0) L.incV += in.incV0;
-        L.incV = fmin(1., L.incV);
+    // Sweep A of an iteration, kUA grid points per lane visit (both species
+    // of a point in the same lane): the previous iteration's mixed density m
+    // (AdmixDensities :4921-4992: none at the first iteration, where m is the
+    // initial density; linear theta n + (1 - theta) m_old with n = TF(V_old,
+    // mu of that iteration); or the Pulay combination sum_p c_p (n_in,p +
+    // theta R_p) over the history entries from the ring head), written to
+    // the history slot mnew unless mix == 0; then -- unless this is the
+    // loop's exit -- UpdateV (:4815-4838) V_s = incV beta m_{s'} + Env_s, the
+    // particle numbers N_s(mu_s) of the TF density of V (GetDensities' first
+    // evaluation, Riemann :7981-8026) and max|V - V_old|, max|V|
+    // (CheckConvergence in potential mode).  Every load of a visit is issued
+    // before its values are used (the tail visit clamps its indices), so a
+    // lane keeps all of them in flight.
+    template <int NS>
+    __device__ void sweepA_t(Loop &L, int mix, int mold_slot, int mnew_slot, int head, const double *coef,
+                             bool update, double *N, double &dvmax, double &vmax) {
+        const unsigned G = (unsigned)in.G;
+        const long long SGn = SG();
+        const int nslot = in.scope + 1, pI = L.pI;
+        const T *mold = nin + (long long)mold_slot * SGn;
+        T *mnew = nin + (long long)mnew_slot * SGn;
+        const bool pair = in.contact && NS == 2;
+        const double beta = in.beta, incV = L.incV, th0 = L.thMix[0], th1 = L.thMix[kBS - 1];
+        const bool need_vold = update || mix == 1;
+        const int np0 = mix == 2 ? min(pI, kMixC) : 0, np1 = mix == 2 ? pI - np0 : 0;
+        unsigned off0[kMixC], off1[kMixC];
+#pragma unroll
+        for (int q = 0; q < kMixC; q++) {
+            off0[q] = (unsigned)(((head + q) % nslot) * SGn);
+            off1[q] = (unsigned)(((head + kMixC + q) % nslot) * SGn);
+        }
+        double n0 = 0., n1 = 0., dv = 0., vm = 0.;
+        for (unsigned i0 = threadIdx.x; i0 < G; i0 += kUA * kBT) {
+            unsigned ii[kUA];
+            bool ok[kUA];
+#pragma unroll
+            for (int u = 0; u < kUA; u++) {
+                const unsigned i = i0 + u * kBT;
+                ok[u] = i < G;
+                ii[u] = ok[u] ? i : G - 1;
+            }
+            double vo[kUA][kBS] = {}, m[kUA][kBS] = {}, env[kUA][kBS] = {};
+            if (need_vold) {
+#pragma unroll
+                for (int u = 0; u < kUA; u++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) vo[u][s] = (double)OldV[s * G + ii[u]];
+            }
+            if (mix != 2) {
+#pragma unroll
+                for (int u = 0; u < kUA; u++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) m[u][s] = (double)mold[s * G + ii[u]];
+            }
+            if (update) {
+#pragma unroll
+                for (int u = 0; u < kUA; u++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) env[u][s] = (double)Env[s * G + ii[u]];
+            }
+            if (mix == 1) {
+#pragma unroll
+                for (int u = 0; u < kUA; u++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) {
+                        const double th = s == 0 ? th0 : th1;
+                        m[u][s] = (double)(T)(th * tfT(L.muMix[s], vo[u][s]) + (1. - th) * m[u][s]);
+                    }
+            } else if (mix == 2) {
+                double acc[kUA][kBS] = {};
+                mix_any<NS>(np0, acc, ii, off0, coef, th0, th1);
+                if (np1 > 0) mix_any<NS>(np1, acc, ii, off1, coef + kMixC, th0, th1);
+#pragma unroll
+                for (int u = 0; u < kUA; u++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) m[u][s] = (double)(T)acc[u][s];
+            }
+            if (mix != 0) {
+#pragma unroll
+                for (int u = 0; u < kUA; u++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++)
+                        if (ok[u]) mnew[s * G + ii[u]] = (T)m[u][s];
+            }
+            if (update) {
+#pragma unroll
+                for (int u = 0; u < kUA; u++) {
+                    if (!ok[u]) continue;
+                    const double w = trapw(in, ii[u]);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) {
+                        double acc = 0.;
+                        if (in.contact) acc += (pair ? m[u][1 - s] : 0.) * beta;
+                        if (incV < 1.) acc *= incV;
+                        const T vT = (T)(acc + env[u][s]);
+                        V[s * G + ii[u]] = vT;
+                        const double v = (double)vT;
+                        const double nv = w * tfval(L.mu[s], v);
+                        if (s == 0) n0 += nv;
+                        else n1 += nv;
+                        dv = fmax(dv, fabs(v - vo[u][s]));
+                        vm = fmax(vm, fabs(v));
+                    }
+                }
+            }
+        }
+        // algorithmic bytes of the sweep
+        double fields = 0.;
+        if (mix == 1) fields += 2.;                       // V_old, m_old
+        else if (mix == 2) fields += 2. * (double)pI;    // n_in and R of every entry
+        else if (update) fields += 1.;                   // m (first iteration)
+        if (mix != 0) fields += 1.;                      // m written
+        if (update) fields += (mix == 1 ? 0. : 1.) + 2.;  // V_old (if not read above), Env, V written
+        L.bytes += fields * (double)SGn * sizeof(T);
+        if (update) {
+            double v[3] = {n0, n1, 0.};
+            bsum_n<3>(v, 2, sh);
+            N[0] = v[0] * in.trapD;
+            N[1] = v[1] * in.trapD;
+            dvmax = bmax(dv, sh);
+            vmax = bmax(vm, sh);
+        }
+    }
+    __device__ void sweepA(Loop &L, int mix, int mold_slot, int mnew_slot, int head, const double *coef,
+                           bool update, double *N, double &dvmax, double &vmax) {
+        if (in.S == 2) sweepA_t<2>(L, mix, mold_slot, mnew_slot, head, coef, update, N, dvmax, vmax);
+        else sweepA_t<1>(L, mix, mold_slot, mnew_slot, head, coef, update, N, dvmax, vmax);
     }
 
-    // CheckConvergence (:4759-4782) with RelDev (:4784-4802)
-    __device__ void convergence(Loop &L) {
+    // Sweep B after the mu search, kUB grid points per lane visit: the density
+    // n = TF(V, mu_eval) (not stored), the density-mode overlaps <n,n>, <m,m>,
+    // <n,m> per species (CheckConvergence :4759-4782 with RelDev :4784-4802)
+    // and, when Pulay mixing runs, the push of (n_in = m, R = n - m) into
+    // history slot ps (m already lives there) with the new Gram row
+    // <R_new, R_j> over NC retained entries (PulayMixer :5005-5167); every
+    // load of a visit (V, m and the NC stored residuals) is issued first.  With
+    // more than kGC retained entries the remaining chunks of kGC re-read R_new
+    // (FIRST false).  acc: [0, kGC) the row, kGC <R_new, R_new>, then the
+    // overlaps.
+    template <int NS, int NC, bool FIRST>
+    __device__ void sweepB_t(Loop &L, bool push, const T *m, T *r, const unsigned *offj, double *acc) {
+        const unsigned G = (unsigned)in.G;
+        const bool dens_mode = in.convmode != 1;
+        for (unsigned i0 = threadIdx.x; i0 < G; i0 += kUB * kBT) {
+            unsigned ii[kUB];
+            bool ok[kUB];
+#pragma unroll
+            for (int u = 0; u < kUB; u++) {
+                const unsigned i = i0 + u * kBT;
+                ok[u] = i < G;
+                ii[u] = ok[u] ? i : G - 1;
+            }
+            T vv[kUB][NS], mv[kUB][NS], x[NC > 0 ? NC : 1][kUB][NS];
+#pragma unroll
+            for (int u = 0; u < kUB; u++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    if (FIRST) {
+                        vv[u][s] = V[s * G + ii[u]];
+                        mv[u][s] = m[s * G + ii[u]];
+                    } else
+                        vv[u][s] = r[s * G + ii[u]];
+                }
+#pragma unroll
+            for (int j = 0; j < NC; j++)
+#pragma unroll
+                for (int u = 0; u < kUB; u++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) x[j][u][s] = res[offj[j] + s * G + ii[u]];
+#pragma unroll
+            for (int u = 0; u < kUB; u++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    double rv;
+                    if (FIRST) {
+                        const double nv = tfT(L.muEval[s], (double)vv[u][s]), mm = (double)mv[u][s];
+                        const T rT = (T)(nv - mm);
+                        rv = (double)rT;
+                        if (push && ok[u]) r[s * G + ii[u]] = rT;
+                        if (dens_mode && ok[u]) {
+                            acc[kGC + 1 + 3 * s] += nv * nv;
+                            acc[kGC + 2 + 3 * s] += mm * mm;
+                            acc[kGC + 3 + 3 * s] += nv * mm;
+                        }
+                        if (push && ok[u]) acc[kGC] += rv * rv;
+                    } else
+                        rv = (double)vv[u][s];
+                    if (ok[u])
+#pragma unroll
+                        for (int j = 0; j < NC; j++) acc[j] += rv * (double)x[j][u][s];
+                }
+        }
+    }
+    template <int NS, bool FIRST>
+    __device__ void sweepB_nc(int nc, Loop &L, bool push, const T *m, T *r, const unsigned *offj, double *acc) {
+        switch (nc) {
+        case 0: sweepB_t<NS, 0, FIRST>(L, push, m, r, offj, acc); break;
+        case 1: sweepB_t<NS, 1, FIRST>(L, push, m, r, offj, acc); break;
+        case 2: sweepB_t<NS, 2, FIRST>(L, push, m, r, offj, acc); break;
+        case 3: sweepB_t<NS, 3, FIRST>(L, push, m, r, offj, acc); break;
+        case 4: sweepB_t<NS, 4, FIRST>(L, push, m, r, offj, acc); break;
+        case 5: sweepB_t<NS, 5, FIRST>(L, push, m, r, offj, acc); break;
+        case 6: sweepB_t<NS, 6, FIRST>(L, push, m, r, offj, acc); break;
+        case 7: sweepB_t<NS, 7, FIRST>(L, push, m, r, offj, acc); break;
+        default: sweepB_t<NS, 8, FIRST>(L, push, m, r, offj, acc); break;
+        }
+    }
+    // Results: row[0..I-1] (thread 0), ov[3 s + {0,1,2}] (every thread).
+    __device__ void sweepB(Loop &L, bool push, int ps, int head, int I, double *row, double *ov) {
+        const long long SGn = SG();
+        const int nslot = in.scope + 1, nOld = push ? I - 1 : 0;
+        const bool dens_mode = in.convmode != 1;
+        const T *m = nin + (long long)ps * SGn;
+        T *r = res + (long long)ps * SGn;
+        for (int j0 = 0; j0 == 0 || j0 < nOld; j0 += kGC) {
+            const int nc = min(kGC, nOld - j0);
+            unsigned offj[kGC];
+#pragma unroll
+            for (int j = 0; j < kGC; j++) offj[j] = (unsigned)(((head + j0 + j) % nslot) * SGn);
+            double acc[kGC + 1 + 3 * kBS];
+#pragma unroll
+            for (int j = 0; j < kGC + 1 + 3 * kBS; j++) acc[j] = 0.;
+            if (j0 == 0) {
+                if (in.S == 2) sweepB_nc<2, true>(nc, L, push, m, r, offj, acc);
+                else sweepB_nc<1, true>(nc, L, push, m, r, offj, acc);
+            } else {
+                if (in.S == 2) sweepB_nc<2, false>(nc, L, push, m, r, offj, acc);
+                else sweepB_nc<1, false>(nc, L, push, m, r, offj, acc);
+            }
+            if (j0 == 0) {
+                bsum_n<kGC + 1 + 3 * kBS>(acc, kGC + 1 + 3 * kBS, sh);
+                if (threadIdx.x == 0) {
+                    for (int j = 0; j < nc; j++) row[j] = acc[j];
+                    if (push) row[nOld] = acc[kGC];
+                }
+                for (int k = 0; k < 3 * kBS; k++) ov[k] = acc[kGC + 1 + k];
+                // V and m read; R written and the stored residuals read when pushing
+                L.bytes += ((push ? 1. + (double)nc : 0.) + ((push || dens_mode) ? 2. : 0.)) * (double)SGn * sizeof(T);
+            } else {
+                bsum_n<kGC + 1 + 3 * kBS>(acc, nc, sh);
+                if (threadIdx.x == 0)
+                    for (int j = 0; j < nc; j++) row[j0 + j] = acc[j];
+                L.bytes += (1. + (double)nc) * (double)SGn * sizeof(T);
+            }
+        }
+    }
+
+    // CheckConvergence (:4759-4782): the exit counter from this iteration's
+    // CC (potential mode: max|V - V_old|; density mode: max_s RelDev weighted
+    // by the species' share of N)
+    __device__ void convergence(Loop &L, double dvmax, double vmax, const double *ov) {
         const double mp = 100. * 1.0e-16;
         if (L.SCcount > 0) {
             L.CC = 0.;
@@ -384,42 +711,12 @@ template <typename T> struct Inst {
             for (int s = 0; s < in.S; s++) tot += L.TmpAb[s];
             double crit = in.sccrit;
             if (in.convmode == 1) {
-                double m = 0., vm = 0.;
-                stream(
-                    SG(),
-                    [&](long long idx) {
-                        Pair x;
-                        x.a = (double)V[idx];
-                        x.b = (double)OldV[idx];
-                        return x;
-                    },
-                    [&](long long, Pair x) {
-                        m = fmax(m, fabs(x.a - x.b));
-                        vm = fmax(vm, fabs(x.a));
-                    });
-                L.CC = bmax(m, sh);
-                if (floor_rel > 0.) crit = fmax(crit, floor_rel * bmax(vm, sh));
+                L.CC = dvmax;
+                if (floor_rel > 0.) crit = fmax(crit, floor_rel * vmax);
             } else {
                 double maxtest = 0.;
                 for (int s = 0; s < in.S; s++) {
-                    const T *a = Den + (long long)s * in.G, *b = OldDen + (long long)s * in.G;
-                    double aa = 0., bb = 0., ab = 0.;
-                    stream(
-                        in.G,
-                        [&](long long i) {
-                            Pair x;
-                            x.a = (double)a[i];
-                            x.b = (double)b[i];
-                            return x;
-                        },
-                        [&](long long, Pair p) {
-                            aa += p.a * p.a;
-                            bb += p.b * p.b;
-                            ab += p.a * p.b;
-                        });
-                    aa = bsum(aa, sh);
-                    bb = bsum(bb, sh);
-                    ab = bsum(ab, sh);
+                    const double aa = ov[3 * s], bb = ov[3 * s + 1], ab = ov[3 * s + 2];
                     double rd = (aa + bb > kMPb) ? 1. - 2. * ab / (aa + bb) : 0.;
                     double test = rd * L.TmpAb[s] / tot;
                     if (test > maxtest) {
@@ -452,76 +749,6 @@ template <typename T> struct Inst {
             }
         } else
             L.CC = 1.2345678901234567890;
-    }
-
-    // PulayMixer, mixer 1, Euclidean metric (:5005-5167); history slots are
-    // addressed through a ring (logical entry k lives in physical slot
-    // (head + k) mod (scope + 1)).  Returns whether coefficients were found.
-    __device__ bool pulay(Loop &L, int &head, double *coef) {
-        const int scope = in.scope, ld = scope + 1, nslot = scope + 1;
-        const long long SGn = SG();
-        int I = L.pI;
-        {  // push n_in = OldDen, R = Den - OldDen into logical entry I
-            int ps = (head + I) % nslot;
-            T *a = nin + (long long)ps * SGn, *r = res + (long long)ps * SGn;
-            stream(
-                SGn,
-                [&](long long idx) {
-                    Pair x;
-                    x.a = (double)OldDen[idx];
-                    x.b = (double)Den[idx];
-                    return x;
-                },
-                [&](long long idx, Pair x) {
-                    a[idx] = (T)x.a;
-                    r[idx] = (T)(x.b - x.a);
-                });
-        }
-        I++;
-        int shift = 0;
-        if (I > scope) {  // drop the oldest entry
-            head = (head + 1) % nslot;
-            I--;
-            shift = 1;
-        }
-        L.pI = I;
-        __syncthreads();
-        // the new Gram row: dots of the newest residual with every stored one
-        double *row = W, *A = W + (kMaxScope + 1);
-        const T *rn = res + (long long)((head + I - 1) % nslot) * SGn;
-        for (int j = 0; j < I; j++) {
-            const T *rj = res + (long long)((head + j) % nslot) * SGn;
-            double acc = 0.;
-            stream(
-                SGn,
-                [&](long long idx) {
-                    Pair x;
-                    x.a = (double)rn[idx];
-                    x.b = (double)rj[idx];
-                    return x;
-                },
-                [&](long long, Pair x) { acc += x.a * x.b; });
-            acc = bsum(acc, sh);
-            if (threadIdx.x == 0) row[j] = acc;
-        }
-        __syncthreads();
-        // the Gram matrix from the cache (shifted when the oldest entry was dropped) and
-        // the coefficients, by thread 0; flag and coefficients broadcast through LDS
-        if (threadIdx.x == 0) {
-            for (int i = 0; i < I; i++)
-                for (int j = 0; j < I; j++) {
-                    double v;
-                    if (i == I - 1) v = row[j];
-                    else if (j == I - 1) v = row[i];
-                    else v = M[(i + shift) * ld + (j + shift)];
-                    A[i * I + j] = v;
-                }
-            for (int i = 0; i < I; i++)
-                for (int j = 0; j < I; j++) M[i * ld + j] = A[i * I + j];
-            W[kWork - 1] = solve_coeff(I, A, W + 2 * (kMaxScope + 1) * (kMaxScope + 1), coef) ? 1. : 0.;
-        }
-        __syncthreads();
-        return W[kWork - 1] != 0.;
     }
 
     // Full-pivoting Householder QR inverse (the decomposition Eigen's
@@ -644,67 +871,21 @@ template <typename T> struct Inst {
         return true;
     }
 
-    // AdmixDensities (:4921-4992)
-    __device__ void admix(Loop &L, int &head) {
-        const int S = in.S;
-        double nA = 0., nD = 0.;
-        for (int s = 0; s < S; s++) {
-            nD += (in.Ab[s] - L.TmpAb[s]) * (in.Ab[s] - L.TmpAb[s]);
-            nA += in.Ab[s] * in.Ab[s];
-        }
-        double rel = sqrt(nD) / sqrt(nA), mod = 1.;
-        if (rel > kMPb) mod = fmin(1., pow(fabs(1. - rel), 30.));
-        if (L.TrappedQ)
-            for (int s = 0; s < S; s++) L.theta[s] *= mod;
-        double *coef = W + kWork - 2 - (kMaxScope + 1);
-        bool ok = false;
-        if (in.mixer > 0 && L.SCcount > 0 && L.CC < 1.0e+300) ok = pulay(L, head, coef);
-        const long long SGn = SG();
-        if (ok) {
-            L.PulMix++;
-            const int nslot = in.scope + 1, pI = L.pI;
-            const double th0 = L.theta[0], th1 = L.theta[kBS - 1];
-            const long long G = in.G;
-            for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-                const double th = idx >= G ? th1 : th0;  // S <= 2
-                double acc = 0.;
-                int slot = head;  // physical slot of logical entry p
-#pragma unroll 4
-                for (int p = 0; p < pI; p++) {
-                    const long long off = (long long)slot * SGn + idx;
-                    acc += coef[p] * ((double)nin[off] + th * (double)res[off]);
-                    slot = slot + 1 == nslot ? 0 : slot + 1;
-                }
-                Den[idx] = (T)acc;
-            }
-        } else {
-            L.LinMix++;
-            const double th0 = L.theta[0], th1 = L.theta[kBS - 1];
-            const long long G = in.G;
-            stream(
-                SGn,
-                [&](long long idx) {
-                    Pair x;
-                    x.a = (double)Den[idx];
-                    x.b = (double)OldDen[idx];
-                    return x;
-                },
-                [&](long long idx, Pair x) {
-                    const double th = idx >= G ? th1 : th0;  // S <= 2
-                    Den[idx] = (T)(th * x.a + (1. - th) * x.b);
-                });
-        }
-        __syncthreads();
-        if (L.TrappedQ)
-            for (int s = 0; s < S; s++) L.theta[s] *= 1. / mod;
-    }
 };
 
 // One launch: each workgroup takes instances from the queue until it is empty.
+// Per SC iteration of an instance (GetData :210-225) the fields are swept
+// twice, plus one sweep per mu-bisection density evaluation: sweep A (the
+// previous iteration's mixing, UpdateV, N at the current mu, potential-mode
+// convergence), the mu search (EnforceConstraints), sweep B (the density,
+// density-mode convergence, Pulay push and Gram row).  The density n of an
+// iteration is never stored: it is TF(V, mu_eval), evaluated where it is
+// used; the mixed density m lives in the Pulay history slot that the next
+// push takes as its n_in.
 template <typename T>
 __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
     __shared__ double sh[kBT];
-    __shared__ double Mcache[(kMaxScope + 1) * (kMaxScope + 1)];
+    __shared__ double Gcache[(kMaxScope + 1) * (kMaxScope + 1)];
     __shared__ double Work[kWork];
     __shared__ int nextIdx;
     T *base = a.slots + (long long)blockIdx.x * a.slot_stride;
@@ -712,44 +893,34 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
         __syncthreads();
         if (threadIdx.x == 0) nextIdx = atomicAdd(a.next, 1);
         __syncthreads();
-        const int k = nextIdx;
-        if (k >= a.n) break;
+        if (__builtin_amdgcn_readfirstlane(nextIdx) >= a.n) break;
+        // queue position -> instance (largest first); uniform, so the instance's
+        // parameters are read with scalar loads into scalar registers
+        const int k = __builtin_amdgcn_readfirstlane(a.order[__builtin_amdgcn_readfirstlane(nextIdx)]);
+        const unsigned long long tstart = __builtin_amdgcn_s_memrealtime();
         const BInst &in = a.inst[k];
-        const long long SGn = (long long)in.S * in.G;
+        const long long SGn = (long long)in.S * in.G, G = in.G;
+        const int S = in.S, scope = in.scope, nslot = scope + 1, ld = scope + 1;
         Inst<T> I{in};
         I.Env = base;
         I.V = base + SGn;
         I.OldV = base + 2 * SGn;
-        I.Den = base + 3 * SGn;
-        I.OldDen = base + 4 * SGn;
-        I.nin = base + 5 * SGn;
-        I.res = I.nin + (long long)(in.scope + 1) * SGn;
+        I.nin = base + 3 * SGn;
+        I.res = I.nin + (long long)nslot * SGn;
         I.sh = sh;
-        I.M = Mcache;
         I.W = Work;
         I.floor_rel = a.floor_rel;
         I.stall_max = a.stall_max;
         I.relacc = fmax(in.RelAcc, a.relacc_floor);
         double *outDen = a.fields + in.out_off, *outV = outDen + SGn;
-        // InitializeEnvironments (:6362-6390): Env, and V <- Env (or the stored start)
-        for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-            int s = idx >= in.G ? 1 : 0;  // S <= 2
-            long long i = idx - (long long)s * in.G;
-            double r[3];
-            coords(in, i, r);
-            double v = 0.;
-            if (in.env[s] == 1) v = in.c0[s] * pow(r[0] * r[0] + r[1] * r[1] + r[2] * r[2], 1.);
-            else if (in.env[s] == 3) v = in.c0[s] * ((1. - in.c1[s]) * r[0] * r[0] + (1. + in.c1[s]) * r[1] * r[1]);
-            I.Env[idx] = (T)v;
-            I.V[idx] = (T)(a.polish ? a.start[in.out_off + SGn + idx] : v);
-        }
-        __syncthreads();
+        double *row = Work, *A = Work + (kMaxScope + 1), *coef = Work + kWork - 2 - (kMaxScope + 1);
         Loop L;
-        for (int s = 0; s < in.S; s++) {
+        for (int s = 0; s < S; s++) {
             L.mu[s] = a.polish ? a.out[k].mu[s] : in.mu0[s];
             L.theta[s] = in.theta0[s];
             L.TmpAb[s] = L.OldAb[s] = L.OldMu[s] = L.PrevMu[s] = 0.;
-            L.muEval[s] = L.mu[s];
+            L.muEval[s] = L.muMix[s] = L.mu[s];
+            L.thMix[s] = L.theta[s];
         }
         L.DMM = a.polish ? a.out[k].DMM : in.DMM0;
         L.RelAcc = I.relacc;
@@ -757,53 +928,182 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
         L.CC = L.OldCC = 0.;
         L.best = 1.0e+300;
         L.SCcount = L.ExitSCcount = L.ExitSCloop = L.LinMix = L.PulMix = L.TrappedQ = 0;
-        L.warn = L.err = L.bis = L.pI = L.stall = 0;
-        int head = 0;
-        I.densities(L);  // initial densities (:194-201)
+        L.warn = L.err = L.bis = L.pI = L.stall = L.nobr = L.prej = 0;
+        // InitializeEnvironments (:6362-6390): Env, and V <- Env (or the stored start); the
+        // particle numbers at the initial mu in the same sweep
+        double n0 = 0., n1 = 0.;
+        for (long long i = threadIdx.x; i < G; i += kBT) {
+            double r[3];
+            coords(in, i, r);
+            const double w = trapw(in, i);
+            for (int s = 0; s < S; s++) {
+                const long long idx = s * G + i;
+                double v = 0.;
+                if (in.env[s] == 1) v = in.c0[s] * pow(r[0] * r[0] + r[1] * r[1] + r[2] * r[2], 1.);
+                else if (in.env[s] == 3) v = in.c0[s] * ((1. - in.c1[s]) * r[0] * r[0] + (1. + in.c1[s]) * r[1] * r[1]);
+                I.Env[idx] = (T)v;
+                const T vT = (T)(a.polish ? a.start[in.out_off + SGn + idx] : v);
+                I.V[idx] = vT;
+                const double nv = w * I.tfval(L.mu[s], (double)vT);
+                if (s == 0) n0 += nv;
+                else n1 += nv;
+            }
+        }
+        L.bytes = 2. * (double)SGn * sizeof(T) + (a.polish ? (double)SGn * 8. : 0.);
+        {
+            double v[3] = {n0, n1, 0.};
+            bsum_n<3>(v, 2, sh);
+            for (int s = 0; s < S; s++) {
+                L.TmpAb[s] = v[s] * in.trapD;
+                L.bytes += (double)G * sizeof(T);  // counted as a density evaluation
+            }
+        }
+        // initial densities (:194-201): EnforceConstraints, then m = n(V, mu_eval) into slot 0
+        I.enforce(L);
+        int head = 0, mslot = 0;
+        {
+            T *m = I.nin;
+            for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
+                const int s = idx >= G ? 1 : 0;
+                m[idx] = (T)I.tfval(L.muEval[s], (double)I.V[idx]);
+            }
+            L.bytes += 2. * (double)SGn * sizeof(T);
+        }
+        __syncthreads();
+        int mix = 0, mold = 0;  // the pending mixing of the previous iteration (0: none)
+        unsigned long long tph[4] = {0, 0, 0, 0};
         while (!L.ExitSCloop && in.maxsc > 0) {
-            // StoreCurrentIteration (:4804): swap the field pointers
-            T *t = I.Den;
-            I.Den = I.OldDen;
-            I.OldDen = t;
-            t = I.V;
+            // StoreCurrentIteration (:4804): swap the potential buffers
+            T *t = I.V;
             I.V = I.OldV;
             I.OldV = t;
             L.OldCC = L.CC;
-            for (int s = 0; s < in.S; s++) {
+            for (int s = 0; s < S; s++) {
                 L.OldMu[s] = L.mu[s];
                 L.OldAb[s] = L.TmpAb[s];
             }
-            I.updateV(L);
-            I.densities(L);
-            I.convergence(L);
+            // sweep A: mixing of the previous iteration + UpdateV + N(mu) + max|dV|
+            double N[kBS] = {0., 0.}, dvmax = 0., vmax = 0.;
+            unsigned long long c0 = __builtin_amdgcn_s_memrealtime();
+            I.sweepA(L, mix, mold, mslot, head, coef, true, N, dvmax, vmax);
+            unsigned long long c1 = __builtin_amdgcn_s_memrealtime();
+            tph[0] += c1 - c0;
+            if (L.SCcount > 0) L.incV += in.incV0;
+            L.incV = fmin(1., L.incV);
+            // GetDensities (:3306-3316): N at the current mu, EnforceConstraints
+            for (int s = 0; s < S; s++) {
+                L.TmpAb[s] = N[s];
+                L.muEval[s] = L.mu[s];
+            }
+            I.enforce(L);
+            c0 = __builtin_amdgcn_s_memrealtime();
+            tph[1] += c0 - c1;
+            // Pulay bookkeeping ahead of the sweep (PulayMixer :5005-5050): the new entry takes
+            // slot mslot (its n_in, the mixed density, is already there); a full history drops
+            // its oldest entry
+            const bool tryPulay = in.mixer > 0 && L.SCcount > 0;
+            int pI = L.pI, nhead = head, shift = 0;
+            if (tryPulay) {
+                pI++;
+                if (pI > scope) {
+                    nhead = (head + 1) % nslot;
+                    pI--;
+                    shift = 1;
+                }
+            }
+            double ov[3 * kBS];
+            I.sweepB(L, tryPulay, mslot, nhead, pI, row, ov);
+            c1 = __builtin_amdgcn_s_memrealtime();
+            tph[2] += c1 - c0;
+            I.convergence(L, dvmax, vmax, ov);
             if (L.SCcount == in.maxsc) L.ExitSCloop = 1;  // Intercept
-            I.admix(L, head);
+            // AdmixDensities (:4921-4992): the mixing itself runs in the next sweep A
+            double nA = 0., nD = 0.;
+            for (int s = 0; s < S; s++) {
+                nD += (in.Ab[s] - L.TmpAb[s]) * (in.Ab[s] - L.TmpAb[s]);
+                nA += in.Ab[s] * in.Ab[s];
+            }
+            double rel = sqrt(nD) / sqrt(nA), mod = 1.;
+            if (rel > kMPb) mod = fmin(1., pow(fabs(1. - rel), 30.));
+            if (L.TrappedQ)
+                for (int s = 0; s < S; s++) L.theta[s] *= mod;
+            for (int s = 0; s < S; s++) {
+                L.thMix[s] = L.theta[s];
+                L.muMix[s] = L.muEval[s];
+            }
+            bool ok = false;
+            if (tryPulay && L.CC < 1.0e+300) {
+                head = nhead;
+                L.pI = pI;
+                // the Gram matrix from the cache (shifted when the oldest entry was dropped) and
+                // the coefficients, by thread 0; flag and coefficients broadcast through LDS
+                if (threadIdx.x == 0) {
+                    for (int i = 0; i < pI; i++)
+                        for (int j = 0; j < pI; j++) {
+                            double v;
+                            if (i == pI - 1) v = row[j];
+                            else if (j == pI - 1) v = row[i];
+                            else v = Gcache[(i + shift) * ld + (j + shift)];
+                            A[i * pI + j] = v;
+                        }
+                    for (int i = 0; i < pI; i++)
+                        for (int j = 0; j < pI; j++) Gcache[i * ld + j] = A[i * pI + j];
+                    Work[kWork - 1] =
+                        Inst<T>::solve_coeff(pI, A, Work + 2 * (kMaxScope + 1) * (kMaxScope + 1), coef) ? 1. : 0.;
+                }
+                __syncthreads();
+                ok = Work[kWork - 1] != 0.;
+                if (!ok) L.prej++;
+                tph[3] += __builtin_amdgcn_s_memrealtime() - c1;
+                mold = mslot;
+                mslot = (head + L.pI) % nslot;  // the free slot takes the next mixed density
+            } else
+                mold = mslot;  // linear mixing in place
+            if (ok) L.PulMix++;
+            else L.LinMix++;
+            mix = ok ? 2 : 1;
+            if (L.TrappedQ)
+                for (int s = 0; s < S; s++) L.theta[s] *= 1. / mod;
             L.SCcount++;  // PrepareNextIteration
         }
+        // the last iteration's mixing (the loop exits after AdmixDensities)
+        if (mix != 0) {
+            double N[kBS], dvmax, vmax;
+            T *t = I.V;  // sweep A reads the mixing's V_old from OldV
+            I.V = I.OldV;
+            I.OldV = t;
+            I.sweepA(L, mix, mold, mslot, head, coef, false, N, dvmax, vmax);
+            t = I.V;
+            I.V = I.OldV;
+            I.OldV = t;
+        }
+        __syncthreads();
+        const T *Den = I.nin + (long long)mslot * SGn;  // the mixed density
         // energies (GetEnergy :5414-5436): contact, environment, TF dispersal, AuxEkin[0]
         double Eint = 0.;
-        if (in.contact && in.S > 1) {
+        if (in.contact && S > 1) {
             double acc = 0.;
-            for (long long i = threadIdx.x; i < in.G; i += kBT)
-                acc += trapw(in, i) * (double)I.Den[i] * (double)I.Den[in.G + i];
+            for (long long i = threadIdx.x; i < G; i += kBT) acc += trapw(in, i) * (double)Den[i] * (double)Den[G + i];
             Eint = in.beta * bsum(acc, sh) * in.trapD;
         }
         BOut o{};
         double Etot = Eint;
         const double DD = (double)in.D;
-        for (int s = 0; s < in.S; s++) {
-            const T *n = I.Den + (long long)s * in.G, *v = I.V + (long long)s * in.G, *e = I.Env + (long long)s * in.G;
+        for (int s = 0; s < S; s++) {
+            const T *n = Den + (long long)s * G, *v = I.V + (long long)s * G, *e = I.Env + (long long)s * G;
             double ee = 0., ed = 0., ea = 0.;
-            for (long long i = threadIdx.x; i < in.G; i += kBT) {
+            for (long long i = threadIdx.x; i < G; i += kBT) {
                 double w = trapw(in, i), nn = (double)n[i];
                 ee += w * (double)e[i] * nn;
                 double tt = L.mu[s] - (double)v[i];
                 ed += w * (tt > 0. ? pow(tt, (DD + 2.) / 2.) : 0.);
                 ea += w * pow(nn > 0. ? nn : 0., (DD + 2.) / DD);
             }
-            ee = bsum(ee, sh) * in.trapD;
-            ed = bsum(ed, sh) * in.trapD;
-            ea = bsum(ea, sh) * in.trapD;
+            double v3[3] = {ee, ed, ea};
+            bsum_n<3>(v3, 3, sh);
+            ee = v3[0] * in.trapD;
+            ed = v3[1] * in.trapD;
+            ea = v3[2] * in.trapD;
             double aux0 = in.units > 0 ? in.aux0pref * ea : 0.;
             double disp = (in.ekin0 ? in.dispPref * ed : 0.) + (in.ekin1 ? aux0 : 0.);
             o.Eenv[s] = ee;
@@ -811,9 +1111,11 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             o.Aux0[s] = aux0;
             Etot += disp + ee;
         }
+        L.bytes += ((in.contact && S > 1) ? 2. * (double)G : 0.) * sizeof(T) + 3. * (double)SGn * sizeof(T);
         // retire: fields and results out
+        L.bytes += 2. * (double)SGn * sizeof(T) + 2. * (double)SGn * 8.;
         for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-            outDen[idx] = (double)I.Den[idx];
+            outDen[idx] = (double)Den[idx];
             outV[idx] = (double)I.V[idx];
         }
         if (threadIdx.x == 0) {
@@ -824,11 +1126,17 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             o.errorCount = L.err;
             o.warningCount = L.warn + (in.maxsc > 0 && L.SCcount >= in.maxsc ? 1 : 0);
             o.bisection = L.bis;
+            o.bracket_failures = L.nobr;
+            o.pulay_rejections = L.prej;
             o.CC = L.CC;
             o.DMM = L.DMM;
             o.Etot = Etot;
             o.Eint = Eint;
-            for (int s = 0; s < in.S; s++) {
+            o.bytes = L.bytes;
+            o.t0 = tstart;
+            o.t1 = __builtin_amdgcn_s_memrealtime();
+            for (int q = 0; q < 4; q++) o.tph[q] = tph[q];
+            for (int s = 0; s < S; s++) {
                 o.mu[s] = L.mu[s];
                 o.N[s] = L.TmpAb[s];
             }
@@ -839,6 +1147,7 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
 
 }  // namespace
 
+
 struct mpdpft_batch {
     int device = 0, n = 0, flags = 0, nslots = 0, scope = 0;
     long long Gmax = 0, arena = 0;
@@ -846,7 +1155,9 @@ struct mpdpft_batch {
     std::vector<mpdpft_params> params;
     BInst *d_inst = nullptr;
     BOut *d_out = nullptr;
-    int *d_next = nullptr;
+    int *d_next = nullptr, *d_order = nullptr;
+    hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [phase][start, stop] around k_batch_solve
+    double kernel_ms[2] = {0., 0.}, kernel_bytes[2] = {0., 0.};
     void *d_slots = nullptr;
     double *d_fields = nullptr, *d_start = nullptr;
     std::vector<BOut> out;
@@ -865,6 +1176,7 @@ static int bfail(mpdpft_batch *b, int rc, const std::string &m) {
 // InitializeHardCodedParameters :863-896, for the TF family).
 static int prepare(mpdpft_batch *b, const mpdpft_params &p, BInst &o, int i) {
     auto bad = [&](const std::string &m) { return bfail(b, MPDPFT_EINVAL, "instance " + std::to_string(i) + ": " + m); };
+    if (const char *why = params_structure_error(&p)) return bad(why);
     if (p.DIM != 2 && p.DIM != 3) return bad("DIM must be 2 or 3");
     if (p.S < 1 || p.S > kBS) return bad("S must be 1 or 2");
     if (p.DensityExpression != 7) return bad("the batch runs DensityExpression 7 (TF)");
@@ -887,6 +1199,7 @@ static int prepare(mpdpft_batch *b, const mpdpft_params &p, BInst &o, int i) {
     o.D = p.DIM;
     o.S = p.S;
     o.EL = p.steps + 1;
+    o.invEL = 1. / (double)o.EL;
     o.G = 1;
     for (int d = 0; d < o.D; d++) o.G *= o.EL;
     const double x1 = -p.edge / 2., x2 = p.edge / 2.;
@@ -946,14 +1259,10 @@ extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, in
     *out = b;
     b->n = n;
     b->flags = flags;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(b, MPDPFT_ENODEV, "no GPU visible");
-    b->device = device < 0 ? 0 : device;
-    BHIP(hipSetDevice(b->device));
     b->inst.resize(n);
     b->params.assign(p, p + n);
     long long off = 0;
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < n; i++) {  // every instance is checked before the device is touched
         int rc = prepare(b, p[i], b->inst[i], i);
         if (rc) return rc;
         b->inst[i].out_off = off;
@@ -962,6 +1271,10 @@ extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, in
         b->scope = std::max(b->scope, b->inst[i].scope);
     }
     b->arena = off;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(b, MPDPFT_ENODEV, "no GPU visible");
+    b->device = device < 0 ? 0 : device;
+    BHIP(hipSetDevice(b->device));
     // slots: resident workgroups; at most one per instance
     hipDeviceProp_t prop;
     BHIP(hipGetDeviceProperties(&prop, b->device));
@@ -973,9 +1286,21 @@ extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, in
     BHIP(hipMalloc((void **)&b->d_out, sizeof(BOut) * n));
     BHIP(hipMalloc((void **)&b->d_next, sizeof(int)));
     BHIP(hipMalloc((void **)&b->d_fields, sizeof(double) * (size_t)off));
-    const long long per = (5LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
+    const long long per = (3LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
     BHIP(hipMalloc(&b->d_slots, sizeof(double) * (size_t)per * b->nslots));
     BHIP(hipMemcpy(b->d_inst, b->inst.data(), sizeof(BInst) * n, hipMemcpyHostToDevice));
+    // queue order: largest instance first (S G decides the cost of an SC iteration), ties by index,
+    // so the longest solves start at once and the short ones fill in behind them
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    const char *ord = std::getenv("MPDPFT_BATCH_ORDER");  // "index": the instances' own order
+    if (!(ord && std::strcmp(ord, "index") == 0)) std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (long long)b->inst[x].S * b->inst[x].G > (long long)b->inst[y].S * b->inst[y].G;
+    });
+    BHIP(hipMalloc((void **)&b->d_order, sizeof(int) * n));
+    BHIP(hipMemcpy(b->d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    for (int ph = 0; ph < 2; ph++)
+        for (int e = 0; e < 2; e++) BHIP(hipEventCreate(&b->ev[ph][e]));
     b->out.resize(n);
     return MPDPFT_OK;
 }
@@ -990,10 +1315,11 @@ constexpr double kF32Floor = 1.0e-6;
 constexpr int kF32Stall = 16;
 constexpr double kF32RelAcc = 1.0e-8;
 
-template <typename T> static int run(mpdpft_batch *b, int polish) {
-    const long long per = (5LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
+template <typename T> static int run(mpdpft_batch *b, int polish, int phase) {
+    const long long per = (3LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
     BArgs<T> a;
     a.inst = b->d_inst;
+    a.order = b->d_order;
     a.n = b->n;
     a.next = b->d_next;
     a.slots = (T *)b->d_slots;
@@ -1007,8 +1333,10 @@ template <typename T> static int run(mpdpft_batch *b, int polish) {
     a.stall_max = sizeof(T) == 4 ? kF32Stall : 0;
     a.relacc_floor = sizeof(T) == 4 ? kF32RelAcc : 0.;
     BHIP(hipMemsetAsync(b->d_next, 0, sizeof(int), b->st));
+    BHIP(hipEventRecord(b->ev[phase][0], b->st));
     hipLaunchKernelGGL(k_batch_solve<T>, dim3(b->nslots), dim3(kBT), 0, b->st, a);
     BHIP(hipGetLastError());
+    BHIP(hipEventRecord(b->ev[phase][1], b->st));
     return MPDPFT_OK;
 }
 
@@ -1021,16 +1349,21 @@ extern "C" int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *res) {
     b->phase_seconds[0] = b->phase_seconds[1] = 0.;
     b->phase_iterations[0] = b->phase_iterations[1] = 0;
     b->phase_evaluations[0] = b->phase_evaluations[1] = 0;
-    auto count = [&](int ph) -> int {  // iterations of the phase just run (host copy of the results)
+    b->kernel_ms[0] = b->kernel_ms[1] = b->kernel_bytes[0] = b->kernel_bytes[1] = 0.;
+    auto count = [&](int ph) -> int {  // iterations, bytes and kernel time of the phase just run
         BHIP(hipMemcpy(b->out.data(), b->d_out, sizeof(BOut) * b->n, hipMemcpyDeviceToHost));
         for (int i = 0; i < b->n; i++) {
             b->phase_iterations[ph] += b->out[i].SCcount;
             b->phase_evaluations[ph] += b->out[i].bisection;
+            b->kernel_bytes[ph] += b->out[i].bytes;
         }
+        float ms = 0.f;
+        BHIP(hipEventElapsedTime(&ms, b->ev[ph][0], b->ev[ph][1]));
+        b->kernel_ms[ph] = ms;
         return MPDPFT_OK;
     };
     if (fp32) {
-        rc = run<float>(b, 0);
+        rc = run<float>(b, 0, 0);
         if (rc) return rc;
         // FP64 polish from the FP32 result: its potential and mu start a full FP64 loop
         if (!b->d_start) BHIP(hipMalloc((void **)&b->d_start, sizeof(double) * (size_t)b->arena));
@@ -1039,14 +1372,14 @@ extern "C" int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *res) {
         b->phase_seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if ((rc = count(0))) return rc;
         auto t1 = std::chrono::steady_clock::now();
-        rc = run<double>(b, 1);
+        rc = run<double>(b, 1, 1);
         if (rc) return rc;
         BHIP(hipStreamSynchronize(b->st));
         b->phase_seconds[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         b->seconds = b->phase_seconds[0] + b->phase_seconds[1];
         if ((rc = count(1))) return rc;
     } else {
-        rc = run<double>(b, 0);
+        rc = run<double>(b, 0, 1);
         if (rc) return rc;
         BHIP(hipStreamSynchronize(b->st));
         b->seconds = b->phase_seconds[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1067,6 +1400,8 @@ extern "C" int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *res) {
         r.DeltamuModifier = o.DMM;
         r.TotalEnergy = o.Etot;
         r.bisection_steps = o.bisection;
+        r.bracket_failures = o.bracket_failures;
+        r.pulay_rejections = o.pulay_rejections;
         r.solve_seconds = b->seconds;
         const mpdpft_params &p = b->params[i];
         for (int s = 0; s < r.S; s++) {
@@ -1103,6 +1438,23 @@ extern "C" int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, 
     return MPDPFT_OK;
 }
 
+extern "C" int mpdpft_batch_kernel_stats(const mpdpft_batch *b, double *kernel_ms, double *bytes,
+                                         double *instance_us) {
+    if (!b) return MPDPFT_EINVAL;
+    for (int ph = 0; ph < 2; ph++) {
+        if (kernel_ms) kernel_ms[ph] = b->kernel_ms[ph];
+        if (bytes) bytes[ph] = b->kernel_bytes[ph];
+    }
+    if (instance_us)  // the last phase's per-instance time between being taken from the queue and retiring
+        for (int i = 0; i < b->n; i++) {
+            instance_us[i] = (double)(b->out[i].t1 - b->out[i].t0) * 0.01;
+            if (std::getenv("MPDPFT_BATCH_PHASES"))  // diagnostics: sweep A, mu search, sweep B, Pulay solve
+                std::fprintf(stderr, "batch phases %d %.1f %.1f %.1f %.1f us\n", i, b->out[i].tph[0] * 0.01,
+                             b->out[i].tph[1] * 0.01, b->out[i].tph[2] * 0.01, b->out[i].tph[3] * 0.01);
+        }
+    return MPDPFT_OK;
+}
+
 extern "C" const char *mpdpft_batch_last_error(const mpdpft_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
 extern "C" void mpdpft_batch_destroy(mpdpft_batch *b) {
@@ -1111,6 +1463,10 @@ extern "C" void mpdpft_batch_destroy(mpdpft_batch *b) {
     (void)hipFree(b->d_inst);
     (void)hipFree(b->d_out);
     (void)hipFree(b->d_next);
+    (void)hipFree(b->d_order);
+    for (int ph = 0; ph < 2; ph++)
+        for (int e = 0; e < 2; e++)
+            if (b->ev[ph][e]) (void)hipEventDestroy(b->ev[ph][e]);
     (void)hipFree(b->d_slots);
     (void)hipFree(b->d_fields);
     (void)hipFree(b->d_start);

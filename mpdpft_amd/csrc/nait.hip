@@ -1449,6 +1449,7 @@ struct NaitTables {
     int *d_counts = nullptr;    // [3][G] per-point evaluations, iterMax and relAcc returns
     double Mpp[MPDPFT_MAX_SPECIES][3];
     double last_itermax[MPDPFT_MAX_SPECIES], last_relacc[MPDPFT_MAX_SPECIES];
+    int maxiter_capped[MPDPFT_MAX_SPECIES];  // the maxIter cap was reported in this solve
 };
 
 static std::string default_tables_path() {
@@ -1670,8 +1671,10 @@ void nait_free(mpdpft_ctx *c) {
 
 void nait_reset_state(mpdpft_ctx *c) {
     if (!c->nait) return;
-    for (int s = 0; s < c->S; s++)
+    for (int s = 0; s < c->S; s++) {
         for (int k = 0; k < 3; k++) c->nait->Mpp[s][k] = c->p.Mpp[k];
+        c->nait->maxiter_capped[s] = 0;
+    }
 }
 
 int nait_partials(mpdpft_ctx *c, int s, double mu, double *n, double *dU, double *dg2, double *dl) {
@@ -1816,6 +1819,16 @@ void nait_adapt(mpdpft_ctx *c) {
             t->Mpp[s][1] += 1.;
         }
         t->Mpp[s][0] = std::min(t->Mpp[s][0], 50.);
+        // the reference raises maxIter without bound; the device walk's stack holds kStack - 4
+        // levels, so maxIter stops there (noted once per solve in the control stream)
+        if (t->Mpp[s][1] > (double)(kStack - 4)) {
+            t->Mpp[s][1] = (double)(kStack - 4);
+            if (!t->maxiter_capped[s]) {
+                t->maxiter_capped[s] = 1;
+                c->control << " AdaptnAiT[s=" << s << "]: maxIter held at " << (kStack - 4)
+                           << " (depth of the quadrature stack)\n";
+            }
+        }
     }
 }
 

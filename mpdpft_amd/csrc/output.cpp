@@ -234,6 +234,7 @@ static void cell(const std::vector<double> &X, double t, int &l, double &f) {
 
 extern "C" int mpdpft_import_v(const char *path, const mpdpft_params *p, double *out, int *warnings) {
     if (!path || !p || !out) return MPDPFT_EINVAL;
+    if (params_structure_error(p)) return MPDPFT_EINVAL;
     if (p->DIM < 1 || p->DIM > 3 || p->S < 1 || p->S > MPDPFT_MAX_SPECIES || p->steps < 1) return MPDPFT_EINVAL;
     const int D = p->DIM, S = p->S, EL = p->steps + 1;
     FILE *f = std::fopen(path, "r");

@@ -18,6 +18,22 @@ extern "C" int mpdpft_abi_version(void) { return MPDPFT_ABI_VERSION; }
 extern "C" size_t mpdpft_params_sizeof(void) { return sizeof(mpdpft_params); }
 extern "C" size_t mpdpft_result_sizeof(void) { return sizeof(mpdpft_result); }
 
+// Structural checks every consumer of a params record runs first: the
+// layout version and every list count against its fixed array (a larger count
+// would index past the struct).  Returns nullptr or the reason.
+const char *params_structure_error(const mpdpft_params *p) {
+    if (p->abi_version != MPDPFT_ABI_VERSION) return "abi_version does not match this library (mpdpft_params_default)";
+    if (p->S < 1 || p->S > MPDPFT_MAX_SPECIES) return "S out of range";
+    if (p->n_interactions < 0 || p->n_interactions > MPDPFT_MAX_LIST) return "n_interactions out of range";
+    if (p->n_schedule < 0 || p->n_schedule > MPDPFT_MAX_LIST) return "n_schedule out of range";
+    if (p->n_regularize < 0 || p->n_regularize > MPDPFT_MAX_LIST) return "n_regularize out of range";
+    if (p->n_ekin < 0 || p->n_ekin > MPDPFT_MAX_LIST) return "n_ekin out of range";
+    if (p->n_Mpp < 0 || p->n_Mpp > MPDPFT_MAX_LIST) return "n_Mpp out of range";
+    if (p->n_mpp < 0 || p->n_mpp > MPDPFT_MAX_MPP) return "n_mpp out of range";
+    if (p->n_centres < 0 || p->n_centres > MPDPFT_MAX_CENTRES) return "n_centres out of range";
+    return nullptr;
+}
+
 extern "C" void mpdpft_params_default(mpdpft_params *p) {
     std::memset(p, 0, sizeof(*p));
     p->abi_version = MPDPFT_ABI_VERSION;

@@ -148,6 +148,7 @@ static int integral_from_partials(mpdpft_ctx *c, double *out) {
 // set-up: InitializeHardCodedParameters (grid part, :862-896, :1105-1106)
 // ---------------------------------------------------------------------------
 static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
+    if (const char *why = params_structure_error(p)) return ctx_fail(c, MPDPFT_EINVAL, why);
     if (p->DIM != 2 && p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "DIM must be 2 or 3");
     static const int ok_steps[] = {16, 32, 64, 96, 128, 256};
     bool ok = false;
@@ -216,6 +217,7 @@ static void twiddles(int n, std::vector<double2> &w) {
 }
 
 static int taper_geometry(mpdpft_ctx *c);
+static void reset_loop_scalars(mpdpft_ctx *c);
 
 extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **out) {
     if (!p || !out) return MPDPFT_EINVAL;
@@ -394,6 +396,7 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
     c->nait_plain = nk && std::strcmp(nk, "plain") == 0;
     c->nait_generic = nk && std::strcmp(nk, "reuse") == 0;
     if (p->DensityExpression == 5) RC(nait_init(c));
+    reset_loop_scalars(c);  // the stage entry points may run before any loop
     return MPDPFT_OK;
 }
 
@@ -512,10 +515,10 @@ static int enforce_constraints(mpdpft_ctx *c) {
                 }
             }
             if (c->TmpAb[s] >= target) hiFound[s] = true;
-            if (!hiFound[s] || !loFound[s]) {
+            if (!hiFound[s] || !loFound[s]) {  // :4657-4661 (printed, not counted as a warning)
                 c->muVec[s] = c->PreviousmuVec[s];
                 abortQ = true;
-                c->warningCount++;
+                c->bracket_failures++;
                 c->control << "mu-ranges not found for s=" << s << " --- reset muVec to PreviousmuVec\n";
             }
         }
@@ -934,7 +937,7 @@ static bool fullpiv_inverse(int n, const double *A, double *inv) {
 // PulayMixer, mixer 1, Euclidean metric (:5005-5167).  History slots hold
 // n_in and the residual R = n_out - n_in; only the newest Gram row/column is
 // computed (one fused multi-dot pass), older entries come from the cache.
-static int pulay_mixer(mpdpft_ctx *c, int *ok) {
+static int pulay_push_row(mpdpft_ctx *c) {
     int scope = c->scope, ld = scope + 1;
     int I = c->pI;
     HIPC(mk::pulay_push(c->SG, c->d_OldDen, c->d_Den, c->nin[I], c->res[I], c->st));
@@ -955,16 +958,26 @@ static int pulay_mixer(mpdpft_ctx *c, int *ok) {
     std::vector<double> row(I);
     HIPC(mk::multi_dot(c->SG, c->res[I - 1], c->res.data(), I, c->d_partials, c->st));
     RC(ctx_reduce(c, I, 1., row.data()));
-    std::vector<double> old(c->ResMat), M((size_t)I * I);
+    std::vector<double> old(c->ResMat);
     for (int i = 0; i < I; i++)
         for (int j = 0; j < I; j++) {
             double v;
             if (i == I - 1) v = row[j];
             else if (j == I - 1) v = row[i];
             else v = old[(i + shift) * ld + (j + shift)];
-            M[i * I + j] = v;
             c->ResMat[i * ld + j] = v;
         }
+    return MPDPFT_OK;
+}
+
+static int pulay_mixer(mpdpft_ctx *c, int *ok) {
+    RC(pulay_push_row(c));
+    if (c->fault_kind == MPDPFT_FAULT_DUPLICATE_PULAY && c->SCcount == c->fault_at)
+        RC(pulay_push_row(c));  // forcing hook: the same (n_in, R) twice, a singular Gram matrix
+    const int I = c->pI, ld = c->scope + 1;
+    std::vector<double> M((size_t)I * I);
+    for (int i = 0; i < I; i++)
+        for (int j = 0; j < I; j++) M[i * I + j] = c->ResMat[i * ld + j];
     std::vector<double> inv((size_t)I * I);
     *ok = 0;
     if (fullpiv_inverse(I, M.data(), inv.data())) {
@@ -991,7 +1004,10 @@ static int pulay_mixer(mpdpft_ctx *c, int *ok) {
             c->control << "PulayMixer validation of ResMat_inv failed: TestVal = " << tv << " (not 0)\n";
         }
     }
-    if (!*ok) c->control << "PulayMixer: success = 0 ResidualMatrix is ill-conditioned\n";
+    if (!*ok) {
+        c->pulay_rejections++;
+        c->control << "PulayMixer: success = 0 ResidualMatrix is ill-conditioned\n";
+    }
     return MPDPFT_OK;
 }
 
@@ -1240,7 +1256,10 @@ static std::string vec_str(const double *v, int n) {
     return o.str();
 }
 
-static int init_loop(mpdpft_ctx *c) {
+// The loop's scalar state at its start (mu, theta, T, DeltamuModifier,
+// RelAcc, counters), InitializeHardCodedParameters / GetData (:858-1383,
+// :135-201).
+static void reset_loop_scalars(mpdpft_ctx *c) {
     const mpdpft_params &p = c->p;
     for (int s = 0; s < c->S; s++) {
         c->muVec[s] = p.muVec[s];
@@ -1256,11 +1275,17 @@ static int init_loop(mpdpft_ctx *c) {
     c->warningCount = c->importWarnings;
     c->importWarnings = 0;
     c->errorCount = c->bisection_steps = 0;
+    c->bracket_failures = c->pulay_rejections = 0;
     c->MaxIteration = p.MaxIteration;
     c->CC = c->OldCC = 0.;
     for (int s = 0; s < c->S; s++) c->thetaVec0[s] = p.thetaVec[s];
     c->delfield_valid = -1;
     nait_reset_state(c);
+}
+
+static int init_loop(mpdpft_ctx *c) {
+    const mpdpft_params &p = c->p;
+    reset_loop_scalars(c);
     c->pulay_mixer = (int)(p.Mixer[0] + 0.5);
     c->scope = std::max(1, (int)p.Mixer[1]);
     if ((int)c->nin.size() != c->scope + 1) {
@@ -1279,6 +1304,7 @@ static int init_loop(mpdpft_ctx *c) {
     // initial densities (:194-201)
     RC(get_densities(c));
     RC(field_stats(c));
+    c->loop_ready = 1;
     return MPDPFT_OK;
 }
 
@@ -1345,6 +1371,11 @@ static int iteration(mpdpft_ctx *c) {
         c->OldAb[s] = c->TmpAb[s];
     }
     RC(update_v(c, c->d_OldDen));
+    if (c->fault_kind == MPDPFT_FAULT_NAN_POTENTIAL && c->SCcount == c->fault_at) {  // forcing hook
+        const double nan = std::nan("");
+        HIPC(hipMemcpyAsync(c->d_V + c->CentreIndex, &nan, sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPC(hipStreamSynchronize(c->st));  // nan is a stack value
+    }
     for (int s = 0; s < c->S; s++)  // PreparationsForSCdensity (:4506-4529)
         if (c->TVec[s] < 0.) c->TVec[s] = std::max(std::fabs(c->muVec[s] - c->VMin[s]), std::fabs(c->muVec[s] - c->VMax[s])) / 50.;
     RC(get_densities(c));
@@ -1417,9 +1448,9 @@ static int iteration(mpdpft_ctx *c) {
     return MPDPFT_OK;
 }
 
-extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
-    if (!c || !r) return MPDPFT_EINVAL;
-    if (!c->st) return ctx_fail(c, MPDPFT_EINVAL, "context was not created");
+// The loop's start as in GetData (:135-201): V = Env (or the imported
+// potential), the initial densities.
+static int loop_begin(mpdpft_ctx *c) {
     const mpdpft_params &p = c->p;
     HIPC(hipSetDevice(c->device));
     c->control.str("");
@@ -1427,7 +1458,6 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
     c->control << " ***** mpdpft_hip: self-consistent DPFT loop on " << "gfx950 *****\n";
     c->control << c->notes.str();
     c->notes.str("");
-    auto t0 = std::chrono::steady_clock::now();
     if (c->V0.empty())  // InitializeEnvironments: V <- Env (:6388)
         HIPC(hipMemcpyAsync(c->d_V, c->d_Env, sizeof(double) * c->SG, hipMemcpyDeviceToDevice, c->st));
     else  // LoadV (InterpolVQ = 1, :184)
@@ -1439,20 +1469,13 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
     c->fp32_best = 1.0e300;
     RC(init_loop(c));
     c->PrintSC = p.Print > 0 ? 1 : 0;
-    if (p.maxSCcount > 0) {
-        c->control << "\n S E L F C O N S I S T E N T   L O O P ...\n\n";
-        while (!c->ExitSCloop) RC(iteration(c));
-        if (c->SCcount >= p.maxSCcount) {
-            c->warningCount++;
-            c->control << "\n Warning!!! SC loop not converged (CC=" << c->CC << ") \n";
-        }
-    }
-    HIPC(hipStreamSynchronize(c->st));
-    auto t1 = std::chrono::steady_clock::now();
-    RC(get_energy(c));
-    HIPC(hipStreamSynchronize(c->st));
-    c->solve_seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (p.maxSCcount > 0) c->control << "\n S E L F C O N S I S T E N T   L O O P ...\n\n";
+    else c->ExitSCloop = 1;
+    return MPDPFT_OK;
+}
 
+static void fill_result(const mpdpft_ctx *c, mpdpft_result *r) {
+    const mpdpft_params &p = c->p;
     std::memset(r, 0, sizeof(*r));
     r->S = c->S;
     r->SCcount = c->SCcount;
@@ -1467,6 +1490,8 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
     r->solve_seconds = c->solve_seconds;
     r->bisection_steps = c->bisection_steps;
     r->fp32_iterations = c->fp32_iterations;
+    r->bracket_failures = c->bracket_failures;
+    r->pulay_rejections = c->pulay_rejections;
     for (int s = 0; s < c->S; s++) {
         r->muVec[s] = c->muVec[s];
         r->FinalAbundances[s] = c->TmpAb[s];
@@ -1476,7 +1501,109 @@ extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
         for (int k = 0; k < 4; k++) r->AuxEint[s][k] = c->AuxEint[s][k];
     }
     for (int a = 0; a < p.n_interactions; a++) r->InteractionEnergies[a] = c->IntE[a];
+}
+
+static void clear_energies(mpdpft_ctx *c) {
+    c->Etot = c->EInt = 0.;
+    for (int s = 0; s < MPDPFT_MAX_SPECIES; s++) {
+        c->DispE[s] = c->EnvE[s] = 0.;
+        for (int k = 0; k < 4; k++) c->AuxEkin[s][k] = c->AuxEint[s][k] = 0.;
+    }
+    for (int a = 0; a < MPDPFT_MAX_LIST; a++) c->IntE[a] = 0.;
+}
+
+extern "C" int mpdpft_solve(mpdpft_ctx *c, mpdpft_result *r) {
+    if (!c || !r) return MPDPFT_EINVAL;
+    if (!c->st) return ctx_fail(c, MPDPFT_EINVAL, "context was not created");
+    const mpdpft_params &p = c->p;
+    auto t0 = std::chrono::steady_clock::now();
+    RC(loop_begin(c));
+    while (!c->ExitSCloop) RC(iteration(c));
+    if (p.maxSCcount > 0 && c->SCcount >= p.maxSCcount) {
+        c->warningCount++;
+        c->control << "\n Warning!!! SC loop not converged (CC=" << c->CC << ") \n";
+    }
+    HIPC(hipStreamSynchronize(c->st));
+    auto t1 = std::chrono::steady_clock::now();
+    RC(get_energy(c));
+    HIPC(hipStreamSynchronize(c->st));
+    c->solve_seconds = std::chrono::duration<double>(t1 - t0).count();
+    fill_result(c, r);
     return (r->converged || p.maxSCcount == 0) ? MPDPFT_OK : MPDPFT_ENOTCONV;
+}
+
+extern "C" int mpdpft_loop_begin(mpdpft_ctx *c) {
+    if (!c) return MPDPFT_EINVAL;
+    if (!c->st) return ctx_fail(c, MPDPFT_EINVAL, "context was not created");
+    clear_energies(c);
+    return loop_begin(c);
+}
+
+extern "C" int mpdpft_step(mpdpft_ctx *c, mpdpft_result *r, int *exit_loop) {
+    if (!c) return MPDPFT_EINVAL;
+    if (!c->loop_ready) return ctx_fail(c, MPDPFT_EINVAL, "mpdpft_step: call mpdpft_loop_begin first");
+    HIPC(hipSetDevice(c->device));
+    if (!c->ExitSCloop) {
+        RC(iteration(c));
+        if (c->ExitSCloop && c->p.maxSCcount > 0 && c->SCcount >= c->p.maxSCcount) {
+            c->warningCount++;
+            c->control << "\n Warning!!! SC loop not converged (CC=" << c->CC << ") \n";
+        }
+    }
+    HIPC(hipStreamSynchronize(c->st));
+    if (r) fill_result(c, r);
+    if (exit_loop) *exit_loop = c->ExitSCloop;
+    return MPDPFT_OK;
+}
+
+extern "C" int mpdpft_energy(mpdpft_ctx *c, mpdpft_result *r) {
+    if (!c || !r) return MPDPFT_EINVAL;
+    if (!c->st) return ctx_fail(c, MPDPFT_EINVAL, "context was not created");
+    HIPC(hipSetDevice(c->device));
+    RC(get_energy(c));
+    HIPC(hipStreamSynchronize(c->st));
+    fill_result(c, r);
+    return MPDPFT_OK;
+}
+
+extern "C" int mpdpft_enforce_constraints(mpdpft_ctx *c, double *mu, double *N) {
+    if (!c || !mu) return MPDPFT_EINVAL;
+    if (!c->st) return ctx_fail(c, MPDPFT_EINVAL, "context was not created");
+    HIPC(hipSetDevice(c->device));
+    for (int s = 0; s < c->S; s++) c->muVec[s] = mu[s];
+    const int before = c->bracket_failures;
+    c->delfield_valid = -1;  // the potential may have been replaced
+    RC(get_densities(c));
+    HIPC(hipStreamSynchronize(c->st));
+    for (int s = 0; s < c->S; s++) {
+        mu[s] = c->muVec[s];
+        if (N) N[s] = c->TmpAb[s];
+    }
+    return c->bracket_failures > before ? MPDPFT_ENOBRACKET : MPDPFT_OK;
+}
+
+extern "C" int mpdpft_mix(mpdpft_ctx *c, int sc_count, const double *n_out) {
+    if (!c || !n_out || sc_count < 0) return MPDPFT_EINVAL;
+    if (!c->loop_ready) return ctx_fail(c, MPDPFT_EINVAL, "mpdpft_mix: call mpdpft_loop_begin first (Pulay history)");
+    HIPC(hipSetDevice(c->device));
+    std::swap(c->d_Den, c->d_OldDen);  // the current density is this step's n_in
+    HIPC(hipMemcpyAsync(c->d_Den, n_out, sizeof(double) * c->SG, hipMemcpyHostToDevice, c->st));
+    const int keep = c->SCcount, before = c->pulay_rejections;
+    c->SCcount = sc_count;
+    const int rc = admix(c);
+    c->SCcount = keep;
+    RC(rc);
+    HIPC(hipStreamSynchronize(c->st));
+    return c->pulay_rejections > before ? MPDPFT_EPULAY : MPDPFT_OK;
+}
+
+extern "C" int mpdpft_inject_fault(mpdpft_ctx *c, int kind, int at) {
+    if (!c) return MPDPFT_EINVAL;
+    if (kind != 0 && kind != MPDPFT_FAULT_DUPLICATE_PULAY && kind != MPDPFT_FAULT_NAN_POTENTIAL)
+        return ctx_fail(c, MPDPFT_EINVAL, "mpdpft_inject_fault: unknown kind");
+    c->fault_kind = at < 0 ? 0 : kind;
+    c->fault_at = at;
+    return MPDPFT_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -62,6 +62,9 @@ struct mpdpft_ctx {
     double DeltamuModifier = 0, RelAcc = 0, RelAccOriginal = 0, incV = 1, incV0 = 1, CC = 0;
     int SCcount = 0, ExitSCcount = 0, ExitSCloop = 0, LinMix = 0, PulMix = 0, TrappedQ = 0;
     int warningCount = 0, errorCount = 0, bisection_steps = 0, PrintSC = 0, MaxIteration = 4;
+    int bracket_failures = 0, pulay_rejections = 0;  // recoverable events of the current solve
+    int fault_kind = 0, fault_at = -1;               // mpdpft_inject_fault
+    int loop_ready = 0;                              // mpdpft_loop_begin has run (Pulay slots allocated)
     double EInt = 0, Etot = 0;
     double IntE[MPDPFT_MAX_LIST], DispE[MPDPFT_MAX_SPECIES], EnvE[MPDPFT_MAX_SPECIES];
     double AuxEkin[MPDPFT_MAX_SPECIES][4];
@@ -92,6 +95,7 @@ struct mpdpft_ctx {
 };
 
 // internal helpers shared between translation units
+const char *params_structure_error(const mpdpft_params *p);  // params.cpp
 int ctx_fail(mpdpft_ctx *c, int code, const std::string &msg);
 int hip_fail(mpdpft_ctx *c, hipError_t e, const char *what);
 int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method);

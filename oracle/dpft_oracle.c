@@ -242,7 +242,7 @@ struct orc_state {
     double TVec[MPDPFT_MAX_SPECIES];
     double DeltamuModifier, RelAcc, RelAccOriginal, incrementalV, incrementalVOriginal, CC;
     int SCcount, ExitSCcount, ExitSCloop, LinMix, PulMix, TrappedQ;
-    int warningCount, errorCount, bisection_steps;
+    int warningCount, errorCount, bisection_steps, bracket_failures, pulay_rejections;
     /* Pulay */
     int pulay_mixer, pulay_scope, pulay_I;
     double **nin, **nout, **res; /* history entries, each [S][G] */
@@ -1274,10 +1274,10 @@ static int enforce_constraints(orc_state *o) {
                     }
                 }
                 if (o->TmpAb[s] >= Ab) MAXfound[s] = 1;
-                if (!MAXfound[s] || !MINfound[s]) {
+                if (!MAXfound[s] || !MINfound[s]) { /* :4657-4661: printed there, not a warning */
                     o->muVec[s] = o->PreviousmuVec[s];
                     abortQ = 1;
-                    o->warningCount++;
+                    o->bracket_failures++;
                     break;
                 }
             }
@@ -1646,6 +1646,7 @@ static int pulay_mixer(orc_state *o) {
         }
     double tv;
     int ok = oracle_pulay_coefficients(I, M, o->coeff, &tv);
+    if (!ok) o->pulay_rejections++;
     return ok;
 }
 
@@ -1920,8 +1921,8 @@ static int orc_iteration(orc_state *o) {
     if (o->SCcount == o->p.maxSCcount) o->ExitSCloop = 1;
     oracle_nait_intercept(o);
     if (getenv("ORACLE_VERBOSE"))
-        fprintf(stderr, "SC %d CC %.6e mu %.12f N %.12f DM %.4f\n", o->SCcount, o->CC, o->muVec[0], o->TmpAb[0],
-                o->DeltamuModifier);
+        fprintf(stderr, "SC %d CC %.8e mu %.15g N %.15g DM %.4f bis %d Lin %d Pul %d\n", o->SCcount, o->CC, o->muVec[0],
+                o->TmpAb[0], o->DeltamuModifier, o->bisection_steps, o->LinMix, o->PulMix);
     admix_densities(o);
     inject_schedule(o);
     o->SCcount++; /* PrepareNextIteration (:5317) */
@@ -1970,6 +1971,8 @@ int oracle_solve_v0(const mpdpft_params *p, const double *v0, mpdpft_result *r, 
     r->PulMix = o->PulMix;
     r->errorCount = o->errorCount;
     r->warningCount = o->warningCount;
+    r->bracket_failures = o->bracket_failures;
+    r->pulay_rejections = o->pulay_rejections;
     r->CC = o->CC;
     r->DeltamuModifier = o->DeltamuModifier;
     r->TotalEnergy = o->Etot;
@@ -1989,6 +1992,62 @@ int oracle_solve_v0(const mpdpft_params *p, const double *v0, mpdpft_result *r, 
     int conv = o->ExitSCcount >= 3 || p->maxSCcount == 0;
     orc_free(o);
     return conv ? 0 : MPDPFT_ENOTCONV;
+}
+
+/* GetDensities + EnforceConstraints on a given potential v ([S][G]) from
+ * mu (in/out), with the scalar state of a loop's start (:3306-3316,
+ * :4569-4743); N, den ([S][G]) out; *nobracket counts the species whose mu
+ * bracket was not found.  The oracle of mpdpft_enforce_constraints. */
+int oracle_enforce(const mpdpft_params *p, const double *v, double *mu, double *N, double *den, int *nobracket) {
+    orc_state *o = orc_create(p);
+    memcpy(o->V, v, sizeof(double) * (size_t)(o->S * o->G));
+    for (int s = 0; s < o->S; s++) o->muVec[s] = mu[s];
+    int rc = get_densities(o);
+    for (int s = 0; s < o->S; s++) {
+        mu[s] = o->muVec[s];
+        if (N) N[s] = o->TmpAb[s];
+    }
+    memcpy(den, o->Den, sizeof(double) * (size_t)(o->S * o->G));
+    if (nobracket) *nobracket = o->bracket_failures;
+    orc_free(o);
+    return rc;
+}
+
+/* A sequence of AdmixDensities calls (:4921-4992, PulayMixer :5005-5167):
+ * the current density starts as n_in0 ([S][G]); call k (SCcount k) mixes
+ * n_outs[k] with it; out: the mixed density after the last call,
+ * *rejections the Pulay solves rejected.  The oracle of mpdpft_mix. */
+int oracle_mix_sequence(const mpdpft_params *p, int nsteps, const double *n_in0, const double *n_outs, double *out,
+                        int *rejections) {
+    orc_state *o = orc_create(p);
+    const long long SG = (long long)o->S * o->G;
+    o->pulay_mixer = (int)(o->p.Mixer[0] + 0.5);
+    o->pulay_scope = (int)o->p.Mixer[1];
+    if (o->pulay_scope < 1) o->pulay_scope = 1;
+    int n = o->pulay_scope + 1;
+    o->nin = (double **)calloc((size_t)n, sizeof(double *));
+    o->nout = (double **)calloc((size_t)n, sizeof(double *));
+    o->res = (double **)calloc((size_t)n, sizeof(double *));
+    for (int i = 0; i < n; i++) {
+        o->nin[i] = dalloc(SG);
+        o->nout[i] = dalloc(SG);
+        o->res[i] = dalloc(SG);
+    }
+    o->ResMat = dalloc((long long)n * n);
+    for (int s = 0; s < o->S; s++) o->TmpAb[s] = o->p.Abundances[s];
+    memcpy(o->Den, n_in0, sizeof(double) * (size_t)SG);
+    for (int k = 0; k < nsteps; k++) {
+        double *t = o->OldDen; /* StoreCurrentIteration: the current density becomes n_in */
+        o->OldDen = o->Den;
+        o->Den = t;
+        memcpy(o->Den, n_outs + (long long)k * SG, sizeof(double) * (size_t)SG);
+        o->SCcount = k;
+        admix_densities(o);
+    }
+    memcpy(out, o->Den, sizeof(double) * (size_t)SG);
+    if (rejections) *rejections = o->pulay_rejections;
+    orc_free(o);
+    return 0;
 }
 
 double oracle_time_iterations(const mpdpft_params *p, int n_iter, int *bisection_steps) {

@@ -65,6 +65,9 @@ double oracle_exp_i0(double x);
 
 /* Timed partial workload for the CPU baseline: n_iter SC iterations (no
  * convergence test) starting from the initial density; returns seconds. */
+int oracle_enforce(const mpdpft_params *p, const double *v, double *mu, double *N, double *den, int *nobracket);
+int oracle_mix_sequence(const mpdpft_params *p, int nsteps, const double *n_in0, const double *n_outs, double *out,
+                        int *rejections);
 double oracle_time_iterations(const mpdpft_params *p, int n_iter, int *bisection_steps);
 
 #ifdef __cplusplus

@@ -53,6 +53,8 @@ def lib():
         L.oracle_pulay_coefficients.argtypes = [ctypes.c_int, dp, dp, dp]
         L.oracle_hartree_isolated_direct.argtypes = [P, dp, dp]
         L.oracle_cube_inverse_r.restype = ctypes.c_double
+        L.oracle_enforce.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_mix_sequence.argtypes = [P, ctypes.c_int, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
         L.oracle_time_iterations.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.oracle_time_iterations.restype = ctypes.c_double
         for n in ("oracle_curlyA", "oracle_airy", "oracle_exp_i0"):
@@ -134,6 +136,33 @@ def density(p, s, mu, v):
     rc = lib().oracle_density(ctypes.byref(p), s, mu, _dp(v), _dp(out), ctypes.byref(N))
     assert rc == 0, rc
     return out, N.value
+
+
+def enforce(p, v, mu):
+    """GetDensities + EnforceConstraints on potential v ([S][G]) from mu:
+    (mu, N, density [S][G], bracket failures)."""
+    G = grid_size(p)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    m = np.ascontiguousarray(mu, dtype=np.float64).copy()
+    N = np.zeros(p.S)
+    den = np.zeros(p.S * G)
+    nb = ctypes.c_int()
+    rc = lib().oracle_enforce(ctypes.byref(p), _dp(v), _dp(m), _dp(N), _dp(den), ctypes.byref(nb))
+    assert rc == 0, rc
+    return m, N, den.reshape(p.S, G), nb.value
+
+
+def mix_sequence(p, n_in0, n_outs):
+    """AdmixDensities called with SCcount 0, 1, ... on n_outs[k], starting
+    from n_in0: (mixed density [S][G], Pulay rejections)."""
+    G = grid_size(p)
+    n0 = np.ascontiguousarray(n_in0, dtype=np.float64).reshape(-1)
+    no = np.ascontiguousarray(n_outs, dtype=np.float64).reshape(-1)
+    k = no.size // (p.S * G)
+    out = np.zeros(p.S * G)
+    rj = ctypes.c_int()
+    lib().oracle_mix_sequence(ctypes.byref(p), k, _dp(n0), _dp(no), _dp(out), ctypes.byref(rj))
+    return out.reshape(p.S, G), rj.value
 
 
 def env(p):

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_layouts_and_version():
     lib = _abi.load_library()
-    assert lib.mpdpft_abi_version() == 2
+    assert lib.mpdpft_abi_version() == 3
     assert lib.mpdpft_params_sizeof() == ctypes.sizeof(_abi.Params)
     assert lib.mpdpft_result_sizeof() == ctypes.sizeof(_abi.Result)
 
@@ -111,3 +111,26 @@ def test_cli_fails_loudly_without_a_gpu(tmp_path):
     assert "no HIP device" in (tmp_path / "mpDPFT_Control.dat").read_text()
     assert not (tmp_path / "mpDPFT_Den.dat").exists() and not (tmp_path / "mpDPFT_V.dat").exists()
 
+
+
+@pytest.mark.parametrize("field,value", [("n_interactions", 17), ("n_schedule", 17), ("n_regularize", 17),
+                                         ("n_ekin", 17), ("n_Mpp", 17), ("n_mpp", 65), ("n_centres", 257),
+                                         ("n_interactions", -1), ("abi_version", 1), ("S", 9)])
+def test_counts_beyond_their_arrays_are_rejected(tmp_path, field, value):
+    """A list count larger than its fixed array (or a params record of another
+    layout version) is EINVAL at every entry point that takes params, before
+    any device work: mpdpft_create, mpdpft_batch_create, mpdpft_import_v."""
+    p = configs.c0(steps=16)
+    setattr(p, field, value)
+    with pytest.raises(mpdpft_amd.MpdpftError) as e:
+        mpdpft_amd.Context(p)
+    assert e.value.code == _abi.EINVAL
+    assert "out of range" in str(e.value) or "abi_version" in str(e.value)
+    with pytest.raises(mpdpft_amd.MpdpftError) as e:
+        mpdpft_amd.Batch([configs.c0(steps=16), p])
+    assert e.value.code == _abi.EINVAL and "instance 1" in str(e.value)
+    v = tmp_path / "mpDPFT_V.dat"
+    v.write_text("0 0 1 1 \n")
+    with pytest.raises(mpdpft_amd.MpdpftError) as e:
+        mpdpft_amd.import_v(v, p)
+    assert e.value.code == _abi.EINVAL
