@@ -188,13 +188,14 @@ def batch_leg(steps, warmup, plain_n, oracle_energies):
     b64 = Batch(insts)
     for _ in range(warmup):
         b64.solve()
-    ms = by = 0.0
+    ms = sb = kb = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         r64 = b64.solve()
-        (_, k_ms), (_, k_by), us = b64.kernel_stats()
+        (_, k_ms), (_, k_sb), (_, k_kb), us = b64.kernel_stats()
         ms += k_ms
-        by += k_by
+        sb += k_sb
+        kb += k_kb
     elapsed = time.perf_counter() - t0
     f64 = [b64.fields(i) for i in range(N_BATCH)]
     b64.close()
@@ -203,15 +204,18 @@ def batch_leg(steps, warmup, plain_n, oracle_energies):
                           "mean_instance_ms": round(float(us[grid == g].mean()) / 1e3, 2)}
                  for g in (64, 128, 256)}
     avg_ms = ms / steps
-    bytes_per_launch = by / steps
+    bytes_per_launch = sb / steps
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     traffic = read_summary_value(os.path.join(ROOT, "profiles", "batch_c3_512_traffic.md"), "traffic_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_batch_solve<double>",
             "avg_launch_ms": round(avg_ms, 3), "algorithmic_bytes_per_launch": bytes_per_launch,
-            "note": "one launch solves all 512 instances; algorithmic bytes = every field access of every "
-                    "instance's loop (mu evaluations, density, UpdateV, convergence, Pulay push + Gram row, "
-                    "mixing, energies, retire) counted once by the kernel itself"}
+            "fused_kernel_bytes_per_launch": kb / steps,
+            "note": "one launch solves all 512 instances; algorithmic bytes = per SC iteration of every instance "
+                    "the reference stages' inputs + outputs (UpdateV 3 SG, N(mu) G per evaluation, density 2 SG, "
+                    "convergence 2 SG, Pulay Gram row (I+1) SG and combination (2I+1) SG or linear mixing 3 SG; "
+                    "8-byte fields), counted per instance by the kernel; the fused sweeps themselves move "
+                    "fused_kernel_bytes_per_launch"}
     # FP32 fields + FP64 polish
     b32 = Batch(insts, fp32=True)
     b32.solve()
@@ -220,7 +224,7 @@ def batch_leg(steps, warmup, plain_n, oracle_energies):
     for _ in range(n32):
         r32 = b32.solve()
     t32 = (time.perf_counter() - t0) / n32
-    (ms32, by32, _) = b32.kernel_stats()
+    (ms32, by32, _, _) = b32.kernel_stats()
     (ph_s, ph_it, ph_bis) = b32.phase_stats()
     f32 = [b32.fields(i) for i in range(N_BATCH)]
     b32.close()
@@ -381,6 +385,20 @@ def adjoint_leg(steps, h, n_coords, deadline):
     }
 
 
+def copy_leg():
+    """A plain device copy of known size (256 MiB of FP64 read, 256 MiB
+    written, three times): the calibration kernel for the FETCH_SIZE and
+    WRITE_SIZE counters of a rocprof counter run."""
+    import torch
+    n = 1 << 25
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"bytes_read_per_launch": 8 * n, "bytes_written_per_launch": 8 * n, "launches": 3}
+
+
 def main():
     t_begin = time.perf_counter()
     ap = argparse.ArgumentParser()
@@ -389,7 +407,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--deadline", type=float, default=450.0, help="seconds after which no further leg starts")
     ap.add_argument("--legs", default="cpu,batch,c1,adjoint",
-                    help="comma-separated subset of cpu, batch, plain, c1, adjoint")
+                    help="comma-separated subset of cpu, batch, plain, c1, adjoint, copy (counter calibration)")
     ap.add_argument("--plain", type=int, default=512, help="instances the plain FP64 loop solves")
     ap.add_argument("--c1-grid", type=int, default=128)
     ap.add_argument("--maxsc", type=int, default=300)
@@ -407,7 +425,7 @@ def main():
 
     skipped, errors = {}, {}
     value = ms_per_step = None
-    roof = cpu = batch = c1 = adjoint = None
+    roof = cpu = batch = c1 = adjoint = copy = None  # noqa: F841
 
     def run(name, est_s, fn):
         if name not in legs:
@@ -436,6 +454,7 @@ def main():
             ms_per_step = 1e3 * sec
         c1 = run("c1", 70, lambda: c1_leg(args.c1_grid, args.maxsc))
         adjoint = run("adjoint", 60, lambda: adjoint_leg(args.adjoint_grid, 1e-4, args.fd_coords, deadline))
+        copy = run("copy", 5, copy_leg) if "copy" in legs else None
     finally:
         if _PARAMS_FILE and os.path.exists(_PARAMS_FILE):
             os.unlink(_PARAMS_FILE)
@@ -459,7 +478,7 @@ def main():
             "batch": batch,
             "c1": c1,
             "adjoint": adjoint,
-            "skipped_legs": skipped,
+            "skipped_legs": {k: v for k, v in skipped.items() if k != "copy"},
             "leg_errors": errors,
             "bench_seconds": round(time.perf_counter() - t_begin, 1),
         }

@@ -352,11 +352,16 @@ int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, long long *
                              long long *bisection_steps);
 /* The last solve's k_batch_solve launches (the persistent kernel), each argument may be NULL:
  * kernel_ms[phase] is the launch's duration from HIP events recorded on the batch's stream around
- * it, bytes[phase] its algorithmic traffic (every field a stage of an instance's loop reads or
- * writes, counted once per access at the field's storage width, summed over the instances), phase
- * [0] the FP32 loop and [1] the FP64 loop or polish; instance_us[i] (n values) the time instance i
- * spent in the last phase between leaving the queue and retiring (device clock, 100 MHz). */
-int mpdpft_batch_kernel_stats(const mpdpft_batch *b, double *kernel_ms, double *bytes, double *instance_us);
+ * it; stage_bytes[phase] the loop's algorithmic traffic counted per reference stage (inputs +
+ * outputs of UpdateV, every N(mu) evaluation, the density, CheckConvergence, the Pulay Gram row
+ * (I + 1 fields) and combination (2I + 1) or linear mixing (3), energies, fields out), at the
+ * fields' storage width, summed over the instances; kernel_bytes[phase] what the fused sweeps of
+ * this kernel access (fewer: the density is never stored, one sweep serves several stages);
+ * phase [0] the FP32 loop and [1] the FP64 loop or polish; instance_us[i] (n values) the time
+ * instance i spent in the last phase between leaving the queue and retiring (device clock,
+ * 100 MHz). */
+int mpdpft_batch_kernel_stats(const mpdpft_batch *b, double *kernel_ms, double *stage_bytes, double *kernel_bytes,
+                              double *instance_us);
 const char *mpdpft_batch_last_error(const mpdpft_batch *b);
 void mpdpft_batch_destroy(mpdpft_batch *b);
 

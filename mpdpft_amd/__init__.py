@@ -263,14 +263,17 @@ class Batch:
         return (sec[0], sec[1]), (its[0], its[1]), (bis[0], bis[1])
 
     def kernel_stats(self):
-        """(kernel_ms[2], algorithmic bytes[2], per-instance microseconds[n]) of the
-        last solve's persistent launches: [0] the FP32 loop, [1] the FP64 loop or
-        polish (HIP events on the batch's stream)."""
+        """(kernel_ms[2], stage bytes[2], kernel bytes[2], per-instance
+        microseconds[n]) of the last solve's persistent launches: [0] the FP32
+        loop, [1] the FP64 loop or polish (HIP events on the batch's stream;
+        stage bytes: the reference stages' inputs + outputs, kernel bytes: the
+        fused sweeps' own accesses)."""
         ms = (ctypes.c_double * 2)()
-        by = (ctypes.c_double * 2)()
+        sb = (ctypes.c_double * 2)()
+        kb = (ctypes.c_double * 2)()
         us = np.zeros(self.n)
-        self._check(self._lib.mpdpft_batch_kernel_stats(self._h, ms, by, _dp(us)))
-        return (ms[0], ms[1]), (by[0], by[1]), us
+        self._check(self._lib.mpdpft_batch_kernel_stats(self._h, ms, sb, kb, _dp(us)))
+        return (ms[0], ms[1]), (sb[0], sb[1]), (kb[0], kb[1]), us
 
     def _check(self, rc):
         if rc != _abi.OK:

@@ -161,7 +161,7 @@ def load_library(path=None):
     lib.mpdpft_batch_seconds.restype = ctypes.c_double
     lib.mpdpft_batch_phase_stats.argtypes = [vp, vp, vp, vp]
     lib.mpdpft_batch_phase_stats.restype = ctypes.c_int
-    lib.mpdpft_batch_kernel_stats.argtypes = [vp, vp, vp, vp]
+    lib.mpdpft_batch_kernel_stats.argtypes = [vp, vp, vp, vp, vp]
     lib.mpdpft_batch_kernel_stats.restype = ctypes.c_int
     lib.mpdpft_batch_last_error.argtypes = [vp]
     lib.mpdpft_batch_last_error.restype = ctypes.c_char_p

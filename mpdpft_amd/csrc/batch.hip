@@ -43,7 +43,9 @@ constexpr int kWork = 5 * (kMaxScope + 1) * (kMaxScope + 1);  // >= row + A + qr
 constexpr int kGC = 8;   // stored Pulay entries dotted per sweep of the new residual
 constexpr int kMixC = 8; // Pulay entries combined per load batch of the mixing
 constexpr int kUA = 2;   // grid points per lane visit in sweep A
-constexpr int kUB = 1;   // grid points per lane visit in sweep B (more costs registers and time)
+constexpr int kTeamMax = 2;  // workgroups that share one instance (the largest instances)
+constexpr int kRedMax = 16;  // values per team reduction
+constexpr int kUB = 2;   // grid points per lane visit in sweep B
 
 struct BInst {
     int D, S, EL, env[kBS];
@@ -63,15 +65,31 @@ struct BOut {
     int pulay_rejections, pad;
     double CC, DMM, Etot, Eint;
     double mu[kBS], N[kBS], Edisp[kBS], Eenv[kBS], Aux0[kBS];
-    double bytes;                 // algorithmic bytes the instance's loop moved (each field access once)
+    double bytes;                 // bytes the instance's fused sweeps moved (each field access once)
+    double sbytes;                // the reference stages' (inputs + outputs) bytes of the same loop
     unsigned long long t0, t1;    // s_memrealtime (100 MHz) when the instance was taken and retired
     unsigned long long tph[4];    // time in sweep A, the mu search, sweep B, the Pulay solve (100 MHz ticks)
 };
 
+// Per-instance exchange area of a team: each member's block partials, by
+// reduction generation parity, and the arrival counter of the team barrier.
+struct TeamArea {
+    unsigned count;
+    unsigned pad[63];
+    double part[2][kTeamMax][kRedMax];
+};
+
+// A queue entry: instance, rank in its team, team size.
+struct Ticket {
+    int inst, rank, size, pad;
+};
+
 template <typename T> struct BArgs {
     const BInst *inst;
-    const int *order;     // queue order of the instances
-    int n;
+    const Ticket *tickets;  // queue: one ticket per team member, largest instances first
+    int n;                  // tickets
+    TeamArea *team;         // [instances]
+    int *team_err;          // set when a team barrier waited past its bound
     int *next;            // queue head (atomic)
     T *slots;             // per slot: (5 + 2 (scope + 1)) S Gmax values
     long long slot_stride, Gmax;
@@ -155,14 +173,14 @@ __device__ __forceinline__ void bsum_n(double (&v)[NV], int n, double *sh) {
         if (k < n) v[k] = uni((sh[k] + sh[NV + k]) + (sh[2 * NV + k] + sh[3 * NV + k]));
 }
 
-// Visit the points i = threadIdx.x + k kBT < n in increasing k, kU at a time:
+// Visit the points i = lo + threadIdx.x + k kBT < n in increasing k, kU at a time:
 // the kU loads of a step are issued before their values are used, so a lane
 // keeps kU loads in flight (one workgroup per instance is latency bound
 // otherwise).  Per-lane accumulation order is that of the plain strided loop.
 constexpr int kU = 16;
 template <class Ld, class Use>
-__device__ __forceinline__ void stream(long long n, Ld ld, Use use) {
-    long long i = threadIdx.x;
+__device__ __forceinline__ void stream(long long lo, long long n, Ld ld, Use use) {
+    long long i = lo + threadIdx.x;
     for (; i + (kU - 1) * kBT < n; i += kU * kBT) {
         decltype(ld(0LL)) x[kU];
 #pragma unroll
@@ -228,7 +246,8 @@ struct Loop {
     double mu[kBS], PrevMu[kBS], OldMu[kBS], TmpAb[kBS], OldAb[kBS], theta[kBS], muEval[kBS];
     double DMM, RelAcc, incV, CC, OldCC, best;
     double thMix[kBS], muMix[kBS];  // theta and mu_eval of the iteration whose mixing is pending
-    double bytes;  // algorithmic bytes: every field a stage reads or writes, counted once per access
+    double bytes;   // bytes the fused sweeps move: every field access of this kernel, counted once
+    double sbytes;  // the loop's stage bytes: (inputs + outputs) of each reference stage per field
     int SCcount, ExitSCcount, ExitSCloop, LinMix, PulMix, TrappedQ, warn, err, bis, pI, stall, nobr, prej;
 };
 
@@ -240,36 +259,104 @@ template <typename T> struct Inst {
     double floor_rel;                              // reduced-precision stop floor (0: the criterion as given)
     int stall_max;                                 // reduced-precision stall exit (0: none)
     double relacc;                                 // particle-number accuracy of the mu search
+    unsigned lo, hi;                               // this workgroup's grid points [lo, hi) of the instance
+    int TS, rank;                                  // team size and rank (TS = 1: the whole instance)
+    TeamArea *tm;                                  // the team's exchange area
+    unsigned gen;                                  // team reductions done so far
+    int *err;
     __device__ long long SG() const { return (long long)in.S * in.G; }
 
+    // Team reductions: block values (identical in every thread), then, for a
+    // team, every member's values through the exchange area summed in rank
+    // order, so every member gets the same bits.  The members are resident
+    // workgroups running the same control flow; the wait is bounded anyway.
+    template <int NV>
+    __device__ __forceinline__ void team_combine(double (&v)[NV], int n, bool is_max, int first_max = 1 << 30) {
+        if (TS == 1) return;
+        const unsigned g = gen++;
+        double(*part)[kRedMax] = tm->part[g & 1];
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < n; k++)
+                __hip_atomic_store(&part[rank][k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&tm->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned target = (g + 1) * (unsigned)TS;
+            unsigned spins = 0;
+            while (__hip_atomic_load(&tm->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1u << 27)) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            if (k < n) {
+                double acc = __hip_atomic_load(&part[0][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int r = 1; r < TS; r++) {
+                    const double x = __hip_atomic_load(&part[r][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    acc = (is_max || k >= first_max) ? fmax(acc, x) : acc + x;
+                }
+                v[k] = uni(acc);
+            }
+    }
+    template <int NV>
+    __device__ __forceinline__ void tsum_n(double (&v)[NV], int n) {
+        bsum_n<NV>(v, n, sh);
+        team_combine<NV>(v, n, false);
+    }
+    // v[0], v[1] summed, v[2], v[3] maximised over the team, in one exchange
+    __device__ __forceinline__ void tsum2_max2(double (&v)[4]) {
+        double sv[2] = {v[0], v[1]};
+        bsum_n<2>(sv, 2, sh);
+        const double m2 = bmax(v[2], sh), m3 = bmax(v[3], sh);
+        v[0] = sv[0];
+        v[1] = sv[1];
+        v[2] = m2;
+        v[3] = m3;
+        team_combine<4>(v, 4, false, 2);
+    }
+    __device__ __forceinline__ double tsum(double x) {
+        double v[1] = {bsum(x, sh)};
+        team_combine<1>(v, 1, false);
+        return v[0];
+    }
+    __device__ __forceinline__ double tmax(double x) {
+        double v[1] = {bmax(x, sh)};
+        team_combine<1>(v, 1, true);
+        return v[0];
+    }
+
     // N_s(mu) of the TF density (GetTFdensity :4415-4456 + Riemann :7981-8026)
-    __device__ double evalN(int s, double mu) {
+    __device__ __forceinline__ double evalN(int s, double mu) {
         const T *v = V + (long long)s * in.G;
         const double pref = in.pref;
         const bool d2 = in.D == 2;
         double acc = 0.;
         stream(
-            in.G, [&](long long i) { return (double)v[i]; },
+            lo, hi, [&](long long i) { return (double)v[i]; },
             [&](long long i, double vi) {
                 double t = mu - vi;
                 double val = t > 0. ? (d2 ? pref * t : pref * t * sqrt(t)) : 0.;
                 acc += trapw(in, i) * val;
             });
-        return bsum(acc, sh) * in.trapD;
+        return tsum(acc) * in.trapD;
     }
-    __device__ double N_at(Loop &L, int s, double mu) {
+    __device__ __forceinline__ double N_at(Loop &L, int s, double mu) {
         L.muEval[s] = mu;
         L.bytes += (double)in.G * sizeof(T);
+        L.sbytes += (double)in.G * sizeof(T);
         return evalN(s, mu);
     }
-    __device__ double N_eval(Loop &L, int s, double mu) {  // a density evaluation of the bisection
+    __device__ __forceinline__ double N_eval(Loop &L, int s, double mu) {  // a density evaluation of the bisection
         L.bis++;
         return N_at(L, s, mu);
     }
 
     // EnforceConstraints (mpDPFT.cpp:4569-4743), the same step sequence as the
     // single-instance solver; the density of the last evaluated mu is kept.
-    __device__ void enforce(Loop &L) {
+    __device__ __forceinline__ void enforce(Loop &L) {
         const int S = in.S;
         L.TrappedQ = 0;
         bool abortQ = false;
@@ -467,7 +554,7 @@ template <typename T> struct Inst {
     // before its values are used (the tail visit clamps its indices), so a
     // lane keeps all of them in flight.
     template <int NS>
-    __device__ void sweepA_t(Loop &L, int mix, int mold_slot, int mnew_slot, int head, const double *coef,
+    __device__ __forceinline__ void sweepA_t(Loop &L, int mix, int mold_slot, int mnew_slot, int head, const double *coef,
                              bool update, double *N, double &dvmax, double &vmax) {
         const unsigned G = (unsigned)in.G;
         const long long SGn = SG();
@@ -485,14 +572,14 @@ template <typename T> struct Inst {
             off1[q] = (unsigned)(((head + kMixC + q) % nslot) * SGn);
         }
         double n0 = 0., n1 = 0., dv = 0., vm = 0.;
-        for (unsigned i0 = threadIdx.x; i0 < G; i0 += kUA * kBT) {
+        for (unsigned i0 = lo + threadIdx.x; i0 < hi; i0 += kUA * kBT) {
             unsigned ii[kUA];
             bool ok[kUA];
 #pragma unroll
             for (int u = 0; u < kUA; u++) {
                 const unsigned i = i0 + u * kBT;
-                ok[u] = i < G;
-                ii[u] = ok[u] ? i : G - 1;
+                ok[u] = i < hi;
+                ii[u] = ok[u] ? i : hi - 1;
             }
             double vo[kUA][kBS] = {}, m[kUA][kBS] = {}, env[kUA][kBS] = {};
             if (need_vold) {
@@ -568,15 +655,15 @@ template <typename T> struct Inst {
         if (update) fields += (mix == 1 ? 0. : 1.) + 2.;  // V_old (if not read above), Env, V written
         L.bytes += fields * (double)SGn * sizeof(T);
         if (update) {
-            double v[3] = {n0, n1, 0.};
-            bsum_n<3>(v, 2, sh);
+            double v[4] = {n0, n1, dv, vm};
+            tsum2_max2(v);
             N[0] = v[0] * in.trapD;
             N[1] = v[1] * in.trapD;
-            dvmax = bmax(dv, sh);
-            vmax = bmax(vm, sh);
+            dvmax = v[2];
+            vmax = v[3];
         }
     }
-    __device__ void sweepA(Loop &L, int mix, int mold_slot, int mnew_slot, int head, const double *coef,
+    __device__ __forceinline__ void sweepA(Loop &L, int mix, int mold_slot, int mnew_slot, int head, const double *coef,
                            bool update, double *N, double &dvmax, double &vmax) {
         if (in.S == 2) sweepA_t<2>(L, mix, mold_slot, mnew_slot, head, coef, update, N, dvmax, vmax);
         else sweepA_t<1>(L, mix, mold_slot, mnew_slot, head, coef, update, N, dvmax, vmax);
@@ -593,17 +680,17 @@ template <typename T> struct Inst {
     // (FIRST false).  acc: [0, kGC) the row, kGC <R_new, R_new>, then the
     // overlaps.
     template <int NS, int NC, bool FIRST>
-    __device__ void sweepB_t(Loop &L, bool push, const T *m, T *r, const unsigned *offj, double *acc) {
+    __device__ __forceinline__ void sweepB_t(Loop &L, bool push, const T *m, T *r, const unsigned *offj, double *acc) {
         const unsigned G = (unsigned)in.G;
         const bool dens_mode = in.convmode != 1;
-        for (unsigned i0 = threadIdx.x; i0 < G; i0 += kUB * kBT) {
+        for (unsigned i0 = lo + threadIdx.x; i0 < hi; i0 += kUB * kBT) {
             unsigned ii[kUB];
             bool ok[kUB];
 #pragma unroll
             for (int u = 0; u < kUB; u++) {
                 const unsigned i = i0 + u * kBT;
-                ok[u] = i < G;
-                ii[u] = ok[u] ? i : G - 1;
+                ok[u] = i < hi;
+                ii[u] = ok[u] ? i : hi - 1;
             }
             T vv[kUB][NS], mv[kUB][NS], x[NC > 0 ? NC : 1][kUB][NS];
 #pragma unroll
@@ -647,7 +734,7 @@ template <typename T> struct Inst {
         }
     }
     template <int NS, bool FIRST>
-    __device__ void sweepB_nc(int nc, Loop &L, bool push, const T *m, T *r, const unsigned *offj, double *acc) {
+    __device__ __forceinline__ void sweepB_nc(int nc, Loop &L, bool push, const T *m, T *r, const unsigned *offj, double *acc) {
         switch (nc) {
         case 0: sweepB_t<NS, 0, FIRST>(L, push, m, r, offj, acc); break;
         case 1: sweepB_t<NS, 1, FIRST>(L, push, m, r, offj, acc); break;
@@ -661,7 +748,7 @@ template <typename T> struct Inst {
         }
     }
     // Results: row[0..I-1] (thread 0), ov[3 s + {0,1,2}] (every thread).
-    __device__ void sweepB(Loop &L, bool push, int ps, int head, int I, double *row, double *ov) {
+    __device__ __forceinline__ void sweepB(Loop &L, bool push, int ps, int head, int I, double *row, double *ov) {
         const long long SGn = SG();
         const int nslot = in.scope + 1, nOld = push ? I - 1 : 0;
         const bool dens_mode = in.convmode != 1;
@@ -683,7 +770,7 @@ template <typename T> struct Inst {
                 else sweepB_nc<1, false>(nc, L, push, m, r, offj, acc);
             }
             if (j0 == 0) {
-                bsum_n<kGC + 1 + 3 * kBS>(acc, kGC + 1 + 3 * kBS, sh);
+                tsum_n<kGC + 1 + 3 * kBS>(acc, kGC + 1 + 3 * kBS);
                 if (threadIdx.x == 0) {
                     for (int j = 0; j < nc; j++) row[j] = acc[j];
                     if (push) row[nOld] = acc[kGC];
@@ -692,7 +779,7 @@ template <typename T> struct Inst {
                 // V and m read; R written and the stored residuals read when pushing
                 L.bytes += ((push ? 1. + (double)nc : 0.) + ((push || dens_mode) ? 2. : 0.)) * (double)SGn * sizeof(T);
             } else {
-                bsum_n<kGC + 1 + 3 * kBS>(acc, nc, sh);
+                tsum_n<kGC + 1 + 3 * kBS>(acc, nc);
                 if (threadIdx.x == 0)
                     for (int j = 0; j < nc; j++) row[j0 + j] = acc[j];
                 L.bytes += (1. + (double)nc) * (double)SGn * sizeof(T);
@@ -703,7 +790,7 @@ template <typename T> struct Inst {
     // CheckConvergence (:4759-4782): the exit counter from this iteration's
     // CC (potential mode: max|V - V_old|; density mode: max_s RelDev weighted
     // by the species' share of N)
-    __device__ void convergence(Loop &L, double dvmax, double vmax, const double *ov) {
+    __device__ __forceinline__ void convergence(Loop &L, double dvmax, double vmax, const double *ov) {
         const double mp = 100. * 1.0e-16;
         if (L.SCcount > 0) {
             L.CC = 0.;
@@ -888,26 +975,44 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
     __shared__ double Gcache[(kMaxScope + 1) * (kMaxScope + 1)];
     __shared__ double Work[kWork];
     __shared__ int nextIdx;
-    T *base = a.slots + (long long)blockIdx.x * a.slot_stride;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) nextIdx = atomicAdd(a.next, 1);
         __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(nextIdx) >= a.n) break;
-        // queue position -> instance (largest first); uniform, so the instance's
-        // parameters are read with scalar loads into scalar registers
-        const int k = __builtin_amdgcn_readfirstlane(a.order[__builtin_amdgcn_readfirstlane(nextIdx)]);
+        const int tix = __builtin_amdgcn_readfirstlane(nextIdx);
+        if (tix >= a.n) break;
+        // the ticket names the instance (largest first), this workgroup's rank in its
+        // team and the team size; uniform, so the instance's parameters sit in scalar registers
+        const int k = __builtin_amdgcn_readfirstlane(a.tickets[tix].inst);
+        const int rank = __builtin_amdgcn_readfirstlane(a.tickets[tix].rank);
+        const int TS = __builtin_amdgcn_readfirstlane(a.tickets[tix].size);
         const unsigned long long tstart = __builtin_amdgcn_s_memrealtime();
         const BInst &in = a.inst[k];
         const long long SGn = (long long)in.S * in.G, G = in.G;
         const int S = in.S, scope = in.scope, nslot = scope + 1, ld = scope + 1;
         Inst<T> I{in};
+        I.TS = TS;
+        I.rank = rank;
+        I.tm = a.team + k;
+        I.gen = 0;
+        I.err = a.team_err;
+        I.sh = sh;
+        // a team works in its rank-0 member's field slot, each member on its own
+        // contiguous, 64-aligned share of the grid points
+        int slot_wg = blockIdx.x;
+        if (TS > 1) {
+            double v[1] = {rank == 0 ? (double)blockIdx.x : 0.};
+            I.template team_combine<1>(v, 1, false);
+            slot_wg = (int)v[0];
+        }
+        T *base = a.slots + (long long)slot_wg * a.slot_stride;
+        I.lo = rank == 0 ? 0u : ((unsigned)(G * rank / TS) & ~63u);
+        I.hi = rank == TS - 1 ? (unsigned)G : ((unsigned)(G * (rank + 1) / TS) & ~63u);
         I.Env = base;
         I.V = base + SGn;
         I.OldV = base + 2 * SGn;
         I.nin = base + 3 * SGn;
         I.res = I.nin + (long long)nslot * SGn;
-        I.sh = sh;
         I.W = Work;
         I.floor_rel = a.floor_rel;
         I.stall_max = a.stall_max;
@@ -932,7 +1037,7 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
         // InitializeEnvironments (:6362-6390): Env, and V <- Env (or the stored start); the
         // particle numbers at the initial mu in the same sweep
         double n0 = 0., n1 = 0.;
-        for (long long i = threadIdx.x; i < G; i += kBT) {
+        for (long long i = I.lo + threadIdx.x; i < I.hi; i += kBT) {
             double r[3];
             coords(in, i, r);
             const double w = trapw(in, i);
@@ -950,9 +1055,10 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             }
         }
         L.bytes = 2. * (double)SGn * sizeof(T) + (a.polish ? (double)SGn * 8. : 0.);
+        L.sbytes = L.bytes + (double)SGn * sizeof(T);  // Env, V written; N at the initial mu
         {
             double v[3] = {n0, n1, 0.};
-            bsum_n<3>(v, 2, sh);
+            I.template tsum_n<3>(v, 2);
             for (int s = 0; s < S; s++) {
                 L.TmpAb[s] = v[s] * in.trapD;
                 L.bytes += (double)G * sizeof(T);  // counted as a density evaluation
@@ -963,11 +1069,10 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
         int head = 0, mslot = 0;
         {
             T *m = I.nin;
-            for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-                const int s = idx >= G ? 1 : 0;
-                m[idx] = (T)I.tfval(L.muEval[s], (double)I.V[idx]);
-            }
+            for (long long i = I.lo + threadIdx.x; i < I.hi; i += kBT)
+                for (int s = 0; s < S; s++) m[s * G + i] = (T)I.tfval(L.muEval[s], (double)I.V[s * G + i]);
             L.bytes += 2. * (double)SGn * sizeof(T);
+            L.sbytes += 2. * (double)SGn * sizeof(T);  // initial density: V in, n out
         }
         __syncthreads();
         int mix = 0, mold = 0;  // the pending mixing of the previous iteration (0: none)
@@ -986,6 +1091,8 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             double N[kBS] = {0., 0.}, dvmax = 0., vmax = 0.;
             unsigned long long c0 = __builtin_amdgcn_s_memrealtime();
             I.sweepA(L, mix, mold, mslot, head, coef, true, N, dvmax, vmax);
+            // stage bytes: UpdateV (Env and the other species' density in, V out), N at the current mu
+            L.sbytes += ((in.contact && S == 2) ? 3. : 2.) * (double)SGn * sizeof(T) + (double)SGn * sizeof(T);
             unsigned long long c1 = __builtin_amdgcn_s_memrealtime();
             tph[0] += c1 - c0;
             if (L.SCcount > 0) L.incV += in.incV0;
@@ -1013,6 +1120,7 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             }
             double ov[3 * kBS];
             I.sweepB(L, tryPulay, mslot, nhead, pI, row, ov);
+            L.sbytes += 4. * (double)SGn * sizeof(T);  // density (V in, n out); convergence (two fields in)
             c1 = __builtin_amdgcn_s_memrealtime();
             tph[2] += c1 - c0;
             I.convergence(L, dvmax, vmax, ov);
@@ -1061,6 +1169,10 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
                 mold = mslot;  // linear mixing in place
             if (ok) L.PulMix++;
             else L.LinMix++;
+            // stage bytes of AdmixDensities: Gram row (I + 1 fields) and Pulay combination (2I + 1),
+            // or linear mixing (n, n_old in, n out)
+            L.sbytes += (tryPulay && L.CC < 1.0e+300 ? (double)(L.pI + 1) : 0.) * (double)SGn * sizeof(T) +
+                        (ok ? (double)(2 * L.pI + 1) : 3.) * (double)SGn * sizeof(T);
             mix = ok ? 2 : 1;
             if (L.TrappedQ)
                 for (int s = 0; s < S; s++) L.theta[s] *= 1. / mod;
@@ -1083,8 +1195,9 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
         double Eint = 0.;
         if (in.contact && S > 1) {
             double acc = 0.;
-            for (long long i = threadIdx.x; i < G; i += kBT) acc += trapw(in, i) * (double)Den[i] * (double)Den[G + i];
-            Eint = in.beta * bsum(acc, sh) * in.trapD;
+            for (long long i = I.lo + threadIdx.x; i < I.hi; i += kBT)
+                acc += trapw(in, i) * (double)Den[i] * (double)Den[G + i];
+            Eint = in.beta * I.tsum(acc) * in.trapD;
         }
         BOut o{};
         double Etot = Eint;
@@ -1092,7 +1205,7 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
         for (int s = 0; s < S; s++) {
             const T *n = Den + (long long)s * G, *v = I.V + (long long)s * G, *e = I.Env + (long long)s * G;
             double ee = 0., ed = 0., ea = 0.;
-            for (long long i = threadIdx.x; i < G; i += kBT) {
+            for (long long i = I.lo + threadIdx.x; i < I.hi; i += kBT) {
                 double w = trapw(in, i), nn = (double)n[i];
                 ee += w * (double)e[i] * nn;
                 double tt = L.mu[s] - (double)v[i];
@@ -1100,7 +1213,7 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
                 ea += w * pow(nn > 0. ? nn : 0., (DD + 2.) / DD);
             }
             double v3[3] = {ee, ed, ea};
-            bsum_n<3>(v3, 3, sh);
+            I.template tsum_n<3>(v3, 3);
             ee = v3[0] * in.trapD;
             ed = v3[1] * in.trapD;
             ea = v3[2] * in.trapD;
@@ -1112,13 +1225,20 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             Etot += disp + ee;
         }
         L.bytes += ((in.contact && S > 1) ? 2. * (double)G : 0.) * sizeof(T) + 3. * (double)SGn * sizeof(T);
+        L.sbytes += ((in.contact && S > 1) ? 2. * (double)G : 0.) * sizeof(T) + 3. * (double)SGn * sizeof(T) +
+                    4. * (double)SGn * sizeof(T);  // energies; retire as a copy (two fields in, two out)
         // retire: fields and results out
         L.bytes += 2. * (double)SGn * sizeof(T) + 2. * (double)SGn * 8.;
-        for (long long idx = threadIdx.x; idx < SGn; idx += kBT) {
-            outDen[idx] = (double)Den[idx];
-            outV[idx] = (double)I.V[idx];
+        for (long long i = I.lo + threadIdx.x; i < I.hi; i += kBT)
+            for (int s = 0; s < S; s++) {
+                outDen[s * G + i] = (double)Den[s * G + i];
+                outV[s * G + i] = (double)I.V[s * G + i];
+            }
+        if (TS > 1) {  // the rank-0 slot is free again only when every member has copied its share out
+            double v[1] = {0.};
+            I.template tsum_n<1>(v, 1);
         }
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && rank == 0) {
             o.SCcount = L.SCcount;
             o.converged = L.ExitSCcount >= 3;
             o.LinMix = L.LinMix;
@@ -1133,6 +1253,7 @@ __global__ __launch_bounds__(kBT) void k_batch_solve(BArgs<T> a) {
             o.Etot = Etot;
             o.Eint = Eint;
             o.bytes = L.bytes;
+            o.sbytes = L.sbytes;
             o.t0 = tstart;
             o.t1 = __builtin_amdgcn_s_memrealtime();
             for (int q = 0; q < 4; q++) o.tph[q] = tph[q];
@@ -1155,9 +1276,12 @@ struct mpdpft_batch {
     std::vector<mpdpft_params> params;
     BInst *d_inst = nullptr;
     BOut *d_out = nullptr;
-    int *d_next = nullptr, *d_order = nullptr;
+    int *d_next = nullptr, *d_team_err = nullptr;
+    Ticket *d_tickets = nullptr;
+    TeamArea *d_team = nullptr;
+    int n_tickets = 0, n_teams = 0;
     hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [phase][start, stop] around k_batch_solve
-    double kernel_ms[2] = {0., 0.}, kernel_bytes[2] = {0., 0.};
+    double kernel_ms[2] = {0., 0.}, kernel_bytes[2] = {0., 0.}, stage_bytes[2] = {0., 0.};
     void *d_slots = nullptr;
     double *d_fields = nullptr, *d_start = nullptr;
     std::vector<BOut> out;
@@ -1278,9 +1402,19 @@ extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, in
     // slots: resident workgroups; at most one per instance
     hipDeviceProp_t prop;
     BHIP(hipGetDeviceProperties(&prop, b->device));
+    // slots: the workgroups resident at once (a persistent launch larger than that only adds
+    // workgroups that start after the queue is empty)
+    int per_cu = 0;
+    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_batch_solve<double>),
+                                                      kBT, 0));
+    int per_cu32 = 0;
+    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu32, reinterpret_cast<const void *>(&k_batch_solve<float>),
+                                                      kBT, 0));
+    per_cu = std::max(1, std::min(per_cu, per_cu32));
     const char *env = std::getenv("MPDPFT_BATCH_SLOTS");
-    int want = env ? std::atoi(env) : 2 * prop.multiProcessorCount;
-    b->nslots = std::max(1, std::min(n, want));
+    int want = env ? std::atoi(env) : per_cu * prop.multiProcessorCount;
+    want = std::min(want, per_cu * prop.multiProcessorCount);
+    b->nslots = std::max(1, std::min(n * kTeamMax, want));
     BHIP(hipStreamCreate(&b->st));
     BHIP(hipMalloc((void **)&b->d_inst, sizeof(BInst) * n));
     BHIP(hipMalloc((void **)&b->d_out, sizeof(BOut) * n));
@@ -1297,8 +1431,28 @@ extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, in
     if (!(ord && std::strcmp(ord, "index") == 0)) std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return (long long)b->inst[x].S * b->inst[x].G > (long long)b->inst[y].S * b->inst[y].G;
     });
-    BHIP(hipMalloc((void **)&b->d_order, sizeof(int) * n));
-    BHIP(hipMemcpy(b->d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    // teams: an instance of at least team_points grid points is solved by kTeamMax workgroups
+    // (its SC loop would otherwise outlast the rest of the batch on one CU).  A team's tickets
+    // are consecutive and tickets are taken in order by running workgroups, so at most the
+    // workgroup holding the newest ticket waits for a partner that is not yet running, and
+    // that partner starts as soon as any other workgroup retires an instance
+    const char *tp = std::getenv("MPDPFT_BATCH_TEAM_POINTS");  // 0: no teams
+    const long long team_points = tp ? std::atoll(tp) : 40000;
+    int team_inst = 0;
+    for (int i = 0; i < n; i++)
+        if (team_points > 0 && b->inst[i].G >= team_points) team_inst++;
+    const bool teams = team_inst > 0 && b->nslots >= kTeamMax;
+    std::vector<Ticket> tickets;
+    for (int i : order) {
+        const int ts = (teams && b->inst[i].G >= team_points) ? kTeamMax : 1;
+        for (int r = 0; r < ts; r++) tickets.push_back(Ticket{i, r, ts, 0});
+    }
+    b->n_tickets = (int)tickets.size();
+    b->n_teams = teams ? team_inst : 0;
+    BHIP(hipMalloc((void **)&b->d_tickets, sizeof(Ticket) * tickets.size()));
+    BHIP(hipMemcpy(b->d_tickets, tickets.data(), sizeof(Ticket) * tickets.size(), hipMemcpyHostToDevice));
+    BHIP(hipMalloc((void **)&b->d_team, sizeof(TeamArea) * n));
+    BHIP(hipMalloc((void **)&b->d_team_err, sizeof(int)));
     for (int ph = 0; ph < 2; ph++)
         for (int e = 0; e < 2; e++) BHIP(hipEventCreate(&b->ev[ph][e]));
     b->out.resize(n);
@@ -1319,8 +1473,10 @@ template <typename T> static int run(mpdpft_batch *b, int polish, int phase) {
     const long long per = (3LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
     BArgs<T> a;
     a.inst = b->d_inst;
-    a.order = b->d_order;
-    a.n = b->n;
+    a.tickets = b->d_tickets;
+    a.n = b->n_tickets;
+    a.team = b->d_team;
+    a.team_err = b->d_team_err;
     a.next = b->d_next;
     a.slots = (T *)b->d_slots;
     a.slot_stride = per;  // values of T per slot (the arena is sized for doubles)
@@ -1333,6 +1489,8 @@ template <typename T> static int run(mpdpft_batch *b, int polish, int phase) {
     a.stall_max = sizeof(T) == 4 ? kF32Stall : 0;
     a.relacc_floor = sizeof(T) == 4 ? kF32RelAcc : 0.;
     BHIP(hipMemsetAsync(b->d_next, 0, sizeof(int), b->st));
+    BHIP(hipMemsetAsync(b->d_team, 0, sizeof(TeamArea) * b->n, b->st));
+    BHIP(hipMemsetAsync(b->d_team_err, 0, sizeof(int), b->st));
     BHIP(hipEventRecord(b->ev[phase][0], b->st));
     hipLaunchKernelGGL(k_batch_solve<T>, dim3(b->nslots), dim3(kBT), 0, b->st, a);
     BHIP(hipGetLastError());
@@ -1350,12 +1508,17 @@ extern "C" int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *res) {
     b->phase_iterations[0] = b->phase_iterations[1] = 0;
     b->phase_evaluations[0] = b->phase_evaluations[1] = 0;
     b->kernel_ms[0] = b->kernel_ms[1] = b->kernel_bytes[0] = b->kernel_bytes[1] = 0.;
+    b->stage_bytes[0] = b->stage_bytes[1] = 0.;
     auto count = [&](int ph) -> int {  // iterations, bytes and kernel time of the phase just run
+        int terr = 0;
+        BHIP(hipMemcpy(&terr, b->d_team_err, sizeof(int), hipMemcpyDeviceToHost));
+        if (terr) return bfail(b, MPDPFT_EHIP, "a team barrier of the batch kernel did not complete");
         BHIP(hipMemcpy(b->out.data(), b->d_out, sizeof(BOut) * b->n, hipMemcpyDeviceToHost));
         for (int i = 0; i < b->n; i++) {
             b->phase_iterations[ph] += b->out[i].SCcount;
             b->phase_evaluations[ph] += b->out[i].bisection;
             b->kernel_bytes[ph] += b->out[i].bytes;
+            b->stage_bytes[ph] += b->out[i].sbytes;
         }
         float ms = 0.f;
         BHIP(hipEventElapsedTime(&ms, b->ev[ph][0], b->ev[ph][1]));
@@ -1438,12 +1601,13 @@ extern "C" int mpdpft_batch_phase_stats(const mpdpft_batch *b, double *seconds, 
     return MPDPFT_OK;
 }
 
-extern "C" int mpdpft_batch_kernel_stats(const mpdpft_batch *b, double *kernel_ms, double *bytes,
-                                         double *instance_us) {
+extern "C" int mpdpft_batch_kernel_stats(const mpdpft_batch *b, double *kernel_ms, double *stage_bytes,
+                                         double *kernel_bytes, double *instance_us) {
     if (!b) return MPDPFT_EINVAL;
     for (int ph = 0; ph < 2; ph++) {
         if (kernel_ms) kernel_ms[ph] = b->kernel_ms[ph];
-        if (bytes) bytes[ph] = b->kernel_bytes[ph];
+        if (stage_bytes) stage_bytes[ph] = b->stage_bytes[ph];
+        if (kernel_bytes) kernel_bytes[ph] = b->kernel_bytes[ph];
     }
     if (instance_us)  // the last phase's per-instance time between being taken from the queue and retiring
         for (int i = 0; i < b->n; i++) {
@@ -1463,7 +1627,9 @@ extern "C" void mpdpft_batch_destroy(mpdpft_batch *b) {
     (void)hipFree(b->d_inst);
     (void)hipFree(b->d_out);
     (void)hipFree(b->d_next);
-    (void)hipFree(b->d_order);
+    (void)hipFree(b->d_tickets);
+    (void)hipFree(b->d_team);
+    (void)hipFree(b->d_team_err);
     for (int ph = 0; ph < 2; ph++)
         for (int e = 0; e < 2; e++)
             if (b->ev[ph][e]) (void)hipEventDestroy(b->ev[ph][e]);

@@ -83,3 +83,28 @@ def test_batch_fp32_with_fp64_polish():
         assert abs(rb[i].TotalEnergy - r.TotalEnergy) <= 1e-6 * abs(r.TotalEnergy)
         assert rel(bd, den) < 1e-6
     b.close()
+
+
+def test_batch_teams_match_single_workgroup_solves(monkeypatch):
+    """Instances of 256^2 points are solved by two-workgroup teams (each member
+    sweeps half the points, reductions exchanged through global memory); the
+    results equal the one-workgroup solves of the same instances to the FP64
+    tolerance (the reductions combine in another order)."""
+    print("case batch teams", flush=True)
+    insts = configs.c3(n=4, seed=9, grids=(256, 256, 256))
+    b = Batch(insts)
+    rt = b.solve()
+    ft = [b.fields(i) for i in range(len(insts))]
+    b.close()
+    monkeypatch.setenv("MPDPFT_BATCH_TEAM_POINTS", "0")
+    b = Batch(insts)
+    r1 = b.solve()
+    for i in range(len(insts)):
+        d, V = b.fields(i)
+        print("instance %d: SCcount team %d single %d, rel E %.2e" % (
+            i, rt[i].SCcount, r1[i].SCcount, abs(rt[i].TotalEnergy - r1[i].TotalEnergy) / abs(r1[i].TotalEnergy)),
+            flush=True)
+        assert rt[i].converged and abs(rt[i].SCcount - r1[i].SCcount) <= 1
+        assert abs(rt[i].TotalEnergy - r1[i].TotalEnergy) <= RTOL * abs(r1[i].TotalEnergy)
+        assert rel(ft[i][0], d) < RTOL and rel(ft[i][1], V) < RTOL
+    b.close()
