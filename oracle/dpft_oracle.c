@@ -33,11 +33,22 @@
  *    Fermi gas, the deep-well limit (Airy -> TF, Laplacian term included) and
  *    its first moment correction (int t^3 Ai = 2).
  *  - mu at convergence: the root of N(mu) = N on the exported potential
- *    (scipy brentq), to RelAcc.
- *  - mu bisection path, convergence, mixing: invariants (monotone N(mu), exit rule,
- *    linear/Pulay fixed points) -- parity unpinned beyond these invariants and
- *    the full-loop closed forms; the same holds for AdaptnAiT and for
- *    EkinAiT03D's outer MaxIteration loop.
+ *    (scipy brentq), to RelAcc, for TF and for nAiT (converged C1 at 16^3).
+ *  - EnforceConstraints on a traced search (oracle_enforce_trace): N(mu)
+ *    monotone over every evaluation, walk steps in one direction growing as
+ *    (1 + k), the root inside the last walk step, bisection steps halving,
+ *    exit at the first |N - N0| / 2N0 <= RelAcc; 20 failed walk steps restore
+ *    mu and count a bracket failure.
+ *  - CheckConvergence (oracle_convergence_sequence): the overlap measure's
+ *    closed form (1 - c)^2 / (1 + c^2) x N_s / N for n -> c n, CC(n, n) = 0
+ *    and the steady rule, the potential-mode maximum, ExitSCcount >= 3 with
+ *    a reset on a step above the criterion.
+ *  - EkinAiT03D with its outer MaxIteration loop: the converged C1 16^3
+ *    energy against scipy quadrature of the whole field (5e-12).
+ *  - Mixing: the linear/Pulay fixed points and the full-pivot QR solve above;
+ *    the Pulay history bookkeeping is pinned only through the full-loop
+ *    closed forms.  AdaptnAiT's Mpp increments: parity unpinned beyond the
+ *    converged-density checks of the C1 and C2 solves.
  */
 #include "dpft_oracle.h"
 
@@ -243,6 +254,8 @@ struct orc_state {
     double DeltamuModifier, RelAcc, RelAccOriginal, incrementalV, incrementalVOriginal, CC;
     int SCcount, ExitSCcount, ExitSCloop, LinMix, PulMix, TrappedQ;
     int warningCount, errorCount, bisection_steps, bracket_failures, pulay_rejections;
+    double *trace_mu, *trace_N; /* optional record of the mu search's density evaluations */
+    int *trace_phase, trace_max, trace_n;
     /* Pulay */
     int pulay_mixer, pulay_scope, pulay_I;
     double **nin, **nout, **res; /* history entries, each [S][G] */
@@ -1189,6 +1202,16 @@ int oracle_density(const mpdpft_params *p, int s, double mu, const double *v, do
     return rc;
 }
 
+/* Record of the mu search (test probe): mu and N of each density evaluation
+ * and its phase (1 walking mu down, 2 walking up, 3 bisection). */
+static void trace_eval(orc_state *o, int s, int phase) {
+    if (!o->trace_mu || o->trace_n >= o->trace_max) return;
+    o->trace_mu[o->trace_n] = o->muVec[s];
+    o->trace_N[o->trace_n] = o->TmpAb[s];
+    o->trace_phase[o->trace_n] = phase;
+    o->trace_n++;
+}
+
 /* EnforceConstraints (mpDPFT.cpp:4569-4743): mu bracketing then bisection. */
 static int enforce_constraints(orc_state *o) {
     int S = o->S;
@@ -1234,6 +1257,7 @@ static int enforce_constraints(orc_state *o) {
                     get_density(o, s);
                     o->bisection_steps++;
                     o->TmpAb[s] = integrate(o, o->Den + (long long)s * o->G);
+                    trace_eval(o, s, 1);
                     rc++;
                     if (rc == 1) {
                         double diff1 = fabs(o->TmpAb[s] - Ab);
@@ -1264,6 +1288,7 @@ static int enforce_constraints(orc_state *o) {
                         get_density(o, s);
                         o->bisection_steps++;
                         o->TmpAb[s] = integrate(o, o->Den + (long long)s * o->G);
+                        trace_eval(o, s, 2);
                         rc++;
                         if (rc == 1) {
                             double diff1 = fabs(o->TmpAb[s] - Ab);
@@ -1299,6 +1324,7 @@ static int enforce_constraints(orc_state *o) {
                     get_density(o, s);
                     o->bisection_steps++;
                     o->TmpAb[s] = integrate(o, o->Den + (long long)s * o->G);
+                    trace_eval(o, s, 3);
                     if (o->TmpAb[s] > 0. && nh < 4096) hist[nh++] = o->TmpAb[s];
                     int trap = 0;
                     if (nh > 5) {
@@ -1998,6 +2024,60 @@ int oracle_solve_v0(const mpdpft_params *p, const double *v0, mpdpft_result *r, 
  * mu (in/out), with the scalar state of a loop's start (:3306-3316,
  * :4569-4743); N, den ([S][G]) out; *nobracket counts the species whose mu
  * bracket was not found.  The oracle of mpdpft_enforce_constraints. */
+int oracle_enforce_trace(const mpdpft_params *p, const double *v, double *mu, double *N, double *den,
+                         int *nobracket, double *trace_mu, double *trace_N, int *trace_phase, int trace_max,
+                         int *trace_n) {
+    orc_state *o = orc_create(p);
+    o->trace_mu = trace_mu;
+    o->trace_N = trace_N;
+    o->trace_phase = trace_phase;
+    o->trace_max = trace_max;
+    memcpy(o->V, v, sizeof(double) * (size_t)(o->S * o->G));
+    for (int s = 0; s < o->S; s++) o->muVec[s] = mu[s];
+    int rc = get_densities(o);
+    for (int s = 0; s < o->S; s++) {
+        mu[s] = o->muVec[s];
+        if (N) N[s] = o->TmpAb[s];
+    }
+    if (den) memcpy(den, o->Den, sizeof(double) * (size_t)(o->S * o->G));
+    if (nobracket) *nobracket = o->bracket_failures;
+    if (trace_n) *trace_n = o->trace_n;
+    orc_free(o);
+    return rc;
+}
+
+/* CheckConvergence (:4759-4782) over a sequence of loop states (test probe):
+ * state k has density dens[k], potential Vs[k] ([S][G] each), particle
+ * numbers Ns[k] and mu mus[k] ([S]); iteration k compares state k with state
+ * k - 1 (k = 0 is SCcount 0).  CC per iteration out; returns the iteration
+ * after which the loop exits (ExitSCcount reaching 3), or -1. */
+int oracle_convergence_sequence(const mpdpft_params *p, int n, const double *dens, const double *Vs, const double *Ns,
+                                const double *mus, double *CC) {
+    orc_state *o = orc_create(p);
+    const long long SG = (long long)o->S * o->G;
+    int exit_at = -1;
+    for (int k = 0; k < n && exit_at < 0; k++) {
+        o->SCcount = k;
+        memcpy(o->Den, dens + k * SG, sizeof(double) * (size_t)SG);
+        memcpy(o->V, Vs + k * SG, sizeof(double) * (size_t)SG);
+        if (k > 0) {
+            memcpy(o->OldDen, dens + (k - 1) * SG, sizeof(double) * (size_t)SG);
+            memcpy(o->OldV, Vs + (k - 1) * SG, sizeof(double) * (size_t)SG);
+        }
+        for (int s = 0; s < o->S; s++) {
+            o->OldAb[s] = k > 0 ? Ns[(k - 1) * o->S + s] : 0.;
+            o->OldmuVec[s] = k > 0 ? mus[(k - 1) * o->S + s] : 0.;
+            o->TmpAb[s] = Ns[k * o->S + s];
+            o->muVec[s] = mus[k * o->S + s];
+        }
+        check_convergence(o);
+        CC[k] = o->CC;
+        if (o->ExitSCloop) exit_at = k;
+    }
+    orc_free(o);
+    return exit_at;
+}
+
 int oracle_enforce(const mpdpft_params *p, const double *v, double *mu, double *N, double *den, int *nobracket) {
     orc_state *o = orc_create(p);
     memcpy(o->V, v, sizeof(double) * (size_t)(o->S * o->G));

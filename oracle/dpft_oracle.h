@@ -65,6 +65,11 @@ double oracle_exp_i0(double x);
 
 /* Timed partial workload for the CPU baseline: n_iter SC iterations (no
  * convergence test) starting from the initial density; returns seconds. */
+int oracle_enforce_trace(const mpdpft_params *p, const double *v, double *mu, double *N, double *den,
+                         int *nobracket, double *trace_mu, double *trace_N, int *trace_phase, int trace_max,
+                         int *trace_n);
+int oracle_convergence_sequence(const mpdpft_params *p, int n, const double *dens, const double *Vs, const double *Ns,
+                                const double *mus, double *CC);
 int oracle_enforce(const mpdpft_params *p, const double *v, double *mu, double *N, double *den, int *nobracket);
 int oracle_mix_sequence(const mpdpft_params *p, int nsteps, const double *n_in0, const double *n_outs, double *out,
                         int *rejections);

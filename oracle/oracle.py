@@ -54,6 +54,9 @@ def lib():
         L.oracle_hartree_isolated_direct.argtypes = [P, dp, dp]
         L.oracle_cube_inverse_r.restype = ctypes.c_double
         L.oracle_enforce.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_enforce_trace.argtypes = [P, dp, dp, dp, dp, ip, dp, dp, ip, ctypes.c_int, ip]
+        L.oracle_convergence_sequence.argtypes = [P, ctypes.c_int, dp, dp, dp, dp, dp]
         L.oracle_mix_sequence.argtypes = [P, ctypes.c_int, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
         L.oracle_time_iterations.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.oracle_time_iterations.restype = ctypes.c_double
@@ -150,6 +153,36 @@ def enforce(p, v, mu):
     rc = lib().oracle_enforce(ctypes.byref(p), _dp(v), _dp(m), _dp(N), _dp(den), ctypes.byref(nb))
     assert rc == 0, rc
     return m, N, den.reshape(p.S, G), nb.value
+
+
+def enforce_trace(p, v, mu, max_evals=4096):
+    """enforce() with the record of its density evaluations: (mu, N, bracket
+    failures, trace) where trace is a list of (phase, mu, N) with phase 1 =
+    walking mu down, 2 = walking up, 3 = bisection."""
+    G = grid_size(p)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    m = np.ascontiguousarray(mu, dtype=np.float64).copy()
+    N = np.zeros(p.S)
+    tm, tN = np.zeros(max_evals), np.zeros(max_evals)
+    tp = (ctypes.c_int * max_evals)()
+    nb, nt = ctypes.c_int(), ctypes.c_int()
+    rc = lib().oracle_enforce_trace(ctypes.byref(p), _dp(v), _dp(m), _dp(N), None, ctypes.byref(nb), _dp(tm), _dp(tN),
+                                    tp, max_evals, ctypes.byref(nt))
+    assert rc == 0, rc
+    return m, N, nb.value, [(tp[i], tm[i], tN[i]) for i in range(nt.value)]
+
+
+def convergence_sequence(p, dens, Vs, Ns, mus):
+    """CheckConvergence over states k = 0..n-1 (arrays [n][S][G], [n][S][G],
+    [n][S], [n][S]): (CC per iteration, exit iteration or -1)."""
+    dens = np.ascontiguousarray(dens, dtype=np.float64)
+    Vs = np.ascontiguousarray(Vs, dtype=np.float64)
+    Ns = np.ascontiguousarray(Ns, dtype=np.float64)
+    mus = np.ascontiguousarray(mus, dtype=np.float64)
+    n = dens.shape[0]
+    CC = np.zeros(n)
+    ex = lib().oracle_convergence_sequence(ctypes.byref(p), n, _dp(dens), _dp(Vs), _dp(Ns), _dp(mus), _dp(CC))
+    return CC, ex
 
 
 def mix_sequence(p, n_in0, n_outs):
