@@ -365,6 +365,36 @@ int mpdpft_batch_kernel_stats(const mpdpft_batch *b, double *kernel_ms, double *
 const char *mpdpft_batch_last_error(const mpdpft_batch *b);
 void mpdpft_batch_destroy(mpdpft_batch *b);
 
+/* ---- batched solve without device allocation ----------------------------- */
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  mpdpft_batch_create + mpdpft_batch_solve +
+ * mpdpft_batch_get_fields in one call, same kernels and same results, with
+ * no hipMalloc, hipFree, stream or event creation: the instance descriptors,
+ * results, queue, team areas, the FP32 phase's start copy and each running
+ * workgroup's field slot live in one static device array of
+ * MPDPFT_NOALLOC_SCRATCH_BYTES per GPU, which the library holds until the
+ * process ends.  The solve runs as many workgroups at once as slots of the
+ * largest instance fit beside the descriptors (a 256^2 two-species instance
+ * with Pulay scope 8 needs 22 MB per slot, a 64^2 one 1.4 MB); instances that
+ * leave no room for one slot are rejected with MPDPFT_EINVAL.
+ *   instances  n host parameter sets (n <= MPDPFT_NOALLOC_MAX_INSTANCES), as
+ *              for mpdpft_batch_create; flags 0 or MPDPFT_BATCH_FP32
+ *   fields     device memory of `device` (fields_len doubles) that receives
+ *              each instance's converged [S][G] density then [S][G] potential,
+ *              instance after instance (instance i at sum_{j<i} 2 S_j G_j)
+ *   results    n host results, as mpdpft_batch_solve reports them
+ *   err        optional (may be NULL) buffer of err_len bytes for the message
+ *              of a failure
+ * Runs on the device's null stream and returns after the solve finished.
+ * Every call on one GPU uses that GPU's one array: calls that overlapped on a
+ * GPU would race, so the library runs a process's calls one at a time; other
+ * processes hold their own arrays. */
+#define MPDPFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define MPDPFT_NOALLOC_MAX_INSTANCES 4096
+int mpdpft_batch_solve_noalloc(const mpdpft_params *instances, int n, int device, int flags, double *fields,
+                               size_t fields_len, mpdpft_result *results, char *err, size_t err_len);
+
 /* ---- drop-in: run in a directory like ./mpDPFT --------------------------- */
 /* Reads dir/mpDPFT.input (and dir/mpDPFT_V.dat when InterpolVQ = 1), polls
  * dir/mpDPFTmanualSCloopBreakQ.dat, solves, writes dir/mpDPFT_Den.dat,

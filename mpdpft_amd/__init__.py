@@ -10,8 +10,8 @@ import numpy as np
 from . import _abi
 from ._abi import Params, Result, load_library
 
-__all__ = ["Params", "Result", "default_params", "parse_input", "Context", "Batch", "run_dir", "load_library",
-           "MpdpftError"]
+__all__ = ["Params", "Result", "default_params", "parse_input", "Context", "Batch", "batch_solve_noalloc",
+           "batch_fields_len", "run_dir", "load_library", "MpdpftError"]
 
 
 class MpdpftError(RuntimeError):
@@ -287,6 +287,29 @@ class Batch:
         if rc != _abi.OK:
             raise MpdpftError(rc, "batch_get_fields")
         return den.reshape(p.S, -1), V.reshape(p.S, -1)
+
+
+def batch_fields_len(instances):
+    """Doubles the converged fields of these instances take in the output of
+    batch_solve_noalloc (instance i's [S][G] density then [S][G] potential at
+    the sum of the earlier instances' 2 S G)."""
+    return sum(2 * p.S * (p.steps + 1) ** p.DIM for p in instances)
+
+
+def batch_solve_noalloc(instances, fields_ptr, fields_len, device=0, fp32=False):
+    """The batched solve without device allocation (mpdpft_batch_solve_noalloc):
+    fields_ptr is the address of device memory on `device` holding fields_len
+    doubles, which receives the converged fields; returns the results."""
+    lib = load_library()
+    n = len(instances)
+    arr = (Params * n)(*instances)
+    res = (Result * n)()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.mpdpft_batch_solve_noalloc(arr, n, device, _abi.BATCH_FP32 if fp32 else 0, ctypes.c_void_p(fields_ptr),
+                                        fields_len, res, err, len(err))
+    if rc != _abi.OK:
+        raise MpdpftError(rc, "batch_solve_noalloc: " + err.value.decode())
+    return list(res)
 
 
 def import_v(path, params):

@@ -86,9 +86,11 @@ EXPORTS = [
     "mpdpft_batch_phase_stats", "mpdpft_batch_kernel_stats", "mpdpft_batch_last_error", "mpdpft_batch_destroy",
     "mpdpft_adjoint", "mpdpft_centre_gradient", "mpdpft_centre_forces", "mpdpft_set_centres",
     "mpdpft_loop_begin", "mpdpft_step", "mpdpft_energy", "mpdpft_enforce_constraints", "mpdpft_mix",
-    "mpdpft_inject_fault",
+    "mpdpft_inject_fault", "mpdpft_batch_solve_noalloc",
 ]
 BATCH_FP32 = 1
+NOALLOC_SCRATCH_BYTES = 64 << 20
+NOALLOC_MAX_INSTANCES = 4096
 FAULT_DUPLICATE_PULAY = 1
 FAULT_NAN_POTENTIAL = 2
 
@@ -167,6 +169,9 @@ def load_library(path=None):
     lib.mpdpft_batch_last_error.restype = ctypes.c_char_p
     lib.mpdpft_batch_destroy.argtypes = [vp]
     lib.mpdpft_batch_destroy.restype = None
+    lib.mpdpft_batch_solve_noalloc.argtypes = [P, _i, _i, _i, vp, ctypes.c_size_t, R, ctypes.c_char_p,
+                                               ctypes.c_size_t]
+    lib.mpdpft_batch_solve_noalloc.restype = _i
     lib.mpdpft_adjoint.argtypes = [vp, dp, dp, ctypes.POINTER(AdjointInfo)]
     lib.mpdpft_adjoint.restype = _i
     lib.mpdpft_centre_gradient.argtypes = [vp, dp, dp]

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1298,8 +1299,11 @@ static int bfail(mpdpft_batch *b, int rc, const std::string &m) {
 
 // mpdpft_params of one instance -> BInst (the set-up of mpdpft_create,
 // InitializeHardCodedParameters :863-896, for the TF family).
-static int prepare(mpdpft_batch *b, const mpdpft_params &p, BInst &o, int i) {
-    auto bad = [&](const std::string &m) { return bfail(b, MPDPFT_EINVAL, "instance " + std::to_string(i) + ": " + m); };
+static int prepare(char (&msg)[256], const mpdpft_params &p, BInst &o, int i) {
+    auto bad = [&](const char *m) {
+        std::snprintf(msg, sizeof msg, "instance %d: %s", i, m);
+        return MPDPFT_EINVAL;
+    };
     if (const char *why = params_structure_error(&p)) return bad(why);
     if (p.DIM != 2 && p.DIM != 3) return bad("DIM must be 2 or 3");
     if (p.S < 1 || p.S > kBS) return bad("S must be 1 or 2");
@@ -1377,86 +1381,93 @@ static int prepare(mpdpft_batch *b, const mpdpft_params &p, BInst &o, int i) {
         if (e_ != hipSuccess) return bfail(b, MPDPFT_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, int flags, mpdpft_batch **out) {
-    if (!p || n < 1 || !out) return MPDPFT_EINVAL;
-    mpdpft_batch *b = new mpdpft_batch();
-    *out = b;
-    b->n = n;
-    b->flags = flags;
-    b->inst.resize(n);
-    b->params.assign(p, p + n);
-    long long off = 0;
-    for (int i = 0; i < n; i++) {  // every instance is checked before the device is touched
-        int rc = prepare(b, p[i], b->inst[i], i);
-        if (rc) return rc;
-        b->inst[i].out_off = off;
-        off += 2LL * b->inst[i].S * b->inst[i].G;
-        b->Gmax = std::max(b->Gmax, b->inst[i].G);
-        b->scope = std::max(b->scope, b->inst[i].scope);
-    }
-    b->arena = off;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(b, MPDPFT_ENODEV, "no GPU visible");
-    b->device = device < 0 ? 0 : device;
-    BHIP(hipSetDevice(b->device));
-    // slots: resident workgroups; at most one per instance
-    hipDeviceProp_t prop;
-    BHIP(hipGetDeviceProperties(&prop, b->device));
-    // slots: the workgroups resident at once (a persistent launch larger than that only adds
-    // workgroups that start after the queue is empty)
-    int per_cu = 0;
-    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_batch_solve<double>),
-                                                      kBT, 0));
-    int per_cu32 = 0;
-    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu32, reinterpret_cast<const void *>(&k_batch_solve<float>),
-                                                      kBT, 0));
-    per_cu = std::max(1, std::min(per_cu, per_cu32));
-    const char *env = std::getenv("MPDPFT_BATCH_SLOTS");
-    int want = env ? std::atoi(env) : per_cu * prop.multiProcessorCount;
-    want = std::min(want, per_cu * prop.multiProcessorCount);
-    b->nslots = std::max(1, std::min(n * kTeamMax, want));
-    BHIP(hipStreamCreate(&b->st));
-    BHIP(hipMalloc((void **)&b->d_inst, sizeof(BInst) * n));
-    BHIP(hipMalloc((void **)&b->d_out, sizeof(BOut) * n));
-    BHIP(hipMalloc((void **)&b->d_next, sizeof(int)));
-    BHIP(hipMalloc((void **)&b->d_fields, sizeof(double) * (size_t)off));
-    const long long per = (3LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
-    BHIP(hipMalloc(&b->d_slots, sizeof(double) * (size_t)per * b->nslots));
-    BHIP(hipMemcpy(b->d_inst, b->inst.data(), sizeof(BInst) * n, hipMemcpyHostToDevice));
-    // queue order: largest instance first (S G decides the cost of an SC iteration), ties by index,
-    // so the longest solves start at once and the short ones fill in behind them
-    std::vector<int> order(n);
+// Queue order and teams, shared by both solve entry points.  Order: largest
+// instance first (S G decides the cost of an SC iteration), ties by index, so
+// the longest solves start at once and the short ones fill in behind them
+// (MPDPFT_BATCH_ORDER=index: the instances' own order).  Teams: an instance of
+// at least team_points grid points is solved by kTeamMax workgroups (its SC
+// loop would otherwise outlast the rest of the batch on one CU).  A team's
+// tickets are consecutive and tickets are taken in order by running
+// workgroups, so at most the workgroup holding the newest ticket waits for a
+// partner that is not yet running, and that partner starts as soon as any
+// other workgroup retires an instance; with fewer than kTeamMax slots there
+// are no teams.  order: n ints of work space; tickets: room for kTeamMax n.
+// Returns the number of tickets.
+static int make_tickets(const BInst *inst, int n, int nslots, int *order, Ticket *tickets, int *n_teams) {
     for (int i = 0; i < n; i++) order[i] = i;
-    const char *ord = std::getenv("MPDPFT_BATCH_ORDER");  // "index": the instances' own order
-    if (!(ord && std::strcmp(ord, "index") == 0)) std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return (long long)b->inst[x].S * b->inst[x].G > (long long)b->inst[y].S * b->inst[y].G;
-    });
-    // teams: an instance of at least team_points grid points is solved by kTeamMax workgroups
-    // (its SC loop would otherwise outlast the rest of the batch on one CU).  A team's tickets
-    // are consecutive and tickets are taken in order by running workgroups, so at most the
-    // workgroup holding the newest ticket waits for a partner that is not yet running, and
-    // that partner starts as soon as any other workgroup retires an instance
+    const char *ord = std::getenv("MPDPFT_BATCH_ORDER");
+    if (!(ord && std::strcmp(ord, "index") == 0))
+        std::sort(order, order + n, [&](int x, int y) {
+            const long long cx = (long long)inst[x].S * inst[x].G, cy = (long long)inst[y].S * inst[y].G;
+            return cx != cy ? cx > cy : x < y;
+        });
     const char *tp = std::getenv("MPDPFT_BATCH_TEAM_POINTS");  // 0: no teams
     const long long team_points = tp ? std::atoll(tp) : 40000;
     int team_inst = 0;
     for (int i = 0; i < n; i++)
-        if (team_points > 0 && b->inst[i].G >= team_points) team_inst++;
-    const bool teams = team_inst > 0 && b->nslots >= kTeamMax;
-    std::vector<Ticket> tickets;
-    for (int i : order) {
-        const int ts = (teams && b->inst[i].G >= team_points) ? kTeamMax : 1;
-        for (int r = 0; r < ts; r++) tickets.push_back(Ticket{i, r, ts, 0});
+        if (team_points > 0 && inst[i].G >= team_points) team_inst++;
+    const bool teams = team_inst > 0 && nslots >= kTeamMax;
+    int nt = 0;
+    for (int q = 0; q < n; q++) {
+        const int i = order[q];
+        const int ts = (teams && inst[i].G >= team_points) ? kTeamMax : 1;
+        for (int r = 0; r < ts; r++) tickets[nt++] = Ticket{i, r, ts, 0};
     }
-    b->n_tickets = (int)tickets.size();
-    b->n_teams = teams ? team_inst : 0;
-    BHIP(hipMalloc((void **)&b->d_tickets, sizeof(Ticket) * tickets.size()));
-    BHIP(hipMemcpy(b->d_tickets, tickets.data(), sizeof(Ticket) * tickets.size(), hipMemcpyHostToDevice));
-    BHIP(hipMalloc((void **)&b->d_team, sizeof(TeamArea) * n));
-    BHIP(hipMalloc((void **)&b->d_team_err, sizeof(int)));
-    for (int ph = 0; ph < 2; ph++)
-        for (int e = 0; e < 2; e++) BHIP(hipEventCreate(&b->ev[ph][e]));
-    b->out.resize(n);
-    return MPDPFT_OK;
+    *n_teams = teams ? team_inst : 0;
+    return nt;
+}
+
+// Resident workgroups of the persistent kernel on one device (both field
+// types), optionally capped by MPDPFT_BATCH_SLOTS.
+static hipError_t resident_slots(int device, int *slots) {
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return e;
+    int per_cu = 0, per_cu32 = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_batch_solve<double>),
+                                                     kBT, 0);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu32, reinterpret_cast<const void *>(&k_batch_solve<float>),
+                                                     kBT, 0);
+    if (e != hipSuccess) return e;
+    per_cu = std::max(1, std::min(per_cu, per_cu32));
+    const char *env = std::getenv("MPDPFT_BATCH_SLOTS");
+    const int all = per_cu * prop.multiProcessorCount;
+    *slots = env ? std::max(1, std::min(std::atoi(env), all)) : all;
+    return hipSuccess;
+}
+
+// Values of T per slot (the slot arena is sized for doubles): Env, V, OldV and
+// the Pulay history of n_in and residual per entry plus the mixed density.
+static long long slot_values(int scope, long long Gmax) { return (3LL + 2LL * (scope + 1)) * kBS * Gmax; }
+
+// Device buffers of one solve (allocated by mpdpft_batch_create, or carved
+// from the static scratch array by the allocation-free entry point).
+struct DevBuffers {
+    BInst *inst;
+    BOut *out;
+    Ticket *tickets;
+    TeamArea *team;
+    int *next, *team_err;
+    void *slots;
+    double *fields, *start;
+    int n, n_tickets, nslots, scope;
+    long long Gmax;
+};
+
+// Zero fill of 32-bit words: the queue head, the team areas and the team
+// error flag are cleared by a kernel on the solve's stream before each launch.
+namespace {
+__global__ void k_clear_words(unsigned *p, long long nwords) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+}  // namespace
+
+static void clear_words(void *p, size_t bytes, hipStream_t st) {
+    const long long nw = (long long)(bytes / sizeof(unsigned));
+    const int blocks = (int)std::min<long long>(1024, (nw + 255) / 256);
+    hipLaunchKernelGGL(k_clear_words, dim3(std::max(1, blocks)), dim3(256), 0, st, (unsigned *)p, nw);
 }
 
 // Reduced-precision stop: FP32 fields hold V to a few ulp of max|V|, so the
@@ -1469,32 +1480,124 @@ constexpr double kF32Floor = 1.0e-6;
 constexpr int kF32Stall = 16;
 constexpr double kF32RelAcc = 1.0e-8;
 
-template <typename T> static int run(mpdpft_batch *b, int polish, int phase) {
-    const long long per = (3LL + 2LL * (b->scope + 1)) * kBS * b->Gmax;
+// One persistent launch over the whole queue on stream st; ev0/ev1 (may be
+// null) are recorded around the kernel.
+template <typename T>
+static hipError_t launch(const DevBuffers &d, int polish, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     BArgs<T> a;
-    a.inst = b->d_inst;
-    a.tickets = b->d_tickets;
-    a.n = b->n_tickets;
-    a.team = b->d_team;
-    a.team_err = b->d_team_err;
-    a.next = b->d_next;
-    a.slots = (T *)b->d_slots;
-    a.slot_stride = per;  // values of T per slot (the arena is sized for doubles)
-    a.Gmax = b->Gmax;
-    a.fields = b->d_fields;
-    a.out = b->d_out;
+    a.inst = d.inst;
+    a.tickets = d.tickets;
+    a.n = d.n_tickets;
+    a.team = d.team;
+    a.team_err = d.team_err;
+    a.next = d.next;
+    a.slots = (T *)d.slots;
+    a.slot_stride = slot_values(d.scope, d.Gmax);
+    a.Gmax = d.Gmax;
+    a.fields = d.fields;
+    a.out = d.out;
     a.polish = polish;
-    a.start = b->d_start;
+    a.start = d.start;
     a.floor_rel = sizeof(T) == 4 ? kF32Floor : 0.;
     a.stall_max = sizeof(T) == 4 ? kF32Stall : 0;
     a.relacc_floor = sizeof(T) == 4 ? kF32RelAcc : 0.;
-    BHIP(hipMemsetAsync(b->d_next, 0, sizeof(int), b->st));
-    BHIP(hipMemsetAsync(b->d_team, 0, sizeof(TeamArea) * b->n, b->st));
-    BHIP(hipMemsetAsync(b->d_team_err, 0, sizeof(int), b->st));
-    BHIP(hipEventRecord(b->ev[phase][0], b->st));
-    hipLaunchKernelGGL(k_batch_solve<T>, dim3(b->nslots), dim3(kBT), 0, b->st, a);
-    BHIP(hipGetLastError());
-    BHIP(hipEventRecord(b->ev[phase][1], b->st));
+    clear_words(d.next, sizeof(int), st);
+    clear_words(d.team, sizeof(TeamArea) * (size_t)d.n, st);
+    clear_words(d.team_err, sizeof(int), st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_solve<T>, dim3(d.nslots), dim3(kBT), 0, st, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+// BOut of each instance -> the mpdpft_result mpdpft_solve reports for it.
+static void fill_results(const BOut *out, const BInst *inst, const mpdpft_params *params, int n, double seconds,
+                         mpdpft_result *res) {
+    for (int i = 0; i < n; i++) {
+        const BOut &o = out[i];
+        mpdpft_result &r = res[i];
+        std::memset(&r, 0, sizeof(r));
+        r.S = inst[i].S;
+        r.SCcount = o.SCcount;
+        r.converged = o.converged;
+        r.LinMix = o.LinMix;
+        r.PulMix = o.PulMix;
+        r.errorCount = o.errorCount;
+        r.warningCount = o.warningCount;
+        r.CC = o.CC;
+        r.DeltamuModifier = o.DMM;
+        r.TotalEnergy = o.Etot;
+        r.bisection_steps = o.bisection;
+        r.bracket_failures = o.bracket_failures;
+        r.pulay_rejections = o.pulay_rejections;
+        r.solve_seconds = seconds;
+        const mpdpft_params &p = params[i];
+        for (int s = 0; s < r.S; s++) {
+            r.muVec[s] = o.mu[s];
+            r.FinalAbundances[s] = o.N[s];
+            r.DispersalEnergies[s] = o.Edisp[s];
+            r.EnvironmentalEnergies[s] = o.Eenv[s];
+            r.AuxEkin[s][0] = o.Aux0[s];
+        }
+        for (int a = 0; a < p.n_interactions; a++) r.InteractionEnergies[a] = p.Interactions[a] == 1 ? o.Eint : 0.;
+    }
+}
+
+extern "C" int mpdpft_batch_create(const mpdpft_params *p, int n, int device, int flags, mpdpft_batch **out) {
+    if (!p || n < 1 || !out) return MPDPFT_EINVAL;
+    mpdpft_batch *b = new mpdpft_batch();
+    *out = b;
+    b->n = n;
+    b->flags = flags;
+    b->inst.resize(n);
+    b->params.assign(p, p + n);
+    long long off = 0;
+    for (int i = 0; i < n; i++) {  // every instance is checked before the device is touched
+        char msg[256];
+        int rc = prepare(msg, p[i], b->inst[i], i);
+        if (rc) return bfail(b, rc, msg);
+        b->inst[i].out_off = off;
+        off += 2LL * b->inst[i].S * b->inst[i].G;
+        b->Gmax = std::max(b->Gmax, b->inst[i].G);
+        b->scope = std::max(b->scope, b->inst[i].scope);
+    }
+    b->arena = off;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(b, MPDPFT_ENODEV, "no GPU visible");
+    b->device = device < 0 ? 0 : device;
+    BHIP(hipSetDevice(b->device));
+    // slots: the workgroups resident at once (a persistent launch larger than that only adds
+    // workgroups that start after the queue is empty), at most kTeamMax per instance
+    int want = 0;
+    BHIP(resident_slots(b->device, &want));
+    b->nslots = std::max(1, std::min(n * kTeamMax, want));
+    BHIP(hipStreamCreate(&b->st));
+    BHIP(hipMalloc((void **)&b->d_inst, sizeof(BInst) * n));
+    BHIP(hipMalloc((void **)&b->d_out, sizeof(BOut) * n));
+    BHIP(hipMalloc((void **)&b->d_next, sizeof(int)));
+    BHIP(hipMalloc((void **)&b->d_fields, sizeof(double) * (size_t)off));
+    BHIP(hipMalloc(&b->d_slots, sizeof(double) * (size_t)slot_values(b->scope, b->Gmax) * b->nslots));
+    BHIP(hipMemcpy(b->d_inst, b->inst.data(), sizeof(BInst) * n, hipMemcpyHostToDevice));
+    std::vector<int> order(n);
+    std::vector<Ticket> tickets((size_t)kTeamMax * n);
+    b->n_tickets = make_tickets(b->inst.data(), n, b->nslots, order.data(), tickets.data(), &b->n_teams);
+    BHIP(hipMalloc((void **)&b->d_tickets, sizeof(Ticket) * b->n_tickets));
+    BHIP(hipMemcpy(b->d_tickets, tickets.data(), sizeof(Ticket) * b->n_tickets, hipMemcpyHostToDevice));
+    BHIP(hipMalloc((void **)&b->d_team, sizeof(TeamArea) * n));
+    BHIP(hipMalloc((void **)&b->d_team_err, sizeof(int)));
+    for (int ph = 0; ph < 2; ph++)
+        for (int e = 0; e < 2; e++) BHIP(hipEventCreate(&b->ev[ph][e]));
+    b->out.resize(n);
+    return MPDPFT_OK;
+}
+
+template <typename T> static int run(mpdpft_batch *b, int polish, int phase) {
+    DevBuffers d{b->d_inst, b->d_out, b->d_tickets, b->d_team, b->d_next, b->d_team_err, b->d_slots,
+                 b->d_fields, b->d_start, b->n, b->n_tickets, b->nslots, b->scope, b->Gmax};
+    BHIP(launch<T>(d, polish, b->st, b->ev[phase][0], b->ev[phase][1]));
     return MPDPFT_OK;
 }
 
@@ -1548,34 +1651,7 @@ extern "C" int mpdpft_batch_solve(mpdpft_batch *b, mpdpft_result *res) {
         b->seconds = b->phase_seconds[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if ((rc = count(1))) return rc;
     }
-    for (int i = 0; i < b->n; i++) {
-        const BOut &o = b->out[i];
-        mpdpft_result &r = res[i];
-        std::memset(&r, 0, sizeof(r));
-        r.S = b->inst[i].S;
-        r.SCcount = o.SCcount;
-        r.converged = o.converged;
-        r.LinMix = o.LinMix;
-        r.PulMix = o.PulMix;
-        r.errorCount = o.errorCount;
-        r.warningCount = o.warningCount;
-        r.CC = o.CC;
-        r.DeltamuModifier = o.DMM;
-        r.TotalEnergy = o.Etot;
-        r.bisection_steps = o.bisection;
-        r.bracket_failures = o.bracket_failures;
-        r.pulay_rejections = o.pulay_rejections;
-        r.solve_seconds = b->seconds;
-        const mpdpft_params &p = b->params[i];
-        for (int s = 0; s < r.S; s++) {
-            r.muVec[s] = o.mu[s];
-            r.FinalAbundances[s] = o.N[s];
-            r.DispersalEnergies[s] = o.Edisp[s];
-            r.EnvironmentalEnergies[s] = o.Eenv[s];
-            r.AuxEkin[s][0] = o.Aux0[s];
-        }
-        for (int a = 0; a < p.n_interactions; a++) r.InteractionEnergies[a] = p.Interactions[a] == 1 ? o.Eint : 0.;
-    }
+    fill_results(b->out.data(), b->inst.data(), b->params.data(), b->n, b->seconds, res);
     return MPDPFT_OK;
 }
 
@@ -1638,4 +1714,144 @@ extern "C" void mpdpft_batch_destroy(mpdpft_batch *b) {
     (void)hipFree(b->d_start);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
+}
+
+// ---- allocation-free solve ------------------------------------------------
+// The whole batch solve with no device allocation: instance descriptors,
+// results, the queue, the team areas, the FP32 phase's start copy and every
+// running workgroup's field slot are carved from `scratch`, one static array
+// per GPU that the library holds until the process ends.  Host code reaches
+// the array only through the address hipGetSymbolAddress returns on each
+// device (kept in na_dev); the array's own name is a host-side stand-in.
+constexpr size_t kScratchBytes = MPDPFT_NOALLOC_SCRATCH_BYTES;
+constexpr int kNoAllocMaxInst = MPDPFT_NOALLOC_MAX_INSTANCES;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct NoAllocDevice {
+    unsigned char *ptr;  // this device's copy of scratch
+    int slots;           // resident workgroups of k_batch_solve
+};
+static NoAllocDevice na_dev[kNoAllocMaxDevices];
+// host staging of the calls (static: the entry point allocates nothing), and the
+// lock that keeps calls from several threads of a process one at a time
+static std::mutex na_lock;
+static BInst na_inst[kNoAllocMaxInst];
+static BOut na_out[kNoAllocMaxInst];
+static Ticket na_tickets[kTeamMax * kNoAllocMaxInst];
+static int na_order[kNoAllocMaxInst];
+
+extern "C" int mpdpft_batch_solve_noalloc(const mpdpft_params *p, int n, int device, int flags, double *fields,
+                                          size_t fields_len, mpdpft_result *res, char *err, size_t err_len) {
+    char msg[256];
+    auto fail = [&](int rc, const char *m) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", m);
+        return rc;
+    };
+#define NAHIP(x)                                                                                   \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) {                                                                    \
+            std::snprintf(msg, sizeof msg, "%s: %s", #x, hipGetErrorString(e_));                   \
+            return fail(MPDPFT_EHIP, msg);                                                         \
+        }                                                                                          \
+    } while (0)
+    if (!p || !fields || !res || n < 1) return fail(MPDPFT_EINVAL, "null instances, fields or results, or n < 1");
+    if (n > kNoAllocMaxInst) {
+        std::snprintf(msg, sizeof msg, "%d instances: the allocation-free solve takes at most %d", n, kNoAllocMaxInst);
+        return fail(MPDPFT_EINVAL, msg);
+    }
+    if (flags & ~MPDPFT_BATCH_FP32) return fail(MPDPFT_EINVAL, "unknown flags");
+    std::lock_guard<std::mutex> guard(na_lock);
+    long long off = 0, Gmax = 0;
+    int scope = 0;
+    for (int i = 0; i < n; i++) {
+        int rc = prepare(msg, p[i], na_inst[i], i);
+        if (rc) return fail(rc, msg);
+        na_inst[i].out_off = off;
+        off += 2LL * na_inst[i].S * na_inst[i].G;
+        Gmax = std::max(Gmax, na_inst[i].G);
+        scope = std::max(scope, na_inst[i].scope);
+    }
+    if ((unsigned long long)off > (unsigned long long)fields_len) {
+        std::snprintf(msg, sizeof msg, "fields holds %zu doubles; the instances' densities and potentials need %lld",
+                      fields_len, off);
+        return fail(MPDPFT_EINVAL, msg);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MPDPFT_ENODEV, "no GPU visible");
+    if (device < 0) device = 0;
+    if (device >= ndev || device >= kNoAllocMaxDevices) return fail(MPDPFT_EINVAL, "device index out of range");
+    NAHIP(hipSetDevice(device));
+    NoAllocDevice &dv = na_dev[device];
+    if (!dv.ptr) {  // first call on this device: its copy of scratch and the slot count
+        void *ptr = nullptr;
+        NAHIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        NAHIP(resident_slots(device, &dv.slots));
+        dv.ptr = (unsigned char *)ptr;
+    }
+    // fields must be device memory of this GPU with room for off doubles
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, fields) != hipSuccess ||
+        (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) || attr.device != device) {
+        (void)hipGetLastError();
+        return fail(MPDPFT_EINVAL, "fields is not device memory of the given GPU");
+    }
+    void *base = nullptr;
+    size_t extent = 0;
+    NAHIP(hipMemGetAddressRange(&base, &extent, fields));
+    if ((unsigned char *)fields + sizeof(double) * (size_t)off > (unsigned char *)base + extent)
+        return fail(MPDPFT_EINVAL, "fields ends inside the instances' densities and potentials");
+    // the array's layout for this call: fixed parts, then as many field slots as fit
+    const bool fp32 = (flags & MPDPFT_BATCH_FP32) != 0;
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = used;
+        used += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t at_inst = take(sizeof(BInst) * n), at_out = take(sizeof(BOut) * n);
+    const size_t at_tick = take(sizeof(Ticket) * kTeamMax * n), at_team = take(sizeof(TeamArea) * n);
+    const size_t at_next = take(sizeof(int)), at_terr = take(sizeof(int));
+    const size_t at_start = fp32 ? take(sizeof(double) * (size_t)off) : 0;
+    const size_t slot_bytes = (sizeof(double) * (size_t)slot_values(scope, Gmax) + 255) & ~(size_t)255;
+    const size_t fit = used < kScratchBytes ? (kScratchBytes - used) / slot_bytes : 0;
+    if (fit < 1) {
+        std::snprintf(msg, sizeof msg,
+                      "the instances need %zu bytes of scratch for their descriptors%s and %zu per workgroup slot; "
+                      "the allocation-free solve has %zu",
+                      used, fp32 ? " and start fields" : "", slot_bytes, kScratchBytes);
+        return fail(MPDPFT_EINVAL, msg);
+    }
+    const int nslots = (int)std::max<long long>(1, std::min<long long>({(long long)n * kTeamMax, dv.slots, (long long)fit}));
+    int n_teams = 0;
+    const int n_tickets = make_tickets(na_inst, n, nslots, na_order, na_tickets, &n_teams);
+    unsigned char *sp = dv.ptr;
+    NAHIP(hipMemcpy(sp + at_inst, na_inst, sizeof(BInst) * n, hipMemcpyHostToDevice));
+    NAHIP(hipMemcpy(sp + at_tick, na_tickets, sizeof(Ticket) * n_tickets, hipMemcpyHostToDevice));
+    DevBuffers d{(BInst *)(sp + at_inst), (BOut *)(sp + at_out), (Ticket *)(sp + at_tick), (TeamArea *)(sp + at_team),
+                 (int *)(sp + at_next), (int *)(sp + at_terr), sp + used, fields,
+                 fp32 ? (double *)(sp + at_start) : nullptr, n, n_tickets, nslots, scope, Gmax};
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = nullptr;  // the null stream: no stream is created
+    int terr = 0;
+    if (fp32) {
+        NAHIP(launch<float>(d, 0, st, nullptr, nullptr));
+        NAHIP(hipMemcpyAsync(d.start, fields, sizeof(double) * (size_t)off, hipMemcpyDeviceToDevice, st));
+        NAHIP(hipStreamSynchronize(st));
+        NAHIP(hipMemcpy(&terr, d.team_err, sizeof(int), hipMemcpyDeviceToHost));
+        if (terr) return fail(MPDPFT_EHIP, "a team barrier of the batch kernel did not complete");
+        NAHIP(launch<double>(d, 1, st, nullptr, nullptr));
+    } else {
+        NAHIP(launch<double>(d, 0, st, nullptr, nullptr));
+    }
+    NAHIP(hipStreamSynchronize(st));
+    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    NAHIP(hipMemcpy(&terr, d.team_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (terr) return fail(MPDPFT_EHIP, "a team barrier of the batch kernel did not complete");
+    NAHIP(hipMemcpy(na_out, d.out, sizeof(BOut) * n, hipMemcpyDeviceToHost));
+    fill_results(na_out, na_inst, p, n, seconds, res);
+    if (err && err_len) err[0] = 0;
+    return MPDPFT_OK;
+#undef NAHIP
 }
