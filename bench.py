@@ -129,10 +129,11 @@ def _oracle_run(indices, threads):
 
 
 def cpu_baseline_leg(budget_s):
-    """C3 on the host: sequentially with all threads (the reference's runY jobs
-    one after another), one instance per core (independent jobs side by side),
-    and sequentially on one thread, each on a bounded prefix of the seeded
-    instance list."""
+    """C3 on the host, on the first 2 x cores seeded instances: one instance
+    per core (independent jobs side by side: the value), sequentially with
+    all threads (the reference's runY jobs one after another) and sequentially
+    on one thread (the last two on a prefix of that sample when the budget
+    runs out)."""
     cores = host_threads()
     t_start = time.perf_counter()
     # one instance per core: 2 x cores instances, one worker process each (OMP 1)
@@ -151,16 +152,17 @@ def cpu_baseline_leg(budget_s):
             raise RuntimeError("oracle worker failed: " + err[-400:])
         par += json.loads(out.strip().splitlines()[-1])
     t_par = time.perf_counter() - t0
-    # sequential, all threads: the first instances until ~25 % of the budget is used
-    seq = []
-    t0 = time.perf_counter()
-    k = 0
-    while time.perf_counter() - t0 < 0.25 * budget_s and k < 64:
-        seq += _oracle_run(range(k, k + 4), cores)
-        k += 4
+    # sequential on all threads, then sequential on one thread, over the same instances (the
+    # first n_par, in chunks of 4 until each mode has used its share of the budget)
+    def sequential(threads, share):
+        done, t0, k = [], time.perf_counter(), 0
+        while k < n_par and time.perf_counter() - t0 < share * budget_s:
+            done += _oracle_run(range(k, min(k + 4, n_par)), threads)
+            k += 4
+        return done
+    seq = sequential(cores, 0.25)
     t_seq = sum(x[1] for x in seq)
-    # sequential, one thread: the first 4 instances
-    one = _oracle_run(range(4), 1)
+    one = sequential(1, 0.5)
     t_one = sum(x[1] for x in one)
     return {
         "value": round(len(par) / t_par, 3),
