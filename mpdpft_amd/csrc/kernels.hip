@@ -858,14 +858,17 @@ __device__ __noinline__ double xc_v(int t, int units, double n) {
     return v;
 }
 __device__ __noinline__ double xc_v_f(int t, int units, double n) { return (double)xc_lda_v_f(t, units, (float)n); }
-// DiracExchange (mpDPFT.cpp:2766-2769): -gammaH 2 (D n / pi)^{1/D} / (D - 1), D > 1
+// DiracExchange (mpDPFT.cpp:2766-2769): -gammaH 2 (D n / pi)^{1/D} / (D - 1), D > 1;
+// the D-th root as cbrt (3D) or sqrt (2D) rather than pow(., 1/D) (1/3 is not a
+// double, so the two differ by about an ulp)
+__device__ __forceinline__ double dirac_root(double x, int D) { return D == 3 ? cbrt(x) : sqrt(x); }
 __device__ __noinline__ double dirac_v(double n, int D, double gammaH) {
     const double DD = (double)D;
-    return D > 1 ? -gammaH * (2. * pow(d_pos(n) * DD / PI_D, 1. / DD)) / (DD - 1.) : 0.;
+    return D > 1 ? -gammaH * (2. * dirac_root(d_pos(n) * DD / PI_D, D)) / (DD - 1.) : 0.;
 }
 __device__ __noinline__ double dirac_v_f(double n, int D, double gammaH) {
-    float nf = (float)d_pos(n), Df = (float)D;
-    return D > 1 ? (double)(-(float)gammaH * (2.f * powf(nf * Df / (float)PI_D, 1.f / Df)) / (Df - 1.f)) : 0.;
+    float nf = (float)d_pos(n), Df = (float)D, x = nf * Df / (float)PI_D;
+    return D > 1 ? (double)(-(float)gammaH * (2.f * (D == 3 ? cbrtf(x) : sqrtf(x))) / (Df - 1.f)) : 0.;
 }
 __device__ __noinline__ double renorm_v(double n_s, double n_o, double a) {
     const double es = renorm_eta(n_s, a), eo = renorm_eta(n_o, a);
@@ -887,12 +890,13 @@ __device__ __noinline__ double vwn_v(double n) {
 // ---------------------------------------------------------------------------
 // potentials
 // ---------------------------------------------------------------------------
+// one species per blockIdx.y
 __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const double *__restrict__ den,
                                                   const double *__restrict__ env, double *V) {
-    size_t G = g.G, tot = G * (size_t)vp.S;
-    for (size_t idx = blockIdx.x * 256 + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * 256) {
-        int s = (int)(idx / G);
-        size_t i = idx - (size_t)s * G;
+    const size_t G = g.G;
+    const int s = blockIdx.y;
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < G; i += (size_t)gridDim.x * 256) {
+        const size_t idx = (size_t)s * G + i;
         double ns = den[idx];
         double acc = 0.;
         for (int a = 0; a < vp.n_inter; a++) {
@@ -925,7 +929,8 @@ __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const d
 }
 hipError_t update_v(const Grid &g, const VintParams &vp, const double *den_all, const double *env, double *V,
                     hipStream_t st) {
-    hipLaunchKernelGGL(k_update_v, dim3(2048), dim3(256), 0, st, g, vp, den_all, env, V);
+    const unsigned blocks = (unsigned)std::min<size_t>(2048 / (size_t)vp.S + 1, (g.G + 255) / 256);
+    hipLaunchKernelGGL(k_update_v, dim3(blocks, vp.S), dim3(256), 0, st, g, vp, den_all, env, V);
     return hipGetLastError();
 }
 
@@ -941,7 +946,7 @@ __global__ __launch_bounds__(256) void k_vint_dvdn(Grid g, VintParams vp, const 
                 acc += 2. * vp.beta;
             } else if (t == 11) {  // v = -gammaH 2 (D n / pi)^{1/D} / (D - 1)
                 if (g.D > 1 && ns > 0.)
-                    acc += -vp.gammaH * 2. / (DD - 1.) * pow(ns * DD / PI_D, 1. / DD) / (DD * ns);
+                    acc += -vp.gammaH * 2. / (DD - 1.) * dirac_root(ns * DD / PI_D, g.D) / (DD * ns);
             } else if (t == -1 || t == -12) {
                 acc += xc_lda_dvdn(t, vp.units, ns);
             }

@@ -10,8 +10,9 @@ pickles).  Cases:
   c1_nait_16  the drop-in functional (nAiT T = 0.05, isolated Hartree, Dirac,
               PW92; Z = 10, a = 0.5, N = 10, 20 bohr cube) at 16^3, converged
               at max|dV| < 1e-8
-  c1_nait_16_tight  the same converged at max|dV| < 1e-11 (the fixed point to
-              well below the 1e-10 comparison)
+  c1_nait_16_tight  the same converged at max|dV| < 1e-11 and with mu
+              bisected to RelAcc 2e-12 (the fixed point to well below the
+              1e-10 comparison)
   c2_nait_16  the precision workload's physics (Z = N = 18, a = 0.4, 24 bohr)
               at 16^3, FP64, converged at max|dV| < 1e-8
   c4_nait_16  the differentiation workload's functional on a 3-centre cluster
@@ -59,8 +60,11 @@ def c1_nait_16():
 
 
 def c1_nait_16_tight():
-    """C1 at 16^3 converged at max|dV| < 1e-11: the fixed point itself."""
+    """C1 at 16^3 converged at max|dV| < 1e-11 with mu bisected to RelAcc
+    2e-12: the fixed point itself (RelAcc 1e-11 would leave mu ~1e-10 apart
+    between two converged solves)."""
     p = configs.c1(steps=16, density_expression=5, hartree_mode=1, tol=1e-11)
+    p.RelAcc = 2e-12
     rc, r, den, V = O.solve(p)
     assert rc == 0 and r.converged
     out = result_dict(r)
