@@ -1,4 +1,4 @@
-// params.cpp -- defaults and the mpDPFT.input reader (host only).
+// params.cpp -- defaults, the mpDPFT.input reader and the mpDPFT_V.dat reader (host only).
 //
 // The reader keeps the reference's positional contract (GetInputParameters,
 // mpDPFT.cpp:621-721): line n holds parameter n, each value followed by a
@@ -6,6 +6,7 @@
 // numbers until the first token that is not a number; Mixer reads 3 values.
 // Lines after the 47th are extension keys written as "value //Key ...".
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -222,5 +223,114 @@ extern "C" int mpdpft_parse_input(const char *path, mpdpft_params *p) {
     }
     if (n < 47 || bad) return MPDPFT_EINVAL;
     *p = q;
+    return MPDPFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// LoadV/ImportV (InterpolVQ = 1, mpDPFT.cpp:6430-6502): mpDPFT_V.dat read and
+// interpolated multilinearly onto this run's lattice (host only).
+// ---------------------------------------------------------------------------
+// Cell index and fraction of t on the sorted axis X (n >= 2): the cell
+// [X[l], X[l+1]] that holds t, the first or last cell outside the axis range
+// (so the multilinear form extrapolates linearly there).
+static void cell(const std::vector<double> &X, double t, int &l, double &f) {
+    int n = (int)X.size(), lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int m = (lo + hi) / 2;
+        if (X[m] >= t) hi = m;
+        else lo = m;
+    }
+    l = lo;
+    f = (t - X[l]) / (X[l + 1] - X[l]);
+}
+
+extern "C" int mpdpft_import_v(const char *path, const mpdpft_params *p, double *out, int *warnings) {
+    if (!path || !p || !out) return MPDPFT_EINVAL;
+    if (params_structure_error(p)) return MPDPFT_EINVAL;
+    if (p->DIM < 1 || p->DIM > 3 || p->S < 1 || p->S > MPDPFT_MAX_SPECIES || p->steps < 1) return MPDPFT_EINVAL;
+    const int D = p->DIM, S = p->S, EL = p->steps + 1;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return MPDPFT_EIO;
+    std::vector<std::vector<double>> rows;
+    std::vector<char> buf(1 << 16);
+    std::string line;
+    while (std::fgets(buf.data(), (int)buf.size(), f)) {
+        line += buf.data();
+        if (line.empty() || line.back() != '\n') {
+            if (!std::feof(f)) continue;
+        }
+        std::istringstream is(line);
+        std::vector<double> r;
+        double v;
+        while (is >> v) r.push_back(v);
+        if (!r.empty()) rows.push_back(std::move(r));
+        line.clear();
+    }
+    std::fclose(f);
+    int warn = 0;
+    const int N = (int)rows.size();
+    if (N < 2) return MPDPFT_EIO;
+    const int n = (int)(std::pow((double)N, 1. / (double)D) + 0.5);
+    size_t nD = 1;
+    for (int d = 0; d < D; d++) nD *= (size_t)n;
+    if (n < 2 || nD != (size_t)N) return MPDPFT_EIO;
+    if ((int)rows[0].size() != D + S) warn++;  // "LoadV: Error!!! dimensional mismatch"
+    for (const auto &r : rows)
+        if ((int)r.size() < D + S) return MPDPFT_EIO;
+    // axes from the first columns: x slowest (row i n^(D-1)), ..., last axis fastest
+    std::vector<std::vector<double>> X(D, std::vector<double>(n));
+    for (int d = 0; d < D; d++) {
+        size_t stride = 1;
+        for (int e = d + 1; e < D; e++) stride *= (size_t)n;
+        for (int i = 0; i < n; i++) X[d][i] = rows[(size_t)i * stride][d];
+    }
+    // target lattice (as mpdpft_create builds it)
+    std::vector<double> lat(EL);
+    const double x1 = -p->edge / 2., x2 = p->edge / 2., del = (x2 - x1) / (double)p->steps;
+    for (int i = 0; i < EL; i++) lat[i] = x1 + del * (double)i;
+    std::vector<std::vector<int>> cl(D, std::vector<int>(EL));
+    std::vector<std::vector<double>> cf(D, std::vector<double>(EL));
+    for (int d = 0; d < D; d++)
+        for (int i = 0; i < EL; i++) cell(X[d], lat[i], cl[d][i], cf[d][i]);
+    size_t G = 1;
+    for (int d = 0; d < D; d++) G *= (size_t)EL;
+    for (int s = 0; s < S; s++) {
+        const int col = D + s;
+        bool nan = false;
+        for (const auto &r : rows) nan |= r[col] != r[col];
+        if (nan) warn++;  // "LoadV: Error!!! ImportField ... is NAN"
+        for (size_t g = 0; g < G; g++) {
+            int idx[3] = {0, 0, 0};
+            size_t rem = g;
+            for (int d = D - 1; d >= 0; d--) {
+                idx[d] = (int)(rem % EL);
+                rem /= EL;
+            }
+            // multilinear: sum over the 2^D cell corners
+            double acc = 0.;
+            for (int corner = 0; corner < (1 << D); corner++) {
+                double w = 1.;
+                size_t r = 0;
+                for (int d = 0; d < D; d++) {
+                    int b = (corner >> (D - 1 - d)) & 1;
+                    double t = cf[d][idx[d]];
+                    w *= b ? t : 1. - t;
+                    r = r * (size_t)n + (size_t)(cl[d][idx[d]] + b);
+                }
+                acc += w * rows[r][col];
+            }
+            out[(size_t)s * G + g] = acc;
+        }
+    }
+    // integral cross-check (the reference warns above 5% asymmetry)
+    double imp = 0., cur = 0.;
+    for (int s = 0; s < S; s++) {
+        for (const auto &r : rows) imp += r[D + s];
+        for (size_t g = 0; g < G; g++) cur += out[(size_t)s * G + g];
+    }
+    imp *= std::pow(2. * std::fabs(rows[0][0]), (double)D) / (double)N;
+    cur *= std::pow(del, (double)D);
+    if (std::fabs((imp - cur) / (imp + cur)) > 0.05) warn++;
+    if (warnings) *warnings = warn;
     return MPDPFT_OK;
 }
