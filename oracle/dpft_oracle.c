@@ -1017,6 +1017,166 @@ static void renorm_contact(orc_state *o, int EnergyQ, int ss, double *out) {
     }
 }
 
+/* DeltaEkin2D3Dcrossover (mpDPFT.cpp:2906-2923), 2D, Units 1 (0 otherwise):
+ * l = floor(-1/2 + sqrt(1/4 + 4 pi n / alpha)) filled transverse levels of the
+ * oscillator omega_z = alpha; for l > 0 the energy density
+ * E0 + n V0 + n^2 V1 with V0 = l alpha / 2, V1 = -pi l / (l + 1),
+ * E0 = -alpha^2 l (l + 1)(l + 2) / (48 pi), and the potential V0 + 2 n V1. */
+static void crossover_2d3d(orc_state *o, int EnergyQ, int ss, double *out) {
+    const double *n = o->Den + (long long)ss * o->G, a = o->p.alpha;
+    for (long long i = 0; i < o->G; i++) {
+        out[i] = 0.;
+        if (o->p.Units != 1) continue;
+        double d = n[i], l = floor(-0.5 + sqrt(0.25 + 4. * PI_ * d / a));
+        if (l > MP_) {
+            double V0 = 0.5 * l * a, V1 = -PI_ * l / (l + 1.);
+            double E0 = -a * a * l * (l + 1.) * (l + 2.) / (48. * PI_);
+            out[i] = EnergyQ ? E0 + d * V0 + d * d * V1 : V0 + d * 2. * V1;
+        }
+    }
+}
+
+/* TabFunc_2D3D_eta.dat (Load2D3Deta, mpDPFT.cpp:2927-2950): 31 x 31 triplets
+ * (eta1, eta2, eta3)(l1, l2) in row-major order, each line "l1 l2 eta1 eta2 eta3". */
+static double g_eta[31 * 31][3];
+static int g_eta_rows = -1;
+int oracle_load_eta(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) return MPDPFT_EIO;
+    char line[512];
+    int n = 0;
+    while (n < 31 * 31 && fgets(line, sizeof line, f)) {
+        int a, b;
+        double e0, e1, e2;
+        if (sscanf(line, "%d %d %lf %lf %lf", &a, &b, &e0, &e1, &e2) != 5) continue;
+        g_eta[n][0] = e0;
+        g_eta[n][1] = e1;
+        g_eta[n][2] = e2;
+        n++;
+    }
+    fclose(f);
+    g_eta_rows = n;
+    return n > 0 ? 0 : MPDPFT_EIO;
+}
+
+/* Quasi2Dcontact (mpDPFT.cpp:2952-2967), 2D, two species, Units 1 and beta != 0
+ * (0 otherwise): l_s = max(0, floor(-1/2 + sqrt(1/4 + 4 pi n_s / alpha))),
+ * G = beta sqrt(alpha), f1 = alpha / (2 pi); energy density (species 0 focal)
+ * G (n1 n2 eta1(l1,l2) + f1 (n1 eta2(l1,l2) + n2 eta2(l2,l1)) + f1^2 eta3(l1,l2)),
+ * potential of the focal species G (n_other eta1(l1,l2) + f1 eta2(l1,l2)).
+ * Levels beyond the loaded table give NaN (the reference reads past its table). */
+static void quasi2d_contact(orc_state *o, int EnergyQ, int ss, double *out) {
+    long long G = o->G;
+    int s1 = ss, s2 = ss == 0 ? 1 : 0;
+    double a = o->p.alpha, f1 = a / (2. * PI_), f2 = f1 * f1, g = o->p.beta * sqrt(a);
+    for (long long i = 0; i < G; i++) {
+        out[i] = 0.;
+        if (!(o->p.Units == 1 && fabs(o->p.beta) > MP_)) continue;
+        double d1 = o->Den[s1 * G + i], d2 = o->Den[s2 * G + i];
+        int l1 = (int)floor(-0.5 + sqrt(0.25 + 4. * PI_ * d1 / a)), l2 = (int)floor(-0.5 + sqrt(0.25 + 4. * PI_ * d2 / a));
+        if (l1 < 0) l1 = 0;
+        if (l2 < 0) l2 = 0;
+        int k12 = l1 * 31 + l2, k21 = l2 * 31 + l1;
+        if (l1 > 30 || l2 > 30 || k12 >= g_eta_rows || k21 >= g_eta_rows) {
+            out[i] = NAN;
+            continue;
+        }
+        if (EnergyQ)
+            out[i] = g * (d1 * d2 * g_eta[k12][0] + f1 * (d1 * g_eta[k12][1] + d2 * g_eta[k21][1]) + f2 * g_eta[k12][2]);
+        else
+            out[i] = g * (d2 * g_eta[k12][0] + f1 * g_eta[k12][1]);
+    }
+}
+
+/* DipolarInteractionPos (mpDPFT.cpp:2719-2757), 3D, position space: with
+ * n0 = Abundances[ss], the centred spectrum of n has n0 e^{-q^2} subtracted from
+ * its real part and is multiplied by q_z^2/q^2 - 1/3 + 8 pi/3 (zero for q^2 <=
+ * 0.01 dk^2); after the backward transform n0/(8 pi^3) zf(|r|) is added,
+ * zf(z) = (pi/z^3)(e^{-z^2/4} sqrt(pi) z (4 + z^2) - 4 pi erf(z/2)) +
+ * pi sqrt(pi) e^{-z^2/4}(8 pi/3 - 1/3) (z > 1e-12; pi sqrt(pi) 8 pi/3 at 0), and
+ * the potential is alpha times the real part.  Its energy integrand is 0. */
+static void dipolar_pos(orc_state *o, int EnergyQ, int ss, double *out) {
+    long long G = o->G;
+    if (EnergyQ || o->D != 3) {
+        memset(out, 0, sizeof(double) * (size_t)G);
+        return;
+    }
+    const double *n = o->Den + (long long)ss * G;
+    double n0 = o->p.Abundances[ss], qtest = 0.01 * o->dk * o->dk, sqrtpi = sqrt(PI_);
+    for (long long i = 0; i < G; i++) {
+        o->CF[i].re = n[i];
+        o->CF[i].im = 0.;
+    }
+    fft_centered(o, o->CF, -1);
+    for (long long i = 0; i < G; i++) {
+        double k[3];
+        kvec_at(o, i, k);
+        double qk2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2], ql2 = k[2] * k[2];
+        if (qk2 > qtest) {
+            double f = (ql2 / qk2) - 1. / 3. + (8. * PI_) / 3.;
+            o->CF[i].re = (o->CF[i].re - n0 * EXPs(-qk2)) * f;
+            o->CF[i].im *= f;
+        } else {
+            o->CF[i].re = 0.;
+            o->CF[i].im = 0.;
+        }
+    }
+    fft_centered(o, o->CF, +1);
+    for (long long i = 0; i < G; i++) {
+        double z = o->rnorm[i], zf;
+        if (z > 1.0e-12)
+            zf = (PI_ / (z * z * z)) * (EXPs(-z * z / 4.) * sqrtpi * z * (4. + z * z) - 4. * PI_ * erf(z / 2.)) +
+                 PI_ * sqrtpi * EXPs(-z * z / 4.) * (-1. / 3. + 8. * PI_ / 3.);
+        else
+            zf = PI_ * sqrtpi * (8. * PI_ / 3.);
+        out[i] = o->p.alpha * (o->CF[i].re + (n0 / (8. * PI_ * PI_ * PI_)) * zf);
+    }
+}
+
+/* DipolarInteraction (mpDPFT.cpp:2969-2990), 3D, one species, Units 1 (0
+ * otherwise), momentum space: for each grid point q (a wavevector) the
+ * trapezoid integral over grid points k of (k_z/|k|^2 - 1/3) where p = q + k
+ * lies inside |p| < edge/2 and n(p) > n(q) (p rounded to its grid index as
+ * IndexOfVec, :3083), times alpha / (2 (2 pi)^3).  Its energy is 0. */
+static void dipolar_k(orc_state *o, int EnergyQ, int ss, double *out) {
+    long long G = o->G;
+    if (EnergyQ || o->D != 3 || o->p.Units != 1) {
+        memset(out, 0, sizeof(double) * (size_t)G);
+        return;
+    }
+    const double *n = o->Den + (long long)ss * G;
+    const double pref = o->p.alpha / (2. * pow(2. * PI_, 3.));
+    const int EL = o->EL;
+    const double x0 = -o->p.edge / 2.;
+#pragma omp parallel
+    {
+        double *tf = dalloc(G);
+#pragma omp for schedule(dynamic, 16)
+        for (long long i = 0; i < G; i++) {
+            double q[3], k[3], pk[3];
+            vec_at(o, i, q);
+            double rhop = n[i];
+            for (long long j = 0; j < G; j++) {
+                vec_at(o, j, k);
+                for (int d = 0; d < 3; d++) pk[d] = q[d] + k[d];
+                double npk = sqrt(pk[0] * pk[0] + pk[1] * pk[1] + pk[2] * pk[2]);
+                tf[j] = 0.;
+                if (npk < 0.5 * o->p.edge) {
+                    long long idx = 0;
+                    for (int d = 0; d < 3; d++) {
+                        int ci = (int)(0.5 + (double)(EL - 1) * ((pk[d] - x0) / o->p.edge));
+                        idx = idx * EL + ci;
+                    }
+                    if (n[idx] - rhop > 0.)
+                        tf[j] = k[2] / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]) - 1. / 3.;
+                }
+            }
+            out[i] = pref * integrate(o, tf);
+        }
+        free(tf);
+    }
+}
+
 /* GombasCorrelation (mpDPFT.cpp:2774-2796): energy density
  * e(n) = -gammaH n (a1 t/(a2 + c t) + (b1/c) ln(1 + b2 c t)), t = POS(n)^(1/3);
  * the potential (3D, Units 2 only; 0 otherwise) is de/dn. */
@@ -1094,9 +1254,29 @@ static int getvint(orc_state *o, int type, int ss, double *out) {
         gombas(o, 0, ss, out);
     } else if (type == 13) {
         vwn(o, 0, ss, out);
+    } else if (type == 14) {
+        crossover_2d3d(o, 0, ss, out);
+    } else if (type == 15 && o->S == 1 && o->D == 3) {
+        dipolar_k(o, 0, ss, out);
+    } else if (type == 17) {
+        dipolar_pos(o, 0, ss, out);
+    } else if (type == 24 && o->D == 2 && o->S == 2) {
+        quasi2d_contact(o, 0, ss, out);
     } else
         return MPDPFT_EINVAL;
     return 0;
+}
+
+/* The energy integrand of interaction 14 (species s) or 24 (species s focal). */
+int oracle_energy_density(const mpdpft_params *p, int type, int s, const double *den, double *out) {
+    orc_state *o = orc_create(p);
+    memcpy(o->Den, den, sizeof(double) * (size_t)(o->S * o->G));
+    int rc = 0;
+    if (type == 14 && o->D == 2) crossover_2d3d(o, 1, s, out);
+    else if (type == 24 && o->D == 2 && o->S == 2) quasi2d_contact(o, 1, s, out);
+    else rc = MPDPFT_EINVAL;
+    orc_free(o);
+    return rc;
 }
 
 int oracle_vint(const mpdpft_params *p, int type, int s, const double *den, double *vint) {
@@ -1770,7 +1950,19 @@ static double interaction_energy(orc_state *o, int type) { /* :5805-5918 */
         else vwn(o, 1, 0, o->Tmp2);
         return integrate(o, o->Tmp2);
     }
-    return 0.;
+    if (o->D == 2 && type == 14) { /* summed over species (:5879-5886) */
+        memset(o->Tmp1, 0, sizeof(double) * (size_t)G);
+        for (int ss = 0; ss < o->S; ss++) {
+            crossover_2d3d(o, 1, ss, o->Tmp2);
+            for (long long i = 0; i < G; i++) o->Tmp1[i] += o->Tmp2[i];
+        }
+        return integrate(o, o->Tmp1);
+    }
+    if (o->D == 2 && o->S == 2 && type == 24) { /* counted once, species 0 focal (:5890-5898) */
+        quasi2d_contact(o, 1, 0, o->Tmp2);
+        return integrate(o, o->Tmp2);
+    }
+    return 0.; /* 15, 17 and the rest: no interaction energy (:5887-5889, default :5917) */
 }
 
 /* GetAuxEint (:5518-5549): S = 1, DIM 3, Units 2 only (zeros otherwise). */

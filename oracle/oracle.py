@@ -122,6 +122,28 @@ def delfield(p, f, method=5):
     return dict(zip(("dx", "dy", "dz", "gradsq", "laplacian"), outs))
 
 
+def load_eta(path):
+    """Load a TabFunc_2D3D_eta.dat file for interaction 24."""
+    L = lib()
+    L.oracle_load_eta.argtypes = [ctypes.c_char_p]
+    rc = L.oracle_load_eta(str(path).encode())
+    assert rc == 0, rc
+
+
+def energy_density(p, type_, s, den):
+    """The energy integrand of interaction 14 or 24 on den ([S][G])."""
+    L = lib()
+    P = ctypes.POINTER(type(p))
+    L.oracle_energy_density.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+    G = grid_size(p)
+    den = np.ascontiguousarray(den, dtype=np.float64).reshape(-1)
+    out = np.zeros(G)
+    rc = L.oracle_energy_density(ctypes.byref(p), type_, s, _dp(den), _dp(out))
+    assert rc == 0, rc
+    return out
+
+
 def vint(p, type_, s, den):
     G = grid_size(p)
     den = np.ascontiguousarray(den, dtype=np.float64).reshape(-1)
