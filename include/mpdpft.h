@@ -399,8 +399,13 @@ int mpdpft_batch_solve_noalloc(const mpdpft_params *instances, int n, int device
 /* Reads dir/mpDPFT.input (and dir/mpDPFT_V.dat when InterpolVQ = 1), polls
  * dir/mpDPFTmanualSCloopBreakQ.dat, solves, writes dir/mpDPFT_Den.dat,
  * mpDPFT_V.dat and mpDPFT_Control.dat in the reference formats (ExportData,
- * :12399-12520). */
+ * :12399-12520).  Sets the data directory (below) to dir. */
 int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res);
+
+/* Directory of the TabFunc_*.dat tables that contexts read at mpdpft_create
+ * (process-wide; "" or NULL = the current directory, where the reference reads
+ * them): TabFunc_2D3D_eta.dat for interaction 24 (Load2D3Deta, :2927-2950). */
+int mpdpft_set_data_dir(const char *dir);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ from . import _abi
 from ._abi import Params, Result, load_library
 
 __all__ = ["Params", "Result", "default_params", "parse_input", "Context", "Batch", "batch_solve_noalloc",
-           "batch_fields_len", "run_dir", "load_library", "MpdpftError"]
+           "batch_fields_len", "run_dir", "set_data_dir", "load_library", "MpdpftError"]
 
 
 class MpdpftError(RuntimeError):
@@ -324,6 +324,12 @@ def import_v(path, params):
     if rc != _abi.OK:
         raise MpdpftError(rc, "import_v(%s)" % path)
     return out.reshape(S, G), w.value
+
+
+def set_data_dir(path):
+    """Directory of the TabFunc_*.dat tables contexts read (TabFunc_2D3D_eta.dat
+    for interaction 24); None or "" = the current directory."""
+    load_library().mpdpft_set_data_dir(str(path or "").encode())
 
 
 def run_dir(path, device=-1):

@@ -86,7 +86,7 @@ EXPORTS = [
     "mpdpft_batch_phase_stats", "mpdpft_batch_kernel_stats", "mpdpft_batch_last_error", "mpdpft_batch_destroy",
     "mpdpft_adjoint", "mpdpft_centre_gradient", "mpdpft_centre_forces", "mpdpft_set_centres",
     "mpdpft_loop_begin", "mpdpft_step", "mpdpft_energy", "mpdpft_enforce_constraints", "mpdpft_mix",
-    "mpdpft_inject_fault", "mpdpft_batch_solve_noalloc",
+    "mpdpft_inject_fault", "mpdpft_batch_solve_noalloc", "mpdpft_set_data_dir",
 ]
 BATCH_FP32 = 1
 NOALLOC_SCRATCH_BYTES = 64 << 20
@@ -149,6 +149,8 @@ def load_library(path=None):
     lib.mpdpft_run_dir.restype = _i
     lib.mpdpft_import_v.argtypes = [ctypes.c_char_p, P, dp, ctypes.POINTER(_i)]
     lib.mpdpft_import_v.restype = _i
+    lib.mpdpft_set_data_dir.argtypes = [ctypes.c_char_p]
+    lib.mpdpft_set_data_dir.restype = _i
     lib.mpdpft_set_initial_potential.argtypes = [vp, dp]
     lib.mpdpft_set_initial_potential.restype = _i
     lib.mpdpft_set_break_file.argtypes = [vp, ctypes.c_char_p]

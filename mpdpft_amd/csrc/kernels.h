@@ -32,7 +32,8 @@ enum MulKind {
     MUL_DERIV_LAP = 3,   // re <- i k_z e^{sigma k^2}, im <- -|k|^2 e^{2 sigma k^2}
     MUL_HARTREE_PER = 4, // reference periodic Hartree with Gaussian compensation
     MUL_SECOND = 5,      // re <- -k_a^2 (method 4 Laplacian pieces)
-    MUL_HARTREE_2D = 6   // 2D periodic Hartree: (f - f0 e^{-k^2}) / |k|, 0 for |k| <= dkzero
+    MUL_HARTREE_2D = 6,  // 2D periodic Hartree: (f - f0 e^{-k^2}) / |k|, 0 for |k| <= dkzero
+    MUL_DIPOLAR = 7      // 3D dipolar (position space): (f - n0 e^{-k^2}) (k_z^2/k^2 - 1/3 + 8 pi/3), 0 for k^2 <= dkzero
 };
 struct MulParams {
     int kind;
@@ -47,12 +48,14 @@ struct MulParams {
     double dxD;           // dx^D
     double dkzero;        // 1e-6 dk^2
     int regularize;       // RegularizeCoulombSingularity
+    double n0;            // MUL_DIPOLAR: the subtracted Gaussian's amplitude (and post 4's)
 };
 // Backward transform: f = L^-D * IDFT(M * spec) written to re/im real grid
 // fields (either may be null) with periodic-image fill, out = scale*value
 // (+ out if accumulate).  post: 0 none, 1 POS clip of re.
 // post 2 adds the Gaussian compensation f0 erf(r/2)/(4 pi r) and multiplies
-// by c1 = 4 pi gammaH (needs lat, the lattice table, and m.f0slot).
+// by c1 = 4 pi gammaH (needs lat, the lattice table, and m.f0slot); post 4
+// adds the dipolar compensation n0 zf(|r|)/(8 pi^3) and multiplies by c1 = alpha.
 // The first pass reads 'spec' and writes 'work' (may equal spec), so a
 // spectrum can feed several backward transforms.
 hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, const MulParams &m, double *re,
@@ -110,12 +113,19 @@ struct VintParams {
     int units;
     int hartree_in_V;
     int fp32;  // pointwise interaction potentials evaluated in FP32
+    double alpha;                 // omega_z / mu0 mu^2 prefactor (interactions 14, 24)
+    const double *eta;            // [eta_rows][3] eta triplets of interaction 24
+    int eta_rows;
+    const double *dip_r, *dip_k;  // [S][G] dipolar potentials of 17 and 15 (computed beforehand)
 };
 hipError_t update_v(const Grid &g, const VintParams &vp, const double *den_all, const double *env, double *V,
                     hipStream_t st);
 // out = inc * sum_a dV_a/dn of the pointwise interactions (10, 11, -1, -12) of one
 // species at density den (the diagonal of the interaction Jacobian without Hartree).
 hipError_t vint_dvdn(const Grid &g, const VintParams &vp, const double *den, double *out, hipStream_t st);
+// DipolarInteraction (momentum space, 3D): out[i] = scale sum_j w_j (k_z/|k|^2 - 1/3) over the grid
+// points k_j with |q_i + k_j| < edge/2 and n(q_i + k_j) > n(q_i); w_j the trapezoid weights.
+hipError_t dipolar_k(const Grid &g, const double *n, double scale, double *out, hipStream_t st);
 
 // RelDev dots per species + field stats: partials layout per species s:
 // [<a,a>, <b,b>, <a,b>] sums and [denmin, denmax, vmin, vmax].
@@ -172,12 +182,16 @@ enum EnergyOp {
     E_GOMBAS = 8,        // Gombas correlation energy density of a, scale p0 (= gammaH)
     E_VWN = 9,           // VWN correlation energy density of a (eV, density in 1/Angstrom^3)
     E_QT2 = 10,          // (2 * 0.131163 * (p0 - a))^2 where p0 > a, else 0 (a = V, p0 = mu)
-    E_RENORM = 11        // renormalised 2D contact energy density of (a, b) = (n_0, n_1), p0 = scattering length
+    E_RENORM = 11,       // renormalised 2D contact energy density of (a, b) = (n_0, n_1), p0 = scattering length
+    E_CROSSOVER = 12,    // 2D-3D crossover energy density of a, p0 = alpha
+    E_QUASI2D = 13       // quasi-2D contact energy density of (a, b) = (n_0, n_1), p0 = alpha, p1 = beta, eta
 };
 struct EnergyParams {
     int op;
     double p0, p1;       // op parameters (mu, power, threshold, scale)
     int xc_id, units;
+    const double *eta;   // E_QUASI2D: [eta_rows][3]
+    int eta_rows;
 };
 hipError_t energy_integrand(const Grid &g, const EnergyParams &ep, const double *a, const double *b,
                             const double *c, const double *d, double *partials, hipStream_t st);

@@ -340,6 +340,26 @@ template <int D, int KIND = -1> struct LdMulD {
             }
             return o;
         }
+        case MUL_DIPOLAR: {  // DipolarInteractionPos (mpDPFT.cpp:2735-2746), 3D; dkzero = 0.01 dk^2
+            double kk2 = 0.;
+            int par = 0;
+            for (int d = 0; d < D; d++) {
+                double kh = m.kq_h[q[d]];
+                kk2 += kh * kh;
+                par += q[d];
+            }
+            const double kz = m.kq_h[q[D - 1]], ql2 = kz * kz;
+            if (kk2 > m.dkzero) {
+                const double f = (ql2 / kk2) - 1. / 3. + (8. * PI_D) / 3.;
+                const double sg = (par & 1) ? -1. : 1.;
+                o.x = (m.dxD * v.x - sg * m.n0 * d_exp(-kk2)) * f;
+                o.y = m.dxD * v.y * f;
+            } else {
+                o.x = 0.;
+                o.y = 0.;
+            }
+            return o;
+        }
         default:
             return v;
         }
@@ -379,7 +399,8 @@ struct StGrid {
     int EL, s, D;
     const double *lat;
     const double *f0slot;
-    double c1;  // 4 pi gammaH for post 2, 2 pi gammaH for post 3
+    double c1;  // 4 pi gammaH for post 2, 2 pi gammaH for post 3, alpha for post 4
+    double n0;  // post 4: the dipolar compensation's amplitude
     // post 3: 2 pi gammaH (val + f0 e^{-z^2/8} I0(z^2/8) / (4 sqrt(pi))) (mpDPFT.cpp:2647-2655)
     __device__ double hart2d(int idx, double val) const {
         int a = idx / EL, b = idx - a * EL;
@@ -405,12 +426,25 @@ struct StGrid {
         double zf = z > 1.0e-10 ? PI_D * erf(z / 2.) / z : sqrt(PI_D);
         return c1 * (val + (*f0slot) * zf / (4. * PI_D * PI_D));
     }
+    // post 4: alpha (val + n0 zf(|r|) / (8 pi^3)) (DipolarInteractionPos, mpDPFT.cpp:2749-2755)
+    __device__ double dipolar(int idx, double val) const {
+        int a = idx / (EL * EL), rem = idx - a * EL * EL, b = rem / EL, c = rem - b * EL;
+        const double z = sqrt(lat[a] * lat[a] + lat[b] * lat[b] + lat[c] * lat[c]), sp = sqrt(PI_D);
+        double zf;
+        if (z > 1.0e-12)
+            zf = (PI_D / (z * z * z)) * (d_exp(-z * z / 4.) * sp * z * (4. + z * z) - 4. * PI_D * erf(z / 2.)) +
+                 PI_D * sp * d_exp(-z * z / 4.) * (-1. / 3. + 8. * PI_D / 3.);
+        else
+            zf = PI_D * sp * (8. * PI_D / 3.);
+        return c1 * (val + (n0 / (8. * PI_D * PI_D * PI_D)) * zf);
+    }
     __device__ void put(int idx, double2 v) const {
         if (re) {
             double a = scale * v.x;
             if (post == 1) a = d_pos(a);
             else if (post == 2) a = hart(idx, a);
             else if (post == 3) a = hart2d(idx, a);
+            else if (post == 4) a = dipolar(idx, a);
             re[idx] = a;
         }
         if (im) im[idx] = scale * v.y;
@@ -547,7 +581,7 @@ hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, 
     int s = g.s;
     StBuf sb{work, s, s};
     LdBuf lb{work, s, s};
-    StGrid sg{re, im, scale, post, g.EL, s, g.D, g.lat, m.f0slot, c1};
+    StGrid sg{re, im, scale, post, g.EL, s, g.D, g.lat, m.f0slot, c1, m.n0};
     if (g.D == 3) {
         // DelField's multipliers (with the (k, smoothing) table) compiled per kind
         if (m.kind == MUL_DERIV_PAIR && m.kek) {
@@ -870,6 +904,36 @@ __device__ __noinline__ double dirac_v_f(double n, int D, double gammaH) {
     float nf = (float)d_pos(n), Df = (float)D, x = nf * Df / (float)PI_D;
     return D > 1 ? (double)(-(float)gammaH * (2.f * (D == 3 ? cbrtf(x) : sqrtf(x))) / (Df - 1.f)) : 0.;
 }
+// DeltaEkin2D3Dcrossover (mpDPFT.cpp:2906-2923), 2D, Units 1: l = floor(-1/2 +
+// sqrt(1/4 + 4 pi n / alpha)) filled transverse levels; for l > 0 the energy
+// density E0 + n V0 + n^2 V1 (V0 = l alpha / 2, V1 = -pi l / (l + 1), E0 =
+// -alpha^2 l (l + 1)(l + 2) / (48 pi)) and the potential V0 + 2 n V1.
+__device__ __noinline__ double crossover(double n, double a, int energy) {
+    const double l = floor(-0.5 + sqrt(0.25 + 4. * PI_D * n / a));
+    if (!(l > 1.0e-12)) return 0.;
+    const double V0 = 0.5 * l * a, V1 = -PI_D * l / (l + 1.);
+    if (energy) return -a * a * l * (l + 1.) * (l + 2.) / (48. * PI_D) + n * V0 + n * n * V1;
+    return V0 + n * 2. * V1;
+}
+// Quasi2Dcontact (mpDPFT.cpp:2952-2967), 2D, two species, Units 1, beta != 0:
+// levels l_s = max(0, floor(-1/2 + sqrt(1/4 + 4 pi n_s / alpha))), G = beta
+// sqrt(alpha), f1 = alpha / (2 pi); potential of the focal species (n1)
+// G (n2 eta1(l1,l2) + f1 eta2(l1,l2)), energy density G (n1 n2 eta1(l1,l2) +
+// f1 (n1 eta2(l1,l2) + n2 eta2(l2,l1)) + f1^2 eta3(l1,l2)); eta = the reference's
+// 31 x 31 table (NaN for levels beyond the rows given, where the reference reads
+// past its table).
+__device__ __noinline__ double quasi2d(double n1, double n2, double a, double beta, const double *eta, int rows,
+                                       int energy) {
+    int l1 = (int)floor(-0.5 + sqrt(0.25 + 4. * PI_D * n1 / a)), l2 = (int)floor(-0.5 + sqrt(0.25 + 4. * PI_D * n2 / a));
+    l1 = l1 < 0 ? 0 : l1;
+    l2 = l2 < 0 ? 0 : l2;
+    const int k12 = l1 * 31 + l2, k21 = l2 * 31 + l1;
+    if (l1 > 30 || l2 > 30 || k12 >= rows || k21 >= rows) return __builtin_nan("");
+    const double f1 = a / (2. * PI_D), f2 = f1 * f1, G = beta * sqrt(a);
+    if (energy)
+        return G * (n1 * n2 * eta[3 * k12] + f1 * (n1 * eta[3 * k12 + 1] + n2 * eta[3 * k21 + 1]) + f2 * eta[3 * k12 + 2]);
+    return G * (n2 * eta[3 * k12] + f1 * eta[3 * k12 + 1]);
+}
 __device__ __noinline__ double renorm_v(double n_s, double n_o, double a) {
     const double es = renorm_eta(n_s, a), eo = renorm_eta(n_o, a);
     return PI_D * (d_pos(n_o) + 1.0e-16) * (qmc_beta(0, es) + qmc_beta(0, eo) + 0.5 * qmc_beta(1, es) * es * es);
@@ -920,6 +984,14 @@ __global__ __launch_bounds__(256) void k_update_v(Grid g, VintParams vp, const d
                 v = gombas_v(ns, vp.gammaH);
             } else if (t == 13 && g.D == 3 && vp.units == 2) {
                 v = vwn_v(ns);
+            } else if (t == 14 && vp.units == 1) {
+                v = crossover(ns, vp.alpha, 0);
+            } else if (t == 24) {  // two species, Units 1, beta != 0 (checked at create)
+                v = quasi2d(ns, den[(size_t)(1 - s) * G + i], vp.alpha, vp.beta, vp.eta, vp.eta_rows, 0);
+            } else if (t == 15) {
+                v = vp.dip_k ? vp.dip_k[idx] : 0.;
+            } else if (t == 17) {
+                v = vp.dip_r ? vp.dip_r[idx] : 0.;
             }
             acc += v;
         }
@@ -956,6 +1028,47 @@ __global__ __launch_bounds__(256) void k_vint_dvdn(Grid g, VintParams vp, const 
 }
 hipError_t vint_dvdn(const Grid &g, const VintParams &vp, const double *den, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_vint_dvdn, dim3(2048), dim3(256), 0, st, g, vp, den, out);
+    return hipGetLastError();
+}
+
+// DipolarInteraction (mpDPFT.cpp:2969-2990), 3D, momentum space: one block per
+// grid point q_i sums, over the grid points k_j in a fixed order, the trapezoid
+// weight times k_z/|k|^2 - 1/3 where p = q_i + k_j lies inside |p| < edge/2 and
+// n(p) > n(q_i), p rounded to its grid index as IndexOfVec (:3083) does.
+__global__ __launch_bounds__(256) void k_dipolar_k(Grid g, const double *__restrict__ n, double scale,
+                                                   double *out) {
+    __shared__ double sh[4];
+    const int EL = g.EL;
+    const double x0 = -g.edge / 2.;
+    for (size_t i = blockIdx.x; i < g.G; i += gridDim.x) {
+        double q[3];
+        coords(g, (int)i, q);
+        const double rhop = n[i];
+        double acc = 0.;
+        for (size_t j = threadIdx.x; j < g.G; j += 256) {
+            double k[3], pk[3];
+            coords(g, (int)j, k);
+            for (int d = 0; d < 3; d++) pk[d] = q[d] + k[d];
+            double tf = 0.;
+            if (sqrt(pk[0] * pk[0] + pk[1] * pk[1] + pk[2] * pk[2]) < 0.5 * g.edge) {
+                size_t idx = 0;
+                for (int d = 0; d < 3; d++) {
+                    const int ci = (int)(0.5 + (double)(EL - 1) * ((pk[d] - x0) / g.edge));
+                    idx = idx * (size_t)EL + (size_t)ci;
+                }
+                if (n[idx] - rhop > 0.) tf = k[2] / (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]) - 1. / 3.;
+            }
+            acc += trap_weight(g, (int)j) * tf;
+        }
+        const double r = block_sum(acc, sh);
+        if (threadIdx.x == 0) out[i] = scale * r;
+        __syncthreads();
+    }
+}
+hipError_t dipolar_k(const Grid &g, const double *n, double scale, double *out, hipStream_t st) {
+    if (g.D != 3) return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)std::min<size_t>(g.G, 65536);
+    hipLaunchKernelGGL(k_dipolar_k, dim3(blocks), dim3(256), 0, st, g, n, scale, out);
     return hipGetLastError();
 }
 
@@ -1170,6 +1283,8 @@ __global__ __launch_bounds__(256) void k_energy(Grid g, EnergyParams ep, const d
             }
             break;
         }
+        case E_CROSSOVER: v = crossover(a[i], ep.p0, 1); break;
+        case E_QUASI2D: v = quasi2d(a[i], b[i], ep.p0, ep.p1, ep.eta, ep.eta_rows, 1); break;
         case E_GRADCORR: {
             double rho = fabs(a[i]);
             v = rho > ep.p0 ? b[i] / rho : 0.;

@@ -337,6 +337,7 @@ static int run_one(const std::string &d, const mpdpft_params &p, int device, mpd
 // mpDPFT_ControlTask.dat gets one line per task.
 extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     std::string d = dir ? dir : "";
+    mpdpft_set_data_dir(d.c_str());  // the TabFunc files sit beside mpDPFT.input, as for the reference
     mpdpft_params p;
     mpdpft_params_default(&p);
     int rc = mpdpft_parse_input(join(d, "mpDPFT.input").c_str(), &p);

@@ -147,6 +147,40 @@ static int integral_from_partials(mpdpft_ctx *c, double *out) {
 // ---------------------------------------------------------------------------
 // set-up: InitializeHardCodedParameters (grid part, :862-896, :1105-1106)
 // ---------------------------------------------------------------------------
+// Directory of the TabFunc_*.dat files the reference reads from its working
+// directory (mpdpft_set_data_dir; "" = the current directory).
+static std::string g_data_dir;
+extern "C" int mpdpft_set_data_dir(const char *dir) {
+    g_data_dir = dir ? dir : "";
+    return MPDPFT_OK;
+}
+
+// TabFunc_2D3D_eta.dat (Load2D3Deta, mpDPFT.cpp:2927-2950): 31 x 31 triplets
+// (eta1, eta2, eta3)(l1, l2) in row-major order, one per line "l1 l2 eta1 eta2 eta3".
+static int load_eta_table(mpdpft_ctx *c) {
+    const std::string path =
+        (g_data_dir.empty() ? std::string() : g_data_dir + (g_data_dir.back() == '/' ? "" : "/")) +
+        "TabFunc_2D3D_eta.dat";
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return ctx_fail(c, MPDPFT_EIO, "interaction 24 needs " + path);
+    std::vector<double> eta;
+    char line[512];
+    while ((int)eta.size() < 3 * 31 * 31 && std::fgets(line, sizeof line, f)) {
+        int a, b;
+        double e0, e1, e2;
+        if (std::sscanf(line, "%d %d %lf %lf %lf", &a, &b, &e0, &e1, &e2) != 5) continue;
+        eta.push_back(e0);
+        eta.push_back(e1);
+        eta.push_back(e2);
+    }
+    std::fclose(f);
+    if (eta.empty()) return ctx_fail(c, MPDPFT_EIO, "no eta triplets in " + path);
+    c->eta_rows = (int)(eta.size() / 3);
+    RC(dmalloc(c, &c->d_eta, eta.size()));
+    HIPC(hipMemcpy(c->d_eta, eta.data(), sizeof(double) * eta.size(), hipMemcpyHostToDevice));
+    return MPDPFT_OK;
+}
+
 static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
     if (const char *why = params_structure_error(p)) return ctx_fail(c, MPDPFT_EINVAL, why);
     if (p->DIM != 2 && p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "DIM must be 2 or 3");
@@ -169,10 +203,21 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
     if ((int)p->Mixer[1] > 60) return ctx_fail(c, MPDPFT_EINVAL, "Pulay scope must be <= 60");
     for (int a = 0; a < p->n_interactions; a++) {
         int t = p->Interactions[a];
-        if (!(t == 0 || t == 1 || t == 8 || t == 9 || t == 10 || t == 11 || t == 12 || t == 13 || t == -1 || t == -12))
+        if (!(t == 0 || t == 1 || t == 8 || t == 9 || t == 10 || t == 11 || t == 12 || t == 13 || t == 14 || t == 15 ||
+              t == 17 || t == 24 || t == -1 || t == -12))
             return ctx_fail(c, MPDPFT_EINVAL, "unsupported interaction type " + std::to_string(t));
         if (t == 9 && !(p->DIM == 2 && p->S == 2))
             return ctx_fail(c, MPDPFT_EINVAL, "interaction 9 (renormalised contact) needs DIM 2 and two species");
+        if (t == 14 && p->DIM != 2)
+            return ctx_fail(c, MPDPFT_EINVAL, "interaction 14 (2D-3D crossover) needs DIM 2");
+        if (t == 24 && !(p->DIM == 2 && p->S == 2))
+            return ctx_fail(c, MPDPFT_EINVAL, "interaction 24 (quasi-2D contact) needs DIM 2 and two species");
+        if (t == 15 && !(p->DIM == 3 && p->S == 1))
+            return ctx_fail(c, MPDPFT_EINVAL, "interaction 15 (dipolar, momentum space) needs DIM 3 and one species");
+        if (t == 17 && p->DIM != 3)
+            return ctx_fail(c, MPDPFT_EINVAL, "interaction 17 (dipolar, position space) needs DIM 3");
+        if ((t == 14 || t == 15 || t == 24) && p->alpha == 0.)
+            return ctx_fail(c, MPDPFT_EINVAL, "interactions 14, 15, 24 need alpha != 0");
     }
     for (int r = 0; r < p->n_regularize; r++)
         if (p->regularize[r] != 0) return ctx_fail(c, MPDPFT_EINVAL, "regularize must be 0");
@@ -319,8 +364,13 @@ extern "C" int mpdpft_create(const mpdpft_params *p, int device, mpdpft_ctx **ou
 
     c->g = mk::Grid{c->D, c->s, c->EL, c->G, c->F, p->edge, c->dx, c->d_lat};
 
-    for (int a = 0; a < p->n_interactions; a++)
-        if (p->Interactions[a] == 8) c->has_hartree = 1;
+    for (int a = 0; a < p->n_interactions; a++) {
+        const int t = p->Interactions[a];
+        if (t == 8) c->has_hartree = 1;
+        if (t == 15 && !c->d_dip_k) RC(dmalloc(c, &c->d_dip_k, SG));
+        if (t == 17 && !c->d_dip_r) RC(dmalloc(c, &c->d_dip_r, SG));
+        if (t == 24 && !c->d_eta) RC(load_eta_table(c));
+    }
     if (c->has_hartree && p->HartreeMode == 1) {
         int M = 2 * c->s;
         size_t tot = (size_t)M * M * (c->D == 3 ? M : 1);
@@ -658,11 +708,58 @@ static int hartree(mpdpft_ctx *c, const double *n, double *out) {
 
 // UpdateV + GetVint (:4815-4850): V_s = inc * sum_a Vint_a + Env_s from the
 // mixed density (held in OldDen after the pointer swap of StoreCurrentIteration).
+// The dipolar potentials of interactions 17 (DipolarInteractionPos, one
+// transform pair per species) and 15 (DipolarInteraction, the pair sum), each
+// into its [S][G] field for k_update_v to add in list order.
+static int dipolar_fields(mpdpft_ctx *c, const double *den) {
+    const mpdpft_params &p = c->p;
+    const size_t G = c->G;
+    if (c->d_dip_r) {
+        for (int s = 0; s < c->S; s++) {
+            stage_begin(c);
+            HIPC(mk::fft_forward_real(c->g, den + s * G, nullptr, c->d_spec, c->d_tw, c->d_scal + 200, c->st));
+            mk::MulParams m{};
+            m.kind = mk::MUL_DIPOLAR;
+            m.kq = c->d_kq;
+            m.kq_h = c->d_kqh;
+            m.dxD = std::pow(c->dx, 3.);
+            m.dkzero = 0.01 * c->dk * c->dk;
+            m.n0 = p.Abundances[s];
+            HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_spec, m, c->d_dip_r + s * G, nullptr,
+                                       std::pow(1. / p.edge, 3.), 4, p.alpha, c->d_tw, c->st));
+            stage_end(c, "dipolar_pos", 2., 2. * 32. * 3. * (double)c->F, 0.);
+        }
+    }
+    if (c->d_dip_k) {
+        if (p.Units == 1) {
+            double scale = p.alpha / (2. * std::pow(2. * kPI, 3.));
+            const double a = (p.edge / 2. - (-p.edge / 2.)) / ((double)c->EL - 1.);
+            for (int d = 0; d < 3; d++) scale *= a;
+            stage_begin(c);
+            HIPC(mk::dipolar_k(c->g, den, scale, c->d_dip_k, c->st));
+            stage_end(c, "dipolar_k", 1., 8. * (double)G, 0.);
+        } else {
+            HIPC(mk::fill(G, 0., c->d_dip_k, c->st));
+        }
+    }
+    return MPDPFT_OK;
+}
+
+static void vint_extra(const mpdpft_ctx *c, mk::VintParams &vp) {
+    vp.alpha = c->p.alpha;
+    vp.eta = c->d_eta;
+    vp.eta_rows = c->eta_rows;
+    vp.dip_r = c->d_dip_r;
+    vp.dip_k = c->d_dip_k;
+}
+
 static int update_v(mpdpft_ctx *c, const double *den) {
     const mpdpft_params &p = c->p;
     if (c->has_hartree)
         for (int s = 0; s < c->S; s++) RC(hartree(c, den + s * c->G, c->d_V + s * c->G));
+    RC(dipolar_fields(c, den));
     mk::VintParams vp{};
+    vint_extra(c, vp);
     vp.n_inter = p.n_interactions;
     for (int a = 0; a < p.n_interactions; a++) vp.inter[a] = p.Interactions[a];
     vp.S = c->S;
@@ -1115,6 +1212,28 @@ static int interaction_energy(mpdpft_ctx *c, int type, double *E) {  // :5805-59
         ep.op = type == 12 ? mk::E_GOMBAS : mk::E_VWN;
         ep.p0 = p.gammaH;
         return energy_integral(c, ep, c->d_Den, nullptr, nullptr, nullptr, E);
+    }
+    if (c->D == 2 && type == 14) {  // DeltaEkin2D3Dcrossover energy, summed over species (:5879-5886)
+        if (p.Units != 1) return MPDPFT_OK;
+        ep.op = mk::E_CROSSOVER;
+        ep.p0 = p.alpha;
+        double acc = 0.;
+        for (int s = 0; s < c->S; s++) {
+            double v;
+            RC(energy_integral(c, ep, c->d_Den + s * G, nullptr, nullptr, nullptr, &v));
+            acc += v;
+        }
+        *E = acc;
+        return MPDPFT_OK;
+    }
+    if (c->D == 2 && c->S == 2 && type == 24) {  // Quasi2Dcontact energy, counted once (:5890-5898)
+        if (!(p.Units == 1 && std::fabs(p.beta) > kMP)) return MPDPFT_OK;
+        ep.op = mk::E_QUASI2D;
+        ep.p0 = p.alpha;
+        ep.p1 = p.beta;
+        ep.eta = c->d_eta;
+        ep.eta_rows = c->eta_rows;
+        return energy_integral(c, ep, c->d_Den, c->d_Den + G, nullptr, nullptr, E);
     }
     if (c->D == 3 && c->S == 1 && type == 11) {  // DiracExchange energy (:2762-2764)
         double DD = 3.;
@@ -1736,7 +1855,9 @@ extern "C" int mpdpft_interaction_potential(mpdpft_ctx *c, int type, int s, cons
     if (type == 8) {
         RC(hartree(c, c->d_OldDen + s * G, c->d_tmp1));
     } else {
+        if (type == 15 || type == 17) RC(dipolar_fields(c, c->d_OldDen));
         mk::VintParams vp{};
+        vint_extra(c, vp);
         vp.n_inter = 1;
         vp.inter[0] = type;
         vp.S = c->S;

@@ -74,6 +74,9 @@ struct mpdpft_ctx {
     int verbose = 0;  // MPDPFT_VERBOSE: one line per SC iteration on stderr
     int nait_plain = 0;  // MPDPFT_NAIT_KERNEL=plain: per-node nAiT kernel without panel reuse
     int nait_generic = 0;  // MPDPFT_NAIT_KERNEL=reuse: panel reuse without the fixed lane layout (any BP)
+    double *d_eta = nullptr;  // quasi-2D contact's eta table [eta_rows][3] (interaction 24)
+    int eta_rows = 0;
+    double *d_dip_r = nullptr, *d_dip_k = nullptr;  // [S][G] dipolar potentials of interactions 17 and 15
     std::vector<double> V0;   // initial potential of a solve (empty: V = Env)
     std::ostringstream notes; // lines for the next solve's control stream (LoadV)
     int importWarnings = 0;   // warnings of that import, added to the solve's count
