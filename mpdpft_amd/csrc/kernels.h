@@ -137,9 +137,12 @@ hipError_t max_abs_diff(size_t n, const double *a, const double *b, double *part
 // Pulay helpers over n = S*G entries.
 hipError_t pulay_push(size_t n, const double *nin, const double *nout, double *slot_nin, double *slot_res,
                       hipStream_t st);
-// partials[j*kRedBlocks + blk] = partial <res_new, res_j>, j < nh
+// partials[j*kRedBlocks + blk] = partial <res_new, res_j>, j < nh (FP64 matrix cores)
 hipError_t multi_dot(size_t n, const double *x, const double *const *ys, int nh, double *partials,
                      hipStream_t st);
+// out = w(q) x for S packed spectra, w = 2^(axes at the Nyquist index): the multiplicity of
+// each mode among the (s+1)^D entries of the wrap-filled centred spectrum (Pulay momentum metric)
+hipError_t weight_nyquist(const Grid &g, int S, const double2 *x, double2 *out, hipStream_t st);
 // den[s] = sum_p c_p (nin_p + theta_s res_p)
 hipError_t pulay_mix(const Grid &g, int S, int I, const double *const *nin, const double *const *res,
                      const double *coeff_dev, const double *theta_dev, double *den, hipStream_t st);

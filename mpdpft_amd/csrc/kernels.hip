@@ -1144,24 +1144,65 @@ __global__ void k_pulay_push(size_t n, const double *nin, const double *nout, do
 struct PtrList {
     const double *p[64];
 };
-__global__ __launch_bounds__(256) void k_multi_dot(size_t n, const double *x, PtrList ys, int nh, double *partials) {
-    __shared__ double sh[4];
-    double acc[16];
+// Pulay Gram row <y_j, x> (j < nh) on the FP64 matrix cores: per tile of 16
+// history rows, v_mfma_f64_16x16x4f64 multiplies A (16 rows x 4 elements, lane
+// l holding row l%16's element at slot l/16) by B (4 x 16, only column 0 = x
+// nonzero: lanes 0, 16, 32, 48) into C; C[i][0] accumulates <y_i, x>.  The 4
+// element slots of one MFMA are the lane groups' own contiguous runs of kMdT
+// elements (lane group g takes [c*4kMdT + g*kMdT, +kMdT) of chunk c), so every
+// element of x meets the same element of each row exactly once.  C[i][j] sits in
+// lane j + 16*(i%4), register i/4 (A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k), so C[i][0] is in lane 16*(i%4).  Per block the four waves' rows are
+// summed in a fixed order.
+typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kMdT = 4;
+__global__ __launch_bounds__(256) void k_multi_dot(size_t n, const double *__restrict__ x, PtrList ys, int nh,
+                                                   double *partials) {
+    __shared__ double sh[4][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const size_t chunk = 4 * kMdT;
     for (int base = 0; base < nh; base += 16) {
-        int cnt = nh - base < 16 ? nh - base : 16;
+        const double *row = base + r < nh ? ys.p[base + r] : nullptr;
+        mfma_f64x4 acc = {0., 0., 0., 0.};
+        for (size_t c = (size_t)blockIdx.x * 4 + wave; c * chunk < n; c += (size_t)gridDim.x * 4) {
+            const size_t k0 = c * chunk + (size_t)g * kMdT;
+            double a[kMdT], b[kMdT];
 #pragma unroll
-        for (int j = 0; j < 16; j++) acc[j] = 0.;
-        for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-            double xv = x[i];
+            for (int t = 0; t < kMdT; t++) {
+                const size_t k = k0 + t;
+                a[t] = (row && k < n) ? row[k] : 0.;
+                b[t] = (r == 0 && k < n) ? x[k] : 0.;
+            }
 #pragma unroll
-            for (int j = 0; j < 16; j++)
-                if (j < cnt) acc[j] += xv * ys.p[base + j][i];
+            for (int t = 0; t < kMdT; t++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], acc, 0, 0, 0);
         }
-        for (int j = 0; j < cnt; j++) {
-            double r = block_sum(acc[j], sh);
-            if (threadIdx.x == 0) partials[(size_t)(base + j) * kRedBlocks + blockIdx.x] = r;
-            __syncthreads();
+        if (r == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) sh[wave][g + 4 * v] = acc[v];
         }
+        __syncthreads();
+        if (threadIdx.x < 16 && base + (int)threadIdx.x < nh) {
+            const int i = threadIdx.x;
+            partials[(size_t)(base + i) * kRedBlocks + blockIdx.x] = (sh[0][i] + sh[1][i]) + (sh[2][i] + sh[3][i]);
+        }
+        __syncthreads();
+    }
+}
+
+// x' = w(q) x over S packed spectra of F points: w = 2^(number of axes at the
+// Nyquist index s/2), the multiplicity of that mode among the (s+1)^D entries of
+// the reference's wrap-filled centred spectrum.
+__global__ void k_weight_nyquist(Grid g, int S, const double2 *x, double2 *out) {
+    const size_t F = g.F, tot = F * (size_t)S;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
+        size_t q = idx % F;
+        double w = 1.;
+        for (int d = 0; d < g.D; d++) {
+            if ((int)(q % (size_t)g.s) == g.s / 2) w *= 2.;
+            q /= (size_t)g.s;
+        }
+        const double2 v = x[idx];
+        out[idx] = double2{w * v.x, w * v.y};
     }
 }
 
@@ -1339,6 +1380,10 @@ hipError_t multi_dot(size_t n, const double *x, const double *const *ys, int nh,
     PtrList pl;
     for (int j = 0; j < 64; j++) pl.p[j] = j < nh ? ys[j] : nullptr;
     hipLaunchKernelGGL(k_multi_dot, dim3(kRedBlocks), dim3(256), 0, st, n, x, pl, nh, partials);
+    return hipGetLastError();
+}
+hipError_t weight_nyquist(const Grid &g, int S, const double2 *x, double2 *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_weight_nyquist, dim3(gblocks(g.F * S)), dim3(256), 0, st, g, S, x, out);
     return hipGetLastError();
 }
 hipError_t pulay_mix(const Grid &g, int S, int I, const double *const *nin, const double *const *res,

@@ -198,8 +198,6 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
     if (p->Symmetry != 0) return ctx_fail(c, MPDPFT_EINVAL, "Symmetry must be 0");
     if (p->Noise != 0.) return ctx_fail(c, MPDPFT_EINVAL, "Noise must be 0");
     if ((int)(p->Mixer[0] + 0.5) > 1) return ctx_fail(c, MPDPFT_EINVAL, "Mixer[0] must be 0 or 1");
-    if (std::fabs(p->Mixer[2]) > kMP && p->DIM > 1)
-        return ctx_fail(c, MPDPFT_EINVAL, "momentum-space Pulay metric not supported (Mixer[2] must be 0)");
     if ((int)p->Mixer[1] > 60) return ctx_fail(c, MPDPFT_EINVAL, "Pulay scope must be <= 60");
     for (int a = 0; a < p->n_interactions; a++) {
         int t = p->Interactions[a];
@@ -458,6 +456,8 @@ extern "C" void mpdpft_destroy(mpdpft_ctx *c) {
     for (void *q : c->allocations) hipFree(q);
     for (double *q : c->nin) hipFree(q);
     for (double *q : c->res) hipFree(q);
+    for (double2 *q : c->resF) hipFree(q);
+    if (c->h_spec0) hipHostFree(c->h_spec0);
     if (c->h_scal) hipHostFree(c->h_scal);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -1034,33 +1034,104 @@ static bool fullpiv_inverse(int n, const double *A, double *inv) {
 // PulayMixer, mixer 1, Euclidean metric (:5005-5167).  History slots hold
 // n_in and the residual R = n_out - n_in; only the newest Gram row/column is
 // computed (one fused multi-dot pass), older entries come from the cache.
+// |k| at grid index i (kVecAt, x slowest): the momentum metric's q of history position i.
+static double kvec_norm(const mpdpft_ctx *c, int i) {
+    double q2 = 0.;
+    int rem = i;
+    for (int d = c->D - 1; d >= 0; d--) {
+        const double k = c->k0 + (double)(rem % c->EL) * c->dk;
+        q2 += k * k;
+        rem /= c->EL;
+    }
+    return std::sqrt(q2);
+}
+
+// PulayScalarProduct with the momentum-space metric (mpDPFT.cpp:5212-5227)
+// from the full-spectrum dot D = sum over the (s+1)^D wrap-filled centred
+// entries and the k = 0 product A0 (both summed over species): dk^D [(1 + w
+// AngularFactor(D) sqrt(pi)/2) A0 + (1 + w/q^{D-1})(D - A0) - (G - 1) w/q^{D-1}
+// EXP(-q^2) A0], q = |kVecAt[i]| at the first argument's history position i.
+static double pulay_sp_momentum(const mpdpft_ctx *c, int i, double D, double A0) {
+    const double w = c->pulay_weight, DD = (double)c->D;
+    const double af = c->D == 2 ? 2. * kPI : 4. * kPI;  // AngularFactor, mpDPFT.h:291
+    const double q = kvec_norm(c, i), wq = w / std::pow(q, DD - 1.), x = -q * q;
+    const double ex = x < -700. ? 0. : std::exp(x);
+    const double res = (1. + w * af * std::sqrt(kPI) / 2.) * A0 + (1. + wq) * (D - A0) -
+                       ((double)c->G - 1.) * wq * ex * A0;
+    return std::pow(c->dk, DD) * res;
+}
+
 static int pulay_push_row(mpdpft_ctx *c) {
     int scope = c->scope, ld = scope + 1;
     int I = c->pI;
+    const int S = c->S;
     HIPC(mk::pulay_push(c->SG, c->d_OldDen, c->d_Den, c->nin[I], c->res[I], c->st));
+    if (c->pulay_momentum) {  // the residuals in momentum space (:5028-5034)
+        for (int s = 0; s < S; s++) {
+            HIPC(mk::fft_forward_real(c->g, c->res[I] + s * c->G, nullptr, c->resF[I] + s * c->F, c->d_tw,
+                                      c->d_scal + 200, c->st));
+            HIPC(hipMemcpyAsync(c->h_spec0 + 2 * s, c->resF[I] + s * c->F, sizeof(double2), hipMemcpyDeviceToHost,
+                                c->st));
+        }
+        HIPC(hipStreamSynchronize(c->st));
+        for (int k = 0; k < 2 * S; k++) c->resF0[(size_t)I * 2 * S + k] = c->h_spec0[k];
+    }
     I++;
     int shift = 0;
     if (I > scope) {  // drop the oldest entry (rotate slot pointers)
         double *a = c->nin[0], *b = c->res[0];
+        double2 *f = c->pulay_momentum ? c->resF[0] : nullptr;
+        std::vector<double> f0(c->resF0.begin(), c->resF0.begin() + (c->pulay_momentum ? 2 * S : 0));
         for (int k = 0; k < I - 1; k++) {
             c->nin[k] = c->nin[k + 1];
             c->res[k] = c->res[k + 1];
+            if (f) {
+                c->resF[k] = c->resF[k + 1];
+                for (int q = 0; q < 2 * S; q++) c->resF0[(size_t)k * 2 * S + q] = c->resF0[(size_t)(k + 1) * 2 * S + q];
+            }
         }
         c->nin[I - 1] = a;
         c->res[I - 1] = b;
+        if (f) {
+            c->resF[I - 1] = f;
+            for (int q = 0; q < 2 * S; q++) c->resF0[(size_t)(I - 1) * 2 * S + q] = f0[q];
+        }
         I--;
         shift = 1;
     }
     c->pI = I;
-    std::vector<double> row(I);
-    HIPC(mk::multi_dot(c->SG, c->res[I - 1], c->res.data(), I, c->d_partials, c->st));
-    RC(ctx_reduce(c, I, 1., row.data()));
+    // the new row (I-1, j) and column (i, I-1) of the Gram matrix; Euclidean: one symmetric dot per entry
+    std::vector<double> row(I), col(I);
+    if (c->pulay_momentum) {
+        HIPC(mk::weight_nyquist(c->g, S, c->resF[I - 1], c->d_specx, c->st));
+        std::vector<const double *> ys(I);
+        for (int i = 0; i < I; i++) ys[i] = (const double *)c->resF[i];
+        HIPC(mk::multi_dot(2 * (size_t)S * c->F, (const double *)c->d_specx, ys.data(), I, c->d_partials, c->st));
+        std::vector<double> dots(I);
+        RC(ctx_reduce(c, I, 1., dots.data()));
+        // centred spectrum = dx^D (-1)^{sum q} DFT: products of two carry dx^{2D} (the signs cancel)
+        const double sc = std::pow(c->dx, 2. * c->D);
+        const double *n0 = &c->resF0[(size_t)(I - 1) * 2 * S];
+        for (int i = 0; i < I; i++) {
+            const double *f0 = &c->resF0[(size_t)i * 2 * S];
+            double A0 = 0.;
+            for (int s = 0; s < S; s++) A0 += f0[2 * s] * n0[2 * s] + f0[2 * s + 1] * n0[2 * s + 1];
+            const double D = sc * dots[i];
+            A0 *= sc;
+            row[i] = pulay_sp_momentum(c, I - 1, D, A0);
+            col[i] = pulay_sp_momentum(c, i, D, A0);
+        }
+    } else {
+        HIPC(mk::multi_dot(c->SG, c->res[I - 1], c->res.data(), I, c->d_partials, c->st));
+        RC(ctx_reduce(c, I, 1., row.data()));
+        col = row;
+    }
     std::vector<double> old(c->ResMat);
     for (int i = 0; i < I; i++)
         for (int j = 0; j < I; j++) {
             double v;
             if (i == I - 1) v = row[j];
-            else if (j == I - 1) v = row[i];
+            else if (j == I - 1) v = col[i];
             else v = old[(i + shift) * ld + (j + shift)];
             c->ResMat[i * ld + j] = v;
         }
@@ -1418,6 +1489,20 @@ static int init_loop(mpdpft_ctx *c) {
                 HIPC(hipMalloc((void **)&c->res[i], sizeof(double) * c->SG));
             }
     }
+    // InitializePulayMixer (:7029-7062): the metric weight (k0^2 + dk^2 when Mixer[2] < 0; the
+    // reference leaves it unset when Mixer[2] = 0, taken as 0 here: Euclidean)
+    c->pulay_weight = 0.;
+    if (std::fabs(p.Mixer[2]) > kMP) c->pulay_weight = p.Mixer[2] < 0. ? c->k0 * c->k0 + c->dk * c->dk : p.Mixer[2];
+    c->pulay_momentum = c->pulay_mixer > 0 && c->D > 1 && c->pulay_weight > 0.;
+    if (c->pulay_momentum && (int)c->resF.size() != c->scope + 1) {
+        for (double2 *q : c->resF) hipFree(q);
+        c->resF.assign(c->scope + 1, nullptr);
+        for (int i = 0; i <= c->scope; i++) HIPC(hipMalloc((void **)&c->resF[i], sizeof(double2) * c->S * c->F));
+        if (!c->d_specx) RC(dmalloc(c, &c->d_specx, (size_t)c->S * c->F));
+        if (!c->h_spec0) HIPC(hipHostMalloc((void **)&c->h_spec0, 2 * MPDPFT_MAX_SPECIES * sizeof(double),
+                                            hipHostMallocDefault));
+    }
+    c->resF0.assign((size_t)(c->scope + 1) * 2 * c->S, 0.);
     c->pI = 0;
     c->ResMat.assign((size_t)(c->scope + 1) * (c->scope + 1), 0.);
     // initial densities (:194-201)

@@ -51,6 +51,14 @@ struct mpdpft_ctx {
     int pulay_mixer = 0, scope = 0, pI = 0;
     std::vector<double *> nin, res;
     std::vector<double> ResMat;  // (scope+1)^2
+    // Pulay momentum-space metric (Mixer[2] != 0, DIM > 1): each entry's residual
+    // spectra [S][F] beside it and their k = 0 values on the host
+    int pulay_momentum = 0;
+    double pulay_weight = 0.;
+    std::vector<double2 *> resF;
+    std::vector<double> resF0;   // [scope+1][S][2]
+    double2 *d_specx = nullptr;  // [S][F] Nyquist-weighted spectrum of the newest residual
+    double *h_spec0 = nullptr;   // pinned [S][2]
     double coeff[64];
 
     // SC state (host scalars)

@@ -259,6 +259,9 @@ struct orc_state {
     /* Pulay */
     int pulay_mixer, pulay_scope, pulay_I;
     double **nin, **nout, **res; /* history entries, each [S][G] */
+    cplx **resF;                 /* their residuals' centred spectra [S][G] (momentum-space metric) */
+    double pulay_weight;         /* metric weight (Mixer[2]; k0^2 + dk^2 when negative) */
+    int pulay_momentum;          /* DIM > 1 and weight > 0 */
     double *ResMat;              /* scope x scope cached Gram */
     double coeff[64];
     /* energies */
@@ -424,10 +427,12 @@ static void orc_free(orc_state *o) {
             free(o->nin[i]);
             free(o->nout[i]);
             free(o->res[i]);
+            if (o->resF) free(o->resF[i]);
         }
         free(o->nin);
         free(o->nout);
         free(o->res);
+        free(o->resF);
     }
     free(o->ResMat);
     free(o);
@@ -1810,6 +1815,62 @@ int oracle_pulay_coefficients(int I, const double *A, double *coeff, double *tes
 }
 
 /* PulayMixer, mixer 1 with the Euclidean metric (mpDPFT.cpp:5005-5167). */
+/* PulayScalarProduct (mpDPFT.cpp:5212-5227) with the momentum-space metric:
+ * A0 = Re(Ri[c] conj Rj[c]) at the centre index c (k = 0),
+ * dk^D [(1 + w AngularFactor(D) sqrt(pi)/2) A0 + sum_{l != c} (Re(Ri[l] conj Rj[l])
+ * (1 + w/q^{D-1}) - w/q^{D-1} EXP(-q^2) A0)] over all G entries of the centred
+ * spectra, where q = |kVecAt[i]| is taken at grid index i = the first
+ * argument's history position (the reference indexes kVecAt by i, not l). */
+static double pulay_sp_momentum(const orc_state *o, int s, int i, int j) {
+    const long long G = o->G, c = o->CentreIndex;
+    const cplx *Ri = o->resF[i] + (long long)s * G, *Rj = o->resF[j] + (long long)s * G;
+    const double w = o->pulay_weight, DD = (double)o->D;
+    const double af = o->D == 1 ? 2. : (o->D == 2 ? 2. * PI_ : 4. * PI_); /* AngularFactor, mpDPFT.h:291 */
+    const double at0 = Ri[c].re * Rj[c].re + Ri[c].im * Rj[c].im;
+    double res = (1. + w * af * sqrt(PI_) / 2.) * at0;
+    double kv[3] = {0., 0., 0.};
+    kvec_at(o, i, kv);
+    const double q = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+    const double wq = w / pow(q, DD - 1.), ex = EXPs(-q * q);
+    for (long long l = 0; l < c; l++)
+        res += (Ri[l].re * Rj[l].re + Ri[l].im * Rj[l].im) * (1 + wq) - wq * ex * at0;
+    for (long long l = c + 1; l < G; l++)
+        res += (Ri[l].re * Rj[l].re + Ri[l].im * Rj[l].im) * (1 + wq) - wq * ex * at0;
+    return pow(o->dk, DD) * res;
+}
+
+/* The I x I momentum-metric Gram matrix sum_s PulayScalarProduct(s, i, j) of
+ * residuals res [I][S][G] (test probe; the loop itself caches rows). */
+int oracle_pulay_momentum_gram(const mpdpft_params *p, int I, const double *res, double *M) {
+    if (I < 1 || I > 64) return MPDPFT_EINVAL;
+    orc_state *o = orc_create(p);
+    o->pulay_weight = fabs(p->Mixer[2]) > MP_ ? (p->Mixer[2] < 0. ? o->k0 * o->k0 + o->dk * o->dk : p->Mixer[2]) : 0.;
+    const long long SG = (long long)o->S * o->G;
+    o->resF = (cplx **)calloc((size_t)I, sizeof(cplx *));
+    for (int i = 0; i < I; i++) {
+        o->resF[i] = (cplx *)calloc((size_t)SG, sizeof(cplx));
+        for (int s = 0; s < o->S; s++) {
+            for (long long l = 0; l < o->G; l++) {
+                o->CF[l].re = res[(long long)i * SG + (long long)s * o->G + l];
+                o->CF[l].im = 0.;
+            }
+            fft_centered(o, o->CF, -1);
+            memcpy(o->resF[i] + (long long)s * o->G, o->CF, sizeof(cplx) * (size_t)o->G);
+        }
+    }
+    for (int i = 0; i < I; i++)
+        for (int j = 0; j < I; j++) {
+            double v = 0.;
+            for (int s = 0; s < o->S; s++) v += pulay_sp_momentum(o, s, i, j);
+            M[i * I + j] = v;
+        }
+    for (int i = 0; i < I; i++) free(o->resF[i]);
+    free(o->resF);
+    o->resF = NULL;
+    orc_free(o);
+    return 0;
+}
+
 static int pulay_mixer(orc_state *o) {
     long long SG = (long long)o->S * o->G;
     int scope = o->pulay_scope;
@@ -1818,18 +1879,30 @@ static int pulay_mixer(orc_state *o) {
     memcpy(o->nin[I], o->OldDen, sizeof(double) * (size_t)SG);
     memcpy(o->nout[I], o->Den, sizeof(double) * (size_t)SG);
     for (long long i = 0; i < SG; i++) o->res[I][i] = o->nout[I][i] - o->nin[I][i];
+    if (o->pulay_momentum) /* the residuals in momentum space (:5028-5034) */
+        for (int s = 0; s < o->S; s++) {
+            for (long long l = 0; l < o->G; l++) {
+                o->CF[l].re = o->res[I][(long long)s * o->G + l];
+                o->CF[l].im = 0.;
+            }
+            fft_centered(o, o->CF, -1);
+            memcpy(o->resF[I] + (long long)s * o->G, o->CF, sizeof(cplx) * (size_t)o->G);
+        }
     I++;
     int shift = 0;
     if (I > scope) {
         double *a = o->nin[0], *b = o->nout[0], *c = o->res[0];
+        cplx *cf = o->pulay_momentum ? o->resF[0] : NULL;
         for (int k = 0; k < I - 1; k++) {
             o->nin[k] = o->nin[k + 1];
             o->nout[k] = o->nout[k + 1];
             o->res[k] = o->res[k + 1];
+            if (cf) o->resF[k] = o->resF[k + 1];
         }
         o->nin[I - 1] = a;
         o->nout[I - 1] = b;
         o->res[I - 1] = c;
+        if (cf) o->resF[I - 1] = cf;
         I--;
         shift = 1;
     }
@@ -1844,7 +1917,8 @@ static int pulay_mixer(orc_state *o) {
             if (i == I - 1 || j == I - 1) {
                 v = 0.;
                 for (int s = 0; s < o->S; s++)
-                    v += vecmult(o->res[i] + s * o->G, o->res[j] + s * o->G, o->G);
+                    v += o->pulay_momentum ? pulay_sp_momentum(o, s, i, j)
+                                           : vecmult(o->res[i] + s * o->G, o->res[j] + s * o->G, o->G);
             } else
                 v = old[(i + shift) * (oldn + 1) + (j + shift)];
             M[i * I + j] = v;
@@ -2093,7 +2167,12 @@ static int orc_init_loop(orc_state *o) {
     if (o->pulay_scope < 1) o->pulay_scope = 1;
     if (o->pulay_scope > 60) return MPDPFT_EINVAL;
     if (o->pulay_mixer > 1) return MPDPFT_EINVAL;
-    if (fabs(o->p.Mixer[2]) > MP_ && o->D > 1) return MPDPFT_EINVAL; /* momentum metric: not on this path */
+    o->pulay_weight = 0.;
+    if (fabs(o->p.Mixer[2]) > MP_) {
+        o->pulay_weight = o->p.Mixer[2];
+        if (o->pulay_weight < 0.) o->pulay_weight = o->k0 * o->k0 + o->dk * o->dk;
+    }
+    o->pulay_momentum = o->D > 1 && o->pulay_weight > 0.;
     int n = o->pulay_scope + 1;
     o->nin = (double **)calloc((size_t)n, sizeof(double *));
     o->nout = (double **)calloc((size_t)n, sizeof(double *));
@@ -2102,6 +2181,10 @@ static int orc_init_loop(orc_state *o) {
         o->nin[i] = dalloc(o->S * o->G);
         o->nout[i] = dalloc(o->S * o->G);
         o->res[i] = dalloc(o->S * o->G);
+    }
+    if (o->pulay_momentum) {
+        o->resF = (cplx **)calloc((size_t)n, sizeof(cplx *));
+        for (int i = 0; i < n; i++) o->resF[i] = (cplx *)calloc((size_t)(o->S * o->G), sizeof(cplx));
     }
     o->ResMat = dalloc((long long)n * n);
     return 0;

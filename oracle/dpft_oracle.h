@@ -66,6 +66,8 @@ double oracle_exp_i0(double x);
 int oracle_load_eta(const char *path);
 /* energy integrand of interaction 14 (species s) or 24 (species s focal) */
 int oracle_energy_density(const mpdpft_params *p, int type, int s, const double *den, double *out);
+/* Gram matrix of I residuals [I][S][G] in the Pulay momentum-space metric (Mixer[2] != 0, DIM > 1) */
+int oracle_pulay_momentum_gram(const mpdpft_params *p, int I, const double *res, double *M);
 
 /* Timed partial workload for the CPU baseline: n_iter SC iterations (no
  * convergence test) starting from the initial density; returns seconds. */

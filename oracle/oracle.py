@@ -144,6 +144,20 @@ def energy_density(p, type_, s, den):
     return out
 
 
+def pulay_momentum_gram(p, res):
+    """Gram matrix of residuals res [I][S][G] in the momentum-space metric."""
+    L = lib()
+    P = ctypes.POINTER(type(p))
+    L.oracle_pulay_momentum_gram.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double)]
+    res = np.ascontiguousarray(res, dtype=np.float64)
+    I = res.shape[0]
+    M = np.zeros((I, I))
+    rc = L.oracle_pulay_momentum_gram(ctypes.byref(p), I, _dp(res), _dp(M))
+    assert rc == 0, rc
+    return M
+
+
 def vint(p, type_, s, den):
     G = grid_size(p)
     den = np.ascontiguousarray(den, dtype=np.float64).reshape(-1)
