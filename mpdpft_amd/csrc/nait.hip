@@ -1257,7 +1257,8 @@ __device__ __forceinline__ EkinNodes ekin_nodes(int gs, double dg, double X, dou
 // spread over the lanes.  The host raises MaxIt by one per launch within one
 // energy evaluation; a point's refinements up to the previous cap are the same
 // in every launch, so each launch resumes from the stored last refinement
-// (st_m: its index, -1 once the convergence test stopped the point; st_v: its
+// (st_m: its index, or -(index + 2) once the convergence test stopped the point at
+// that refinement, -1 for the a < thr form; st_v: its
 // value) and evaluates only the new one.
 __global__ __launch_bounds__(64) void k_ekin_ait(const double2 *__restrict__ ap, EkinArgs ea, double *out,
                                                 double *st_v, int *st_m) {
@@ -1334,34 +1335,101 @@ __global__ __launch_bounds__(64) void k_ekin_ait(const double2 *__restrict__ ap,
     if (lane == 0) {
         out[i] = val;
         st_v[i] = val;
-        st_m[i] = done ? -1 : iteration;
+        st_m[i] = done ? -(iteration + 2) : iteration;  // < 0: converged at refinement -(st_m + 2)
     }
 }
 
 // d Ai / dy of airy_fast as computed (polynomial derivative; asymptotic form below -300)
-__device__ double airy_fast_d(const double2 *__restrict__ ap, double y) {
+// Ai and Ai' together (one table lookup): the quintic and its derivative on the
+// interval's monomial coefficients; the asymptotic forms below y = -300.
+__device__ __forceinline__ void airy_pair(const double2 *__restrict__ ap, double y, double &A, double &Ad) {
+    if (y > 100.) {
+        A = Ad = 0.;
+        return;
+    }
     if (y <= -300.) {
-        double z = -y, q = pow(z, 0.25), phi = 2. * z * sqrt(z) / 3. - 0.25 * kPI;
-        double dAdz = (-sin(phi) * sqrt(z) / q - cos(phi) * 0.25 / (q * z)) / sqrt(kPI);
-        return -dAdz;
+        const double z = -y, q = pow(z, 0.25), phi = 2. * z * sqrt(z) / 3. - 0.25 * kPI;
+        A = cos(phi) / (sqrt(kPI) * q);
+        Ad = (sin(phi) * sqrt(z) / q + cos(phi) * 0.25 / (q * z)) / sqrt(kPI);
+        return;
     }
     int k = (int)floor((y - kAX0) * kAINV);
     k = k < 0 ? 0 : (k > kAN - 2 ? kAN - 2 : k);
-    double u = (y - (kAX0 + kAH * (double)k)) * kAINV;
+    const double u = (y - (kAX0 + kAH * (double)k)) * kAINV;
     const double2 *c = ap + 3 * k;
-    double2 c01 = c[0], c23 = c[1], c45 = c[2];
-    return ((((5. * c45.y * u + 4. * c45.x) * u + 3. * c23.y) * u + 2. * c23.x) * u + c01.y) * kAINV;
+    const double2 c01 = c[0], c23 = c[1], c45 = c[2];
+    A = ((((c45.y * u + c45.x) * u + c23.y) * u + c23.x) * u + c01.y) * u + c01.x;
+    Ad = ((((5. * c45.y * u + 4. * c45.x) * u + 3. * c23.y) * u + 2. * c23.x) * u + c01.y) * kAINV;
 }
 
 // Partial derivatives of the EkinAiT03D energy density (k_ekin_ait with the
 // final MaxIt, no resume) with respect to U = V - mu, |grad V|^2 and lap V:
-// the refinement that ends the point's loop is found as in k_ekin_ait, and
-// its trapezoid sum is differentiated with its node count held fixed (the
-// nodes x_j = j X / (gs - 1) move with X = max(1, 100 - U / a)).
-__global__ __launch_bounds__(64) void k_ekin_partials(const double2 *__restrict__ ap, EkinArgs ea, double *eU,
-                                                      double *eg2, double *el) {
+// the refinement that ends the point's loop, differentiated with its node
+// count held fixed (the nodes x_j = j X / (gs - 1) move with X = max(1, 100 -
+// U / a)).  With st_m (the refinements k_ekin_ait recorded for the same V, mu
+// and MaxIt) only that refinement is summed; without it the loop is rerun with
+// the value sums to find it.
+struct EkinPoint {
+    double U, a, Ua, X, xmax, dXdU, dXda, k1, k2, twoa, inva, Ua2;
+};
+// value and derivative sums of refinement `iteration` of one point (wave totals)
+__device__ __forceinline__ void ekin_level(const double2 *__restrict__ ap, const EkinArgs &ea, const EkinPoint &e,
+                                           int iteration, bool derivs, double &r, double &rU, double &ra,
+                                           double &rl) {
     const int lane = threadIdx.x;
-    const long long i = blockIdx.x;
+    const int gs = 64 << iteration;
+    const double dg = (double)(gs - 1), inv = 1. / dg;
+    const int last = ekin_nodes(gs, dg, e.X, e.Ua, ea.tail).last;  // the energy's node range
+    r = rU = ra = rl = 0.;
+    const double c26 = -(ea.c2 / 6.);
+    if (!derivs) {
+        for (int j = lane; j <= last; j += 64) {
+            const double x = ekin_node(j, e.X, dg, inv), y = x + e.Ua;
+            const double wj = (j == 0 || j == gs - 1) ? 0.5 : 1.;
+            const double tt = e.twoa * x, sq = sqrt(tt);
+            const double h = e.k1 * tt * tt * sq - e.k2 * sq;
+            r += wj * (y > 100. ? 0. : airy_fast(ap, y)) * h;
+        }
+        r = wave_total(r);
+        return;
+    }
+    // two nodes per pass (both table reads in flight); sqrt(tt) and 1/sqrt(tt) from one reciprocal square
+    // root with a Newton step (the derivatives need no correctly rounded root)
+#pragma unroll 2
+    for (int j = lane; j <= last; j += 64) {
+        const double x = ekin_node(j, e.X, dg, inv), y = x + e.Ua;
+        const double wj = (j == 0 || j == gs - 1) ? 0.5 : 1.;
+        const double tt = e.twoa * x;
+        double isq = 0., sq = 0.;
+        if (tt > 0.) {
+            isq = rsqrt(tt);
+            isq = isq * (1.5 - 0.5 * tt * isq * isq);
+            sq = tt * isq;
+        }
+        const double h = e.k1 * tt * tt * sq - e.k2 * sq;
+        double AI, AId;
+        airy_pair(ap, y, AI, AId);
+        const double dxU = (double)j * inv * e.dXdU, dxa = (double)j * inv * e.dXda;
+        const double dyU = dxU + e.inva, dya = dxa - e.Ua2;
+        // dh/dx and dh/da (both vanish with x; the x = 0 node contributes no x-derivative)
+        const double t5 = 5. * e.k1 * tt * sq - e.k2 * isq;
+        const double hx = e.a * t5, ha = x * t5;
+        r += wj * AI * h;
+        rU += wj * (AId * dyU * h + AI * hx * dxU);
+        ra += wj * (AId * dya * h + AI * (hx * dxa + ha));
+        rl += wj * AI * c26 * sq;
+    }
+    r = wave_total(r);
+    rU = wave_total(rU);
+    ra = wave_total(ra);
+    rl = wave_total(rl);
+}
+
+__global__ __launch_bounds__(64) void k_ekin_partials(const double2 *__restrict__ ap, EkinArgs ea,
+                                                      const int *__restrict__ st_m, double *eU, double *eg2,
+                                                      double *el) {
+    const int lane = threadIdx.x;
+    const long long i = xcd_point(ea.G);
     if (i >= ea.G) return;
     const double U = ea.V[i] - ea.mu, g2 = fabs(ea.grad2[i]), a = 0.5 * ea.c0 * pow(g2, 1. / 3.);
     const double lap = ea.lap[i];
@@ -1379,52 +1447,48 @@ __global__ __launch_bounds__(64) void k_ekin_partials(const double2 *__restrict_
         }
         return;
     }
-    const double Ua = U / a, X = fmax(1., 100. - Ua), xmax = fmax(1., 100. - U / a);
-    const bool Xvar = 100. - Ua > 1.;
-    const double dXdU = Xvar ? -1. / a : 0., dXda = Xvar ? U / (a * a) : 0.;
-    const double k1 = ea.c1 / 5., k2 = ea.c2 * lap / 6., twoa = 2. * a;
-    double prev = 1., val = 0., dU = 0., da = 0., dl = 0.;
-    for (int iteration = 0;; iteration++) {
-        const int gs = 64 << iteration;
-        const double dg = (double)(gs - 1), inv = 1. / dg;
-        const int last = ekin_nodes(gs, dg, X, Ua, ea.tail).last;  // the energy's node range
-        double r = 0., rU = 0., ra = 0., rl = 0.;
-        for (int j = lane; j <= last; j += 64) {
-            double x = ekin_node(j, X, dg, inv), y = x + Ua;
-            double AI = y > 100. ? 0. : airy_fast(ap, y);
-            double AId = y > 100. ? 0. : airy_fast_d(ap, y);
-            double tt = twoa * x, sq = sqrt(tt);
-            double h = k1 * tt * tt * sq - k2 * sq;
-            double wj = (j == 0 || j == gs - 1) ? 0.5 : 1.;
-            double dxU = (double)j * inv * dXdU, dxa = (double)j * inv * dXda;
-            double dyU = dxU + 1. / a, dya = dxa - U / (a * a);
-            // dh/dx and dh/da (both vanish with x; the x = 0 node contributes no x-derivative)
-            double hx = 0., ha = 0.;
-            if (x > 0.) {
-                hx = 5. * a * k1 * tt * sq - a * k2 / sq;
-                ha = 5. * x * k1 * tt * sq - x * k2 / sq;
-            }
-            r += wj * AI * h;
-            rU += wj * (AId * dyU * h + AI * hx * dxU);
-            ra += wj * (AId * dya * h + AI * (hx * dxa + ha));
-            rl += wj * AI * (-(ea.c2 / 6.) * sq);
-        }
-        r = wave_total(r);
-        rU = wave_total(rU);
-        ra = wave_total(ra);
-        rl = wave_total(rl);
-        val = r * (xmax - 0.) / ((double)gs - 1.);
-        dU = (dXdU * r + X * rU) * inv;
-        da = (dXda * r + X * ra) * inv;
-        dl = X * rl * inv;
-        double ac = (fabs(val) < kMP && fabs(prev) < kMP) ? 0. : fabs((val - prev) / (val + prev));
-        double as = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
-        if (iteration > 0 &&
-            (ac < kMP || (fabs(as) < ea.internal_acc && negligible(ea.cmp, ea.area_pref * val, ea.neg_exp))))
-            break;
-        if (iteration >= ea.MaxIt) break;
-        prev = val;
+    EkinPoint e;
+    e.U = U;
+    e.a = a;
+    e.Ua = U / a;
+    e.X = fmax(1., 100. - e.Ua);
+    e.xmax = fmax(1., 100. - U / a);
+    const bool Xvar = 100. - e.Ua > 1.;
+    e.dXdU = Xvar ? -1. / a : 0.;
+    e.dXda = Xvar ? U / (a * a) : 0.;
+    e.k1 = ea.c1 / 5.;
+    e.k2 = ea.c2 * lap / 6.;
+    e.twoa = 2. * a;
+    e.inva = 1. / a;
+    e.Ua2 = U / (a * a);
+    int level = -1;
+    if (st_m) {  // the refinement k_ekin_ait stopped this point at
+        const int m0 = st_m[i];
+        level = m0 >= 0 ? m0 : (m0 <= -2 ? -m0 - 2 : -1);
     }
+    if (level < 0) {  // rerun the value sums of the refinements to find it
+        double prev = 1.;
+        for (int iteration = 0;; iteration++) {
+            double r, rU, ra, rl;
+            ekin_level(ap, ea, e, iteration, false, r, rU, ra, rl);
+            const double val = r * (e.xmax - 0.) / ((double)(64 << iteration) - 1.);
+            const double ac = (fabs(val) < kMP && fabs(prev) < kMP) ? 0. : fabs((val - prev) / (val + prev));
+            const double as = (val == 0. && prev == 0.) ? 0. : (val - prev) / (val + prev);
+            if ((iteration > 0 &&
+                 (ac < kMP || (fabs(as) < ea.internal_acc && negligible(ea.cmp, ea.area_pref * val, ea.neg_exp)))) ||
+                iteration >= ea.MaxIt) {
+                level = iteration;
+                break;
+            }
+            prev = val;
+        }
+    }
+    double r, rU, ra, rl;
+    ekin_level(ap, ea, e, level, true, r, rU, ra, rl);
+    const double inv = 1. / (double)((64 << level) - 1);
+    const double dU = (e.dXdU * r + e.X * rU) * inv;
+    const double da = (e.dXda * r + e.X * ra) * inv;
+    const double dl = e.X * rl * inv;
     if (lane == 0) {
         eU[i] = dU;
         eg2[i] = g2 > 0. ? da * a / (3. * g2) : 0.;  // a = c0 |g2|^{1/3} / 2
@@ -1446,6 +1510,11 @@ struct NaitTables {
     double *d_ekin_v = nullptr; // [G] last refinement value per point (EkinAiT03D)
     int *d_ekin_m = nullptr;    // [G] its index, -1 when converged
     bool reuse_ok = false;      // the file's geometry is the one the panel-reuse kernel assumes
+    // the last EkinAiT03D evaluation (its d_ekin_m refinements): species, fingerprint of V
+    // (<V,V>, <V,Env>), mu and MaxIteration
+    int ekin_s = -1, ekin_maxit = -1;
+    double ekin_fp[2] = {0., 0.};
+    double ekin_mu = 0.;
     int *d_counts = nullptr;    // [3][G] per-point evaluations, iterMax and relAcc returns
     double Mpp[MPDPFT_MAX_SPECIES][3];
     double last_itermax[MPDPFT_MAX_SPECIES], last_relacc[MPDPFT_MAX_SPECIES];
@@ -1844,6 +1913,19 @@ static double ekin_tail() {
     return (e && *e && *e != '0') ? HUGE_VAL : 25.;
 }
 
+// (<V_s, V_s>, <V_s, Env_s>): tells whether V changed since a recorded evaluation
+static int v_fingerprint(mpdpft_ctx *c, int s, double *fp) {
+    const size_t G = c->G;
+    hipError_t e = mk::conv_dots_S(c->g, 1, c->d_V + s * G, c->d_Env + s * G, c->d_partials, c->st);
+    if (e != hipSuccess) return hip_fail(c, e, "fingerprint");
+    double r[3];
+    int rc = ctx_reduce(c, 3, 1., r);
+    if (rc) return rc;
+    fp[0] = r[0];
+    fp[1] = r[2];
+    return MPDPFT_OK;
+}
+
 // EkinAiT03D (mpDPFT.cpp:5610-5709); c->MaxIteration persists across calls.
 int nait_energy_partials(mpdpft_ctx *c, int s, double *eU, double *eg2, double *el, double *prefactor) {
     NaitTables *t = c->nait;
@@ -1874,7 +1956,14 @@ int nait_energy_partials(mpdpft_ctx *c, int s, double *eU, double *eg2, double *
     ea.MaxIt = c->MaxIteration;
     ea.resume = 0;
     if (c->MaxIteration > 24) return ctx_fail(c, MPDPFT_EINVAL, "EkinAiT03D: MaxIteration above 24");
-    hipLaunchKernelGGL(k_ekin_partials, dim3((unsigned)G), dim3(64), 0, c->st, t->d_ap, ea, eU, eg2, el);
+    // the refinements of the last energy evaluation, if it saw this V (same fingerprint, bit for bit),
+    // mu and MaxIteration (DelField of the same V is the same)
+    double fp[2];
+    if (int rc = v_fingerprint(c, s, fp)) return rc;
+    const bool state = t->ekin_s == s && t->ekin_fp[0] == fp[0] && t->ekin_fp[1] == fp[1] &&
+                       t->ekin_mu == c->muVec[s] && t->ekin_maxit == c->MaxIteration;
+    hipLaunchKernelGGL(k_ekin_partials, dim3(xcd_blocks((long long)G)), dim3(64), 0, c->st, t->d_ap, ea,
+                       state ? t->d_ekin_m : nullptr, eU, eg2, el);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return hip_fail(c, he, "k_ekin_partials");
     return MPDPFT_OK;
@@ -1942,5 +2031,9 @@ int nait_dispersal_energy(mpdpft_ctx *c, int s, double *E) {
         }
     }
     *E = e;
+    t->ekin_s = s;
+    if (int rc = v_fingerprint(c, s, t->ekin_fp)) return rc;
+    t->ekin_mu = c->muVec[s];
+    t->ekin_maxit = c->MaxIteration;
     return MPDPFT_OK;
 }
