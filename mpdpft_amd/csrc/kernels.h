@@ -48,14 +48,13 @@ struct MulParams {
     double dxD;           // dx^D
     double dkzero;        // 1e-6 dk^2
     int regularize;       // RegularizeCoulombSingularity
-    double n0;            // MUL_DIPOLAR: the subtracted Gaussian's amplitude (and post 4's)
+    double n0;            // MUL_DIPOLAR: the subtracted Gaussian's amplitude
 };
 // Backward transform: f = L^-D * IDFT(M * spec) written to re/im real grid
 // fields (either may be null) with periodic-image fill, out = scale*value
 // (+ out if accumulate).  post: 0 none, 1 POS clip of re.
 // post 2 adds the Gaussian compensation f0 erf(r/2)/(4 pi r) and multiplies
-// by c1 = 4 pi gammaH (needs lat, the lattice table, and m.f0slot); post 4
-// adds the dipolar compensation n0 zf(|r|)/(8 pi^3) and multiplies by c1 = alpha.
+// by c1 = 4 pi gammaH (needs lat, the lattice table, and m.f0slot).
 // The first pass reads 'spec' and writes 'work' (may equal spec), so a
 // spectrum can feed several backward transforms.
 hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, const MulParams &m, double *re,
@@ -126,6 +125,9 @@ hipError_t vint_dvdn(const Grid &g, const VintParams &vp, const double *den, dou
 // DipolarInteraction (momentum space, 3D): out[i] = scale sum_j w_j (k_z/|k|^2 - 1/3) over the grid
 // points k_j with |q_i + k_j| < edge/2 and n(q_i + k_j) > n(q_i); w_j the trapezoid weights.
 hipError_t dipolar_k(const Grid &g, const double *n, double scale, double *out, hipStream_t st);
+// DipolarInteractionPos's last step (3D): v = alpha (v + n0 zf(|r|)/(8 pi^3)) in place, v the
+// backward transform of the multiplied spectrum.
+hipError_t dipolar_comp(const Grid &g, double n0, double alpha, double *v, hipStream_t st);
 
 // RelDev dots per species + field stats: partials layout per species s:
 // [<a,a>, <b,b>, <a,b>] sums and [denmin, denmax, vmin, vmax].

@@ -399,8 +399,7 @@ struct StGrid {
     int EL, s, D;
     const double *lat;
     const double *f0slot;
-    double c1;  // 4 pi gammaH for post 2, 2 pi gammaH for post 3, alpha for post 4
-    double n0;  // post 4: the dipolar compensation's amplitude
+    double c1;  // 4 pi gammaH for post 2, 2 pi gammaH for post 3
     // post 3: 2 pi gammaH (val + f0 e^{-z^2/8} I0(z^2/8) / (4 sqrt(pi))) (mpDPFT.cpp:2647-2655)
     __device__ double hart2d(int idx, double val) const {
         int a = idx / EL, b = idx - a * EL;
@@ -426,25 +425,12 @@ struct StGrid {
         double zf = z > 1.0e-10 ? PI_D * erf(z / 2.) / z : sqrt(PI_D);
         return c1 * (val + (*f0slot) * zf / (4. * PI_D * PI_D));
     }
-    // post 4: alpha (val + n0 zf(|r|) / (8 pi^3)) (DipolarInteractionPos, mpDPFT.cpp:2749-2755)
-    __device__ double dipolar(int idx, double val) const {
-        int a = idx / (EL * EL), rem = idx - a * EL * EL, b = rem / EL, c = rem - b * EL;
-        const double z = sqrt(lat[a] * lat[a] + lat[b] * lat[b] + lat[c] * lat[c]), sp = sqrt(PI_D);
-        double zf;
-        if (z > 1.0e-12)
-            zf = (PI_D / (z * z * z)) * (d_exp(-z * z / 4.) * sp * z * (4. + z * z) - 4. * PI_D * erf(z / 2.)) +
-                 PI_D * sp * d_exp(-z * z / 4.) * (-1. / 3. + 8. * PI_D / 3.);
-        else
-            zf = PI_D * sp * (8. * PI_D / 3.);
-        return c1 * (val + (n0 / (8. * PI_D * PI_D * PI_D)) * zf);
-    }
     __device__ void put(int idx, double2 v) const {
         if (re) {
             double a = scale * v.x;
             if (post == 1) a = d_pos(a);
             else if (post == 2) a = hart(idx, a);
             else if (post == 3) a = hart2d(idx, a);
-            else if (post == 4) a = dipolar(idx, a);
             re[idx] = a;
         }
         if (im) im[idx] = scale * v.y;
@@ -581,7 +567,7 @@ hipError_t fft_backward_real(const Grid &g, const double2 *spec, double2 *work, 
     int s = g.s;
     StBuf sb{work, s, s};
     LdBuf lb{work, s, s};
-    StGrid sg{re, im, scale, post, g.EL, s, g.D, g.lat, m.f0slot, c1, m.n0};
+    StGrid sg{re, im, scale, post, g.EL, s, g.D, g.lat, m.f0slot, c1};
     if (g.D == 3) {
         // DelField's multipliers (with the (k, smoothing) table) compiled per kind
         if (m.kind == MUL_DERIV_PAIR && m.kek) {
@@ -1028,6 +1014,33 @@ __global__ __launch_bounds__(256) void k_vint_dvdn(Grid g, VintParams vp, const 
 }
 hipError_t vint_dvdn(const Grid &g, const VintParams &vp, const double *den, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_vint_dvdn, dim3(2048), dim3(256), 0, st, g, vp, den, out);
+    return hipGetLastError();
+}
+
+// DipolarInteractionPos (mpDPFT.cpp:2749-2755), 3D: v = alpha (v + n0 zf(|r|) / (8 pi^3))
+// with zf the transform of the Gaussian the k-space multiplier subtracted.  A
+// pass of its own after the backward transform: inside the transform's store
+// the two extra transcendentals put a stack frame on every backward transform.
+__global__ __launch_bounds__(256) void k_dipolar_comp(Grid g, double n0, double alpha, double *v) {
+    const int EL = g.EL;
+    const double sp = sqrt(PI_D);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.G; i += (size_t)gridDim.x * blockDim.x) {
+        const int a = (int)(i / ((size_t)EL * EL)), rem = (int)(i - (size_t)a * EL * EL), b = rem / EL,
+                  c = rem - b * EL;
+        const double z = sqrt(g.lat[a] * g.lat[a] + g.lat[b] * g.lat[b] + g.lat[c] * g.lat[c]);
+        double zf;
+        if (z > 1.0e-12)
+            zf = (PI_D / (z * z * z)) * (d_exp(-z * z / 4.) * sp * z * (4. + z * z) - 4. * PI_D * erf(z / 2.)) +
+                 PI_D * sp * d_exp(-z * z / 4.) * (-1. / 3. + 8. * PI_D / 3.);
+        else
+            zf = PI_D * sp * (8. * PI_D / 3.);
+        v[i] = alpha * (v[i] + (n0 / (8. * PI_D * PI_D * PI_D)) * zf);
+    }
+}
+hipError_t dipolar_comp(const Grid &g, double n0, double alpha, double *v, hipStream_t st) {
+    if (g.D != 3) return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)std::min<size_t>((g.G + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_dipolar_comp, dim3(blocks), dim3(256), 0, st, g, n0, alpha, v);
     return hipGetLastError();
 }
 

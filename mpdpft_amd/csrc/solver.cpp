@@ -726,8 +726,9 @@ static int dipolar_fields(mpdpft_ctx *c, const double *den) {
             m.dkzero = 0.01 * c->dk * c->dk;
             m.n0 = p.Abundances[s];
             HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_spec, m, c->d_dip_r + s * G, nullptr,
-                                       std::pow(1. / p.edge, 3.), 4, p.alpha, c->d_tw, c->st));
-            stage_end(c, "dipolar_pos", 2., 2. * 32. * 3. * (double)c->F, 0.);
+                                       std::pow(1. / p.edge, 3.), 0, 0., c->d_tw, c->st));
+            HIPC(mk::dipolar_comp(c->g, p.Abundances[s], p.alpha, c->d_dip_r + s * G, c->st));
+            stage_end(c, "dipolar_pos", 2., 2. * 32. * 3. * (double)c->F + 16. * (double)G, 0.);
         }
     }
     if (c->d_dip_k) {
