@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 /* 2: mpdpft_result gained AuxEint; 3: mpdpft_result gained bracket_failures and pulay_rejections */
-#define MPDPFT_ABI_VERSION 3
+#define MPDPFT_ABI_VERSION 4
 
 #define MPDPFT_MAX_SPECIES 8
 #define MPDPFT_MAX_LIST 16
@@ -129,6 +129,10 @@ typedef struct mpdpft_params {
     int WriteFiles;        /* 1: mpdpft_run_dir writes Den/V/Control files (default 1) */
     int n_centres;         /* environment 41 */
     double centres[MPDPFT_MAX_CENTRES][5]; /* x, y, z, Z, a */
+    /* mpDPFT.input line 5 (TaskHyperBox): (lower, upper) pairs of the task
+     * variables, the ranges of TaskType 2/3's starting point and scale */
+    int n_hyperbox;        /* number of values (twice the pairs) */
+    double TaskHyperBox[2 * MPDPFT_MAX_SPECIES];
 } mpdpft_params;
 
 /* Results of one solve (the quantities the reference prints in its final
@@ -296,7 +300,7 @@ int mpdpft_get_stage_stats(mpdpft_ctx *ctx, mpdpft_stage_stats *out, int max, in
  * given dL/dn ([G], one species, plain sum over grid points), returns
  * lambda[i] = dL/dEnv_i, the derivative with respect to an additive change of
  * the external potential at grid point i, with the particle number held
- * fixed.  It linearises the fixed point V = Env + K[n], n = F(V, mu),
+ * fixed (or mu held, when RelAcc = 0 switches the mu search off).  It linearises the fixed point V = Env + K[n], n = F(V, mu),
  * sum w n = N (implicit differentiation, no unrolled iterations) and solves
  * (I - F_V^T P K'^T) lambda = F_V^T P dL/dn by restarted GMRES on the device.
  * Supported: DIM 3, S = 1, DensityExpression 7 (TF) or 5 (nAiT, DelFieldMethod
@@ -320,6 +324,14 @@ int mpdpft_centre_gradient(mpdpft_ctx *ctx, const double *lambda, double *out);
  * interaction and environment energies, dE_disp/dV and dE_disp/dmu of the
  * kinetic energy's quadrature).  EkinTypes 0 only. */
 int mpdpft_centre_forces(mpdpft_ctx *ctx, double *out);
+/* The task variable's gradient for one species (replaces the finite-difference
+ * gradient that ConjugateGradientDescent's mincg takes over whole solves of
+ * functionToBeMinimized, mpDPFT.cpp:8365-8584): out[0] = dE/dN of the last
+ * solve's total energy through the fixed point when the loop holds N
+ * (RelAcc > 0; TaskType 3, GetGlobalMinimum_Abundances :256), or dE/dmu when it
+ * holds mu (RelAcc = 0; TaskType 2, GetGlobalMinimum_muVec :246).  Same
+ * support as mpdpft_centre_forces, any environment. */
+int mpdpft_task_gradient(mpdpft_ctx *ctx, double *out);
 /* Replace the centres list (n x {x, y, z, Z, a}) and rebuild Env of the
  * environment-41 species; the next solve starts from the new Env. */
 int mpdpft_set_centres(mpdpft_ctx *ctx, int n, const double *centres);

@@ -195,6 +195,14 @@ class Context:
         self._check(self._lib.mpdpft_centre_forces(self._h, _dp(out)))
         return out.reshape(-1, 3)
 
+    def task_gradient(self):
+        """dE/dN of the last converged solve's total energy through the fixed point
+        when the loop holds N (RelAcc > 0), dE/dmu when it holds mu (RelAcc = 0);
+        one species (mpdpft_task_gradient)."""
+        out = np.zeros(1)
+        self._check(self._lib.mpdpft_task_gradient(self._h, _dp(out)))
+        return float(out[0])
+
     def set_centres(self, centres):
         """Replace the environment-41 centres (rows x, y, z, Z, a) and rebuild Env."""
         c = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 5)

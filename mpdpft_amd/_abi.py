@@ -46,6 +46,7 @@ class Params(ctypes.Structure):
         ("OutPrecision", _i), ("MaxIteration", _i),
         ("HartreeMode", _i), ("SmoothingKWidth", _d), ("ConvergenceMode", _i), ("Precision", _i),
         ("WriteFiles", _i), ("n_centres", _i), ("centres", (_d * 5) * MAX_CENTRES),
+        ("n_hyperbox", _i), ("TaskHyperBox", _d * (2 * MAX_SPECIES)),
     ]
 
 
@@ -84,7 +85,7 @@ EXPORTS = [
     "mpdpft_import_v", "mpdpft_set_initial_potential", "mpdpft_set_break_file",
     "mpdpft_batch_create", "mpdpft_batch_solve", "mpdpft_batch_get_fields", "mpdpft_batch_seconds",
     "mpdpft_batch_phase_stats", "mpdpft_batch_kernel_stats", "mpdpft_batch_last_error", "mpdpft_batch_destroy",
-    "mpdpft_adjoint", "mpdpft_centre_gradient", "mpdpft_centre_forces", "mpdpft_set_centres",
+    "mpdpft_adjoint", "mpdpft_centre_gradient", "mpdpft_centre_forces", "mpdpft_task_gradient", "mpdpft_set_centres",
     "mpdpft_loop_begin", "mpdpft_step", "mpdpft_energy", "mpdpft_enforce_constraints", "mpdpft_mix",
     "mpdpft_inject_fault", "mpdpft_batch_solve_noalloc", "mpdpft_set_data_dir",
 ]
@@ -180,6 +181,8 @@ def load_library(path=None):
     lib.mpdpft_centre_gradient.restype = _i
     lib.mpdpft_centre_forces.argtypes = [vp, dp]
     lib.mpdpft_centre_forces.restype = _i
+    lib.mpdpft_task_gradient.argtypes = [vp, dp]
+    lib.mpdpft_task_gradient.restype = _i
     lib.mpdpft_set_centres.argtypes = [vp, _i, dp]
     lib.mpdpft_set_centres.restype = _i
     lib.mpdpft_loop_begin.argtypes = [vp]

@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["params.cpp", "kernels.hip", "kernels_fp32.hip", "solver.cpp", "nait.hip", "output.cpp", "batch.hip",
-               "adjoint.hip"]
+               "adjoint.hip", "tasks.cpp"]
 # The FFT passes are instantiated once per block shape, and the tuner picks a
 # shape per length by timing.  With multiply-add contraction the compiler may
 # fuse different products in different instantiations, so the results would

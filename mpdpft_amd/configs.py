@@ -37,7 +37,8 @@ def write_input(p, path, extensions=True):
 
     vals = {
         "System": p.System, "Symmetry": p.Symmetry, "TaskType": p.TaskType,
-        "TaskParameterCount": p.TaskParameterCount, "TaskHyperBox": "0. 0.", "DIM": p.DIM, "Units": p.Units,
+        "TaskParameterCount": p.TaskParameterCount,
+        "TaskHyperBox": vec("TaskHyperBox", p.n_hyperbox) or "0. 0.", "DIM": p.DIM, "Units": p.Units,
         "edge": _fmt(p.edge), "steps": p.steps, "S": S, "Abundances": vec("Abundances", S),
         "RelAcc": _fmt(p.RelAcc), "InternalAcc": _fmt(p.InternalAcc), "K": p.K, "Resources": "0.",
         "DensityExpression": p.DensityExpression, "mpp": vec("mpp", p.n_mpp), "Mpp": vec("Mpp", p.n_Mpp) or "0.",

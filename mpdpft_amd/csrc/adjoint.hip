@@ -12,7 +12,9 @@
 //     P x = x - w (F_mu . x) / (F_mu . w),  F_mu = -dF/dU,
 // (the second row eliminates the chemical potential's multiplier).  The
 // linear system (I - F_V^T P K'^T) lambda = F_V^T P l is solved by restarted
-// GMRES on the device.  F_V^T is the pointwise dF/dU plus the transposed
+// GMRES on the device.  When the loop holds mu instead of N (RelAcc = 0: no
+// mu search in EnforceConstraints, :4578) the constraint row is absent and P
+// is the identity.  F_V^T is the pointwise dF/dU plus the transposed
 // DelField chain (taper with its shell-maximum shift, derivative and
 // Laplacian multipliers, the POS-clipped smoothed |grad V|^2); K'^T is the
 // transposed isolated Hartree convolution plus the pointwise dV_xc/dn.
@@ -52,6 +54,7 @@ struct AdjointState {
     int *istar = nullptr;
     double fmu_w = 0.;  // F_mu . w
     int nait = 0, taper = 0, method = 0, hartree = 0;
+    int fixed_mu = 0;   // the loop holds mu (RelAcc = 0): no constraint row, P = I
     int verbose = 0;  // MPDPFT_VERBOSE at the adjoint call: GMRES residual per restart on stderr
     std::vector<void *> mem;
 };
@@ -62,6 +65,7 @@ using mk::Grid;
 using mk::kRedBlocks;
 constexpr int kT = 256;
 constexpr double kPI = 3.141592653589793;
+constexpr double kMPa = 1.0e-12;  // MP (mpDPFT.cpp:111): RelAcc at or below it holds mu
 
 inline unsigned gblocks(size_t n) { return (unsigned)std::min<size_t>(4096, (n + kT - 1) / kT); }
 
@@ -448,12 +452,11 @@ int delfield_adjoint(mpdpft_ctx *c, const double *mug, const double *mul, double
     return MPDPFT_OK;
 }
 
-// out = F_V^T (P (ell + K'^T lam) - w alpha_add); lam or ell may be null (term dropped)
-int apply_adjoint_map(mpdpft_ctx *c, const double *lam, const double *ell, double *out, double alpha_add = 0.) {
+// fx = F_mu . (ell + K'^T lam), leaving x = ell + K'^T lam in A.x; lam or ell may be null
+int fmu_dot(mpdpft_ctx *c, const double *lam, const double *ell, double *fx) {
     AdjointState &A = *c->adj;
     const Grid &g = c->g;
     const size_t G = c->G;
-    const unsigned nb = gblocks(G);
     const double *h = nullptr;
     if (lam && A.hartree) {
         const int M = 2 * c->s;
@@ -464,9 +467,19 @@ int apply_adjoint_map(mpdpft_ctx *c, const double *lam, const double *ell, doubl
     }
     hipLaunchKernelGGL(k_adj_x, dim3(kRedBlocks), dim3(kT), 0, c->st, G, ell, h, lam ? (const double *)A.xcd : nullptr,
                        lam, (const double *)A.dfdU, A.x, A.partials);
+    HIPC(hipGetLastError());
+    return reduce_to_host(c, A.partials, 1, fx);
+}
+
+// out = F_V^T (P (ell + K'^T lam) - w alpha_add); lam or ell may be null (term dropped)
+int apply_adjoint_map(mpdpft_ctx *c, const double *lam, const double *ell, double *out, double alpha_add = 0.) {
+    AdjointState &A = *c->adj;
+    const Grid &g = c->g;
+    const size_t G = c->G;
+    const unsigned nb = gblocks(G);
     double fx;
-    RC(reduce_to_host(c, A.partials, 1, &fx));
-    const double alpha = fx / A.fmu_w + alpha_add;
+    RC(fmu_dot(c, lam, ell, &fx));
+    const double alpha = A.fixed_mu ? 0. : fx / A.fmu_w + alpha_add;
     hipLaunchKernelGGL(k_adj_nu, dim3(nb), dim3(kT), 0, c->st, g, dxD(c), (const double *)A.x, alpha,
                        (const double *)A.dfdU, (const double *)A.dfdg2, (const double *)A.dfdl, A.direct,
                        A.nait ? A.mug : nullptr, A.nait ? A.mul : nullptr);
@@ -631,6 +644,7 @@ int setup(mpdpft_ctx *c, int restart) {
     A.method = p.DelFieldMethod;
     A.taper = A.nait && p.TaperOff && (A.method == 4 || A.method == 5);
     A.hartree = c->has_hartree;
+    A.fixed_mu = !(p.RelAcc > kMPa && p.TaskType < 100);
     if (A.nait) {
         c->delfield_valid = -1;  // DelField(V) now: its fields and shell maximum are the linearisation's
         RC(nait_partials(c, 0, mu, A.nbuf, A.dfdU, A.dfdg2, A.dfdl));
@@ -728,21 +742,22 @@ extern "C" int mpdpft_centre_gradient(mpdpft_ctx *c, const double *lambda, doubl
     return centre_contract(c, c->adj->t3, out);
 }
 
+namespace {
+
 // The total energy's derivative: E(V, mu, n = F(V, mu); theta) of GetEnergy
 // (:5414-5436) differentiated through the fixed point.  Sources: dE/dn =
 // w (Env + v_xc + K n / 2) + K^T (w n) / 2 (Hartree, Dirac, LDA energies are
 // exact antiderivatives of their potentials); k_V = dE_disp/dV through U and
 // the DelField terms (the kinetic energy's own quadrature, differentiated with
-// its refinements held fixed); k_mu = dE_disp/dmu.  Then
-// dE/dtheta = sum_i (w n + lambda_E)_i dEnv_i/dtheta.
-extern "C" int mpdpft_centre_forces(mpdpft_ctx *c, double *out) {
-    if (!c || !out) return MPDPFT_EINVAL;
+// its refinements held fixed); k_mu = dE_disp/dmu.  Leaves lambda_E in A.w and
+// A.ell = dE/dn; *kmu_out = k_mu.  Then dE/dtheta = sum_i (w n + lambda_E)_i
+// dEnv_i/dtheta, and the constraint's multiplier gives dE/dN (below).
+int energy_adjoint(mpdpft_ctx *c, double *kmu_out) {
     HIPC(hipSetDevice(c->device));
-    RC(centres_ok(c));
     RC(check_supported(c));
     const mpdpft_params &p = c->p;
     for (int e = 0; e < p.n_ekin; e++)
-        if (p.EkinTypes[e] != 0) return ctx_fail(c, MPDPFT_EINVAL, "centre forces: EkinTypes 0 only");
+        if (p.EkinTypes[e] != 0) return ctx_fail(c, MPDPFT_EINVAL, "energy derivatives: EkinTypes 0 only");
     RC(setup(c, 40));
     AdjointState &A = *c->adj;
     const Grid &g = c->g;
@@ -803,17 +818,46 @@ extern "C" int mpdpft_centre_forces(mpdpft_ctx *c, double *out) {
     }
     HIPC(hipGetLastError());
     // rhs = F_V^T (P l_E - w k_mu / F_mu.w) + k_V
-    RC(apply_adjoint_map(c, nullptr, A.ell, A.rhs, kmu / A.fmu_w));
+    RC(apply_adjoint_map(c, nullptr, A.ell, A.rhs, A.fixed_mu ? 0. : kmu / A.fmu_w));
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(kT), 0, c->st, G, 1., (const double *)A.w, 1., A.rhs);
     int its = 0;
     double rr = 0.;
     RC(gmres(c, A.rhs, A.w, 1e-12, 400, &its, &rr));
-    if (rr > 1e-12) return ctx_fail(c, MPDPFT_ENOTCONV, "centre forces: adjoint GMRES did not reach 1e-12");
+    if (rr > 1e-12) return ctx_fail(c, MPDPFT_ENOTCONV, "energy derivatives: adjoint GMRES did not reach 1e-12");
+    *kmu_out = kmu;
+    return MPDPFT_OK;
+}
+
+}  // namespace
+
+extern "C" int mpdpft_centre_forces(mpdpft_ctx *c, double *out) {
+    if (!c || !out) return MPDPFT_EINVAL;
+    HIPC(hipSetDevice(c->device));
+    RC(centres_ok(c));
+    double kmu;
+    RC(energy_adjoint(c, &kmu));
+    AdjointState &A = *c->adj;
+    const unsigned nb = gblocks(c->G);
     // lambda_E + w n
-    hipLaunchKernelGGL(k_weighted, dim3(nb), dim3(kT), 0, c->st, g, dV, (const double *)c->d_Den, A.t3);
-    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(kT), 0, c->st, G, 1., (const double *)A.w, 1., A.t3);
+    hipLaunchKernelGGL(k_weighted, dim3(nb), dim3(kT), 0, c->st, c->g, dxD(c), (const double *)c->d_Den, A.t3);
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(kT), 0, c->st, c->G, 1., (const double *)A.w, 1., A.t3);
     HIPC(hipGetLastError());
     return centre_contract(c, A.t3, out);
+}
+
+// With N held, the Lagrangian E + a.R_V + b (w.n - N) is stationary in V and mu
+// at a = -lambda_E, b = -(k_mu + F_mu.(dE/dn + K'^T lambda_E)) / F_mu.w, and
+// dE/dN = -b.  With mu held (RelAcc = 0) there is no b, and
+// dE/dmu = k_mu + F_mu.(dE/dn + K'^T lambda_E).
+extern "C" int mpdpft_task_gradient(mpdpft_ctx *c, double *out) {
+    if (!c || !out) return MPDPFT_EINVAL;
+    HIPC(hipSetDevice(c->device));
+    double kmu, fx;
+    RC(energy_adjoint(c, &kmu));
+    AdjointState &A = *c->adj;
+    RC(fmu_dot(c, A.w, A.ell, &fx));
+    out[0] = A.fixed_mu ? kmu + fx : (kmu + fx) / A.fmu_w;
+    return MPDPFT_OK;
 }
 
 extern "C" int mpdpft_set_centres(mpdpft_ctx *c, int n, const double *centres) {

@@ -252,8 +252,8 @@ static std::string final_block(mpdpft_ctx *c, const mpdpft_result &r) {
 
 // One GetData call (mpDPFT.cpp:135-244) in directory d with parameters p:
 // LoadV when InterpolVQ is 1, the SC loop, and ExportData when write_files.
-static int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
-                   const std::string &notes, double *mpp_out) {
+int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
+            const std::string &notes, double *mpp_out, double *task_grad, int *grad_rc) {
     mpdpft_ctx *c = nullptr;
     int rc = mpdpft_create(&p, device, &c);
     if (rc) {
@@ -291,6 +291,7 @@ static int run_one(const std::string &d, const mpdpft_params &p, int device, mpd
         mpdpft_destroy(c);
         return src;
     }
+    if (task_grad) *grad_rc = mpdpft_task_gradient(c, task_grad);
     if (write_files && c->p.WriteFiles) {
         std::vector<double> den(c->SG), V(c->SG);
         rc = mpdpft_get_density(c, den.data());
@@ -327,7 +328,8 @@ static int run_one(const std::string &d, const mpdpft_params &p, int device, mpd
     return rc ? rc : src;
 }
 
-// The outer task loop of main (mpDPFTmain.cpp:77-214) for task types 0, 1 and 8,
+// The outer task loop of main (mpDPFTmain.cpp:77-214) for task types 0, 1 and 8
+// (2 and 3, the conjugate-gradient minimisations, are in tasks.cpp),
 // with RetrieveTask's per-task parameter changes (mpDPFT.cpp:6070-6121):
 // 1: rectangular grid of starting chemical potentials mu_s (1 + delta)^p_s,
 //    delta = 1.1^(1/AxesPoints) - 1 (two species); files of the last task only;
@@ -343,13 +345,27 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     int rc = mpdpft_parse_input(join(d, "mpDPFT.input").c_str(), &p);
     if (rc) return rc;
     const int type = p.TaskType;
-    if (type != 0 && type != 1 && type != 8) {
+    if (type != 0 && type != 1 && type != 2 && type != 3 && type != 8) {
         FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");
         if (f) {
-            std::fprintf(f, "TaskType %d is not built (0, 1, 8 are)\n", type);
+            std::fprintf(f, "TaskType %d is not built (0, 1, 2, 3, 8 are)\n", type);
             std::fclose(f);
         }
         return MPDPFT_EINVAL;
+    }
+    if (type == 2 || type == 3) {
+        std::ostringstream task;
+        task << " ***** mpdpft_hip task loop: TaskType " << type << " *****\n";
+        mpdpft_result r{};
+        int rc2 = run_cgd_task(d, p, device, &r, task);
+        if (res) *res = r;
+        FILE *f = std::fopen(join(d, "mpDPFT_ControlTask.dat").c_str(), "w");
+        if (f) {
+            std::string sc = task.str();
+            std::fwrite(sc.data(), 1, sc.size(), f);
+            std::fclose(f);
+        }
+        return rc2;
     }
     int count = (type == 0 || p.TaskParameterCount < 1) ? 1 : p.TaskParameterCount;
     if (type == 1 && p.TaskParameterCount <= 1) count = 1;

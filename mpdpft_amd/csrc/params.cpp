@@ -32,6 +32,7 @@ const char *params_structure_error(const mpdpft_params *p) {
     if (p->n_Mpp < 0 || p->n_Mpp > MPDPFT_MAX_LIST) return "n_Mpp out of range";
     if (p->n_mpp < 0 || p->n_mpp > MPDPFT_MAX_MPP) return "n_mpp out of range";
     if (p->n_centres < 0 || p->n_centres > MPDPFT_MAX_CENTRES) return "n_centres out of range";
+    if (p->n_hyperbox < 0 || p->n_hyperbox > 2 * MPDPFT_MAX_SPECIES) return "n_hyperbox out of range";
     return nullptr;
 }
 
@@ -85,6 +86,7 @@ extern "C" void mpdpft_params_default(mpdpft_params *p) {
     p->ConvergenceMode = 0;
     p->Precision = 0;
     p->WriteFiles = 1;
+    p->n_hyperbox = 2;  // the shipped input's "0. 0."
 }
 
 namespace {
@@ -166,7 +168,7 @@ extern "C" int mpdpft_parse_input(const char *path, mpdpft_params *p) {
         case 1: rc = first(line, &q.Symmetry); break;
         case 2: rc = first(line, &q.TaskType); break;
         case 3: rc = first(line, &q.TaskParameterCount); break;
-        case 4: break;  // TaskHyperBox: task-level, not on this path
+        case 4: rc = fill_open(line, 2 * MPDPFT_MAX_SPECIES, q.TaskHyperBox, &q.n_hyperbox); break;
         case 5: rc = first(line, &q.DIM); break;
         case 6: rc = first(line, &q.Units); break;
         case 7: rc = first(line, &q.edge); break;

@@ -107,6 +107,14 @@ struct mpdpft_ctx {
 
 // internal helpers shared between translation units
 const char *params_structure_error(const mpdpft_params *p);  // params.cpp
+// output.cpp: one GetData call in directory d (LoadV, SC loop, ExportData when
+// write_files); task_grad (may be null) receives mpdpft_task_gradient of the
+// converged context, with its return code in *grad_rc.
+int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
+            const std::string &notes, double *mpp_out, double *task_grad = nullptr, int *grad_rc = nullptr);
+// tasks.cpp: TaskType 2 / 3 (the energy's minimum over muVec / the abundances)
+int run_cgd_task(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res,
+                 std::ostringstream &log);
 int ctx_fail(mpdpft_ctx *c, int code, const std::string &msg);
 int hip_fail(mpdpft_ctx *c, hipError_t e, const char *what);
 int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method);

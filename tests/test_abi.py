@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_layouts_and_version():
     lib = _abi.load_library()
-    assert lib.mpdpft_abi_version() == 3
+    assert lib.mpdpft_abi_version() == 4
     assert lib.mpdpft_params_sizeof() == ctypes.sizeof(_abi.Params)
     assert lib.mpdpft_result_sizeof() == ctypes.sizeof(_abi.Result)
 
@@ -43,6 +43,8 @@ def test_defaults_are_reference_hardcoded_values():
 
 def test_parse_round_trip(tmp_path):
     p = configs.c1(steps=64)
+    p.n_hyperbox = 2
+    p.TaskHyperBox[0], p.TaskHyperBox[1] = 1.5, 4.0
     p.n_centres = 2
     for c, row in enumerate([(1.0, 2.0, 3.0, 1.0, 0.6), (-1.0, 0.5, 0.0, 2.0, 0.4)]):
         for k, v in enumerate(row):
@@ -115,7 +117,8 @@ def test_cli_fails_loudly_without_a_gpu(tmp_path):
 
 @pytest.mark.parametrize("field,value", [("n_interactions", 17), ("n_schedule", 17), ("n_regularize", 17),
                                          ("n_ekin", 17), ("n_Mpp", 17), ("n_mpp", 65), ("n_centres", 257),
-                                         ("n_interactions", -1), ("abi_version", 1), ("S", 9)])
+                                         ("n_interactions", -1), ("abi_version", 1), ("S", 9),
+                                         ("n_hyperbox", 17)])
 def test_counts_beyond_their_arrays_are_rejected(tmp_path, field, value):
     """A list count larger than its fixed array (or a params record of another
     layout version) is EINVAL at every entry point that takes params, before
