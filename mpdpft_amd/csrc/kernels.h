@@ -128,6 +128,15 @@ hipError_t dipolar_k(const Grid &g, const double *n, double scale, double *out, 
 // DipolarInteractionPos's last step (3D): v = alpha (v + n0 zf(|r|)/(8 pi^3)) in place, v the
 // backward transform of the multiplied spectrum.
 hipError_t dipolar_comp(const Grid &g, double n0, double alpha, double *v, hipStream_t st);
+// Getn3pFFT's momentum field (DensityExpression 6) as the standard-order spectrum that the
+// backward transform (scale L^-D, plain store) turns into the density.
+hipError_t n3p_spec(const Grid &g, const double *V, double mu, double Uf, double pref, double2 *spec,
+                    hipStream_t st);
+// Getn3p (DensityExpression 10): n_i = scale sum_j w_j K(r_ij, PV_j), scale = g dx^D.
+hipError_t n3p_bessel(const Grid &g, const double *V, double mu, double mass, double scale, double *n,
+                      hipStream_t st);
+// trapezoid-weighted partial sums of f into partials[0..kRedBlocks)
+hipError_t trap_sum(const Grid &g, const double *f, double *partials, hipStream_t st);
 
 // RelDev dots per species + field stats: partials layout per species s:
 // [<a,a>, <b,b>, <a,b>] sums and [denmin, denmax, vmin, vmax].

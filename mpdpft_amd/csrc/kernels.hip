@@ -1085,6 +1085,162 @@ hipError_t dipolar_k(const Grid &g, const double *n, double scale, double *out, 
     return hipGetLastError();
 }
 
+// ---- the n3' densities (DensityExpression 6 and 10) -------------------------
+// Bessel J_n(x), n = 0..3, x >= 0: the power series for x < 2, Miller's
+// backward recurrence from order 2 floor(x/2) + 44 normalised by
+// J_0 + 2 sum J_2k = 1 for 2 <= x < 25, Hankel's asymptotic expansion with the
+// phase x - (2n+1) pi/4 taken through cos x, sin x for x >= 25 (absolute error
+// below 5e-16 against an independent implementation).
+__device__ double bessel_jn(int n, double x) {
+    if (x < 2.) {
+        double h = 0.5 * x, t = 1., q = -h * h, sum = 0.;
+        for (int k = 1; k <= n; k++) t *= h / (double)k;
+        for (int k = 0; k < 40; k++) {
+            sum += t;
+            t *= q / ((double)(k + 1) * (double)(k + 1 + n));
+            if (fabs(t) < 1e-18 * fabs(sum)) break;
+        }
+        return sum;
+    }
+    if (x < 25.) {
+        const int N = 2 * (int)(0.5 * x) + 44;
+        double jp = 0., jc = 1e-30, norm = 0., jn = 0.;
+        for (int k = N; k >= 1; k--) {
+            const double jm = (2. * (double)k / x) * jc - jp;
+            jp = jc;
+            jc = jm;
+            if (k - 1 == n) jn = jc;
+            if (k - 1 > 0 && ((k - 1) & 1) == 0) norm += 2. * jc;
+        }
+        norm += jc;
+        return jn / norm;
+    }
+    const double mu = 4. * (double)n * (double)n;
+    double P = 0., Q = 0., a = 1., xk = 1.;
+    for (int k = 0; k < 60; k++) {
+        const double term = a / xk;
+        if (k > 0 && fabs(term) < 1e-17) break;
+        if ((k & 1) == 0) P += ((k / 2) & 1) ? -term : term;
+        else Q += (((k - 1) / 2) & 1) ? -term : term;
+        a *= (mu - (double)(2 * k + 1) * (double)(2 * k + 1)) / ((double)(k + 1) * 8.);
+        xk *= x;
+        if (fabs(a / xk) > fabs(term) && k > 2) break;
+    }
+    const double c0 = (n == 0 || n == 3) ? 1. : -1., c1 = (n <= 1) ? 1. : -1.;  // cos, sin of (2n+1) pi/4, over 1/sqrt(2)
+    double sx, cx;
+    sincos(x, &sx, &cx);
+    const double r2 = 0.70710678118654752440;
+    const double cw = r2 * (c0 * cx + c1 * sx), sw = r2 * (c0 * sx - c1 * cx);
+    return sqrt(2. / (PI_D * x)) * (P * cw - Q * sw);
+}
+
+// Getn3pFFT (mpDPFT.cpp:3625-3684): one block per standard-order spectrum entry
+// f; the entry holds the centred momentum q at index (f + s/2) mod s per axis,
+// C(q) = pref sum_r e^{-i q.r} (Uf (mu - V(r)) - q^2/8)_+^{D/2} over the (s+1)^D
+// points r = x0 + index dx, with the centring sign of the backward transform.
+__global__ __launch_bounds__(256) void k_n3p_spec(Grid g, const double *__restrict__ V, double mu, double Uf,
+                                                 double pref, double gph, double2 *__restrict__ spec) {
+    __shared__ double sh[4];
+    const int s = g.s, EL = g.EL, D = g.D;
+    const size_t f = blockIdx.x;
+    int qi[3] = {0, (int)(f / s) % s, (int)(f % s)};
+    if (D == 3) qi[0] = (int)(f / ((size_t)s * s));
+    const double dx = g.edge / (double)s, x0 = -g.edge / 2., k0 = -PI_D / dx, dk = 2. * PI_D / g.edge;
+    double q[3] = {0., 0., 0.}, q2 = 0.;
+    int par = 0;
+    for (int d = 3 - D; d < 3; d++) {
+        par += qi[d];
+        q[d] = k0 + (double)((qi[d] + s / 2) % s) * dk;
+        q2 += q[d] * q[d];
+    }
+    const double q2term = 0.125 * q2;
+    double re = 0., im = 0.;
+    for (size_t r = threadIdx.x; r < g.G; r += 256) {
+        const double t = Uf * (mu - V[r]) - q2term;
+        if (!(t > 0.)) continue;
+        int ri[3] = {0, 0, 0};
+        size_t rem = r;
+        for (int d = 2; d >= 3 - D; d--) {
+            ri[d] = (int)(rem % EL);
+            rem /= EL;
+        }
+        const double amp = D == 2 ? t : t * sqrt(t);
+        double arg = -q[3 - D] * (x0 + (double)ri[3 - D] * dx);
+        for (int d = 4 - D; d < 3; d++) arg -= q[d] * (x0 + (double)ri[d] * dx);
+        double sn, cs;
+        sincos(arg, &sn, &cs);
+        re += cs * amp;
+        im += sn * amp;
+    }
+    re = block_sum(re, sh);
+    im = block_sum(im, sh);
+    if (threadIdx.x == 0) {
+        const double sg = (par & 1) ? -gph : gph;
+        spec[f] = make_double2(sg * pref * re, sg * pref * im);
+    }
+}
+hipError_t n3p_spec(const Grid &g, const double *V, double mu, double Uf, double pref, double2 *spec,
+                    hipStream_t st) {
+    const double gph = (((long long)g.D * g.s / 2) & 1) ? -1. : 1.;
+    hipLaunchKernelGGL(k_n3p_spec, dim3((unsigned)g.F), dim3(256), 0, st, g, V, mu, Uf, pref, gph, spec);
+    return hipGetLastError();
+}
+
+// Getn3p (mpDPFT.cpp:3702-3729): one block per grid point i, n_i = g dx^D sum_j
+// w_j (PV_j / (2 pi r_ij))^D J_D(2 r_ij PV_j), PV_j = sqrt(2 m (mu - V_j)) where
+// mu > V_j, and (PV_j^2 / pi)^D / (8, 48) (D = 2, 3) for r_ij <= 1e-16.
+__global__ __launch_bounds__(256) void k_n3p_bessel(Grid g, const double *__restrict__ V, double mu, double mass,
+                                                   double scale, double *__restrict__ n) {
+    __shared__ double sh[4];
+    const int D = g.D;
+    const int i = (int)blockIdx.x;
+    double ri[3], rj[3];
+    coords(g, i, ri);
+    const double gam = D == 2 ? 0.125 : 0.02083333333333333;
+    double acc = 0.;
+    for (int j = threadIdx.x; j < (int)g.G; j += 256) {
+        double PV = mu - V[j];
+        if (!(PV > 0.)) continue;
+        PV = sqrt(2. * mass * PV);
+        coords(g, j, rj);
+        double rp = (rj[0] - ri[0]) * (rj[0] - ri[0]) + (rj[1] - ri[1]) * (rj[1] - ri[1]);
+        if (D == 3) rp += (rj[2] - ri[2]) * (rj[2] - ri[2]);
+        double si;
+        if (rp > 1e-16) {
+            rp = sqrt(rp);
+            const double z = 2. * rp * PV;
+            const double J = z > 2.0e+12 ? sqrt(2. / (PI_D * z)) * cos(z - 0.5 * (double)D * PI_D - 0.25 * PI_D)
+                                         : bessel_jn(D, z);
+            const double b = PV / (2. * PI_D * rp);
+            si = (D == 2 ? b * b : b * b * b) * J;
+        } else {
+            const double b = PV * PV / PI_D;
+            si = (D == 2 ? b * b : b * b * b) * gam;
+        }
+        acc += trap_weight(g, j) * si;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) n[i] = scale * acc;
+}
+hipError_t n3p_bessel(const Grid &g, const double *V, double mu, double mass, double scale, double *n,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_n3p_bessel, dim3((unsigned)g.G), dim3(256), 0, st, g, V, mu, mass, scale, n);
+    return hipGetLastError();
+}
+
+// trapezoid-weighted partial sums of f into partials[0..kRedBlocks)
+__global__ __launch_bounds__(256) void k_trap_sum(Grid g, const double *__restrict__ f, double *partials) {
+    __shared__ double sh[4];
+    double acc = 0.;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < (int)g.G; i += gridDim.x * 256) acc += trap_weight(g, i) * f[i];
+    const double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+hipError_t trap_sum(const Grid &g, const double *f, double *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_trap_sum, dim3(kRedBlocks), dim3(256), 0, st, g, f, partials);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // convergence, stats
 // ---------------------------------------------------------------------------

@@ -189,8 +189,11 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
     for (int v : ok_steps) ok |= (p->steps == v);
     if (!ok) return ctx_fail(c, MPDPFT_EINVAL, "steps must be one of 16, 32, 64, 96, 128, 256");
     if (p->S < 1 || p->S > MPDPFT_MAX_SPECIES) return ctx_fail(c, MPDPFT_EINVAL, "S out of range");
-    if (p->DensityExpression != 7 && p->DensityExpression != 5)
-        return ctx_fail(c, MPDPFT_EINVAL, "DensityExpression must be 7 (TF) or 5 (nAiT)");
+    if (p->DensityExpression != 7 && p->DensityExpression != 5 && p->DensityExpression != 6 &&
+        p->DensityExpression != 10)
+        return ctx_fail(c, MPDPFT_EINVAL, "DensityExpression must be 7 (TF), 5 (nAiT), 6 or 10 (n3')");
+    if (p->DensityExpression == 6 && p->Units != 1 && p->Units != 2)
+        return ctx_fail(c, MPDPFT_EINVAL, "DensityExpression 6 needs Units 1 or 2");
     if (p->DensityExpression == 5 && p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "nAiT is built for DIM 3");
     if (p->method != 1) return ctx_fail(c, MPDPFT_EINVAL, "integration method must be 1 (Riemann)");
     if (p->TaskType != 0 && p->TaskType != 1 && p->TaskType != 8)  // the task loop itself is mpdpft_run_dir's
@@ -229,6 +232,8 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
     if (p->Precision != 0 && p->Precision != 1) return ctx_fail(c, MPDPFT_EINVAL, "Precision must be 0 or 1");
     if (p->Precision == 1) {
         if (p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "Precision 1: DIM 3");
+        if (p->DensityExpression != 5 && p->DensityExpression != 7)
+            return ctx_fail(c, MPDPFT_EINVAL, "Precision 1: DensityExpression 5 or 7");
         for (int a = 0; a < p->n_interactions; a++)
             if (p->Interactions[a] == 8 && p->HartreeMode != 1)
                 return ctx_fail(c, MPDPFT_EINVAL, "Precision 1: isolated Hartree (HartreeMode 1)");
@@ -483,6 +488,31 @@ static int density(mpdpft_ctx *c, int s, double mu, double *den_out, double *N) 
         stage_begin(c);
         HIPC(mk::tf_density(c->g, c->d_V + s * c->G, mu, pref, den_out, c->d_partials, c->st));
         stage_end(c, "tf_density", (double)c->G, 16. * c->G, 0.);
+        return integral_from_partials(c, N);
+    }
+    if (p.DensityExpression == 6) {  // Getn3pFFT (:3625-3684): O(G F) momentum sum, backward transform
+        const double DD = (double)c->D, Uf = p.Units == 2 ? 0.131163 : 1.;
+        const double pref = std::pow(c->dx, DD) * p.degeneracy * (c->D == 2 ? 2. * kPI : 4. * kPI) *
+                            std::pow(8., DD / 2.) / (std::pow(2. * kPI, DD) * DD * std::pow(2., DD));
+        stage_begin(c);
+        HIPC(mk::n3p_spec(c->g, c->d_V + s * c->G, mu, Uf, pref, c->d_spec, c->st));
+        mk::MulParams m{};
+        m.kind = mk::MUL_NONE;
+        m.kq = c->d_kq;
+        HIPC(mk::fft_backward_real(c->g, c->d_spec, c->d_spec, m, den_out, nullptr,
+                                   std::pow(1. / p.edge, DD), 0, 0., c->d_tw, c->st));
+        HIPC(mk::trap_sum(c->g, den_out, c->d_partials, c->st));
+        stage_end(c, "n3p_fft", (double)c->F * (double)c->G, 8. * (double)c->F * (double)c->G, 0.);
+        return integral_from_partials(c, N);
+    }
+    if (p.DensityExpression == 10) {  // Getn3p (:3702-3729): O(G^2) Bessel-kernel sum
+        const double mass = p.Units == 2 ? 0.131163 : 1.;
+        double scale = p.degeneracy, a = trap_scale(c);
+        for (int d = 0; d < c->D; d++) scale *= a;
+        stage_begin(c);
+        HIPC(mk::n3p_bessel(c->g, c->d_V + s * c->G, mu, mass, scale, den_out, c->st));
+        HIPC(mk::trap_sum(c->g, den_out, c->d_partials, c->st));
+        stage_end(c, "n3p_bessel", (double)c->G * (double)c->G, 8. * (double)c->G * (double)c->G, 0.);
         return integral_from_partials(c, N);
     }
     return nait_density(c, s, mu, den_out, N);

@@ -1362,12 +1362,171 @@ int oracle_nait_density(orc_state *o, int s);
 int oracle_nait_intercept(orc_state *o);
 static void get_energy(orc_state *o);
 
+/* AngularFactor (mpDPFT.h:291) */
+static double angular_factor(int D) {
+    return D == 1 ? 2. : (D == 2 ? 2. * PI_ : 4. * PI_);
+}
+
+/* Bessel J_n(x), n = 0..3, x >= 0 (gsl_sf_bessel_Jn's role in Getn3p, GSL
+ * not in the reference tree): the power series for x < 2, Miller's backward
+ * recurrence from order 2 floor(x/2) + 44 normalised by J_0 + 2 sum J_2k = 1
+ * for 2 <= x < 25, Hankel's asymptotic expansion with the phase
+ * x - (2n+1) pi/4 taken through cos x, sin x for x >= 25. */
+double oracle_bessel_jn(int n, double x) {
+    if (n < 0 || n > 3 || !(x >= 0.)) return NAN;
+    if (x < 2.) {
+        double h = 0.5 * x, t = 1., q = -h * h, sum = 0.;
+        for (int k = 1; k <= n; k++) t *= h / (double)k;  /* (x/2)^n / n! */
+        for (int k = 0; k < 40; k++) {
+            sum += t;
+            t *= q / ((double)(k + 1) * (double)(k + 1 + n));
+            if (fabs(t) < 1e-18 * fabs(sum)) break;
+        }
+        return sum;
+    }
+    if (x < 25.) {
+        int N = 2 * (int)(0.5 * x) + 44;
+        double jp = 0., jc = 1e-30, norm = 0., jn = 0.;
+        for (int k = N; k >= 1; k--) {
+            double jm = (2. * (double)k / x) * jc - jp; /* J_{k-1} */
+            jp = jc;
+            jc = jm;
+            if (k - 1 == n) jn = jc;
+            if (k - 1 > 0 && ((k - 1) & 1) == 0) norm += 2. * jc;
+        }
+        norm += jc; /* J_0 */
+        return jn / norm;
+    }
+    const double mu = 4. * (double)n * (double)n;
+    double P = 0., Q = 0., a = 1., xk = 1.;
+    for (int k = 0; k < 60; k++) {
+        double term = a / xk;
+        if (k > 0 && fabs(term) < 1e-17) break;
+        if ((k & 1) == 0) P += ((k / 2) & 1) ? -term : term;
+        else Q += (((k - 1) / 2) & 1) ? -term : term;
+        double f = (mu - (double)(2 * k + 1) * (double)(2 * k + 1)) / ((double)(k + 1) * 8.);
+        a *= f;
+        xk *= x;
+        if (fabs(a / xk) > fabs(term) && k > 2) break; /* past the smallest term */
+    }
+    /* cos, sin of (2n+1) pi / 4 */
+    static const double cs[4][2] = {{1., 1.}, {-1., 1.}, {-1., -1.}, {1., -1.}};
+    const double r2 = 0.70710678118654752440, cx = cos(x), sx = sin(x);
+    const double cw = r2 * (cs[n][0] * cx + cs[n][1] * sx), sw = r2 * (cs[n][0] * sx - cs[n][1] * cx);
+    return sqrt(2. / (PI_ * x)) * (P * cw - Q * sw);
+}
+
+/* Getn3pFFT, DensityExpression 6 (mpDPFT.cpp:3625-3684): on the centred
+ * momentum grid q, C(q) = sum_r e^{-i q.r} (Uf (mu - V(r)) - q^2/8)_+^{D/2}
+ * (r = x0 + index dx; Uf = 1 for Units 1, 0.131163 for Units 2), times
+ * dx^D g A_D 8^{D/2} / ((2 pi)^D D 2^D), then the backward centred transform;
+ * the density is its real part. */
+static int n3p_fft_density(orc_state *o, int s) {
+    int D = o->D, n = o->EL;
+    double Uf;
+    if (o->p.Units == 1) Uf = 1.;
+    else if (o->p.Units == 2) Uf = 0.131163;
+    else return MPDPFT_EINVAL;
+    double DD = (double)D, x0 = -o->p.edge / 2., dx = o->p.edge / (double)o->s, k0 = -PI_ / dx,
+           dk = 2. * PI_ / o->p.edge;
+    double pref = pow(dx, DD) * o->p.degeneracy * angular_factor(D) * pow(8., DD / 2.) /
+                  (pow(2. * PI_, DD) * DD * pow(2., DD));
+    const double *V = o->V + (long long)s * o->G;
+    double mu = o->muVec[s];
+    long long G = o->G;
+#pragma omp parallel for schedule(dynamic)
+    for (long long qi = 0; qi < G; qi++) {
+        int qidx[3] = {0, 0, 0};
+        long long rem = qi;
+        for (int d = D - 1; d >= 0; d--) {
+            qidx[d] = (int)(rem % n);
+            rem /= n;
+        }
+        double q[3] = {0., 0., 0.}, q2 = 0.;
+        for (int d = 0; d < D; d++) {
+            q[d] = k0 + (double)qidx[d] * dk;
+            q2 += q[d] * q[d];
+        }
+        double q2term = 0.125 * q2, re = 0., im = 0.;
+        for (long long ri = 0; ri < G; ri++) {
+            double t = Uf * (mu - V[ri]) - q2term;
+            if (!(t > 0.)) continue;
+            int ridx[3] = {0, 0, 0};
+            long long rr = ri;
+            for (int d = D - 1; d >= 0; d--) {
+                ridx[d] = (int)(rr % n);
+                rr /= n;
+            }
+            double amp = D == 1 ? sqrt(t) : (D == 2 ? t : t * sqrt(t));
+            double arg = -q[0] * (x0 + (double)ridx[0] * dx);
+            for (int d = 1; d < D; d++) arg -= q[d] * (x0 + (double)ridx[d] * dx);
+            re += cos(arg) * amp;
+            im += sin(arg) * amp;
+        }
+        o->CF[qi].re = pref * re;
+        o->CF[qi].im = pref * im;
+    }
+    fft_centered(o, o->CF, +1);
+    double *den = o->Den + (long long)s * G;
+    for (long long i = 0; i < G; i++) den[i] = o->CF[i].re;
+    return 0;
+}
+
+/* Getn3p, DensityExpression 10 (mpDPFT.cpp:3702-3729): n(r_i) = g
+ * Integrate_j (PV_j / (2 pi r_ij))^D J_D(2 r_ij PV_j), PV_j = sqrt(2 m (mu -
+ * V_j)) where mu > V_j (m = 0.131163 for Units 2, else 1), and the r_ij -> 0
+ * limit (PV_j^2 / pi)^D / (2, 8, 48) for D = 1, 2, 3 (r_ij <= 1e-16). */
+static int n3p_density(orc_state *o, int s) {
+    int D = o->D;
+    double mass = o->p.Units == 2 ? 0.131163 : 1.;
+    static const double gam[3] = {0.5, 0.125, 0.02083333333333333};
+    double DD = (double)D;
+    const double *V = o->V + (long long)s * o->G;
+    double mu = o->muVec[s];
+    long long G = o->G;
+    double *den = o->Den + (long long)s * G;
+#pragma omp parallel
+    {
+        double *si = dalloc(G);
+#pragma omp for schedule(dynamic)
+        for (long long i = 0; i < G; i++) {
+            double ri[3];
+            vec_at(o, i, ri);
+            for (long long j = 0; j < G; j++) {
+                double PV = mu - V[j];
+                if (!(PV > 0.)) {
+                    si[j] = 0.;
+                    continue;
+                }
+                PV = sqrt(2. * mass * PV);
+                double rj[3], rp = 0.;
+                vec_at(o, j, rj);
+                for (int d = 0; d < D; d++) rp += (rj[d] - ri[d]) * (rj[d] - ri[d]);
+                if (rp > 1e-16) {
+                    rp = sqrt(rp);
+                    double z = 2. * rp * PV, J;
+                    if (z > 2.0e+12) J = sqrt(2. / (PI_ * z)) * cos(z - 0.5 * DD * PI_ - 0.25 * PI_);
+                    else J = oracle_bessel_jn(D, z);
+                    si[j] = pow(PV / (2. * PI_ * rp), DD) * J;
+                } else
+                    si[j] = pow(PV * PV / PI_, DD) * gam[D - 1];
+            }
+            den[i] = o->p.degeneracy * integrate(o, si);
+        }
+        free(si);
+    }
+    return 0;
+}
+
+
 /* GetDensity (mpDPFT.cpp:2142-2238). */
 static int get_density(orc_state *o, int s) {
     int rc;
     int de = o->p.DensityExpression;
     if (de == 7) rc = tf_density(o, s);
     else if (de == 5) rc = oracle_nait_density(o, s);
+    else if (de == 6) rc = n3p_fft_density(o, s);
+    else if (de == 10) rc = n3p_density(o, s);
     else return MPDPFT_EINVAL;
     double *n = o->Den + (long long)s * o->G;
     for (int r = 0; r < o->p.n_regularize; r++) regularize(o, o->p.regularize[r], n, o->p.RegularizationThreshold);
@@ -1975,9 +2134,6 @@ static void admix_densities(orc_state *o) { /* :4921-4992 */
 /* ------------------------------------------------------------------------ */
 double oracle_nait_dispersal_energy(orc_state *o, int s); /* dpft_oracle_nait.c */
 
-static double angular_factor(int D) {
-    return D == 1 ? 2. : (D == 2 ? 2. * PI_ : 4. * PI_);
-}
 
 static double interaction_energy(orc_state *o, int type) { /* :5805-5918 */
     long long G = o->G;

@@ -62,6 +62,8 @@ double oracle_ekin_point(double U, double a, double lap, double thr, double acc,
 double oracle_qmc_beta(int der, double x);
 /* e^-x I0(x) for x >= 0 (2D periodic Hartree compensation) */
 double oracle_exp_i0(double x);
+/* Bessel J_n(x), n = 0..3, x >= 0 (Getn3p's kernel) */
+double oracle_bessel_jn(int n, double x);
 /* TabFunc_2D3D_eta.dat (eta triplets of the quasi-2D contact, interaction 24) */
 int oracle_load_eta(const char *path);
 /* energy integrand of interaction 14 (species s) or 24 (species s focal) */

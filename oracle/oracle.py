@@ -65,6 +65,8 @@ def lib():
             getattr(L, n).restype = ctypes.c_double
         L.oracle_qmc_beta.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_qmc_beta.restype = ctypes.c_double
+        L.oracle_bessel_jn.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.oracle_bessel_jn.restype = ctypes.c_double
         L.oracle_polylog.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_nait_point.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.oracle_nait_point.restype = ctypes.c_double
@@ -309,3 +311,9 @@ def qmc_beta(der, x):
 def exp_i0(x):
     """e^-x I0(x) (2D periodic Hartree compensation)."""
     return lib().oracle_exp_i0(float(x))
+
+
+def bessel_jn(n, x):
+    """J_n(x), n = 0..3, elementwise over x >= 0 (Getn3p's kernel)."""
+    f = lib().oracle_bessel_jn
+    return np.array([f(int(n), float(v)) for v in np.atleast_1d(x)])
