@@ -133,6 +133,8 @@ typedef struct mpdpft_params {
      * variables, the ranges of TaskType 2/3's starting point and scale */
     int n_hyperbox;        /* number of values (twice the pairs) */
     double TaskHyperBox[2 * MPDPFT_MAX_SPECIES];
+    int CubeFile;          /* extension key: 1 writes mpDPFT_Den_Cube.dat (DIM 3) in place of mpDPFT_Den.dat,
+                              the environment-41 centres as nuclei */
 } mpdpft_params;
 
 /* Results of one solve (the quantities the reference prints in its final

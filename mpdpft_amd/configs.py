@@ -61,6 +61,8 @@ def write_input(p, path, extensions=True):
             "%d\t\t\t//ConvergenceMode" % p.ConvergenceMode,
             "%d\t\t\t//Precision" % p.Precision,
         ]
+        if p.CubeFile:
+            lines.append("%d\t\t\t//CubeFile" % p.CubeFile)
         for c in range(p.n_centres):
             lines.append("%s\t\t\t//Centre" % " ".join(_fmt(x) for x in p.centres[c]))
     with open(path, "w") as f:

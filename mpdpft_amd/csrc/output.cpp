@@ -250,6 +250,173 @@ static std::string final_block(mpdpft_ctx *c, const mpdpft_result &r) {
     return t.str();
 }
 
+// mpDPFT_Den_Cube.dat (ExportData, mpDPFT.cpp:12458-12490), written in place of
+// mpDPFT_Den.dat when the extension key CubeFile is 1 (the reference writes it
+// for its chemistry environment 10; here the nuclei are the environment-41
+// centres): the nucleus count, three axis lines of steps and spacing, one line
+// per nucleus (charge and position), then species 0's particles per voxel,
+// max(1e-14, n dx^3), over [0, steps)^3 (the periodic images left out), x
+// slowest.  With Units 2 lengths are converted from Angstrom to bohr.
+static int write_cube(mpdpft_ctx *c, const std::string &path, const std::vector<double> &den) {
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    const mpdpft_params &p = c->p;
+    const int prec = p.OutPrecision, EL = c->EL, steps = c->s;
+    double cv = 1., c3 = 1.;
+    if (p.Units == 2) {
+        cv = 0.52917721067;
+        c3 = cv * cv * cv;
+    }
+    const double h = c->dx / cv, vol = h * h * h;
+    const int nuc = p.Environments[0] == MPDPFT_ENV_SOFT_COULOMB_CENTRES ? p.n_centres : 0;
+    std::fprintf(f, "%d 0 0 0\n", nuc);
+    for (int a = 0; a < 3; a++) {
+        std::fprintf(f, "%d ", steps);
+        for (int b = 0; b < 3; b++) {
+            if (a == b) put(f, h, prec);
+            else std::fputc('0', f);
+            if (b < 2) std::fputc(' ', f);
+        }
+        std::fputs(a == 2 ? " \n" : "\n", f);
+    }
+    if (nuc > 0) {
+        for (int k = 0; k < nuc; k++) {
+            std::fprintf(f, "%.16g ", p.centres[k][3]);
+            for (int d = 0; d < 3; d++) std::fprintf(f, "%.16g ", p.centres[k][d] / cv);
+            std::fputc('\n', f);
+        }
+    } else
+        std::fputs("0 0 0 0\n", f);
+    for (int i = 0; i < steps; i++)
+        for (int j = 0; j < steps; j++)
+            for (int k = 0; k < steps; k++) {
+                const size_t idx = ((size_t)i * EL + j) * EL + k;
+                put(f, std::max(1.0e-14, c3 * den[idx] * vol), prec);
+                std::fputc('\n', f);
+            }
+    std::fclose(f);
+    return MPDPFT_OK;
+}
+
+// mpDPFT_Plots.sh and mpDPFT_CombinedPlots.tex (the plot shells of ExportData,
+// mpDPFT.cpp:12802-13063): per species the x-y maps of density, environment
+// and potential from mpDPFT_ContourData.dat (DIM 2, 3), the density and the
+// environment + potential (with mu) along the cut from mpDPFT_CutData.dat, the
+// density cut on a log scale, and for several species an overview of all
+// density cuts.  The script drives gnuplot's pdfcairo terminal, one PDF per
+// figure named after the figure (mpDPFT_Plots-<figure>.pdf); the LaTeX file
+// holds the run's result block and includes every figure.
+static int write_plots(mpdpft_ctx *c, const std::string &dir, const mpdpft_result &r, const std::vector<double> &den,
+                       const std::vector<double> &env, const std::vector<double> &V) {
+    const int S = c->S;
+    const double L2 = c->p.edge / 2.;
+    auto range = [&](const std::vector<double> &f, int s, double &lo, double &hi) {
+        lo = hi = f[(size_t)s * c->G];
+        for (size_t i = 0; i < c->G; i++) {
+            const double v = f[(size_t)s * c->G + i];
+            lo = std::min(lo, v);
+            hi = std::max(hi, v);
+        }
+        lo -= 0.1 * std::fabs(lo);
+        hi += 0.1 * std::fabs(hi);
+        if (std::fabs(hi - lo) < 1e-12) {
+            lo = 0.;
+            hi = 1.;
+        }
+    };
+    std::ostringstream sh, inc;
+    sh << std::setprecision(10);
+    sh << "#!/bin/bash\n"
+       << "# gnuplot figures of mpDPFT_ContourData.dat and mpDPFT_CutData.dat (pdfcairo), one PDF per figure,\n"
+       << "# named as mpDPFT_CombinedPlots.tex includes them\n"
+       << "GNUPLOT=${GNUPLOT:-gnuplot}\n"
+       << "cd \"$(dirname \"$0\")\" || exit 1\n"
+       << "figure() {  # name width height; gnuplot commands on stdin\n"
+       << "  { printf 'set terminal pdfcairo enhanced size %s,%s\\nset output \"%s.pdf\"\\n' \"$2\" \"$3\" \"$1\"; cat; } "
+          "| $GNUPLOT && echo \"Generated $1.pdf\"\n"
+       << "}\n";
+    const char *fields[3] = {"Den", "Env", "V"};
+    const std::vector<double> *src[3] = {&den, &env, &V};
+    auto add = [&](const std::string &name) { inc << "\\includegraphics[width=0.32\\textwidth]{mpDPFT_Plots-" << name << "}\n"; };
+    std::ostringstream cut;
+    cut << " along (" << -L2 << ",0) to (" << L2 << ",0)";
+    for (int s = 0; s < S; s++) {
+        double lo, hi;
+        if (c->D > 1)
+            for (int k = 0; k < 3; k++) {
+                range(*src[k], s, lo, hi);
+                const std::string name = std::string("2Dcontour") + fields[k] + std::to_string(s);
+                sh << "figure mpDPFT_Plots-" << name << " 10cm 10cm <<'EOF'\n"
+                   << "set title '" << fields[k] << s << "'\nset view map\nunset surface\nset style data pm3d\n"
+                   << "set xrange [" << -L2 << ":" << L2 << "]\nset yrange [" << -L2 << ":" << L2 << "]\n"
+                   << "set cbrange [" << lo << ":" << hi << "]\n"
+                   << "splot 'mpDPFT_ContourData.dat' using 1:2:" << 3 * s + 4 + k << " with pm3d notitle\nEOF\n";
+                add(name);
+            }
+        range(den, s, lo, hi);
+        std::string name = "CutDen" + std::to_string(s);
+        sh << "figure mpDPFT_Plots-" << name << " 10cm 6.875cm <<'EOF'\n"
+           << "set title 'Den" << s << cut.str() << "'\nset grid\nset yrange [" << lo << ":" << hi << "]\n"
+           << "plot 'mpDPFT_CutData.dat' using 1:" << 3 * s + 2 << " with lines lw 3 title 'Den" << s << "'\nEOF\n";
+        add(name);
+        name = "LogPlot-CutDen" + std::to_string(s);
+        sh << "figure mpDPFT_Plots-" << name << " 10cm 6.875cm <<'EOF'\n"
+           << "set title 'LogPlotDen" << s << cut.str() << "'\nset logscale y\nset format y '%.0e'\n"
+           << "set yrange [" << std::max(lo, 1e-16) << ":" << std::max(hi, 1e-15) << "]\n"
+           << "plot 'mpDPFT_CutData.dat' using 1:" << 3 * s + 2 << " with lines lw 3 title 'Den" << s << "'\nEOF\n";
+        add(name);
+        double elo, ehi, vlo, vhi;
+        range(env, s, elo, ehi);
+        range(V, s, vlo, vhi);
+        lo = std::min(std::min(elo, vlo), r.muVec[s] - 0.1 * std::fabs(r.muVec[s]));
+        hi = std::max(std::max(ehi, vhi), r.muVec[s] + 0.1 * std::fabs(r.muVec[s]));
+        name = "CutEnv+V" + std::to_string(s);
+        sh << "figure mpDPFT_Plots-" << name << " 10cm 6.875cm <<'EOF'\n"
+           << "set title 'Env+V" << s << cut.str() << "'\nset grid\nset yrange [" << lo << ":" << hi << "]\n"
+           << "mu(x) = " << std::setprecision(17) << r.muVec[s] << std::setprecision(10) << "\n"
+           << "plot 'mpDPFT_CutData.dat' using 1:" << 3 * s + 3 << " with lines lw 3 title 'Env" << s
+           << "', mu(x) dt 2 lw 3 title 'mu', 'mpDPFT_CutData.dat' using 1:" << 3 * s + 4
+           << " with lines lw 2 title 'V" << s << "'\nEOF\n";
+        add(name);
+    }
+    if (S > 1) {
+        double lo = 0., hi = 0.;
+        for (int s = 0; s < S; s++) {
+            double a, b;
+            range(den, s, a, b);
+            lo = s ? std::min(lo, a) : a;
+            hi = s ? std::max(hi, b) : b;
+        }
+        sh << "figure mpDPFT_Plots-Overview-CutDen 10cm 6.875cm <<'EOF'\n"
+           << "set title 'Overview Den" << cut.str() << "'\nset grid\nset yrange [" << lo << ":" << hi << "]\nplot";
+        for (int s = 0; s < S; s++)
+            sh << (s ? ", " : " ") << "'mpDPFT_CutData.dat' using 1:" << 3 * s + 2 << " with lines lw 3 title 'Den"
+               << s << "'";
+        sh << "\nEOF\n";
+        add("Overview-CutDen");
+    }
+    FILE *f = std::fopen(join(dir, "mpDPFT_Plots.sh").c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    std::string t = sh.str();
+    std::fwrite(t.data(), 1, t.size(), f);
+    std::fclose(f);
+    // the result block is LaTeX already ("\\" line ends); escape the underscores of its names
+    std::string block;
+    for (char ch : final_block(c, r)) {
+        if (ch == '_') block += "\\_";
+        else block += ch;
+    }
+    std::ostringstream tex;
+    tex << "\\documentclass[10pt]{article}\n\\usepackage{graphicx}\n\\usepackage[margin=1.5cm]{geometry}\n"
+        << "\\begin{document}\n\\noindent " << block << "\n\\noindent\n" << inc.str() << "\\end{document}\n";
+    f = std::fopen(join(dir, "mpDPFT_CombinedPlots.tex").c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    t = tex.str();
+    std::fwrite(t.data(), 1, t.size(), f);
+    std::fclose(f);
+    return MPDPFT_OK;
+}
+
 // One GetData call (mpDPFT.cpp:135-244) in directory d with parameters p:
 // LoadV when InterpolVQ is 1, the SC loop, and ExportData when write_files.
 int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
@@ -296,8 +463,14 @@ int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_res
         std::vector<double> den(c->SG), V(c->SG);
         rc = mpdpft_get_density(c, den.data());
         if (!rc) rc = mpdpft_get_potential(c, V.data());
-        c->control << "\n O U T P U T ...\n\n ***** export Den ****\n";
-        if (!rc) rc = write_den(c, join(d, "mpDPFT_Den.dat"), den);
+        c->control << "\n O U T P U T ...\n\n";
+        if (c->p.CubeFile && c->D == 3) {
+            c->control << " ***** export Cube file ****\n";
+            if (!rc) rc = write_cube(c, join(d, "mpDPFT_Den_Cube.dat"), den);
+        } else {
+            c->control << " ***** export Den ****\n";
+            if (!rc) rc = write_den(c, join(d, "mpDPFT_Den.dat"), den);
+        }
         if (!(c->D == 3 && c->s > 400)) {
             c->control << " ***** export V ****\n";
             if (!rc) rc = write_v(c, join(d, "mpDPFT_V.dat"), V);
@@ -314,6 +487,8 @@ int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_res
         }
         c->control << " ***** export CutData ****\n";
         if (!rc) rc = write_cut(c, join(d, "mpDPFT_CutData.dat"), den, env, V);
+        c->control << " ***** produce plot shells ****\n";
+        if (!rc) rc = write_plots(c, d, r, den, env, V);
         c->control << final_block(c, r);
         c->control << " " << r.errorCount << " Error(s);  " << r.warningCount << " Warning(s)\n EndOfControlFile\n";
         FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");
