@@ -4,6 +4,7 @@
 // mpDPFT.cpp:12399-12660; final block :12716-12800; WriteControlFile
 // :11989-11995).
 #include <cmath>
+#include <fstream>
 #include <cstdio>
 #include <iomanip>
 #include <sstream>
@@ -242,7 +243,7 @@ static std::string final_block(mpdpft_ctx *c, const mpdpft_result &r) {
     t << "EnvironmentalEnergies        = " << vec_to_str(ee, c->S) << "\\\\";
     t << "InteractionEnergies          = " << vec_to_str(r.InteractionEnergies, p.n_interactions) << "\\\\";
     t << "TotalEnergy                  = " << r.TotalEnergy << "\\\\";
-    t << "FigureOfMerit                = " << 0. << "\\\\";  // GetFigureOfMerit (:13743-13793) is 0 for task types 0, 1, 8
+    t << "FigureOfMerit                = " << c->fom << "\\\\";  // GetFigureOfMerit (:13743-13793); 0 outside task types 5-7
     t << "SCiterations                 = " << r.SCcount - 1 << "/" << p.maxSCcount << "\\\\";
     t << "ConvergenceCheck             = " << std::setprecision(6) << r.CC << "\\\\";
     t << "ComputationalTime            = " << std::setprecision(6) << r.solve_seconds << "\\\\";
@@ -419,8 +420,42 @@ static int write_plots(mpdpft_ctx *c, const std::string &dir, const mpdpft_resul
 
 // One GetData call (mpDPFT.cpp:135-244) in directory d with parameters p:
 // LoadV when InterpolVQ is 1, the SC loop, and ExportData when write_files.
+// LoadReferenceDensity (mpDPFT.cpp:13247-13280), 2D: the n x n table (x index
+// fastest, on the lattice -edge/2 + edge i/(n-1)) interpolated bilinearly onto
+// the run's lattice, value at (x_i, y_j) stored at j EL + i as the reference
+// does; then GetFigureOfMerit for task types 4-7 (:13771-13775):
+// 1 - 2 <n_0, ref> / (|n_0|^2 + |ref|^2), plain sums over the grid points.
+static double figure_of_merit(const mpdpft_ctx *c, const std::vector<double> &tab, const std::vector<double> &den) {
+    const int n = (int)(std::sqrt((double)tab.size()) + 0.5), EL = c->EL;
+    std::vector<double> X(n), ref(c->G);
+    for (int i = 0; i < n; i++) X[i] = -0.5 * c->p.edge + c->p.edge * (double)i / ((double)(n - 1));
+    auto cellof = [&](double t, int &l, double &f) {
+        l = (int)std::floor((t - X[0]) / (X[1] - X[0]));
+        l = std::min(std::max(l, 0), n - 2);
+        f = (t - X[l]) / (X[l + 1] - X[l]);
+    };
+    for (int i = 0; i < EL; i++)
+        for (int j = 0; j < EL; j++) {
+            int a, b;
+            double t, u;
+            cellof(c->lat_h[i], a, t);
+            cellof(c->lat_h[j], b, u);
+            const double f00 = tab[(size_t)b * n + a], f10 = tab[(size_t)b * n + a + 1],
+                         f01 = tab[(size_t)(b + 1) * n + a], f11 = tab[(size_t)(b + 1) * n + a + 1];
+            ref[(size_t)j * EL + i] = (1. - t) * (1. - u) * f00 + t * (1. - u) * f10 + (1. - t) * u * f01 + t * u * f11;
+        }
+    double dot = 0., nn = 0., rr = 0.;
+    for (size_t k = 0; k < c->G; k++) {
+        dot += den[k] * ref[k];
+        nn += den[k] * den[k];
+        rr += ref[k] * ref[k];
+    }
+    return 1. - 2. * dot / (nn + rr);
+}
+
 int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
-            const std::string &notes, double *mpp_out, double *task_grad, int *grad_rc) {
+            const std::string &notes, double *mpp_out, double *task_grad, int *grad_rc,
+            const std::vector<double> *ref, double *fom) {
     mpdpft_ctx *c = nullptr;
     int rc = mpdpft_create(&p, device, &c);
     if (rc) {
@@ -459,6 +494,16 @@ int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_res
         return src;
     }
     if (task_grad) *grad_rc = mpdpft_task_gradient(c, task_grad);
+    if (ref) {
+        std::vector<double> den0(c->SG);
+        rc = mpdpft_get_density(c, den0.data());  // [S][G]: species 0 first
+        if (rc) {
+            mpdpft_destroy(c);
+            return rc;
+        }
+        c->fom = figure_of_merit(c, *ref, den0);
+        if (fom) *fom = c->fom;
+    }
     if (write_files && c->p.WriteFiles) {
         std::vector<double> den(c->SG), V(c->SG);
         rc = mpdpft_get_density(c, den.data());
@@ -503,6 +548,97 @@ int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_res
     return rc ? rc : src;
 }
 
+// Task types 5, 6, 7 (mpDPFTmain.cpp:110-166, RetrieveTask mpDPFT.cpp:6086-6100,
+// ProcessTASK :170-185): a rectangular scan of TaskParameterCount values per
+// variable over the TaskHyperBox, v = lo + (hi - lo) t / (count - 1); type 7
+// varies mpp[0], type 5 mpp[0] and gammaH, type 6 tauVec[0], mpp[0] and
+// gammaH (the last variable fastest).  Each run's figure of merit against
+// TabFunc_ReferenceDensity.dat (2D) goes with its variables into
+// mpDPFT_TASKParameterExploration.dat; the last run writes its files.
+static int run_exploration(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res) {
+    const int type = p.TaskType, nv = type == 5 ? 2 : (type == 6 ? 3 : 1), count = p.TaskParameterCount;
+    std::ostringstream task;
+    task << std::setprecision(16);
+    task << " ***** mpdpft_hip task loop: TaskType " << type << " *****\n";
+    auto fail = [&](int code, const std::string &msg) {
+        task << msg << "\n";
+        FILE *f = std::fopen(join(d, "mpDPFT_ControlTask.dat").c_str(), "w");
+        if (f) {
+            std::string t = task.str();
+            std::fwrite(t.data(), 1, t.size(), f);
+            std::fclose(f);
+        }
+        return code;
+    };
+    if (p.DIM != 2) return fail(MPDPFT_EINVAL, "task types 5-7 compare with a 2D reference density (DIM 2)");
+    if (count < 2) return fail(MPDPFT_EINVAL, "task types 5-7 need TaskParameterCount >= 2");
+    if (p.n_hyperbox < 2 * nv) return fail(MPDPFT_EINVAL, "task types 5-7 need one TaskHyperBox pair per variable");
+    std::vector<double> tab;
+    {
+        std::ifstream in(join(d, "TabFunc_ReferenceDensity.dat"));
+        if (!in) return fail(MPDPFT_EIO, "TabFunc_ReferenceDensity.dat not found");
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream iss(line);
+            double v = 0.;
+            if (!(iss >> v)) v = 0.;
+            tab.push_back(v);
+        }
+        const int n = (int)(std::sqrt((double)tab.size()) + 0.5);
+        if (n < 2 || (size_t)n * n != tab.size())
+            return fail(MPDPFT_EIO, "TabFunc_ReferenceDensity.dat does not hold n x n values");
+    }
+    auto value = [&](int k, int t) {
+        return p.TaskHyperBox[2 * k] + (p.TaskHyperBox[2 * k + 1] - p.TaskHyperBox[2 * k]) * (double)t /
+                                           ((double)count - 1.);
+    };
+    const int total = nv == 1 ? count : (nv == 2 ? count * count : count * count * count);
+    std::ostringstream rows;
+    rows << std::setprecision(16);
+    mpdpft_result r{};
+    int last_rc = MPDPFT_OK;
+    for (int k = 0; k < total; k++) {
+        int t[3] = {0, 0, 0}, rem = k;
+        for (int v = nv - 1; v >= 0; v--) {
+            t[v] = rem % count;
+            rem /= count;
+        }
+        double vec[3];
+        for (int v = 0; v < nv; v++) vec[v] = value(v, t[v]);
+        mpdpft_params q = p;
+        if (type == 7) q.mpp[0] = vec[0];
+        else if (type == 5) {
+            q.mpp[0] = vec[0];
+            q.gammaH = vec[1];
+        } else {
+            q.tauVec[0] = vec[0];
+            q.mpp[0] = vec[1];
+            q.gammaH = vec[2];
+        }
+        double fom = 0.;
+        last_rc = run_one(d, q, device, &r, k == total - 1, "", nullptr, nullptr, nullptr, &tab, &fom);
+        if (last_rc != MPDPFT_OK && last_rc != MPDPFT_ENOTCONV) return fail(last_rc, "run failed");
+        rows << fom << " ";
+        for (int v = 0; v < nv; v++) rows << vec[v] << " ";
+        rows << "\n";
+        task << "task " << k << ": VEC " << vec_to_str(vec, nv) << " FigureOfMerit " << fom << "  TotalEnergy "
+             << r.TotalEnergy << "  converged " << r.converged << "\n";
+    }
+    if (res) *res = r;
+    FILE *f = std::fopen(join(d, "mpDPFT_TASKParameterExploration.dat").c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    std::string t = rows.str();
+    std::fwrite(t.data(), 1, t.size(), f);
+    std::fclose(f);
+    f = std::fopen(join(d, "mpDPFT_ControlTask.dat").c_str(), "w");
+    if (f) {
+        t = task.str();
+        std::fwrite(t.data(), 1, t.size(), f);
+        std::fclose(f);
+    }
+    return last_rc;
+}
+
 // The outer task loop of main (mpDPFTmain.cpp:77-214) for task types 0, 1 and 8
 // (2 and 3, the conjugate-gradient minimisations, are in tasks.cpp),
 // with RetrieveTask's per-task parameter changes (mpDPFT.cpp:6070-6121):
@@ -520,14 +656,15 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     int rc = mpdpft_parse_input(join(d, "mpDPFT.input").c_str(), &p);
     if (rc) return rc;
     const int type = p.TaskType;
-    if (type != 0 && type != 1 && type != 2 && type != 3 && type != 8) {
+    if (type != 0 && type != 1 && type != 2 && type != 3 && type != 5 && type != 6 && type != 7 && type != 8) {
         FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");
         if (f) {
-            std::fprintf(f, "TaskType %d is not built (0, 1, 2, 3, 8 are)\n", type);
+            std::fprintf(f, "TaskType %d is not built (0, 1, 2, 3, 5, 6, 7, 8 are)\n", type);
             std::fclose(f);
         }
         return MPDPFT_EINVAL;
     }
+    if (type >= 5 && type <= 7) return run_exploration(d, p, device, res);
     if (type == 2 || type == 3) {
         std::ostringstream task;
         task << " ***** mpdpft_hip task loop: TaskType " << type << " *****\n";

@@ -88,6 +88,7 @@ struct mpdpft_ctx {
     std::vector<double> V0;   // initial potential of a solve (empty: V = Env)
     std::ostringstream notes; // lines for the next solve's control stream (LoadV)
     int importWarnings = 0;   // warnings of that import, added to the solve's count
+    double fom = 0.;          // GetFigureOfMerit of the last solve (task types 5-7; 0 otherwise)
     std::string break_file;   // ManualSCloopBreakQ file ("" = not polled)
     double OldCC = 0.;        // CC of the previous iteration (InjectSchedule 5)
     double thetaVec0[MPDPFT_MAX_SPECIES] = {};  // thetaVecOriginal (InjectSchedule 9)
@@ -110,8 +111,11 @@ const char *params_structure_error(const mpdpft_params *p);  // params.cpp
 // output.cpp: one GetData call in directory d (LoadV, SC loop, ExportData when
 // write_files); task_grad (may be null) receives mpdpft_task_gradient of the
 // converged context, with its return code in *grad_rc.
+// ref (may be null): the n x n TabFunc_ReferenceDensity.dat table; then the figure of merit against it goes into
+// *fom (and the Control file).
 int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
-            const std::string &notes, double *mpp_out, double *task_grad = nullptr, int *grad_rc = nullptr);
+            const std::string &notes, double *mpp_out, double *task_grad = nullptr, int *grad_rc = nullptr,
+            const std::vector<double> *ref = nullptr, double *fom = nullptr);
 // tasks.cpp: TaskType 2 / 3 (the energy's minimum over muVec / the abundances)
 int run_cgd_task(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res,
                  std::ostringstream &log);

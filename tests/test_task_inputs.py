@@ -50,7 +50,7 @@ def test_empty_hyperbox(tmp_path):
 
 
 def test_unbuilt_task_type(tmp_path):
-    assert "TaskType 5 is not built" in run(tmp_path, base(5, 1e-10))
+    assert "TaskType 9 is not built" in run(tmp_path, base(9, 1e-10))
 
 
 def test_hyperbox_round_trips_through_the_input_file(tmp_path):
@@ -58,3 +58,24 @@ def test_hyperbox_round_trips_through_the_input_file(tmp_path):
     configs.write_input(p, str(tmp_path / "mpDPFT.input"))
     q = mpdpft_amd.parse_input(tmp_path / "mpDPFT.input")
     assert q.n_hyperbox == 2 and list(q.TaskHyperBox[:2]) == [2.5, 7.25]
+
+
+def test_parameter_scans_need_2d_and_a_reference_table(tmp_path):
+    p = base(7, 1e-10, (0.3, 0.7))
+    p.TaskParameterCount = 3
+    p.DIM = 3
+    assert "2D reference density" in run(tmp_path, p)
+    p.DIM = 2
+    assert "TabFunc_ReferenceDensity.dat not found" in run_code(tmp_path, p, _abi.EIO)
+    (tmp_path / "TabFunc_ReferenceDensity.dat").write_text("1\n2\n3\n")
+    assert "n x n values" in run_code(tmp_path, p, _abi.EIO)
+    p.TaskParameterCount = 1
+    assert "TaskParameterCount >= 2" in run(tmp_path, p)
+
+
+def run_code(tmp_path, p, code):
+    configs.write_input(p, str(tmp_path / "mpDPFT.input"))
+    with pytest.raises(mpdpft_amd.MpdpftError) as e:
+        mpdpft_amd.run_dir(tmp_path)
+    assert e.value.code == code
+    return (tmp_path / "mpDPFT_ControlTask.dat").read_text()
