@@ -1483,12 +1483,24 @@ __global__ __launch_bounds__(64) void k_ekin_partials(const double2 *__restrict_
             prev = val;
         }
     }
-    double r, rU, ra, rl;
-    ekin_level(ap, ea, e, level, true, r, rU, ra, rl);
-    const double inv = 1. / (double)((64 << level) - 1);
-    const double dU = (e.dXdU * r + e.X * rU) * inv;
-    const double da = (e.dXda * r + e.X * ra) * inv;
-    const double dl = e.X * rl * inv;
+    // The partials at successive refinements, from the coarsest up to the energy's own: each refinement is a
+    // trapezoid sum on its own nodes, and the partials settle long before the value meets the energy loop's
+    // 1e-12 test (most points run to MaxIteration, 2^22 nodes at 16).  The first refinement from the third on
+    // whose partials all move by less than 1e-10 of themselves since the previous one is taken; the energy's
+    // refinement caps the search.
+    double dU = 0., da = 0., dl = 0.;
+    for (int lv = 0; lv <= level; lv++) {
+        double r, rU, ra, rl;
+        ekin_level(ap, ea, e, lv, true, r, rU, ra, rl);
+        const double inv = 1. / (double)((64 << lv) - 1);
+        const double nU = (e.dXdU * r + e.X * rU) * inv, na = (e.dXda * r + e.X * ra) * inv, nl = e.X * rl * inv;
+        const bool settled = lv >= 2 && fabs(nU - dU) <= 1e-10 * fabs(nU) && fabs(na - da) <= 1e-10 * fabs(na) &&
+                             fabs(nl - dl) <= 1e-10 * fabs(nl);
+        dU = nU;
+        da = na;
+        dl = nl;
+        if (settled) break;
+    }
     if (lane == 0) {
         eU[i] = dU;
         eg2[i] = g2 > 0. ? da * a / (3. * g2) : 0.;  // a = c0 |g2|^{1/3} / 2
@@ -1962,6 +1974,26 @@ int nait_energy_partials(mpdpft_ctx *c, int s, double *eU, double *eg2, double *
     if (int rc = v_fingerprint(c, s, fp)) return rc;
     const bool state = t->ekin_s == s && t->ekin_fp[0] == fp[0] && t->ekin_fp[1] == fp[1] &&
                        t->ekin_mu == c->muVec[s] && t->ekin_maxit == c->MaxIteration;
+    const char *vb = std::getenv("MPDPFT_VERBOSE");
+    if (state && vb && *vb && *vb != '0') {  // refinement levels of the points (stderr)
+        std::vector<int> m(G);
+        hipError_t ce = hipMemcpyAsync(m.data(), t->d_ekin_m, sizeof(int) * G, hipMemcpyDeviceToHost, c->st);
+        if (ce == hipSuccess) ce = hipStreamSynchronize(c->st);
+        if (ce != hipSuccess) return hip_fail(c, ce, "ekin levels");
+        long long hist[26] = {}, open = 0;
+        double nodes = 0.;
+        for (size_t i = 0; i < G; i++) {
+            const int lv = m[i] >= 0 ? m[i] : (m[i] <= -2 ? -m[i] - 2 : -1);
+            if (lv < 0 || lv > 25) continue;
+            hist[lv]++;
+            open += m[i] >= 0;
+            nodes += 64. * std::ldexp(1., lv);
+        }
+        std::fprintf(stderr, "ekin partials: MaxIteration %d, points per level:", c->MaxIteration);
+        for (int l = 0; l < 26; l++)
+            if (hist[l]) std::fprintf(stderr, " %d:%lld", l, hist[l]);
+        std::fprintf(stderr, "; unconverged %lld; nodes %.3e\n", open, nodes);
+    }
     hipLaunchKernelGGL(k_ekin_partials, dim3(xcd_blocks((long long)G)), dim3(64), 0, c->st, t->d_ap, ea,
                        state ? t->d_ekin_m : nullptr, eU, eg2, el);
     hipError_t he = hipGetLastError();
