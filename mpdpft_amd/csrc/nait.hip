@@ -1483,24 +1483,12 @@ __global__ __launch_bounds__(64) void k_ekin_partials(const double2 *__restrict_
             prev = val;
         }
     }
-    // The partials at successive refinements, from the coarsest up to the energy's own: each refinement is a
-    // trapezoid sum on its own nodes, and the partials settle long before the value meets the energy loop's
-    // 1e-12 test (most points run to MaxIteration, 2^22 nodes at 16).  The first refinement from the third on
-    // whose partials all move by less than 1e-10 of themselves since the previous one is taken; the energy's
-    // refinement caps the search.
-    double dU = 0., da = 0., dl = 0.;
-    for (int lv = 0; lv <= level; lv++) {
-        double r, rU, ra, rl;
-        ekin_level(ap, ea, e, lv, true, r, rU, ra, rl);
-        const double inv = 1. / (double)((64 << lv) - 1);
-        const double nU = (e.dXdU * r + e.X * rU) * inv, na = (e.dXda * r + e.X * ra) * inv, nl = e.X * rl * inv;
-        const bool settled = lv >= 2 && fabs(nU - dU) <= 1e-10 * fabs(nU) && fabs(na - da) <= 1e-10 * fabs(na) &&
-                             fabs(nl - dl) <= 1e-10 * fabs(nl);
-        dU = nU;
-        da = na;
-        dl = nl;
-        if (settled) break;
-    }
+    double r, rU, ra, rl;
+    ekin_level(ap, ea, e, level, true, r, rU, ra, rl);
+    const double inv = 1. / (double)((64 << level) - 1);
+    const double dU = (e.dXdU * r + e.X * rU) * inv;
+    const double da = (e.dXda * r + e.X * ra) * inv;
+    const double dl = e.X * rl * inv;
     if (lane == 0) {
         eU[i] = dU;
         eg2[i] = g2 > 0. ? da * a / (3. * g2) : 0.;  // a = c0 |g2|^{1/3} / 2
