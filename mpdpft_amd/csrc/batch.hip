@@ -344,6 +344,31 @@ template <typename T> struct Inst {
             });
         return tsum(acc) * in.trapD;
     }
+    // N_s at K chemical potentials in one sweep over V.  Per value the per-lane
+    // accumulation and the block and team reductions are those of evalN.
+    template <int K>
+    __device__ __forceinline__ void evalN_multi(int s, const double (&mus)[K], double (&out)[K]) {
+        const T *v = V + (long long)s * in.G;
+        const double pref = in.pref;
+        const bool d2 = in.D == 2;
+        double acc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = 0.;
+        stream(
+            lo, hi, [&](long long i) { return (double)v[i]; },
+            [&](long long i, double vi) {
+                const double w = trapw(in, i);
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    double t = mus[k] - vi;
+                    double val = t > 0. ? (d2 ? pref * t : pref * t * sqrt(t)) : 0.;
+                    acc[k] += w * val;
+                }
+            });
+        tsum_n<K>(acc, K);
+#pragma unroll
+        for (int k = 0; k < K; k++) out[k] = acc[k] * in.trapD;
+    }
     __device__ __forceinline__ double N_at(Loop &L, int s, double mu) {
         L.muEval[s] = mu;
         L.bytes += (double)in.G * sizeof(T);
@@ -442,46 +467,88 @@ template <typename T> struct Inst {
                     // the last five positive N values (hist5[4] the newest) and their count
                     double hist5[5] = {0., 0., 0., 0., 0.};
                     int nh = 0;
-                    bool success = false, tooSmall = false;
-                    while (!success && !tooSmall) {
-                        success = true;
-                        L.TmpAb[s] = N_eval(L, s, L.mu[s]);
-                        if (L.TmpAb[s] > 0.) {
-                            if (nh < 5) hist5[nh] = L.TmpAb[s];
-                            else {
-                                for (int h = 0; h < 4; h++) hist5[h] = hist5[h + 1];
-                                hist5[4] = L.TmpAb[s];
-                            }
-                            nh++;
+                    bool success = false, tooSmall = false, done = false;
+                    // Speculation over kSpec decisions: one sweep evaluates N at the current mu (node 0) and
+                    // at every mu the next kSpec - 1 halvings can reach (node j: N too small -> 2j + 1 at
+                    // mu + step/2, too large -> 2j + 2 at mu - step/2, each with its step |mu' - mu|); the
+                    // step-by-step loop then runs on those values and takes its mu and step from the tree, so
+                    // it evaluates at the same points in the same order, and counts what it uses.
+                    constexpr int kSpec = 3, kCand = (1 << kSpec) - 1;
+                    double cm[kCand], cst[kCand], cN[kCand];
+                    while (!done && !success && !tooSmall) {
+                        cm[0] = L.mu[s];
+                        cst[0] = step[s];
+#pragma unroll
+                        for (int j = 0; 2 * j + 2 < kCand; j++) {
+                            cm[2 * j + 1] = cm[j] + 0.5 * cst[j];
+                            cst[2 * j + 1] = fabs(cm[2 * j + 1] - cm[j]);
+                            cm[2 * j + 2] = cm[j] - 0.5 * cst[j];
+                            cst[2 * j + 2] = fabs(cm[2 * j + 2] - cm[j]);
                         }
-                        // trapped: two equal N among the four entries before the newest (:4690-4700)
-                        bool trapped = false;
-                        if (nh > 5)
-                            for (int h = 0; h < 3 && !trapped; h++)
-                                for (int hp = h + 1; hp < 4; hp++)
-                                    if (fabs(hist5[h] - hist5[hp]) < relacc * target) {
-                                        trapped = true;
-                                        L.TrappedQ = 1;
+                        evalN_multi<kCand>(s, cm, cN);
+                        L.bytes += (double)in.G * sizeof(T);
+                        int j = 0;
+                        for (int lvl = 0; lvl < kSpec; lvl++) {
+                            success = true;
+                            L.bis++;
+                            L.sbytes += (double)in.G * sizeof(T);
+                            L.muEval[s] = cm[j];
+                            L.TmpAb[s] = cN[j];
+                            if (L.TmpAb[s] > 0.) {
+                                if (nh < 5) hist5[nh] = L.TmpAb[s];
+                                else {
+                                    for (int h = 0; h < 4; h++) hist5[h] = hist5[h + 1];
+                                    hist5[4] = L.TmpAb[s];
+                                }
+                                nh++;
+                            }
+                            // trapped: two equal N among the four entries before the newest (:4690-4700)
+                            bool trapped = false;
+                            if (nh > 5)
+                                for (int h = 0; h < 3 && !trapped; h++)
+                                    for (int hp = h + 1; hp < 4; hp++)
+                                        if (fabs(hist5[h] - hist5[hp]) < relacc * target) {
+                                            trapped = true;
+                                            L.TrappedQ = 1;
+                                            break;
+                                        }
+                            if (trapped) {
+                                done = true;
+                                break;
+                            }
+                            int child = -1;
+                            for (int sg = -1; sg < 2; sg += 2) {
+                                if ((double)sg * (L.TmpAb[s] - target) / fabs(2. * target) > relacc) {
+                                    // the child's mu and step: from the tree, or past its last level by
+                                    // the same arithmetic
+                                    const int c = sg < 0 ? 2 * j + 1 : 2 * j + 2;
+                                    double nm, nst;
+                                    if (c < kCand) {
+                                        nm = cm[c];
+                                        nst = cst[c];
+                                    } else {
+                                        nm = sg < 0 ? cm[j] + 0.5 * cst[j] : cm[j] - 0.5 * cst[j];
+                                        nst = fabs(nm - cm[j]);
+                                    }
+                                    prev[s] = L.mu[s];
+                                    L.mu[s] = nm;
+                                    if (fabs(L.mu[s]) > 1.0e+20) {
+                                        L.warn++;
+                                        L.mu[s] = L.PrevMu[s];
                                         break;
                                     }
-                        if (trapped) break;
-                        for (int sg = -1; sg < 2; sg += 2) {
-                            if ((double)sg * (L.TmpAb[s] - target) / fabs(2. * target) > relacc) {
-                                prev[s] = L.mu[s];
-                                L.mu[s] -= (double)sg * 0.5 * step[s];
-                                if (fabs(L.mu[s]) > 1.0e+20) {
-                                    L.warn++;
-                                    L.mu[s] = L.PrevMu[s];
-                                    break;
+                                    step[s] = nst;
+                                    if (fabs(step[s]) < kMPb) {
+                                        L.warn++;
+                                        tooSmall = true;
+                                        break;
+                                    }
+                                    success = false;
+                                    child = c;
                                 }
-                                step[s] = fabs(L.mu[s] - prev[s]);
-                                if (fabs(step[s]) < kMPb) {
-                                    L.warn++;
-                                    tooSmall = true;
-                                    break;
-                                }
-                                success = false;
                             }
+                            if (success || tooSmall || child >= kCand) break;
+                            j = child;
                         }
                     }
                 }
