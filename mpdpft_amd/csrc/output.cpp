@@ -155,9 +155,12 @@ static double bilinear(const std::vector<double> &X, const std::vector<double> &
 // per species the density, environment and potential at the cut point (DIM 1:
 // the grid values; DIM 2/3: interpolated in the x-y plane), and in 3D the lattice
 // coordinate and per species the column density sum_k n(i, steps/2, k) dx.
-static int write_cut(mpdpft_ctx *c, const std::string &path, const std::vector<double> &den,
-                     const std::vector<double> &env, const std::vector<double> &V) {
-    const int EL = c->EL, steps = c->s, prec = c->p.OutPrecision;
+// The cut (GetCutPositions :12151-12173, GetFieldInterpolationsAlongCut
+// :12217-12253): its length and fi[(s 3 + p) EL + i], p = density, environment,
+// potential of species s at cut point i.
+double cut_fields(const mpdpft_ctx *c, const std::vector<double> &den, const std::vector<double> &env,
+                  const std::vector<double> &V, std::vector<double> &fi) {
+    const int EL = c->EL, steps = c->s;
     std::vector<double> cx(EL), cy(EL, 0.);
     if (c->D == 1) {
         for (int i = 0; i < EL; i++) cx[i] = c->lat_h[i];
@@ -171,8 +174,7 @@ static int write_cut(mpdpft_ctx *c, const std::string &path, const std::vector<d
     }
     double dv[2] = {cx[0] - cx[EL - 1], cy[0] - cy[EL - 1]};
     const double CutLine = norm(dv, 2);
-    // fi[s][p][i], p = density, environment, potential
-    std::vector<double> fi((size_t)c->S * 3 * EL);
+    fi.assign((size_t)c->S * 3 * EL, 0.);
     const std::vector<double> *src[3] = {&den, &env, &V};
     if (c->D == 1) {
         for (int s = 0; s < c->S; s++)
@@ -189,6 +191,14 @@ static int write_cut(mpdpft_ctx *c, const std::string &path, const std::vector<d
                 for (int i = 0; i < EL; i++) fi[((size_t)s * 3 + pp) * EL + i] = bilinear(X, F, cx[i], cy[i]);
             }
     }
+    return CutLine;
+}
+
+static int write_cut(mpdpft_ctx *c, const std::string &path, const std::vector<double> &den,
+                     const std::vector<double> &env, const std::vector<double> &V) {
+    const int EL = c->EL, steps = c->s, prec = c->p.OutPrecision;
+    std::vector<double> fi;
+    const double CutLine = cut_fields(c, den, env, V, fi);
     FILE *f = std::fopen(path.c_str(), "w");
     if (!f) return MPDPFT_EIO;
     for (int i = 0; i < EL; i++) {
@@ -295,6 +305,55 @@ static int write_cube(mpdpft_ctx *c, const std::string &path, const std::vector<
                 put(f, std::max(1.0e-14, c3 * den[idx] * vol), prec);
                 std::fputc('\n', f);
             }
+    std::fclose(f);
+    return MPDPFT_OK;
+}
+
+// mpDPFT_MovieData.tmp and mpDPFT_Movie.sh (ExportData, mpDPFT.cpp:12660-12691),
+// MovieQ 1 (linear) or 2 (log scale): the stored cut frames, one row per cut
+// point (the coordinate, then per frame and species the density, environment
+// and potential; 4 significant digits, a space before each value, no newline
+// after the last row), and a script that draws every frame's densities with
+// gnuplot (image.<frame>.png) and joins them with ffmpeg.  The script reads the
+// .tmp file (the reference's script names a .dat file it never writes).
+static int write_movie(mpdpft_ctx *c, const std::string &dir) {
+    if (c->movie.empty() || c->frames < 1) return MPDPFT_OK;
+    FILE *f = std::fopen(join(dir, "mpDPFT_MovieData.tmp").c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    const int EL = c->EL, S = c->S;
+    double lo = 0., hi = 0.;
+    bool first = true;
+    for (int i = 0; i < EL; i++) {
+        for (size_t j = 0; j < c->movie[i].size(); j++) {
+            std::fprintf(f, " %.4g", c->movie[i][j]);
+            if (j > 0 && (j - 1) % 3 == 0) {  // a density column
+                const double v = c->movie[i][j];
+                lo = first ? v : std::min(lo, v);
+                hi = first ? v : std::max(hi, v);
+                first = false;
+            }
+        }
+        if (i < EL - 1) std::fputc('\n', f);
+    }
+    std::fclose(f);
+    const double d = hi - lo;
+    std::ostringstream sh;
+    sh << std::setprecision(10);
+    sh << "#!/bin/bash\n# frames of the density cuts in mpDPFT_MovieData.tmp (gnuplot, png), joined by ffmpeg\n"
+       << "GNUPLOT=${GNUPLOT:-gnuplot}\ncd \"$(dirname \"$0\")\" || exit 1\n$GNUPLOT <<'EOF'\n"
+       << "set terminal pngcairo size 800,600\nset grid\n";
+    if (c->p.MovieQ == 2) sh << "set logscale y\nset yrange [" << std::max(1e-16, lo - 0.1 * d) << ":" << hi + 0.1 * d << "]\n";
+    else sh << "set yrange [" << lo - 0.1 * d << ":" << hi + 0.1 * d << "]\n";
+    sh << "do for [frame=0:" << c->frames - 1 << "] {\n  set output sprintf('image.%d.png', frame)\n  plot";
+    for (int s = 0; s < S; s++)
+        sh << (s ? "," : "") << " 'mpDPFT_MovieData.tmp' using 1:(column(" << 2 + 3 * s << " + frame*" << 3 * S
+           << ")) with lines lw 3 title sprintf('Den" << s << " frame %d', frame)";
+    sh << "\n}\nEOF\n"
+       << "ffmpeg -y -framerate " << ((double)c->frames + 1.) / 20. << " -i image.%d.png -pix_fmt yuv420p mpDPFT_Movie.mp4\n";
+    f = std::fopen(join(dir, "mpDPFT_Movie.sh").c_str(), "w");
+    if (!f) return MPDPFT_EIO;
+    const std::string t = sh.str();
+    std::fwrite(t.data(), 1, t.size(), f);
     std::fclose(f);
     return MPDPFT_OK;
 }
@@ -534,6 +593,10 @@ int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_res
         if (!rc) rc = write_cut(c, join(d, "mpDPFT_CutData.dat"), den, env, V);
         c->control << " ***** produce plot shells ****\n";
         if (!rc) rc = write_plots(c, d, r, den, env, V);
+        if (c->p.MovieQ > 0 && r.SCcount > 0) {
+            c->control << " ***** export MovieData ****\n";
+            if (!rc) rc = write_movie(c, d);
+        }
         c->control << final_block(c, r);
         c->control << " " << r.errorCount << " Error(s);  " << r.warningCount << " Warning(s)\n EndOfControlFile\n";
         FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");

@@ -1504,6 +1504,8 @@ static void reset_loop_scalars(mpdpft_ctx *c) {
     nait_reset_state(c);
 }
 
+static int store_movie_frame(mpdpft_ctx *c);
+
 static int init_loop(mpdpft_ctx *c) {
     const mpdpft_params &p = c->p;
     reset_loop_scalars(c);
@@ -1539,7 +1541,32 @@ static int init_loop(mpdpft_ctx *c) {
     // initial densities (:194-201)
     RC(get_densities(c));
     RC(field_stats(c));
+    c->movie.clear();
+    c->frames = 0;
+    if (p.MovieQ) RC(store_movie_frame(c));  // :199
     c->loop_ready = 1;
+    return MPDPFT_OK;
+}
+
+// StoreMovieData (mpDPFT.cpp:12317-12329) with SetupMovieStorage (:6422-6427):
+// one frame of the cut interpolations of the current density, environment and
+// potential per species.
+static int store_movie_frame(mpdpft_ctx *c) {
+    std::vector<double> den(c->SG), env(c->SG), V(c->SG), fi;
+    HIPC(hipMemcpyAsync(den.data(), c->d_Den, sizeof(double) * c->SG, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(env.data(), c->d_Env, sizeof(double) * c->SG, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(V.data(), c->d_V, sizeof(double) * c->SG, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    const double CutLine = cut_fields(c, den, env, V, fi);
+    const int EL = c->EL;
+    if (c->movie.empty()) {
+        c->movie.resize(EL);
+        for (int i = 0; i < EL; i++) c->movie[i].push_back(-CutLine / 2. + (double)i * CutLine / ((double)c->s));
+    }
+    for (int i = 0; i < EL; i++)
+        for (int s = 0; s < c->S; s++)
+            for (int p = 0; p < 3; p++) c->movie[i].push_back(fi[((size_t)s * 3 + p) * EL + i]);
+    c->frames++;
     return MPDPFT_OK;
 }
 
@@ -1680,6 +1707,7 @@ static int iteration(mpdpft_ctx *c) {
         bool pr = c->SCcount % base == 0 && c->SCcount / base < 10;
         c->PrintSC = (pr || c->SCcount % 1000 == 0 || c->SCcount == p.maxSCcount || c->ExitSCcount > 0) ? 1 : 0;
     }
+    if (p.MovieQ && c->PrintSC == 1 && (c->ExitSCcount == 0 || c->ExitSCloop)) RC(store_movie_frame(c));  // :5342
     return MPDPFT_OK;
 }
 

@@ -89,6 +89,10 @@ struct mpdpft_ctx {
     std::ostringstream notes; // lines for the next solve's control stream (LoadV)
     int importWarnings = 0;   // warnings of that import, added to the solve's count
     double fom = 0.;          // GetFigureOfMerit of the last solve (task types 5-7; 0 otherwise)
+    // MovieQ: frames of the cut (StoreMovieData, mpDPFT.cpp:12317-12329); row i starts with the cut coordinate,
+    // then per frame, per species the density, environment and potential at cut point i
+    std::vector<std::vector<double>> movie;
+    int frames = 0;
     std::string break_file;   // ManualSCloopBreakQ file ("" = not polled)
     double OldCC = 0.;        // CC of the previous iteration (InjectSchedule 5)
     double thetaVec0[MPDPFT_MAX_SPECIES] = {};  // thetaVecOriginal (InjectSchedule 9)
@@ -116,6 +120,10 @@ const char *params_structure_error(const mpdpft_params *p);  // params.cpp
 int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
             const std::string &notes, double *mpp_out, double *task_grad = nullptr, int *grad_rc = nullptr,
             const std::vector<double> *ref = nullptr, double *fom = nullptr);
+// output.cpp: the cut through the x-y plane (or the line in 1D): returns its length, fills
+// fi[(s 3 + p) EL + i] with species s's density, environment, potential (p = 0, 1, 2) at cut point i
+double cut_fields(const mpdpft_ctx *c, const std::vector<double> &den, const std::vector<double> &env,
+                  const std::vector<double> &V, std::vector<double> &fi);
 // tasks.cpp: TaskType 2 / 3 (the energy's minimum over muVec / the abundances)
 int run_cgd_task(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res,
                  std::ostringstream &log);

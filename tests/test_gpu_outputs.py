@@ -12,8 +12,9 @@ runs the script when it is installed.
 Cube file: the header of a Gaussian cube (nucleus count, three axis lines,
 the environment-41 centres as nuclei), then max(1e-14, n dx^3) over
 [0, steps)^3 with x slowest, equal to the density of the same solve through
-the C ABI (16 printed digits)."""
-import os
+the C ABI (16 printed digits).
+
+Movie frames (MovieQ 1, 2): the cut of every stored frame, checked below."""
 import re
 import shutil
 import subprocess
@@ -107,3 +108,33 @@ def test_cube_file(tmp_path):
     ref = np.maximum(1e-14, n * dx ** 3)
     assert np.abs(vals - ref).max() <= 1e-15 * ref.max()
     assert vals.min() >= 1e-14
+
+
+@pytest.mark.parametrize("movieq", [1, 2])
+def test_movie_frames(tmp_path, movieq):
+    """MovieQ (StoreMovieData mpDPFT.cpp:12317-12329, the export :12660-12691): one
+    row per cut point, the cut coordinate then per frame and species the density,
+    environment and potential (4 digits); the frame count in the script matches
+    the columns, the first column and the last frame's potential are the CutData
+    ones, and MovieQ 2 plots on a log scale."""
+    print("case movie", movieq, flush=True)
+    p = configs.c0(steps=32, edge=8.0, N=6.0, maxsc=60)
+    p.MovieQ = movieq
+    configs.write_input(p, str(tmp_path / "mpDPFT.input"))
+    run_dir(tmp_path)
+    rows = [l.split() for l in (tmp_path / "mpDPFT_MovieData.tmp").read_text().split("\n")]
+    assert not (tmp_path / "mpDPFT_MovieData.tmp").read_text().endswith("\n")
+    assert len(rows) == p.steps + 1
+    ncol = len(rows[0])
+    assert all(len(r) == ncol for r in rows) and (ncol - 1) % (3 * p.S) == 0
+    frames = (ncol - 1) // (3 * p.S)
+    assert frames >= 2
+    sh = (tmp_path / "mpDPFT_Movie.sh").read_text()
+    assert "do for [frame=0:%d]" % (frames - 1) in sh
+    assert ("set logscale y" in sh) == (movieq == 2)
+    cut = np.loadtxt(tmp_path / "mpDPFT_CutData.dat")
+    m = np.array([[float(v) for v in r] for r in rows])
+    assert np.allclose(m[:, 0], cut[:, 0], rtol=1e-3, atol=1e-12)
+    last = 1 + (frames - 1) * 3 * p.S
+    for s in range(p.S):
+        assert np.allclose(m[:, last + 3 * s + 2], cut[:, 3 + 3 * s], rtol=1e-3, atol=1e-9)
