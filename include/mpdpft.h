@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 /* 2: mpdpft_result gained AuxEint; 3: mpdpft_result gained bracket_failures and pulay_rejections */
-#define MPDPFT_ABI_VERSION 4
+#define MPDPFT_ABI_VERSION 5
 
 #define MPDPFT_MAX_SPECIES 8
 #define MPDPFT_MAX_LIST 16
@@ -133,6 +133,9 @@ typedef struct mpdpft_params {
      * variables, the ranges of TaskType 2/3's starting point and scale */
     int n_hyperbox;        /* number of values (twice the pairs) */
     double TaskHyperBox[2 * MPDPFT_MAX_SPECIES];
+    unsigned long long NoiseSeed; /* extension key: seed of the Noise generator (mt19937_64, uniform on
+                              [-1, 1)); 0 seeds it from the clock as the reference does; TaskType 10's
+                              repetition k uses NoiseSeed + k */
     int CubeFile;          /* extension key: 1 writes mpDPFT_Den_Cube.dat (DIM 3) in place of mpDPFT_Den.dat,
                               the environment-41 centres as nuclei */
 } mpdpft_params;

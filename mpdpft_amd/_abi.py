@@ -47,7 +47,7 @@ class Params(ctypes.Structure):
         ("HartreeMode", _i), ("SmoothingKWidth", _d), ("ConvergenceMode", _i), ("Precision", _i),
         ("WriteFiles", _i), ("n_centres", _i), ("centres", (_d * 5) * MAX_CENTRES),
         ("n_hyperbox", _i), ("TaskHyperBox", _d * (2 * MAX_SPECIES)),
-        ("CubeFile", _i),
+        ("NoiseSeed", ctypes.c_uint64), ("CubeFile", _i),
     ]
 
 

@@ -63,6 +63,8 @@ def write_input(p, path, extensions=True):
         ]
         if p.CubeFile:
             lines.append("%d\t\t\t//CubeFile" % p.CubeFile)
+        if p.NoiseSeed:
+            lines.append("%d\t\t\t//NoiseSeed" % p.NoiseSeed)
         for c in range(p.n_centres):
             lines.append("%s\t\t\t//Centre" % " ".join(_fmt(x) for x in p.centres[c]))
     with open(path, "w") as f:

@@ -159,6 +159,8 @@ hipError_t pulay_mix(const Grid &g, int S, int I, const double *const *nin, cons
                      const double *coeff_dev, const double *theta_dev, double *den, hipStream_t st);
 hipError_t linear_mix(size_t n_per_species, int S, const double *theta_dev, const double *old, double *den,
                       hipStream_t st);
+// den[i] *= factors[i], i < n (ImposeNoise with host-drawn factors)
+hipError_t mul_fields(size_t n, const double *factors, double *den, hipStream_t st);
 
 // DelField's TaperOff shift by the shell maximum (mpDPFT.cpp:7251-7264); the taper itself is
 // applied in fft_forward_taper's first pass.

@@ -1387,6 +1387,12 @@ __global__ __launch_bounds__(256) void k_pulay_mix(size_t G, int S, int I, PtrLi
     }
 }
 
+// ImposeNoise (mpDPFT.cpp:4490-4504): den *= the host-drawn factors 1 + Noise RN
+__global__ void k_mul_fields(size_t n, const double *f, double *den) {
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x)
+        den[idx] *= f[idx];
+}
+
 __global__ void k_linear_mix(size_t G, int S, const double *theta, const double *old, double *den) {
     size_t tot = G * (size_t)S;
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
@@ -1565,6 +1571,10 @@ hipError_t pulay_mix(const Grid &g, int S, int I, const double *const *nin, cons
     }
     hipLaunchKernelGGL(k_pulay_mix, dim3(gblocks(g.G * S)), dim3(256), 0, st, g.G, S, I, a, b, coeff_dev,
                        theta_dev, den);
+    return hipGetLastError();
+}
+hipError_t mul_fields(size_t n, const double *factors, double *den, hipStream_t st) {
+    hipLaunchKernelGGL(k_mul_fields, dim3(gblocks(n)), dim3(256), 0, st, n, factors, den);
     return hipGetLastError();
 }
 hipError_t linear_mix(size_t n_per_species, int S, const double *theta_dev, const double *old, double *den,

@@ -514,7 +514,8 @@ static double figure_of_merit(const mpdpft_ctx *c, const std::vector<double> &ta
 
 int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
             const std::string &notes, double *mpp_out, double *task_grad, int *grad_rc,
-            const std::vector<double> *ref, double *fom) {
+            const std::vector<double> *ref, double *fom, const std::vector<double> *v_start,
+            std::vector<double> *v_out) {
     mpdpft_ctx *c = nullptr;
     int rc = mpdpft_create(&p, device, &c);
     if (rc) {
@@ -542,6 +543,12 @@ int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_res
         c->notes << " ***** load existing V ***** \nLoadV: Interpolate V from mpDPFT_V.dat\n";
         if (warn) c->notes << "LoadV: " << warn << " warning(s) in the imported field\n";
         c->importWarnings = warn;
+    } else if (v_start) {  // InterpolVQ = 2: data.V = task.V (LoadV :6500)
+        rc = mpdpft_set_initial_potential(c, v_start->data());
+        if (rc) {
+            mpdpft_destroy(c);
+            return rc;
+        }
     }
     mpdpft_result r;
     int src = mpdpft_solve(c, &r);
@@ -553,6 +560,14 @@ int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_res
         return src;
     }
     if (task_grad) *grad_rc = mpdpft_task_gradient(c, task_grad);
+    if (v_out) {
+        v_out->resize(c->SG);
+        rc = mpdpft_get_potential(c, v_out->data());
+        if (rc) {
+            mpdpft_destroy(c);
+            return rc;
+        }
+    }
     if (ref) {
         std::vector<double> den0(c->SG);
         rc = mpdpft_get_density(c, den0.data());  // [S][G]: species 0 first
@@ -702,6 +717,73 @@ static int run_exploration(const std::string &d, const mpdpft_params &p, int dev
     return last_rc;
 }
 
+// TaskType 10 (mpDPFTmain.cpp:179-198, RetrieveTask mpDPFT.cpp:6124-6131):
+// max(1, TaskParameterCount) repetitions of the solve, which differ by their
+// Noise (repetition k seeds the generator with NoiseSeed + k; the reference's
+// seed comes from the clock), then one more run started from the potential
+// and mu of the repetition with the lowest total energy (InterpolVQ 2) with
+// Schedule 0, Noise 0 and maxSCcount 0: no SC loop, the densities of that
+// potential and the output files.  mpDPFT_ControlTask.dat gets the
+// reference's lines on each new minimum (or no improvement) with the sorted
+// energy history.
+static int run_repetitions(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res) {
+    const int count = std::max(1, p.TaskParameterCount);
+    std::ostringstream task;
+    task << std::setprecision(16);
+    task << " ***** mpdpft_hip task loop: TaskType 10, " << count << " repetition(s) *****\n";
+    std::vector<double> hist, bestV, V;
+    double minE = 0., bestMu[MPDPFT_MAX_SPECIES] = {};
+    mpdpft_result r{};
+    int last_rc = MPDPFT_OK;
+    for (int k = 0; k <= count; k++) {
+        const bool last = k == count;
+        mpdpft_params q = p;
+        std::ostringstream notes;
+        notes << std::setprecision(16);
+        if (!last) {
+            if (p.NoiseSeed) q.NoiseSeed = p.NoiseSeed + (unsigned long long)k;
+        } else {
+            q.InterpolVQ = 0;  // the potential comes from bestV (InterpolVQ 2)
+            for (int s = 0; s < q.S; s++) q.muVec[s] = bestMu[s];
+            q.n_schedule = 1;
+            q.Schedule[0] = 0;
+            q.Noise = 0.;
+            q.maxSCcount = 0;
+            notes << "InterpolVQ -> 2\nmuVec -> " << vec_to_str(q.muVec, q.S)
+                  << "\nSchedule -> 0\nNoise -> 0\nmaxSCcount -> 0\n";
+        }
+        last_rc = run_one(d, q, device, &r, last, notes.str(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                          last ? &bestV : nullptr, last ? nullptr : &V);
+        if (last_rc != MPDPFT_OK && last_rc != MPDPFT_ENOTCONV) break;
+        if (last) break;
+        hist.push_back(r.TotalEnergy);
+        std::sort(hist.begin(), hist.end());
+        std::ostringstream h;
+        h << std::setprecision(16);
+        for (size_t i = 0; i < hist.size(); i++) h << (i ? " " : "") << hist[i];
+        if (k == 0 || r.TotalEnergy < minE) {
+            minE = r.TotalEnergy;
+            bestV.swap(V);
+            for (int s = 0; s < q.S; s++) bestMu[s] = r.muVec[s];
+            task << " ***** task.Type==10: new minEtot = " << minE << " @ task.count = " << k << "\n";
+            task << "\n       Etot history (sorted) = " << h.str() << "\n\n";
+        } else {
+            task << " ***** task.Type==10: no improvement in minEtot @ task.count = " << k
+                 << "\n       Etot history (sorted) = " << h.str() << "\n\n";
+        }
+    }
+    task << "final: muVec " << vec_to_str(r.muVec, p.S) << "  NVec " << vec_to_str(r.FinalAbundances, p.S)
+         << "  TotalEnergy " << r.TotalEnergy << "\n";
+    if (res) *res = r;
+    FILE *f = std::fopen(join(d, "mpDPFT_ControlTask.dat").c_str(), "w");
+    if (f) {
+        std::string sc = task.str();
+        std::fwrite(sc.data(), 1, sc.size(), f);
+        std::fclose(f);
+    }
+    return last_rc;
+}
+
 // The outer task loop of main (mpDPFTmain.cpp:77-214) for task types 0, 1 and 8
 // (2 and 3, the conjugate-gradient minimisations, are in tasks.cpp),
 // with RetrieveTask's per-task parameter changes (mpDPFT.cpp:6070-6121):
@@ -719,15 +801,24 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     int rc = mpdpft_parse_input(join(d, "mpDPFT.input").c_str(), &p);
     if (rc) return rc;
     const int type = p.TaskType;
-    if (type != 0 && type != 1 && type != 2 && type != 3 && type != 5 && type != 6 && type != 7 && type != 8) {
+    bool sched11 = false;
+    for (int i = 0; i < p.n_schedule; i++) sched11 = sched11 || p.Schedule[i] == 11;
+    const char *refuse = nullptr;
+    if (type != 0 && type != 1 && type != 2 && type != 3 && type != 5 && type != 6 && type != 7 && type != 8 &&
+        type != 10)
+        refuse = "is not built (0, 1, 2, 3, 5, 6, 7, 8, 10 are)";
+    else if (type == 8 && sched11 && p.Noise > 0.)  // its last task zeroes NoiseOriginal alone (:903)
+        refuse = "with Schedule 11 and Noise > 0 is not built";
+    if (refuse) {
         FILE *f = std::fopen(join(d, "mpDPFT_Control.dat").c_str(), "w");
         if (f) {
-            std::fprintf(f, "TaskType %d is not built (0, 1, 2, 3, 5, 6, 7, 8 are)\n", type);
+            std::fprintf(f, "TaskType %d %s\n", type, refuse);
             std::fclose(f);
         }
         return MPDPFT_EINVAL;
     }
     if (type >= 5 && type <= 7) return run_exploration(d, p, device, res);
+    if (type == 10) return run_repetitions(d, p, device, res);
     if (type == 2 || type == 3) {
         std::ostringstream task;
         task << " ***** mpdpft_hip task loop: TaskType " << type << " *****\n";

@@ -145,6 +145,7 @@ int parse_extension(const std::string &key, const std::string &val, mpdpft_param
     if (key == "MaxIteration") return first(val, &p->MaxIteration);
     if (key == "WriteFiles") return first(val, &p->WriteFiles);
     if (key == "CubeFile") return first(val, &p->CubeFile);
+    if (key == "NoiseSeed") return first(val, &p->NoiseSeed);
     if (key == "Centre") {  // x y z Z a
         if (p->n_centres >= MPDPFT_MAX_CENTRES) return -1;
         if (fill_fixed(val, 5, p->centres[p->n_centres])) return -1;

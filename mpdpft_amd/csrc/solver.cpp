@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <iomanip>
+#include <limits>
 
 #include "fft_core.h"
 #include "solver.h"
@@ -196,10 +197,10 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         return ctx_fail(c, MPDPFT_EINVAL, "DensityExpression 6 needs Units 1 or 2");
     if (p->DensityExpression == 5 && p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "nAiT is built for DIM 3");
     if (p->method != 1) return ctx_fail(c, MPDPFT_EINVAL, "integration method must be 1 (Riemann)");
-    if (p->TaskType < 0 || p->TaskType > 8 || p->TaskType == 4)  // the task loop itself is mpdpft_run_dir's
-        return ctx_fail(c, MPDPFT_EINVAL, "TaskType must be 0, 1, 2, 3, 5, 6, 7 or 8");
+    if (p->TaskType < 0 || (p->TaskType > 8 && p->TaskType != 10) || p->TaskType == 4)  // the task loop is run_dir's
+        return ctx_fail(c, MPDPFT_EINVAL, "TaskType must be 0, 1, 2, 3, 5, 6, 7, 8 or 10");
     if (p->Symmetry != 0) return ctx_fail(c, MPDPFT_EINVAL, "Symmetry must be 0");
-    if (p->Noise != 0.) return ctx_fail(c, MPDPFT_EINVAL, "Noise must be 0");
+    if (!(p->Noise >= 0.)) return ctx_fail(c, MPDPFT_EINVAL, "Noise must be >= 0");
     if ((int)(p->Mixer[0] + 0.5) > 1) return ctx_fail(c, MPDPFT_EINVAL, "Mixer[0] must be 0 or 1");
     if ((int)p->Mixer[1] > 60) return ctx_fail(c, MPDPFT_EINVAL, "Pulay scope must be <= 60");
     for (int a = 0; a < p->n_interactions; a++) {
@@ -244,8 +245,9 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         return ctx_fail(c, MPDPFT_EINVAL, "InterpolVQ must be 0 or 1 (2, 3 come from task loops)");
     for (int i = 0; i < p->n_schedule; i++) {
         int sc = p->Schedule[i];
-        if (!(sc == 0 || sc == 5 || sc == 9))
-            return ctx_fail(c, MPDPFT_EINVAL, "Schedule " + std::to_string(sc) + " is not built (0, 5, 9 are)");
+        if (!(sc == 0 || sc == 3 || sc == 4 || sc == 5 || sc == 7 || sc == 9 || sc == 11))
+            return ctx_fail(c, MPDPFT_EINVAL,
+                            "Schedule " + std::to_string(sc) + " is not built (0, 3, 4, 5, 7, 9, 11 are)");
     }
     if (p->DensityExpression == 5)
         for (int s = 0; s < p->S; s++)
@@ -1505,6 +1507,8 @@ static void reset_loop_scalars(mpdpft_ctx *c) {
 }
 
 static int store_movie_frame(mpdpft_ctx *c);
+static void seed_noise(mpdpft_ctx *c);
+static int impose_noise(mpdpft_ctx *c);
 
 static int init_loop(mpdpft_ctx *c) {
     const mpdpft_params &p = c->p;
@@ -1540,6 +1544,9 @@ static int init_loop(mpdpft_ctx *c) {
     c->ResMat.assign((size_t)(c->scope + 1) * (c->scope + 1), 0.);
     // initial densities (:194-201)
     RC(get_densities(c));
+    seed_noise(c);
+    c->noise = c->noise0 = p.Noise;
+    RC(impose_noise(c));
     RC(field_stats(c));
     c->movie.clear();
     c->frames = 0;
@@ -1596,14 +1603,77 @@ static int manual_break(mpdpft_ctx *c) {
     return Q;
 }
 
-// InjectSchedule (mpDPFT.cpp:5229-5315) for the schedules that only retune
-// the mixing: 5 (theta up by 1.1 while CC falls, else halved, clipped to
-// [10/maxSCcount, 0.8]) and 9 (theta ramped linearly towards 0.2).
-static void inject_schedule(mpdpft_ctx *c) {
+// InitializeRandomNumberGenerator (mpDPFT.cpp:6521-6540): the seed is the
+// extension key NoiseSeed, or, when that is 0, the reference's own from the
+// clock (the fractional part of ns/1e10 times 1e10, folded into [1e6, INT_MAX)).
+static void seed_noise(mpdpft_ctx *c) {
+    unsigned long long seed = c->p.NoiseSeed;
+    if (seed == 0) {
+        const double MaxInt = (double)std::numeric_limits<int>::max();
+        const double Now =
+            std::fabs((double)std::chrono::high_resolution_clock::now().time_since_epoch().count() / 1.0e10);
+        double sd = 1.0e10 * (Now - (double)((int)Now));
+        if (sd < 1.0e-6) sd = (double)std::rand();
+        while (sd >= MaxInt) sd /= 2.;
+        while (sd < 1.0e6) sd *= 2.;
+        seed = (unsigned long long)(int)sd;
+    }
+    c->rng.seed(seed);
+}
+
+// ImposeNoise(1) (mpDPFT.cpp:4490-4504): Den_s[i] *= 1 + Noise RN with RN
+// uniform on [-1, 1) (std::uniform_real_distribution on std::mt19937_64, as
+// the reference draws them), species by species in grid order (the
+// reference's order with one thread); only with Noise > 0 and InterpolVQ = 0
+// or a schedule > 0.  The factors are drawn on the host and applied on the
+// device.
+static int impose_noise(mpdpft_ctx *c) {
     const mpdpft_params &p = c->p;
+    bool scheduleQ = false;
+    for (int i = 0; i < p.n_schedule; i++)
+        if (p.Schedule[i] > 0) scheduleQ = true;
+    if (!(c->noise > 0. && (p.InterpolVQ == 0 || scheduleQ))) return MPDPFT_OK;
+    if (!c->d_noise) RC(dmalloc(c, &c->d_noise, c->SG));
+    std::uniform_real_distribution<double> RN(-1.0, 1.0);
+    std::vector<double> f(c->SG);
+    for (size_t k = 0; k < c->SG; k++) f[k] = 1. + c->noise * RN(c->rng);
+    HIPC(hipMemcpyAsync(c->d_noise, f.data(), sizeof(double) * c->SG, hipMemcpyHostToDevice, c->st));
+    HIPC(mk::mul_fields(c->SG, c->d_noise, c->d_Den, c->st));
+    HIPC(hipStreamSynchronize(c->st));  // f lives on this stack frame
+    return MPDPFT_OK;
+}
+
+// InjectSchedule (mpDPFT.cpp:5229-5315): 5 (theta up by 1.1 while CC falls,
+// else halved, clipped to [10/maxSCcount, 0.8]), 9 (theta ramped linearly
+// towards 0.2) and the noise schedules on the mixed density: 3 (at SCcount =
+// i 10^e, i = 1..9; at SCcount 0 all nine i match), 4 and 7 (Noise =
+// 0.1 exp(-40 or -10 SCcount/maxSCcount) every iteration), 11 (Noise =
+// NoiseOriginal exp(-40 SCcount/(0.1 maxSCcount)) during the first tenth of
+// maxSCcount).
+static int inject_schedule(mpdpft_ctx *c) {
+    const mpdpft_params &p = c->p;
+    const double sc_frac = (double)c->SCcount / ((double)p.maxSCcount);
     for (int i = 0; i < p.n_schedule; i++) {
         int sc = p.Schedule[i];
-        if (sc == 5) {
+        if (sc == 3) {
+            const double expon = std::floor(std::log10((double)c->SCcount));
+            for (int k = 1; k < 10; k++)
+                if (k * (int)std::pow(10., expon) == c->SCcount) {
+                    c->control << "InjectSchedule: ImposeNoise on densities\n";
+                    RC(impose_noise(c));
+                }
+        } else if (sc == 4 || sc == 7) {
+            c->noise = 0.1 * std::exp((sc == 4 ? -40. : -10.) * sc_frac);
+            RC(impose_noise(c));
+            if (c->PrintSC) c->control << "InjectSchedule: ImposeNoise of magnitude " << c->noise << "\n";
+        } else if (sc == 11) {
+            const bool early = (double)c->SCcount < 0.1 * (double)p.maxSCcount;
+            if (c->noise0 > kMP && early) {
+                c->noise = c->noise0 * std::exp(-40. * (double)c->SCcount / (0.1 * (double)p.maxSCcount));
+                RC(impose_noise(c));
+            }
+            if (c->PrintSC && early) c->control << "InjectSchedule: ImposeNoise of magnitude " << c->noise << "\n";
+        } else if (sc == 5) {
             for (int s = 0; s < c->S; s++) {
                 if (c->CC < c->OldCC) c->thetaVec[s] *= 1.1;
                 else c->thetaVec[s] *= 0.5;
@@ -1619,6 +1689,7 @@ static void inject_schedule(mpdpft_ctx *c) {
                                                            ((double)p.maxSCcount);
         }
     }
+    return MPDPFT_OK;
 }
 
 static int iteration(mpdpft_ctx *c) {
@@ -1697,7 +1768,7 @@ static int iteration(mpdpft_ctx *c) {
         c->control << " *****                       NVec = " << vec_str(c->TmpAb, c->S) << " *****\n";
     }
     RC(admix(c));
-    inject_schedule(c);
+    RC(inject_schedule(c));
     // PrepareNextIteration (:5317-5389): counter and print cadence
     c->SCcount++;
     if (p.Print > 0) c->PrintSC = 1;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -89,6 +90,11 @@ struct mpdpft_ctx {
     std::ostringstream notes; // lines for the next solve's control stream (LoadV)
     int importWarnings = 0;   // warnings of that import, added to the solve's count
     double fom = 0.;          // GetFigureOfMerit of the last solve (task types 5-7; 0 otherwise)
+    // ImposeNoise: the generator of InitializeRandomNumberGenerator (mpDPFT.cpp:6521-6540), seeded per solve,
+    // data.Noise (schedules 4, 7 and 11 change it during the loop), NoiseOriginal, and the factors' buffer
+    std::mt19937_64 rng;
+    double noise = 0., noise0 = 0.;
+    double *d_noise = nullptr;
     // MovieQ: frames of the cut (StoreMovieData, mpDPFT.cpp:12317-12329); row i starts with the cut coordinate,
     // then per frame, per species the density, environment and potential at cut point i
     std::vector<std::vector<double>> movie;
@@ -117,9 +123,12 @@ const char *params_structure_error(const mpdpft_params *p);  // params.cpp
 // converged context, with its return code in *grad_rc.
 // ref (may be null): the n x n TabFunc_ReferenceDensity.dat table; then the figure of merit against it goes into
 // *fom (and the Control file).
+// v_start (may be null): the initial potential [S][G] (InterpolVQ = 2: task.V); v_out (may be null) receives the
+// final potential.
 int run_one(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res, bool write_files,
             const std::string &notes, double *mpp_out, double *task_grad = nullptr, int *grad_rc = nullptr,
-            const std::vector<double> *ref = nullptr, double *fom = nullptr);
+            const std::vector<double> *ref = nullptr, double *fom = nullptr,
+            const std::vector<double> *v_start = nullptr, std::vector<double> *v_out = nullptr);
 // output.cpp: the cut through the x-y plane (or the line in 1D): returns its length, fills
 // fi[(s 3 + p) EL + i] with species s's density, environment, potential (p = 0, 1, 2) at cut point i
 double cut_fields(const mpdpft_ctx *c, const std::vector<double> &den, const std::vector<double> &env,

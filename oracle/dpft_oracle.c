@@ -228,6 +228,9 @@ int oracle_dft_brute(int n, int sign, const double *in, double *out) {
 /* ------------------------------------------------------------------------ */
 struct orc_state {
     mpdpft_params p;
+    unsigned long long mt[312]; /* ImposeNoise's generator (mt19937_64) */
+    int mti;
+    double noise, noise0;       /* data.Noise, NoiseOriginal */
     int D, s, EL, S;
     long long G, F;
     long long CentreIndex;
@@ -2304,6 +2307,62 @@ static void get_energy(orc_state *o) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Noise (ImposeNoise mpDPFT.cpp:4490-4504, InitializeRandomNumberGenerator   */
+/* :6521-6540): the reference draws from std::mt19937_64 through             */
+/* std::uniform_real_distribution<double>(-1, 1).  MT19937-64 as published   */
+/* by Nishimura and Matsumoto (2000; the parameters of the C++ standard's    */
+/* mt19937_64, [rand.predef]), and the draw as libstdc++'s                   */
+/* generate_canonical<double, 53> makes it from one 64-bit output x:         */
+/* u = double(x) / 2^64 (rounded to nearest; 1 becomes the double below 1),  */
+/* RN = 2u - 1.  Only seeds given by the extension key NoiseSeed are         */
+/* reproducible; 0 seeds from the clock.                                     */
+/* ------------------------------------------------------------------------ */
+void oracle_mt64_seed(unsigned long long *mt, int *mti, unsigned long long seed) {
+    mt[0] = seed;
+    for (int i = 1; i < 312; i++) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (unsigned long long)i;
+    *mti = 312;
+}
+
+unsigned long long oracle_mt64_next(unsigned long long *mt, int *mti) {
+    const unsigned long long upper = 0xFFFFFFFF80000000ULL, lower = 0x7FFFFFFFULL, A = 0xB5026F5AA96619E9ULL;
+    if (*mti >= 312) {
+        for (int i = 0; i < 312; i++) {
+            unsigned long long y = (mt[i] & upper) | (mt[(i + 1) % 312] & lower);
+            mt[i] = mt[(i + 156) % 312] ^ (y >> 1) ^ ((y & 1ULL) ? A : 0ULL);
+        }
+        *mti = 0;
+    }
+    unsigned long long x = mt[(*mti)++];
+    x ^= (x >> 29) & 0x5555555555555555ULL;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+    x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+    x ^= x >> 43;
+    return x;
+}
+
+double oracle_rn(unsigned long long *mt, int *mti) {
+    double u = (double)oracle_mt64_next(mt, mti) / 18446744073709551616.0;
+    if (u >= 1.) u = nextafter(1., 0.);
+    return u * 2. + -1.;
+}
+
+static void orc_seed_noise(orc_state *o) {
+    unsigned long long seed = o->p.NoiseSeed;
+    if (seed == 0) seed = (unsigned long long)time(NULL) % 2147483647ULL + 1000000ULL;
+    oracle_mt64_seed(o->mt, &o->mti, seed);
+}
+
+/* ImposeNoise(1): Den_s[i] *= 1 + Noise RN, species by species in grid order,
+ * when Noise > 0 and InterpolVQ = 0 or a schedule is > 0. */
+static void orc_impose_noise(orc_state *o) {
+    int scheduleQ = 0;
+    for (int i = 0; i < o->p.n_schedule; i++)
+        if (o->p.Schedule[i] > 0) scheduleQ = 1;
+    if (!(o->noise > 0. && (o->p.InterpolVQ == 0 || scheduleQ))) return;
+    for (long long k = 0; k < (long long)o->S * o->G; k++) o->Den[k] *= 1. + o->noise * oracle_rn(o->mt, &o->mti);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Driver, GetData (mpDPFT.cpp:135-244)                                      */
 /* ------------------------------------------------------------------------ */
 static int orc_init_loop(orc_state *o) {
@@ -2315,6 +2374,9 @@ static int orc_init_loop(orc_state *o) {
     memcpy(o->V, o->V0 ? o->V0 : o->Env, sizeof(double) * (size_t)(o->S * o->G));
     int rc = get_densities(o);
     if (rc) return rc;
+    orc_seed_noise(o);
+    o->noise = o->noise0 = o->p.Noise;
+    orc_impose_noise(o);
     field_stats(o);
     /* InitializePulayMixer (mpDPFT.cpp:7029-7062); RAM guard not applied,
      * metric weight 0 (Euclidean) when Mixer[2] == 0. */
@@ -2346,11 +2408,28 @@ static int orc_init_loop(orc_state *o) {
     return 0;
 }
 
-/* InjectSchedule (mpDPFT.cpp:5229-5315), schedules 5 and 9: theta retuning. */
+/* InjectSchedule (mpDPFT.cpp:5229-5315): 5 and 9 retune theta, 3, 4, 7 and 11
+ * put noise on the mixed density. */
 static void inject_schedule(orc_state *o) {
+    const double mx = (double)o->p.maxSCcount, sc_ = (double)o->SCcount;
     for (int i = 0; i < o->p.n_schedule; i++) {
         int sc = o->p.Schedule[i];
-        if (sc == 5) { /* :5261-5264 */
+        if (sc == 3) { /* :5252-5255; log10(0) = -inf: at SCcount 0 every i matches */
+            double expon = floor(log10(sc_));
+            for (int k = 1; k < 10; k++)
+                if (k * (int)pow(10., expon) == o->SCcount) orc_impose_noise(o);
+        } else if (sc == 4) { /* :5256-5260 */
+            o->noise = 0.1 * exp(-40. * sc_ / mx);
+            orc_impose_noise(o);
+        } else if (sc == 7) { /* :5272-5276 */
+            o->noise = 0.1 * exp(-10. * sc_ / mx);
+            orc_impose_noise(o);
+        } else if (sc == 11) { /* :5300-5306 */
+            if (o->noise0 > MP_ && sc_ < 0.1 * mx) {
+                o->noise = o->noise0 * exp(-40. * sc_ / (0.1 * mx));
+                orc_impose_noise(o);
+            }
+        } else if (sc == 5) { /* :5261-5264 */
             for (int s = 0; s < o->S; s++) {
                 o->thetaVec[s] *= o->CC < o->OldCC ? 1.1 : 0.5;
                 double lo = 10. / (double)o->p.maxSCcount;

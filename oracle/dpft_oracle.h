@@ -82,6 +82,10 @@ int oracle_enforce(const mpdpft_params *p, const double *v, double *mu, double *
 int oracle_mix_sequence(const mpdpft_params *p, int nsteps, const double *n_in0, const double *n_outs, double *out,
                         int *rejections);
 double oracle_time_iterations(const mpdpft_params *p, int n_iter, int *bisection_steps);
+/* ImposeNoise's generator: MT19937-64 and the uniform draw on [-1, 1) */
+void oracle_mt64_seed(unsigned long long *mt, int *mti, unsigned long long seed);
+unsigned long long oracle_mt64_next(unsigned long long *mt, int *mti);
+double oracle_rn(unsigned long long *mt, int *mti);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,12 @@ def lib():
             getattr(L, n).restype = ctypes.c_double
         L.oracle_qmc_beta.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_qmc_beta.restype = ctypes.c_double
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.oracle_mt64_seed.argtypes = [u64p, ctypes.POINTER(ctypes.c_int), ctypes.c_uint64]
+        L.oracle_mt64_next.argtypes = [u64p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_mt64_next.restype = ctypes.c_uint64
+        L.oracle_rn.argtypes = [u64p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_rn.restype = ctypes.c_double
         L.oracle_bessel_jn.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_bessel_jn.restype = ctypes.c_double
         L.oracle_polylog.argtypes = [ctypes.c_int, ctypes.c_double]
@@ -311,6 +317,17 @@ def qmc_beta(der, x):
 def exp_i0(x):
     """e^-x I0(x) (2D periodic Hartree compensation)."""
     return lib().oracle_exp_i0(float(x))
+
+
+def noise_draws(seed, n, raw=False):
+    """The first n draws of ImposeNoise's generator seeded with seed: the
+    MT19937-64 outputs (raw) or the uniform values on [-1, 1)."""
+    L = lib()
+    mt = (ctypes.c_uint64 * 312)()
+    mti = ctypes.c_int(0)
+    L.oracle_mt64_seed(mt, ctypes.byref(mti), int(seed))
+    f = L.oracle_mt64_next if raw else L.oracle_rn
+    return [f(mt, ctypes.byref(mti)) for _ in range(n)]
 
 
 def bessel_jn(n, x):

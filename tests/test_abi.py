@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_layouts_and_version():
     lib = _abi.load_library()
-    assert lib.mpdpft_abi_version() == 4
+    assert lib.mpdpft_abi_version() == 5
     assert lib.mpdpft_params_sizeof() == ctypes.sizeof(_abi.Params)
     assert lib.mpdpft_result_sizeof() == ctypes.sizeof(_abi.Result)
 
