@@ -804,9 +804,7 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     bool sched11 = false;
     for (int i = 0; i < p.n_schedule; i++) sched11 = sched11 || p.Schedule[i] == 11;
     const char *refuse = nullptr;
-    if (type != 0 && type != 1 && type != 2 && type != 3 && type != 5 && type != 6 && type != 7 && type != 8 &&
-        type != 10)
-        refuse = "is not built (0, 1, 2, 3, 5, 6, 7, 8, 10 are)";
+    if (type < 0 || type > 10 || type == 4) refuse = "is not built (0-3, 5-10 are)";
     else if (type == 8 && sched11 && p.Noise > 0.)  // its last task zeroes NoiseOriginal alone (:903)
         refuse = "with Schedule 11 and Noise > 0 is not built";
     if (refuse) {
@@ -819,11 +817,11 @@ extern "C" int mpdpft_run_dir(const char *dir, int device, mpdpft_result *res) {
     }
     if (type >= 5 && type <= 7) return run_exploration(d, p, device, res);
     if (type == 10) return run_repetitions(d, p, device, res);
-    if (type == 2 || type == 3) {
+    if (type == 2 || type == 3 || type == 9) {
         std::ostringstream task;
         task << " ***** mpdpft_hip task loop: TaskType " << type << " *****\n";
         mpdpft_result r{};
-        int rc2 = run_cgd_task(d, p, device, &r, task);
+        int rc2 = type == 9 ? run_gd_task(d, p, device, &r, task) : run_cgd_task(d, p, device, &r, task);
         if (res) *res = r;
         FILE *f = std::fopen(join(d, "mpDPFT_ControlTask.dat").c_str(), "w");
         if (f) {

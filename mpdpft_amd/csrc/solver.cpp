@@ -197,8 +197,8 @@ static int validate(mpdpft_ctx *c, const mpdpft_params *p) {
         return ctx_fail(c, MPDPFT_EINVAL, "DensityExpression 6 needs Units 1 or 2");
     if (p->DensityExpression == 5 && p->DIM != 3) return ctx_fail(c, MPDPFT_EINVAL, "nAiT is built for DIM 3");
     if (p->method != 1) return ctx_fail(c, MPDPFT_EINVAL, "integration method must be 1 (Riemann)");
-    if (p->TaskType < 0 || (p->TaskType > 8 && p->TaskType != 10) || p->TaskType == 4)  // the task loop is run_dir's
-        return ctx_fail(c, MPDPFT_EINVAL, "TaskType must be 0, 1, 2, 3, 5, 6, 7, 8 or 10");
+    if (p->TaskType < 0 || p->TaskType > 10 || p->TaskType == 4)  // the task loop is run_dir's
+        return ctx_fail(c, MPDPFT_EINVAL, "TaskType must be 0, 1, 2, 3, 5, 6, 7, 8, 9 or 10");
     if (p->Symmetry != 0) return ctx_fail(c, MPDPFT_EINVAL, "Symmetry must be 0");
     if (!(p->Noise >= 0.)) return ctx_fail(c, MPDPFT_EINVAL, "Noise must be >= 0");
     if ((int)(p->Mixer[0] + 0.5) > 1) return ctx_fail(c, MPDPFT_EINVAL, "Mixer[0] must be 0 or 1");

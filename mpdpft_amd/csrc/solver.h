@@ -136,6 +136,9 @@ double cut_fields(const mpdpft_ctx *c, const std::vector<double> &den, const std
 // tasks.cpp: TaskType 2 / 3 (the energy's minimum over muVec / the abundances)
 int run_cgd_task(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res,
                  std::ostringstream &log);
+// tasks.cpp: TaskType 9 (annealed gradient descent over the abundances)
+int run_gd_task(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res,
+                std::ostringstream &log);
 int ctx_fail(mpdpft_ctx *c, int code, const std::string &msg);
 int hip_fail(mpdpft_ctx *c, hipError_t e, const char *what);
 int ctx_delfield(mpdpft_ctx *c, const double *f_dev, int s, int method);

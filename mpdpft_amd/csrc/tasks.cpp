@@ -18,7 +18,9 @@
 // The start is the HyperBox centre (the reference draws a uniform point of the
 // box from its own random stream).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <random>
 #include <iomanip>
 #include <sstream>
 #include <string>
@@ -264,6 +266,193 @@ int run_cgd_task(const std::string &d, const mpdpft_params &p, int device, mpdpf
     log << std::setprecision(16) << "ConjugateGradientDescent: Emin = " << fr.TotalEnergy << " @ "
         << (type == 2 ? "muVec" : "Abundances") << " = " << vstr(x) << "\n";
     log << "ConjugateGradientDescent: gradient at the minimum = " << vstr(g) << "\n";
+    if (res) *res = fr;
+    return rc;
+}
+
+namespace {
+
+// libstdc++'s std::uniform_real_distribution<double>(a, b) on std::mt19937_64:
+// one 64-bit output x, u = double(x) / 2^64 (1 becomes the double below 1),
+// a + (b - a) u -- spelled out so that the draws do not depend on the
+// library's distribution object.
+double draw(std::mt19937_64 &g, double a, double b) {
+    double u = (double)g() / 18446744073709551616.0;
+    if (u >= 1.) u = std::nextafter(1., 0.);
+    return u * (b - a) + a;
+}
+
+double norm(const std::vector<double> &v) {
+    double s = 0.;
+    for (double x : v) s += x * x;
+    return std::sqrt(s);
+}
+
+}  // namespace
+
+// TaskType 9 (GetGlobalMinimum_Abundances_AnnealedGradientDescent
+// mpDPFT.cpp:604-608, GradientDescent :8586-8633, gradient_descent
+// :8635-8710): the total energy's minimum over the abundances by
+// Barzilai-Borwein gradient descent on forward differences (step gammaStart,
+// one solve per abundance plus the base point) with random annealing kicks,
+// restarted from uniform points of the TaskHyperBox until TaskParameterCount
+// descents end at the same minimum (within 10 AbsoluteTargetAcc in f and in
+// |x - xmin|^2).  The random stream is the reference's (std::mt19937_64, the
+// start point's uniform draws and the kicks' RNpos, RN, RNpos in that order),
+// seeded with NoiseSeed (the reference seeds it from the clock).  Where the
+// reference reads ErrorHistory[-2] at the second iteration (the anneal step's
+// comparison with a history of one entry) this treats the error as not grown.
+// A cap of 10 TaskParameterCount + 10 descents ends a search whose count of
+// matching descents steps past TaskParameterCount (the reference's test is an
+// equality).  The files are those of a last solve at the final point.
+int run_gd_task(const std::string &d, const mpdpft_params &p, int device, mpdpft_result *res,
+                std::ostringstream &log) {
+    log << std::setprecision(16);
+    const int n = p.n_hyperbox / 2;
+    if (n != p.S || n < 1) return MPDPFT_EINVAL;
+    const double *box = p.TaskHyperBox;
+    double scale = 0., avgN = 0.;
+    for (int i = 0; i < n; i++) scale += (box[2 * i + 1] - box[2 * i]) / (double)n;
+    for (int s = 0; s < p.S; s++) avgN += p.Abundances[s] / (double)p.S;
+    scale = std::min(scale, avgN);
+    const double acc = p.RelAcc > kMPt ? p.RelAcc * scale : 1.0e-4;
+    double gammaStart = 1.0e-2 * scale, AnnealMagnitude = 3., fmin = 0.;
+    const double AnnealThreshold = 0.7;
+    const int maxIter = 200, maxDescents = 10 * std::max(1, p.TaskParameterCount) + 10;
+    std::mt19937_64 rng;
+    unsigned long long seed = p.NoiseSeed;
+    if (seed == 0)
+        seed = (unsigned long long)std::chrono::high_resolution_clock::now().time_since_epoch().count() % 2147483647ULL +
+               1000000ULL;
+    rng.seed(seed);
+    std::vector<std::vector<double>> reports, starts;
+    std::vector<double> x(n), xmin(n);
+    int solves = 0;
+    bool success = false, aborted = false;
+    mpdpft_result last{};
+    auto energy = [&](const std::vector<double> &at, double *f, mpdpft_result *r) -> int {
+        mpdpft_params q = p;
+        for (int s = 0; s < n; s++) q.Abundances[s] = at[s];  // data.Abundances = GradDesc.x (:6122)
+        mpdpft_result rr{};
+        int rc = run_one(d, q, device, &rr, false, "", nullptr);
+        solves++;
+        if (rc != MPDPFT_OK && rc != MPDPFT_ENOTCONV) return rc;
+        *f = rr.TotalEnergy;
+        if (r) *r = rr;
+        return rr.SCcount >= p.maxSCcount ? 1 : 0;  // the solve hit maxSCcount: abort the descent
+    };
+    while (!success && (int)reports.size() < maxDescents) {
+        for (int i = 0; i < n; i++) x[i] = box[2 * i] + draw(rng, 0., 1.) * (box[2 * i + 1] - box[2 * i]);
+        starts.push_back(x);
+        // gradient_descent
+        double f = 0., gamma = gammaStart, Error = acc + 1., anneal = 0.;
+        int iter = 0, successCount = 0, fokCount = 0;
+        bool ok = false;
+        std::vector<double> xOld(x), df(n, 0.), dfOld(n, 0.), fdiff(n), dfHist, fHist, errHist;
+        std::vector<std::vector<double>> xHist;
+        while (iter < maxIter && (Error > acc || !ok) && fokCount < 10) {
+            xHist.push_back(x);
+            if (iter > 0) xOld = xHist[xHist.size() - 2];
+            dfOld = df;
+            bool stop = false;
+            for (int i = 0; i <= n && !stop; i++) {
+                std::vector<double> at(x);
+                if (i < n) at[i] += gammaStart;
+                double fi = 0.;
+                int rc = energy(at, &fi, i == n ? &last : nullptr);
+                if (rc < 0) return rc;
+                if (rc == 1) stop = true;
+                else if (i == n) f = fi;
+                else fdiff[i] = fi;
+            }
+            if (stop) {
+                aborted = true;
+                break;
+            }
+            for (int i = 0; i < n; i++) df[i] = (fdiff[i] - f) / gammaStart;
+            dfHist.push_back(norm(df));
+            fHist.push_back(f);
+            const double I = std::min(10., (double)dfHist.size());
+            double dfAv = 0., fAv = 0.;
+            for (int k = 1; k <= (int)(I + 0.5); k++) {
+                dfAv += dfHist[dfHist.size() - k] / I;
+                fAv += fHist[fHist.size() - k] / I;
+            }
+            if (iter > 0) {
+                if (std::fabs(f - fAv) < p.RelAcc * std::fabs(f + fAv)) fokCount++;
+                else fokCount = 0;
+                if (norm(df) < AnnealThreshold * dfAv) {
+                    const double fp = fHist[fHist.size() - 2];
+                    const double asym = (f == 0. && fp == 0.) ? 0. : (f - fp) / (f + fp);
+                    const double r1 = draw(rng, 0., 1.), r2 = draw(rng, -1., 1.), r3 = draw(rng, 0., 1.);
+                    const double heav = (0.5 + 0.5 * std::tanh(10. * asym)) / std::sqrt((double)iter) - r1 > 0. ? 1. : 0.;
+                    anneal = heav * r2 * (AnnealMagnitude + std::pow(AnnealMagnitude * AnnealMagnitude, r3));
+                    const bool grew = errHist.size() >= 2 && errHist[errHist.size() - 1] > errHist[errHist.size() - 2];
+                    gammaStart *= grew ? 0.5 : 1.2;
+                } else
+                    anneal = 0.;
+                double dxdf = 0., df2 = 0.;
+                for (int i = 0; i < n; i++) {
+                    dxdf += (x[i] - xOld[i]) * (df[i] - dfOld[i]);
+                    df2 += (df[i] - dfOld[i]) * (df[i] - dfOld[i]);
+                }
+                gamma = df2 > 0. ? std::fabs(dxdf) / df2 : gammaStart;
+                if (anneal != 0.) gamma *= anneal;
+            }
+            std::vector<double> xDiff(n), xNew(n);
+            for (int i = 0; i < n; i++) xDiff[i] = df[i] * gamma;
+            if (norm(xDiff) == 0.) break;
+            for (int i = 0; i < n; i++) xNew[i] = std::min(std::max(x[i] - xDiff[i], box[2 * i]), box[2 * i + 1]);
+            Error = norm(xDiff);
+            errHist.push_back(Error);
+            successCount = Error <= acc ? successCount + 1 : 0;
+            if (successCount >= 3) ok = true;
+            log << "gradient_descent: iter = " << iter + 1 << "/" << maxIter << ": f = " << f << "\n gamma = " << gamma
+                << "(anneal=" << anneal << ") -> Error = " << Error << "(" << acc << ") ---> xNew = " << vstr(xNew)
+                << " successCount = " << successCount << "/3 step = " << gammaStart << "\n";
+            x = xNew;
+            iter++;
+        }
+        std::vector<double> rep(x);
+        rep.insert(rep.end(), df.begin(), df.end());
+        rep.push_back(f);
+        rep.push_back((double)iter);
+        rep.push_back(Error);
+        rep.push_back(gamma);
+        reports.push_back(rep);
+        if (iter == maxIter) AnnealMagnitude *= 0.7;
+        else AnnealMagnitude /= 0.8;
+        if (reports.size() == 1 || f < fmin) {
+            fmin = f;
+            xmin = x;
+        }
+        int enc = 0;
+        for (size_t r = 0; r < reports.size(); r++) {
+            double dot = 0., a2 = 0., b2 = 0.;
+            for (int i = 0; i < n; i++) {
+                dot += reports[r][i] * xmin[i];
+                a2 += reports[r][i] * reports[r][i];
+                b2 += xmin[i] * xmin[i];
+            }
+            if (std::fabs(reports[r][2 * n] - fmin) < 10. * acc && std::fabs(2. * dot - (a2 + b2)) < 10. * acc) enc++;
+            log << "f = " << reports[r][2 * n] << " @ x = " << vstr(std::vector<double>(reports[r].begin(),
+                                                                                      reports[r].begin() + n))
+                << " [xStart = " << vstr(starts[r]) << "] <--report #" << r + 1 << "\n";
+        }
+        if (enc == p.TaskParameterCount) success = true;
+        log << "GradientDescent: report " << reports.size() << " (x df f iter Error gamma)\n"
+            << "                 fminEncounter   = " << enc << " fmin = " << fmin << " xmin = " << vstr(xmin) << "\n"
+            << "                 AnnealMagnitude = " << AnnealMagnitude << "\n";
+        if (aborted) break;
+    }
+    log << "                 Local minimum f = " << reports.back()[2 * n] << " @ x = " << vstr(x) << "\n";
+    log << "GradientDescent: " << reports.size() << " descent(s), " << solves << " solves"
+        << (success ? "" : (aborted ? " (a solve hit maxSCcount)" : " (descent cap reached)")) << "\n";
+    // the final point's files
+    mpdpft_params q = p;
+    for (int s = 0; s < n; s++) q.Abundances[s] = x[s];
+    mpdpft_result fr{};
+    int rc = run_one(d, q, device, &fr, true, "", nullptr);
     if (res) *res = fr;
     return rc;
 }

@@ -115,3 +115,37 @@ def test_task_loop_minimum_matches_the_oracle(oracle, tmp_path, task_type):
     assert r.converged and E == r.TotalEnergy
     assert abs(x - xo) < 1e-5 * max(abs(xo), 0.1), (x, xo)
     assert abs(E - Eo) < 1e-10 * abs(Eo), (E, Eo)
+
+
+def test_task_type_9_annealed_descent(oracle, tmp_path):
+    """TaskType 9 (GradientDescent mpDPFT.cpp:8586-8710) on the same cluster,
+    abundance box [3, 6], one descent (TaskParameterCount 1), seed 8642: the
+    start is 3 + 3 u with u the generator's first draw (the oracle's MT19937-64
+    gives it), the energies the log prints at its first iterates are the
+    oracle's at those abundances, and the lowest printed energy reaches the
+    oracle's minimum over N (secant on central differences) to 1e-8."""
+    print("case task 9", flush=True)
+    p = cluster()
+    p.TaskType = 9
+    p.TaskParameterCount = 1
+    p.NoiseSeed = 8642
+    p.n_hyperbox = 2
+    p.TaskHyperBox[0], p.TaskHyperBox[1] = 3.0, 6.0
+    configs.write_input(p, str(tmp_path / "mpDPFT.input"))
+    run_dir(tmp_path)
+    text = (tmp_path / "mpDPFT_ControlTask.dat").read_text()
+    start = float(re.search(r"\[xStart = (\S+)\]", text).group(1))
+    u = oracle.noise_draws(8642, 1, raw=True)[0] / 2.0 ** 64
+    assert abs(start - (3.0 + u * 3.0)) <= 1e-15 * start
+    its = re.findall(r"iter = (\d+)/200: f = (\S+)\n gamma = .*?---> xNew = (\S+)", text)
+    assert len(its) >= 3, text[-3000:]
+    xs = [start] + [float(t[2]) for t in its]
+    fs = [float(t[1]) for t in its]
+    for k in (0, 1, 2, len(its) - 1):
+        Eo = oracle_energy(oracle, cluster(xs[k]))
+        assert abs(fs[k] - Eo) <= 1e-10 * abs(Eo), (k, xs[k], fs[k], Eo)
+    xo, Eo = oracle_minimum(oracle, lambda n: cluster(n), 4.0, 4.5)
+    print("task 9: %d iterations, lowest f %.16g at %.10g; oracle minimum %.16g at %.10g" %
+          (len(its), min(fs), xs[fs.index(min(fs))], Eo, xo), flush=True)
+    assert Eo - 1e-10 * abs(Eo) <= min(fs) <= Eo + 1e-8 * abs(Eo)
+    assert "Local minimum f = " in text and (tmp_path / "mpDPFT_Den.dat").exists()

@@ -49,8 +49,9 @@ def test_empty_hyperbox(tmp_path):
     assert "is empty" in run(tmp_path, base(3, 1e-10, (6.0, 3.0)))
 
 
-def test_unbuilt_task_type(tmp_path):
-    assert "TaskType 9 is not built" in run(tmp_path, base(9, 1e-10))
+@pytest.mark.parametrize("task_type", [4, 11, 44])
+def test_unbuilt_task_type(tmp_path, task_type):
+    assert "TaskType %d is not built" % task_type in run(tmp_path, base(task_type, 1e-10))
 
 
 def test_hyperbox_round_trips_through_the_input_file(tmp_path):
